@@ -1,0 +1,310 @@
+"""ORACLE -- test infrastructure, not product code.
+
+ctypes bindings over ``oracle/liboracle.so`` (the C restatement of the
+reference's MB-pol 2B/3B path, see oracle.h) and, where it was built, over
+``oracle/_ref/libmbxref.so`` (the reference's own leaves compiled in place,
+see ref_shim.cpp).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+CPU-baseline leg import this package, and only as the checker.
+
+Parity status per oracle function (what pins it) is recorded in oracle.h.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+TABLES = os.path.join(ROOT, "mbx_amd", "tables")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_szp = ctypes.POINTER(ctypes.c_size_t)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+class _OrcPoly(ctypes.Structure):
+    _fields_ = [("nvars", ctypes.c_int), ("ncoef", ctypes.c_int), ("nmono", ctypes.c_int),
+                ("coef", _dp), ("mono_k", _ip), ("mono_deg", _ip), ("mono_idx", _ip)]
+
+
+class Table:
+    """A polynomial monomial table (mbx_amd/tables/<name>.tbl)."""
+
+    def __init__(self, name):
+        self.name = name
+        path = os.path.join(TABLES, name + ".tbl")
+        lines = [l.split("#")[0].strip() for l in open(path)]
+        lines = [l for l in lines if l]
+        it = iter(lines)
+        hdr = {}
+        for l in it:
+            if l == "coef":
+                break
+            k, v = l.split()
+            hdr[k] = int(v)
+        self.nvars, self.ncoef, self.nmono = hdr["nvars"], hdr["ncoef"], hdr["nmono"]
+        self.coef = np.array([float(next(it)) for _ in range(self.ncoef)])
+        assert next(it).startswith("monomials")
+        k = np.zeros(self.nmono, np.int32)
+        deg = np.zeros(self.nmono, np.int32)
+        idx = -np.ones((self.nmono, 4), np.int32)
+        for m in range(self.nmono):
+            v = [int(t) for t in next(it).split()]
+            k[m], deg[m] = v[0], v[1]
+            idx[m, :v[1]] = v[2:]
+        self.mono_k, self.mono_deg, self.mono_idx = k, deg, idx
+        self.c = _OrcPoly(self.nvars, self.ncoef, self.nmono, _d(self.coef), _i(self.mono_k),
+                          _i(self.mono_deg), _i(self.mono_idx))
+
+
+_TABLES = {}
+
+
+def table(name):
+    if name not in _TABLES:
+        _TABLES[name] = Table(name)
+    return _TABLES[name]
+
+
+def build():
+    """Compile liboracle.so (and the reference library when the checkout is present)."""
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=subprocess.DEVNULL)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "ref"], stdout=subprocess.DEVNULL)
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER(_OrcPoly)
+        L.orc_poly_eval.restype = ctypes.c_double
+        L.orc_poly_eval.argtypes = [P, _dp, _dp, _dp]
+        L.orc_poly_basis.argtypes = [P, _dp, _dp]
+        L.orc_mbpol2b.restype = ctypes.c_double
+        L.orc_mbpol2b.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_mbpol3b.restype = ctypes.c_double
+        L.orc_mbpol3b.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_invert_unit_cell.argtypes = [_dp, _dp]
+        L.orc_fix_monomer_coordinates.argtypes = [_dp, _dp, _dp, ctypes.c_int, _ip, _ip]
+        L.orc_close_image.argtypes = [_dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_clusters.restype = ctypes.c_long
+        L.orc_clusters.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip,
+                                   _ip, ctypes.c_int, _ip, ctypes.c_long]
+        L.orc_water_system.argtypes = [P, P, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                       _dp, _dp, _lp, _ip, ctypes.c_long, _ip, ctypes.c_long, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def poly_eval(tab, x, a=None, grad=True):
+    x = _arr(x)
+    g = np.zeros(tab.nvars) if grad else None
+    a = _arr(a) if a is not None else None
+    e = lib().orc_poly_eval(ctypes.byref(tab.c), _d(a), _d(x), _d(g))
+    return (e, g) if grad else e
+
+
+def poly_basis(tab, x):
+    x = _arr(x)
+    b = np.zeros(tab.ncoef)
+    lib().orc_poly_basis(ctypes.byref(tab.c), _d(x), _d(b))
+    return b
+
+
+def mbpol2b(w1, w2, a=None):
+    """One dimer: returns (E, g1, g2, virial, evaluated)."""
+    w1, w2 = _arr(w1), _arr(w2)
+    g1, g2, v = np.zeros(9), np.zeros(9), np.zeros(9)
+    ev = ctypes.c_int(0)
+    a = _arr(a) if a is not None else None
+    e = lib().orc_mbpol2b(ctypes.byref(table("mbpol_2b").c), _d(a), _d(w1), _d(w2), _d(g1), _d(g2), _d(v),
+                          ctypes.byref(ev))
+    return e, g1, g2, v, ev.value
+
+
+def mbpol3b(w1, w2, w3, a=None):
+    """One trimer: returns (E, g1, g2, g3, virial, evaluated)."""
+    w1, w2, w3 = _arr(w1), _arr(w2), _arr(w3)
+    g1, g2, g3, v = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(9)
+    ev = ctypes.c_int(0)
+    a = _arr(a) if a is not None else None
+    e = lib().orc_mbpol3b(ctypes.byref(table("mbpol_3b").c), _d(a), _d(w1), _d(w2), _d(w3), _d(g1), _d(g2),
+                          _d(g3), _d(v), ctypes.byref(ev))
+    return e, g1, g2, g3, v, ev.value
+
+
+def invert_unit_cell(box):
+    box = _arr(box)
+    bi = np.zeros(9)
+    if lib().orc_invert_unit_cell(_d(box), _d(bi)):
+        raise ValueError("singular box")
+    return bi
+
+
+def fix_monomer_coordinates(xyz, box, nat):
+    xyz = _arr(xyz).copy()
+    box = _arr(box)
+    nat = np.ascontiguousarray(nat, np.int32)
+    fi = np.concatenate([[0], np.cumsum(nat)[:-1]]).astype(np.int32)
+    lib().orc_fix_monomer_coordinates(_d(xyz), _d(box), _d(invert_unit_cell(box)), len(nat), _i(nat), _i(fi))
+    return xyz
+
+
+def close_image(box, xyz1, xyz2):
+    box = _arr(box)
+    xyz2 = _arr(xyz2).copy()
+    lib().orc_close_image(_d(box), _d(invert_unit_cell(box)), _d(_arr(xyz1)), len(xyz2) // 3, _d(xyz2))
+    return xyz2
+
+
+def clusters(nmax, cutoff, first_xyz, box=None, islocal=None, tags=None, use_ghost=False):
+    """AddClusters restatement.  Returns an (n, nmax) int array in the oracle's emission order."""
+    first_xyz = _arr(first_xyz).reshape(-1)
+    nmon = len(first_xyz) // 3
+    loc = np.ones(nmon, np.int32) if islocal is None else np.ascontiguousarray(islocal, np.int32)
+    tag = np.arange(nmon, dtype=np.int32) if tags is None else np.ascontiguousarray(tags, np.int32)
+    b = _arr(box) if box is not None and len(box) else None
+    bi = invert_unit_cell(b) if b is not None else None
+    L = lib()
+    n = L.orc_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), _i(loc), _i(tag),
+                       int(use_ghost), None, 0)
+    out = np.zeros((max(n, 1), nmax), np.int32)
+    L.orc_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), _i(loc), _i(tag),
+                   int(use_ghost), _i(out), n)
+    return out[:n]
+
+
+def water_system(xyz, box, cutoff2b, cutoff3b, lists=False, per_nmer=False):
+    """2B+3B energy/gradient of an all-water system (9 doubles per water, O H H)."""
+    xyz = _arr(xyz).reshape(-1).copy()
+    nmon = len(xyz) // 9
+    b = _arr(box) if box is not None and len(box) else None
+    e2, e3 = ctypes.c_double(), ctypes.c_double()
+    grad = np.zeros(9 * nmon)
+    vir = np.zeros(9)
+    counts = np.zeros(4, np.int64)
+    res = {}
+    d = t = ed = et = None
+    cap2 = cap3 = 0
+    if lists or per_nmer:
+        # first pass for the counts
+        lib().orc_water_system(ctypes.byref(table("mbpol_2b").c), ctypes.byref(table("mbpol_3b").c), nmon,
+                               _d(xyz.copy()), _d(b), cutoff2b, 0.0, ctypes.byref(e2), ctypes.byref(e3),
+                               _d(grad), _d(vir), counts.ctypes.data_as(_lp), None, 0, None, 0, None, None)
+        cap2 = int(counts[0])
+        n3 = len(clusters(3, cutoff3b, fix_first(xyz, b), b))
+        cap3 = n3
+        d = np.zeros((max(cap2, 1), 2), np.int32)
+        t = np.zeros((max(cap3, 1), 3), np.int32)
+        if per_nmer:
+            ed = np.zeros(max(cap2, 1))
+            et = np.zeros(max(cap3, 1))
+    lib().orc_water_system(ctypes.byref(table("mbpol_2b").c), ctypes.byref(table("mbpol_3b").c), nmon, _d(xyz),
+                           _d(b), cutoff2b, cutoff3b, ctypes.byref(e2), ctypes.byref(e3), _d(grad), _d(vir),
+                           counts.ctypes.data_as(_lp), _i(d), cap2, _i(t), cap3, _d(ed), _d(et))
+    res.update(e2b=e2.value, e3b=e3.value, grad=grad, virial=vir, xyz=xyz, n2=int(counts[0]),
+               n2_eval=int(counts[1]), n3=int(counts[2]), n3_eval=int(counts[3]))
+    if lists or per_nmer:
+        res["dimers"] = d[:cap2]
+        res["trimers"] = t[:cap3]
+    if per_nmer:
+        res["e_dimer"] = ed[:cap2]
+        res["e_trimer"] = et[:cap3]
+    return res
+
+
+def fix_first(xyz, box):
+    """First-atom coordinates of each water after FixMonomerCoordinates (if periodic)."""
+    xyz = _arr(xyz).reshape(-1)
+    nmon = len(xyz) // 9
+    if box is not None:
+        xyz = fix_monomer_coordinates(xyz, box, [3] * nmon)
+    return xyz.reshape(nmon, 9)[:, :3].copy()
+
+
+# --------------------------------------------------------------------------------------------
+# The compiled reference leaves (oracle/_ref/libmbxref.so), when built.
+# --------------------------------------------------------------------------------------------
+
+_REF = None
+REF_PATH = os.path.join(HERE, "_ref", "libmbxref.so")
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        L = ctypes.CDLL(REF_PATH)
+        L.ref_poly2b.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp]
+        L.ref_poly2b_nogrd.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
+        L.ref_poly3b.argtypes = [_dp, _dp, _dp, _dp]
+        L.ref_poly3b_nogrd.restype = ctypes.c_double
+        L.ref_poly3b_nogrd.argtypes = [_dp, _dp]
+        L.ref_x2b.restype = ctypes.c_double
+        L.ref_x2b.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp, _dp]
+        L.ref_x3b.restype = ctypes.c_double
+        L.ref_x3b.argtypes = [_dp, _dp, _dp, ctypes.c_size_t, _dp, _dp, _dp, _dp]
+        L.ref_invert_unit_cell.argtypes = [_dp, _dp]
+        L.ref_fix_monomer_coordinates.argtypes = [_dp, ctypes.c_size_t, _dp, _dp, ctypes.c_size_t, _szp, _szp]
+        L.ref_close_dimer_image.argtypes = [_dp, _dp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _dp, _dp]
+        L.ref_add_clusters.restype = ctypes.c_size_t
+        L.ref_add_clusters.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_int, _dp, _dp, _dp,
+                                       ctypes.c_size_t, _szp, _szp, _ip, ctypes.c_int, _szp, ctypes.c_size_t]
+        L.ref_water_2b3b.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_size_t,
+                                     ctypes.c_int, _dp, _dp, _dp, _dp, _szp]
+        _REF = L
+    return _REF
+
+
+def ref_add_clusters(nmax, cutoff, first_xyz, box=None):
+    first_xyz = _arr(first_xyz).reshape(-1)
+    nmon = len(first_xyz) // 3
+    fi = np.arange(nmon, dtype=np.uintp)
+    loc = np.ones(nmon, np.uintp)
+    tag = np.arange(nmon, dtype=np.int32)
+    b = _arr(box) if box is not None else None
+    bi = invert_unit_cell(b) if b is not None else None
+    sz = lambda a: a.ctypes.data_as(_szp)
+    L = ref()
+    n = L.ref_add_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), nmon, sz(fi),
+                           sz(loc), _i(tag), 0, None, 0)
+    out = np.zeros((max(n, 1), nmax), np.uintp)
+    L.ref_add_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), nmon, sz(fi), sz(loc),
+                       _i(tag), 0, sz(out), n)
+    return out[:n].astype(np.int64)
+
+
+def ref_water_2b3b(xyz, box, cutoff2b, cutoff3b, batch=8, nthreads=1):
+    xyz = _arr(xyz).reshape(-1).copy()
+    nmon = len(xyz) // 9
+    b = _arr(box) if box is not None else None
+    e2, e3 = ctypes.c_double(), ctypes.c_double()
+    grad, vir = np.zeros(9 * nmon), np.zeros(9)
+    counts = np.zeros(4, np.uintp)
+    ref().ref_water_2b3b(nmon, _d(xyz), _d(b), cutoff2b, cutoff3b, batch, nthreads, ctypes.byref(e2),
+                         ctypes.byref(e3), _d(grad), _d(vir), counts.ctypes.data_as(_szp))
+    return dict(e2b=e2.value, e3b=e3.value, grad=grad, virial=vir, n2=int(counts[0]), n3=int(counts[1]), xyz=xyz)

@@ -1,0 +1,91 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * A plain-C, CPU-only restatement of the reference MBX 2B/3B short-range path
+ * (MB-pol water), used only by tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg as the checker.  The HIP path never calls into it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference's src/).  Parity pins (tests that would catch a plausible
+ * mistake in each function) are listed next to each declaration.
+ */
+#ifndef MBX_ORACLE_H
+#define MBX_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A polynomial E(x) = sum_k a_k sum_{m in S_k} prod_{i in m} x_i, given as a monomial table
+ * (mbx_amd/tables/<name>.tbl).  mono_idx holds 4 variable indices per monomial (-1 padded).
+ * Restates poly_2b_v6x / poly_3b_v2x (potential/2b/poly-2b-v6x.cpp:12936,
+ * potential/3b/poly-3b-v2x.cpp:26982) by direct monomial summation.
+ * Pinned by: the reference's 21-digit KATs (unittests/unittest-poly-2b.cpp:2537-3012,
+ * unittest-poly-3b.cpp:1720-2160) and the compiled reference polynomials on random x. */
+typedef struct {
+    int nvars;
+    int ncoef;
+    int nmono;
+    const double* coef;  /* [ncoef] */
+    const int* mono_k;   /* [nmono] coefficient index */
+    const int* mono_deg; /* [nmono] */
+    const int* mono_idx; /* [4*nmono] */
+} orc_poly;
+
+/* Returns E; if g != NULL writes dE/dx (nvars values, overwritten). */
+double orc_poly_eval(const orc_poly* p, const double* a, const double* x, double* g);
+/* basis[k] = sum_{m in S_k} prod x  (dE/da_k), ncoef values. */
+void orc_poly_basis(const orc_poly* p, const double* x, double* basis);
+
+/* MB-pol 2B holder for ONE dimer (potential/2b/x2b-v9x.cpp:1514-1820).  w1, w2: O,H,H (9 each),
+ * w2 already imaged next to w1.  g1, g2 (+=) and vir (+=, 9) may be NULL.  Returns sw*P.
+ * 'a' may be NULL for the fitted coefficients.  *evaluated is set to 1 if r_OO <= r2f.
+ * Pinned by: unittest-poly-holder-2b.cpp:975 and unittest-energy2b.cpp:48-97 KATs, the compiled
+ * reference x2b_v9x on random dimers, and central finite differences of E. */
+double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const double* w2, double* g1, double* g2,
+                   double* vir, int* evaluated);
+
+/* MB-pol 3B holder for ONE trimer (potential/3b/x3b-v2x.cpp:1403-1806).  Same conventions. */
+double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const double* w2, const double* w3,
+                   double* g1, double* g2, double* g3, double* vir, int* evaluated);
+
+/* InvertUnitCell (tools/math_tools.cpp:37-64).  Returns 0, or -1 if singular. */
+int orc_invert_unit_cell(const double* box, double* box_inv);
+
+/* FixMonomerCoordinates (bblock/sys_tools.cpp:257-320).  xyz: 3 doubles per site. */
+void orc_fix_monomer_coordinates(double* xyz, const double* box, const double* box_inv, int nmon, const int* nat,
+                                 const int* first_index);
+
+/* GetCloseDimerImage for one n-mer member (bblock/sys_tools.cpp:322-365): shift the nat2
+ * atoms of xyz2 by whole cells so its first atom is the image closest (fractionally) to
+ * xyz1's first atom. */
+void orc_close_image(const double* box, const double* box_inv, const double* xyz1, int nat2, double* xyz2);
+
+/* AddClusters (bblock/sys_tools.cpp:472-641) with XyzToCloud (kdtree/kdtree_utils.h:196-279),
+ * GetFractionalCoordinates (:165-194) and nanoflann's strict radius test
+ * (kdtree/nanoflann.hpp:166-171), by brute force over the point cloud.
+ * first_xyz: 3 doubles per monomer (its first atom).  Writes nmax indices per n-mer to out
+ * (up to cap n-mers) and returns the number found.  Queries all local monomers in order.
+ * Pinned by: unittest-systools.cpp:852-1129 counts, unittest-system.cpp:644-718 explicit
+ * sets, the compiled reference AddClusters, and an O(N^2)/O(N^3) minimum-image brute force. */
+long orc_clusters(int nmax, double cutoff, int nmon, int use_pbc, const double* box, const double* box_inv,
+                  const double* first_xyz, const int* islocal, const int* tag, int use_ghost, int* out, long cap);
+
+/* Pure-water System::Energy-restricted-to-2B+3B (bblock/system.cpp:1995-2138 with Get2B
+ * :2425-2725 and Get3B :2769-3073): wraps xyz in place when box != NULL, lists, images,
+ * evaluates, scatters.  xyz/grad: 9 doubles per water (O,H,H).  Tags are 3*i.
+ * Outputs: e2b, e3b, grad (overwritten), virial (overwritten), counts[4] =
+ * {listed dimers, evaluated dimers, listed trimers, evaluated trimers}.
+ * If e_dimer/e_trimer are non-NULL they receive per-listed-n-mer energies in list order, and
+ * dimers/trimers (if non-NULL, capacity cap2/cap3 n-mers) receive the lists. */
+int orc_water_system(const orc_poly* p2, const orc_poly* p3, int nmon, double* xyz, const double* box,
+                     double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
+                     long* counts, int* dimers, long cap2, int* trimers, long cap3, double* e_dimer,
+                     double* e_trimer);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
