@@ -1,0 +1,107 @@
+/*
+ * mbxg -- C ABI of the MI355X (gfx950) MBX many-body short-range engine.
+ *
+ * One context owns one HIP stream and all device memory it allocates; contexts are
+ * independent (thread-safe across contexts, not within one).  Every function returns 0 on
+ * success or a negative status; mbxg_last_error() describes the last failure of the calling
+ * thread.  No function takes ownership of caller memory.  Units: Angstrom, kcal/mol;
+ * gradients are +dE/dx (force = -grad), as in the reference.
+ *
+ * The engine replaces the reference's 2B/3B orchestration inside bblock::System
+ * (src/bblock/system.cpp:2425 Get2B, :2769 Get3B) together with what they call:
+ * systools::AddClusters (src/bblock/sys_tools.cpp:472), GetCloseDimerImage (:322),
+ * FixMonomerCoordinates (:257), e2b::get_2b_energy (src/potential/2b/energy2b.cpp:206),
+ * e3b::get_3b_energy (src/potential/3b/energy3b.cpp:139) and the MB-pol holders
+ * x2b_v9x::eval (src/potential/2b/x2b-v9x.cpp:1514) / x3b_v2x::eval
+ * (src/potential/3b/x3b-v2x.cpp:1403).  The C++ bblock::System of this project
+ * (include/mbx/system.h) drives it; the reference-compatible extern "C" entry points
+ * (include/mbx/external_call.h) sit on top of that.
+ */
+#ifndef MBXG_H
+#define MBXG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mbxg_ctx mbxg_ctx;
+
+/* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", ...). */
+enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2 };
+/* Terms for mbxg_compute. */
+enum { MBXG_2B = 1, MBXG_3B = 2 };
+/* Status codes. */
+enum { MBXG_OK = 0, MBXG_EINVAL = -1, MBXG_EHIP = -2, MBXG_ESTATE = -3, MBXG_ENOMEM = -4 };
+
+/* Text of the last error raised on this thread ("" if none). */
+const char* mbxg_last_error(void);
+
+/* Create a context on HIP device `device`. */
+int mbxg_create(mbxg_ctx** ctx, int device);
+int mbxg_destroy(mbxg_ctx* ctx);
+
+/* Topology, in the engine's (internal) monomer order; monomer i's atoms follow monomer i-1's.
+ * kind[i]: MBXG_H2O (3 atoms O H H), MBXG_CH4 (5 atoms C H H H H) or MBXG_OTHER (nat[i] atoms,
+ * contributes nothing, like an unknown pair in energy2b.cpp:372-374).  tag[i] orders monomers
+ * for n-mer ownership (the reference uses the first atom's tag, sys_tools.cpp:558-592).
+ * islocal may be NULL (all local).  Replaces System::AddMonomer/Initialize bookkeeping
+ * (system.cpp:848-870, 1794-1969). */
+int mbxg_set_topology(mbxg_ctx* ctx, int nmon, const int* kind, const int* nat, const int* tag,
+                      const unsigned char* islocal);
+
+/* Box as 9 numbers (rows a, b, c), or NULL for the gas phase (System::SetPBC, system.cpp:712). */
+int mbxg_set_box(mbxg_ctx* ctx, const double* box9);
+
+/* 2B / 3B cutoffs (System::Set2bCutoff / Set3bCutoff, system.h:826,834).  0 disables a term. */
+int mbxg_set_cutoffs(mbxg_ctx* ctx, double cutoff2b, double cutoff3b);
+
+/* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163).
+ * a == NULL restores the fitted values. */
+int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
+
+/* Coordinates of all atoms (3*natoms doubles, internal order) from host or device memory.
+ * They become the context's working copy, which mbxg_compute wraps in place under PBC the way
+ * System::SetPBC wraps xyz_ (sys_tools.cpp:257-320). */
+int mbxg_set_coords(mbxg_ctx* ctx, const double* xyz);
+int mbxg_set_coords_device(mbxg_ctx* ctx, const double* d_xyz);
+
+/* Run lists + evaluation for `terms` (MBXG_2B | MBXG_3B) on the context stream.  Results stay
+ * on the device until mbxg_get_results / the device accessors. */
+int mbxg_compute(mbxg_ctx* ctx, int terms, int do_grads);
+
+/* Wait for the context stream and copy results: energies, gradient (3*natoms, internal
+ * order; may be NULL), virial (9; may be NULL; the reference's ordering with [3]=[1],
+ * [6]=[2], [7]=[5]). */
+int mbxg_get_results(mbxg_ctx* ctx, double* e2b, double* e3b, double* grad, double* virial);
+
+/* counts[0..3] = listed dimers, evaluated dimers, listed trimers, evaluated trimers. */
+int mbxg_get_counts(mbxg_ctx* ctx, int64_t* counts);
+
+/* Listed n-mers of the last compute (monomer indices, emission order), up to cap n-mers each;
+ * either pointer may be NULL. */
+int mbxg_get_lists(mbxg_ctx* ctx, int32_t* dimers, int64_t cap2, int32_t* trimers, int64_t cap3);
+
+/* Evaluated n-mers (order 2 or 3) and their energies, up to cap n-mers. */
+int mbxg_get_nmer_energies(mbxg_ctx* ctx, int order, int32_t* nmers, double* energies, int64_t cap);
+
+/* Working (wrapped) coordinates, 3*natoms doubles. */
+int mbxg_get_coords(mbxg_ctx* ctx, double* xyz);
+
+/* Device pointers into the context (valid until the next compute that grows buffers):
+ * gradient (3*natoms) and totals (14 doubles: e2b and its 6 virial components xx xy xz yy yz zz,
+ * then e3b and its 6). */
+int mbxg_device_outputs(mbxg_ctx* ctx, double** d_grad, double** d_totals);
+
+/* The context's HIP stream (hipStream_t), for callers that enqueue around the engine. */
+int mbxg_stream(mbxg_ctx* ctx, void** stream);
+
+/* Block until the context's stream is idle. */
+int mbxg_synchronize(mbxg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

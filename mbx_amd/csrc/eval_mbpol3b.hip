@@ -1,0 +1,210 @@
+// MB-pol 3B energy + gradient for gfx950: one lane per evaluated trimer, one wavefront per block.
+//
+// Restates x2o::x3b_v2x::eval (potential/3b/x3b-v2x.cpp:1403-1806) for trimers that survive
+// the 4.5 A rule (the list builder compacts them):
+//   * monomers 2 and 3 imaged next to monomer 1 (GetCloseTrimerImage, sys_tools.cpp:438-442);
+//   * 36 variables exp(-k (d - d0)) over intra/inter atom pairs (x3b-v2x.cpp:1548-1584);
+//   * the degree-4 polynomial and its gradient (generated, gen/poly_mbpol_3b.inc), with the
+//     fitted coefficients read as uniform scalar loads;
+//   * s = sab*sac + sab*sbc + sac*sbc of cosine switches, chain rule to atoms, switch
+//     derivatives and the per-trimer virial (x3b-v2x.cpp:1614-1726);
+//   * gradients scattered with hardware FP64 atomics; energy and virial reduced per wave.
+// Compiled with -ffp-contract=off; the polynomial's FMAs are explicit.
+#include "mbxg_device.hpp"
+#include "mbxg_kernels.hpp"
+
+#include "gen/poly_mbpol_3b.inc"
+
+namespace mbxg {
+
+namespace {
+
+// Variable table: atom slots 0..8 = Oa Ha1 Ha2 Ob Hb1 Hb2 Oc Hc1 Hc2.
+struct V3 {
+    unsigned char set, a1, a2;
+};
+__constant__ const V3 kVar3[36] = {
+    {0, 1, 2}, {0, 4, 5}, {0, 7, 8}, {1, 0, 1}, {1, 0, 2}, {1, 3, 4}, {1, 3, 5}, {1, 6, 7}, {1, 6, 8},
+    {2, 1, 4}, {2, 1, 5}, {2, 1, 7}, {2, 1, 8}, {2, 2, 4}, {2, 2, 5}, {2, 2, 7}, {2, 2, 8}, {2, 4, 7},
+    {2, 4, 8}, {2, 5, 7}, {2, 5, 8}, {3, 0, 4}, {3, 0, 5}, {3, 0, 7}, {3, 0, 8}, {3, 3, 1}, {3, 3, 2},
+    {3, 3, 7}, {3, 3, 8}, {3, 6, 1}, {3, 6, 2}, {3, 6, 4}, {3, 6, 5}, {4, 0, 3}, {4, 0, 6}, {4, 3, 6}};
+
+__device__ __forceinline__ double kset(int s) {
+    return s == 0 ? p3::kHH_intra : s == 1 ? p3::kOH_intra : s == 2 ? p3::kHH : s == 3 ? p3::kOH : p3::kOO;
+}
+__device__ __forceinline__ double dset(int s) {
+    return s == 0 ? p3::dHH_intra : s == 1 ? p3::dOH_intra : s == 2 ? p3::dHH : s == 3 ? p3::dOH : p3::dOO;
+}
+
+// f_switch (x3b-v2x.cpp:1275-1288) with r3i = 0.
+__device__ __forceinline__ double switch3(double r, double& g) {
+    if (r > p3::r3f) {
+        g = 0.0;
+        return 0.0;
+    } else if (r > p3::r3i) {
+        const double t1 = M_PI / (p3::r3f - p3::r3i);
+        const double x = (r - p3::r3i) * t1;
+        g = -sin(x) * t1 / 2.0;
+        return (1.0 + cos(x)) / 2.0;
+    }
+    g = 0.0;
+    return 1.0;
+}
+
+// Load the 9 atoms of a water trimer, imaging monomers 2 and 3 next to monomer 1.  `z` is an
+// opaque zero so a second call after the polynomial reloads instead of reusing registers.
+__device__ __forceinline__ void load_trimer(const NmerCtx& N, int m1, int m2, int m3, int z, double* w) {
+    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1] + z;
+    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2] + z;
+    const double* c = N.xyz + 3 * (size_t)N.first_atom[m3] + z;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        w[j] = a[j];
+        w[9 + j] = b[j];
+        w[18 + j] = c[j];
+    }
+    if (N.periodic) {
+        const Box& B = N.box[z];
+        double dx, dy, dz;
+        image_shift(B, w[0], w[1], w[2], w[9], w[10], w[11], dx, dy, dz);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) image_apply(B, dx, dy, dz, w[9 + 3 * j], w[10 + 3 * j], w[11 + 3 * j]);
+        image_shift(B, w[0], w[1], w[2], w[18], w[19], w[20], dx, dy, dz);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) image_apply(B, dx, dy, dz, w[18 + 3 * j], w[19 + 3 * j], w[20 + 3 * j]);
+    }
+}
+
+// O-O separations and the switch product s = sab*sac + sab*sbc + sac*sbc (x3b-v2x.cpp:1495-1509,
+// 1614-1620); g* receive the switch derivatives.
+__device__ __forceinline__ double switches(const double* w, double* dab, double* dac, double* dbc, double& rab,
+                                           double& rac, double& rbc, double& sab, double& sac, double& sbc,
+                                           double& gab, double& gac, double& gbc) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        dab[j] = w[j] - w[9 + j];
+        dac[j] = w[j] - w[18 + j];
+        dbc[j] = w[9 + j] - w[18 + j];
+    }
+    rab = sqrt(dab[0] * dab[0] + dab[1] * dab[1] + dab[2] * dab[2]);
+    rac = sqrt(dac[0] * dac[0] + dac[1] * dac[1] + dac[2] * dac[2]);
+    rbc = sqrt(dbc[0] * dbc[0] + dbc[1] * dbc[1] + dbc[2] * dbc[2]);
+    sab = switch3(rab, gab);
+    sac = switch3(rac, gac);
+    sbc = switch3(rbc, gbc);
+    return sab * sac + sab * sbc + sac * sbc;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
+                                                uint64_t coef_addr, int do_grads,
+                                                double* __restrict__ grad, double* __restrict__ e_out,
+                                                double* __restrict__ partials) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const mbxg_cptr C = (mbxg_cptr)coef_addr;
+    double e_sum = 0.0;
+    double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t < n) {
+        const int m1 = list[3 * t], m2 = list[3 * t + 1], m3 = list[3 * t + 2];
+        // phase 1: variables (x3b-v2x.cpp:1548-1584) and the switch product
+        double x[36];
+        double s;
+        {
+            double w[27];
+            load_trimer(N, m1, m2, m3, 0, w);
+#pragma unroll
+            for (int j = 0; j < 36; ++j) {
+                const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
+                const double d0 = w[3 * a1] - w[3 * a2];
+                const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
+                const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
+                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                x[j] = exp(-kset(st) * (d - dset(st)));
+            }
+            double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
+            s = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
+        }
+        double P;
+        if (do_grads) {
+            // phase 2: polynomial and its gradient; only x, g, s and the indices are live
+            double g[36];
+            P = poly_mbpol_3b(C, x, g);
+            // phase 3: reload the trimer, chain rule (x3b-v2x.cpp:63-76, 1638-1686)
+            const int z = opaque_zero();
+            double w[27];
+            load_trimer(N, m1, m2, m3, z, w);
+            double gd[27];
+#pragma unroll
+            for (int j = 0; j < 27; ++j) gd[j] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 36; ++j) {
+                const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
+                const double d0 = w[3 * a1] - w[3 * a2];
+                const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
+                const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
+                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                const double gj = g[j] * s;
+                const double gg = -kset(st) * gj * x[j] / d;
+                gd[3 * a1] += gg * d0;
+                gd[3 * a1 + 1] += gg * d1;
+                gd[3 * a1 + 2] += gg * d2;
+                gd[3 * a2] -= gg * d0;
+                gd[3 * a2 + 1] -= gg * d1;
+                gd[3 * a2 + 2] -= gg * d2;
+            }
+            double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
+            switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
+            gab *= (sac + sbc) * P / rab;
+            gac *= (sab + sbc) * P / rac;
+            gbc *= (sab + sac) * P / rbc;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gd[c] += gab * dab[c] + gac * dac[c];
+                gd[9 + c] += gbc * dbc[c] - gab * dab[c];
+                gd[18 + c] -= gac * dac[c] + gbc * dbc[c];
+            }
+            // virial of this trimer (x3b-v2x.cpp:1687-1726): -sum_atoms w_a g_b
+            const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double acc = 0.0;
+#pragma unroll
+                for (int at = 0; at < 9; ++at) acc -= w[3 * at + va[q]] * gd[3 * at + vb[q]];
+                vir[q] = acc;
+            }
+            const int mon[3] = {m1, m2, m3};
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double* gp = grad + 3 * (size_t)N.first_atom[mon[m]];
+#pragma unroll
+                for (int j = 0; j < 9; ++j) unsafeAtomicAdd(gp + j, gd[9 * m + j]);
+            }
+        } else {
+            P = poly_mbpol_3b_e(C, x);
+        }
+        const double e = s * P;
+        e_out[t] = e;
+        e_sum = e;
+    }
+    // per-wave partials (one wave per block)
+    e_sum = wave_sum(e_sum);
+    double* out = partials + (size_t)blockIdx.x * kPartials;
+    if (threadIdx.x == 0) out[0] = e_sum;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double v = wave_sum(vir[q]);
+        if (threadIdx.x == 0) out[1 + q] = v;
+    }
+}
+
+hipError_t launch_mbpol3b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
+                          double* e_out, double* partials, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + 63) / 64;
+    hipLaunchKernelGGL(k_mbpol3b, dim3(nb), dim3(64), 0, st, list, n, N, (uint64_t)(uintptr_t)coef, do_grads, grad,
+                       e_out, partials);
+    return hipGetLastError();
+}
+
+}  // namespace mbxg

@@ -1,0 +1,194 @@
+"""ctypes binding of the mbxg C ABI (include/mbxg.h) -- the GPU engine behind bblock::System.
+
+The shared library ``mbx_amd/libmbxg.so`` is built in-tree (``__graft_entry__.build()`` or
+``make -C mbx_amd/csrc``).  There is no CPU fallback: if the library or a GPU is missing,
+constructing an :class:`Engine` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmbxg.so")
+
+H2O, CH4, OTHER = 0, 1, 2
+TERM_2B, TERM_3B = 1, 2
+KIND_OF = {"h2o": H2O, "ch4": CH4}
+NAT_OF = {H2O: 3, CH4: 5}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+_LIB = None
+
+# name: (restype, argtypes) for every function include/mbxg.h declares
+PROTOTYPES = {
+    "mbxg_last_error": (ctypes.c_char_p, []),
+    "mbxg_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    "mbxg_destroy": (ctypes.c_int, [_vp]),
+    "mbxg_set_topology": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_ubyte)]),
+    "mbxg_set_box": (ctypes.c_int, [_vp, _dp]),
+    "mbxg_set_cutoffs": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "mbxg_set_coefficients": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.c_int]),
+    "mbxg_set_coords": (ctypes.c_int, [_vp, _dp]),
+    "mbxg_set_coords_device": (ctypes.c_int, [_vp, _vp]),
+    "mbxg_compute": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "mbxg_get_results": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "mbxg_get_counts": (ctypes.c_int, [_vp, _i64p]),
+    "mbxg_get_lists": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i32p, ctypes.c_int64]),
+    "mbxg_get_nmer_energies": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, ctypes.c_int64]),
+    "mbxg_get_coords": (ctypes.c_int, [_vp, _dp]),
+    "mbxg_device_outputs": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "mbxg_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mbxg_synchronize": (ctypes.c_int, [_vp]),
+}
+
+
+class MbxgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmbxg.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MbxgError("mbx_amd/libmbxg.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise MbxgError(lib().mbxg_last_error().decode() or ("mbxg status %d" % rc))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Engine:
+    """One mbxg context (one HIP stream, its own device memory)."""
+
+    def __init__(self, device=0):
+        self._c = _vp()
+        _check(lib().mbxg_create(ctypes.byref(self._c), device))
+        self.natoms = 0
+        self.nmon = 0
+
+    def close(self):
+        if self._c:
+            lib().mbxg_destroy(self._c)
+            self._c = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_topology(self, kinds, nats=None, tags=None, islocal=None):
+        kinds = np.ascontiguousarray(kinds, np.int32)
+        n = len(kinds)
+        nats = np.ascontiguousarray([NAT_OF[k] for k in kinds] if nats is None else nats, np.int32)
+        tags = np.ascontiguousarray(np.arange(n) if tags is None else tags, np.int32)
+        loc = None if islocal is None else np.ascontiguousarray(islocal, np.uint8)
+        _check(lib().mbxg_set_topology(self._c, n, kinds.ctypes.data_as(_ip), nats.ctypes.data_as(_ip),
+                                       tags.ctypes.data_as(_ip),
+                                       loc.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)) if loc is not None else None))
+        self.nmon = n
+        self.natoms = int(nats.sum())
+
+    def set_box(self, box):
+        if box is None or len(box) == 0:
+            _check(lib().mbxg_set_box(self._c, None))
+        else:
+            b = np.ascontiguousarray(box, np.float64)
+            if b.size != 9:
+                raise ValueError("box needs 9 numbers")
+            _check(lib().mbxg_set_box(self._c, _d(b)))
+
+    def set_cutoffs(self, c2, c3):
+        _check(lib().mbxg_set_cutoffs(self._c, float(c2), float(c3)))
+
+    def set_coefficients(self, model, a=None):
+        if a is None:
+            _check(lib().mbxg_set_coefficients(self._c, model.encode(), None, 0))
+        else:
+            a = np.ascontiguousarray(a, np.float64)
+            _check(lib().mbxg_set_coefficients(self._c, model.encode(), _d(a), len(a)))
+
+    def set_coords(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        if xyz.size != 3 * self.natoms:
+            raise ValueError("expected %d coordinates, got %d" % (3 * self.natoms, xyz.size))
+        _check(lib().mbxg_set_coords(self._c, _d(xyz)))
+
+    def set_coords_device(self, ptr):
+        _check(lib().mbxg_set_coords_device(self._c, ctypes.c_void_p(ptr)))
+
+    def compute(self, terms=TERM_2B | TERM_3B, do_grads=True):
+        _check(lib().mbxg_compute(self._c, int(terms), int(bool(do_grads))))
+
+    def results(self, grad=True):
+        e2, e3 = ctypes.c_double(), ctypes.c_double()
+        g = np.zeros(3 * self.natoms) if grad else None
+        v = np.zeros(9)
+        _check(lib().mbxg_get_results(self._c, ctypes.byref(e2), ctypes.byref(e3), _d(g) if grad else None, _d(v)))
+        return e2.value, e3.value, g, v
+
+    def counts(self):
+        c = np.zeros(4, np.int64)
+        _check(lib().mbxg_get_counts(self._c, c.ctypes.data_as(_i64p)))
+        return c
+
+    def lists(self):
+        c = self.counts()
+        d = np.zeros((max(int(c[0]), 1), 2), np.int32)
+        t = np.zeros((max(int(c[2]), 1), 3), np.int32)
+        _check(lib().mbxg_get_lists(self._c, d.ctypes.data_as(_i32p), int(c[0]), t.ctypes.data_as(_i32p), int(c[2])))
+        return d[:c[0]], t[:c[2]]
+
+    def nmer_energies(self, order):
+        c = self.counts()
+        n = int(c[1] if order == 2 else c[3])
+        idx = np.zeros((max(n, 1), order), np.int32)
+        e = np.zeros(max(n, 1))
+        _check(lib().mbxg_get_nmer_energies(self._c, order, idx.ctypes.data_as(_i32p), _d(e), n))
+        return idx[:n], e[:n]
+
+    def coords(self):
+        x = np.zeros(3 * self.natoms)
+        _check(lib().mbxg_get_coords(self._c, _d(x)))
+        return x
+
+    def device_outputs(self):
+        g, t = _vp(), _vp()
+        _check(lib().mbxg_device_outputs(self._c, ctypes.byref(g), ctypes.byref(t)))
+        return g.value, t.value
+
+    def stream(self):
+        s = _vp()
+        _check(lib().mbxg_stream(self._c, ctypes.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        _check(lib().mbxg_synchronize(self._c))
+
+
+def water_engine(nmon, box, cutoff2b, cutoff3b, device=0):
+    """An engine set up for nmon waters (tags = input order)."""
+    e = Engine(device)
+    e.set_topology([H2O] * nmon)
+    e.set_box(box)
+    e.set_cutoffs(cutoff2b, cutoff3b)
+    return e

@@ -1,0 +1,299 @@
+"""Generate straight-line FP64 energy+gradient code for a permutationally-invariant polynomial.
+
+The polynomial is given by a monomial table (mbx_amd/tables/<name>.tbl):
+
+    E(x) = sum_k a_k * sum_{m in S_k} prod_{i in m} x_i ,   deg(m) <= 4.
+
+The generated device function evaluates E and dE/dx for ONE n-mer per lane,
+with every variable and gradient accumulator in registers and every
+coefficient a uniform (scalar) load.  The schedule is a reverse-mode walk of a
+monomial trie built so that the adjoint of every trie node is a product of its
+ancestors' edge variables, which the forward walk already has:
+
+    level-3 node (p,q,r):  R = c_pqr + sum_l C_l x_l          (leaves l)
+                           g_l += C_l * pi_pqr                 (pi_pqr = x_p x_q x_r)
+                           g_r += pi_pq * R ;  R_pq += x_r * R
+    level-2 node (p,q):    g_q += x_p * R_pq ; R_p += x_q * R_pq
+    level-1 node p:        g_p += R_p ;        E += x_p * R_p
+
+so a degree-4 monomial costs two FMAs (one for E, one for the gradient) and
+every interior node three.  Degree-4 monomials are attached to level-3 nodes
+by a greedy set cover (each monomial may drop any one of its four variables),
+which keeps the number of level-3 nodes low.
+
+Output: a C++ include with
+    __device__ __forceinline__ double <fn>(mbxg_cptr C, const double* x, double* g)
+where C is the derived coefficient table: C[q] = a[ck[q]] * cs[q] with the
+(ck, cs) map emitted alongside (the scale cs is 1 except for symmetry-reduced
+tables).  A second function <fn>_e evaluates E only.
+
+Every level-3 node opens with MBXG_NODE_FENCE(C) (poly_support.hpp): a scheduling barrier
+plus an opaque redefinition of the coefficient pointer.  It keeps each node's coefficient
+loads and temporaries local to the node, so the register allocator never sees tens of
+thousands of simultaneously live values, and keeps compile time linear in the code size.
+"""
+import collections
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+TABLES = os.path.join(ROOT, "mbx_amd", "tables")
+
+
+def load_table(name):
+    """Returns (nvars, coef list, [(k, mono tuple)])."""
+    lines = [l.split("#")[0].strip() for l in open(os.path.join(TABLES, name + ".tbl"))]
+    lines = [l for l in lines if l]
+    it = iter(lines)
+    hdr = {}
+    for l in it:
+        if l == "coef":
+            break
+        key, v = l.split()
+        hdr[key] = int(v)
+    coef = [float(next(it)) for _ in range(hdr["ncoef"])]
+    assert next(it).startswith("monomials")
+    monos = []
+    for _ in range(hdr["nmono"]):
+        v = [int(t) for t in next(it).split()]
+        monos.append((v[0], tuple(v[2:])))
+    return hdr["nvars"], coef, monos
+
+
+class Schedule:
+    """Trie schedule over a list of (coefficient slot, monomial) terms."""
+
+    def __init__(self, nvars, terms):
+        self.nvars = nvars
+        # coefficient slots are opaque ints indexing the derived table C
+        by_deg = collections.defaultdict(list)
+        for q, m in terms:
+            by_deg[len(m)].append((q, tuple(sorted(m))))
+        d4 = by_deg[4]
+        # greedy cover of degree-4 monomials by level-3 nodes
+        cand = collections.defaultdict(set)
+        for idx, (_, m) in enumerate(d4):
+            for r in range(4):
+                cand[m[:r] + m[r + 1:]].add(idx)
+        leaves3 = {}  # node -> list of (q, leaf var)
+        left = set(range(len(d4)))
+        heap_order = sorted(cand, key=lambda n: (-len(cand[n]), n))
+        # lazy greedy: re-evaluate the best candidate until its size is current
+        import heapq
+        heap = [(-len(cand[n]), n) for n in heap_order]
+        heapq.heapify(heap)
+        while left:
+            negsz, n = heapq.heappop(heap)
+            cur = cand[n] & left
+            if len(cur) < -negsz:
+                if cur:
+                    heapq.heappush(heap, (-len(cur), n))
+                continue
+            lst = []
+            for idx in sorted(cur):
+                q, m = d4[idx]
+                rest = list(m)
+                for v in n:
+                    rest.remove(v)
+                lst.append((q, rest[0]))
+            leaves3[n] = lst
+            left -= cur
+        self.const3 = {}
+        leaves2 = collections.defaultdict(list)
+        for q, m in by_deg[3]:
+            if m in leaves3:
+                self.const3[m] = q
+            else:
+                leaves2[m[:2]].append((q, m[2]))
+        nodes2 = set(n[:2] for n in leaves3) | set(leaves2)
+        self.const2 = {}
+        leaves1 = collections.defaultdict(list)
+        for q, m in by_deg[2]:
+            if m in nodes2:
+                self.const2[m] = q
+            else:
+                leaves1[m[:1]].append((q, m[1]))
+        nodes1 = set(n[:1] for n in nodes2) | set(leaves1)
+        self.const1 = {}
+        self.leaves0 = []
+        for q, m in by_deg[1]:
+            if m in nodes1:
+                self.const1[m] = q
+            else:
+                self.leaves0.append((q, m[0]))
+        self.const0 = [q for q, m in by_deg[0]]
+        self.leaves3 = {n: sorted(l, key=lambda t: t[1]) for n, l in leaves3.items()}
+        self.leaves2 = {n: sorted(l, key=lambda t: t[1]) for n, l in leaves2.items()}
+        self.leaves1 = {n: sorted(l, key=lambda t: t[1]) for n, l in leaves1.items()}
+        self.nodes1 = sorted(nodes1)
+        self.children1 = collections.defaultdict(list)
+        for n in sorted(nodes2):
+            self.children1[n[:1]].append(n)
+        self.children2 = collections.defaultdict(list)
+        for n in sorted(leaves3):
+            self.children2[n[:2]].append(n)
+
+    def counts(self):
+        n4 = sum(len(v) for v in self.leaves3.values())
+        n3l = sum(len(v) for v in self.leaves2.values())
+        n2l = sum(len(v) for v in self.leaves1.values())
+        ops = (2 * n4 + 3 * len(self.leaves3) + len(self.const3) + 2 * n3l + 3 * sum(len(v) for v in self.children1.values())
+               + len(self.const2) + 2 * n2l + 2 * len(self.nodes1) + len(self.const1) + 2 * len(self.leaves0))
+        return dict(level3_nodes=len(self.leaves3), deg4=n4, deg3_leaves=n3l, deg3_const=len(self.const3),
+                    level2_nodes=sum(len(v) for v in self.children1.values()), deg2_leaves=n2l,
+                    deg2_const=len(self.const2), level1_nodes=len(self.nodes1), fp64_ops_eg=ops)
+
+    # ---------------------------------------------------------------------------------------
+    def ops(self, grad=True):
+        """Linear op list.  Each op is (text template, coefficient slot or None); the template
+        contains '{c}' where the coefficient value goes.  Temporaries get unique names, so the
+        whole body lives at function scope."""
+        o = []
+        w = lambda t, q=None: o.append((t, q))
+        w("double E = %s;" % ("{c}" if self.const0 else "0.0"), self.const0[0] if self.const0 else None)
+        for q, l in self.leaves0:
+            w("E = __fma_rn({c}, x[%d], E);" % l, q)
+            if grad:
+                w("g[%d] += {c};" % l, q)
+        i1 = i2 = i3 = 0
+        for n1 in self.nodes1:
+            p = n1[0]
+            R1 = "R1_%d" % i1
+            i1 += 1
+            if n1 in self.const1:
+                w("double %s = {c};" % R1, self.const1[n1])
+            else:
+                w("double %s = 0.0;" % R1)
+            for n2 in self.children1.get(n1, []):
+                qv = n2[1]
+                P2, R2 = "P2_%d" % i2, "R2_%d" % i2
+                i2 += 1
+                w("const double %s = x[%d] * x[%d];" % (P2, p, qv))
+                if n2 in self.const2:
+                    w("double %s = {c};" % R2, self.const2[n2])
+                else:
+                    w("double %s = 0.0;" % R2)
+                for n3 in self.children2.get(n2, []):
+                    r = n3[2]
+                    P3, R3 = "P3_%d" % i3, "R3_%d" % i3
+                    i3 += 1
+                    if grad:
+                        w("const double %s = %s * x[%d];" % (P3, P2, r))
+                    first = True
+                    for q, l in self.leaves3[n3]:
+                        if first and n3 not in self.const3:
+                            w("double %s = {c} * x[%d];" % (R3, l), q)
+                        else:
+                            if first:
+                                w("double %s = {c};" % R3, self.const3[n3])
+                            w("%s = __fma_rn({c}, x[%d], %s);" % (R3, l, R3), q)
+                        first = False
+                        if grad:
+                            w("g[%d] = __fma_rn({c}, %s, g[%d]);" % (l, P3, l), q)
+                    if grad:
+                        w("g[%d] = __fma_rn(%s, %s, g[%d]);" % (r, P2, R3, r))
+                    w("%s = __fma_rn(x[%d], %s, %s);" % (R2, r, R3, R2))
+                for q, l in self.leaves2.get(n2, []):
+                    w("%s = __fma_rn({c}, x[%d], %s);" % (R2, l, R2), q)
+                    if grad:
+                        w("g[%d] = __fma_rn({c}, %s, g[%d]);" % (l, P2, l), q)
+                if grad:
+                    w("g[%d] = __fma_rn(x[%d], %s, g[%d]);" % (qv, p, R2, qv))
+                w("%s = __fma_rn(x[%d], %s, %s);" % (R1, qv, R2, R1))
+            for q, l in self.leaves1.get(n1, []):
+                w("%s = __fma_rn({c}, x[%d], %s);" % (R1, l, R1), q)
+                if grad:
+                    w("g[%d] = __fma_rn({c}, x[%d], g[%d]);" % (l, p, l), q)
+            if grad:
+                w("g[%d] += %s;" % (p, R1))
+            w("E = __fma_rn(x[%d], %s, E);" % (p, R1))
+        return o
+
+    def emit(self, fn, grad=True, region_coefs=12, region_ops=64):
+        """C++ source of the evaluation function.
+
+        The op list is cut into regions of at most `region_coefs` distinct coefficients and
+        about `region_ops` arithmetic ops.  Region r's coefficients are loaded (as named SGPR
+        values) during region r-1, and a fence separates regions, so scalar-load latency is
+        covered by the previous region's FMAs while only two regions' coefficients are live.
+        """
+        ops = self.ops(grad)
+        regions = [[]]
+        coefs = [[]]
+        nar = 0
+        for t, q in ops:
+            cur = coefs[-1]
+            arith = True
+            if q is not None and q not in cur and (len(cur) >= region_coefs or nar >= region_ops):
+                regions.append([])
+                coefs.append([])
+                nar = 0
+                cur = coefs[-1]
+            if q is not None and q not in cur:
+                cur.append(q)
+            regions[-1].append((t, q))
+            nar += arith
+        out = []
+        w = out.append
+        if grad:
+            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x, double* g) {" % fn)
+            w("    for (int i = 0; i < %d; ++i) g[i] = 0.0;" % self.nvars)
+        else:
+            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x) {" % fn)
+        name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
+        for j, q in enumerate(coefs[0]):
+            w("    const double %s = C[%d];" % (name(0, q), q))
+        for r, reg in enumerate(regions):
+            w("    MBXG_NODE_FENCE(C);")
+            if r + 1 < len(regions):
+                for q in coefs[r + 1]:
+                    w("    const double %s = C[%d];" % (name(r + 1, q), q))
+            for t, q in reg:
+                w("    " + (t.replace("{c}", name(r, q)) if q is not None else t))
+        w("    return E;")
+        w("}")
+        return "\n".join(out) + "\n"
+
+def full_terms(monos):
+    """Terms of the full polynomial: coefficient slot q == coefficient index k, scale 1."""
+    return [(k, m) for k, m in monos], None
+
+
+def generate(name, fn, out_dir):
+    nvars, coef, monos = load_table(name)
+    terms, _ = full_terms(monos)
+    s = Schedule(nvars, terms)
+    c = s.counts()
+    hdr = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+           "// Schedule: %s" % ", ".join("%s=%d" % kv for kv in c.items()),
+           "// Coefficient slots index the fitted coefficient vector directly (%d slots)." % len(coef),
+           "#pragma once",
+           "#include \"poly_support.hpp\"", ""]
+    src = "\n".join(hdr) + s.emit(fn, grad=True) + "\n" + s.emit(fn + "_e", grad=False)
+    path = os.path.join(out_dir, "poly_%s.inc" % name)
+    with open(path, "w") as f:
+        f.write(src)
+    # fitted coefficients, for the host runtime's default table
+    cpath = os.path.join(out_dir, "coef_%s.inc" % name)
+    with open(cpath, "w") as f:
+        f.write("// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit.\n" % name)
+        f.write("#pragma once\n")
+        f.write("static const int kNCoef_%s = %d;\n" % (name, len(coef)))
+        f.write("static const double kFit_%s[%d] = {\n" % (name, len(coef)))
+        for i in range(0, len(coef), 4):
+            f.write("    " + ", ".join("%.17g" % v for v in coef[i:i + 4]) + ",\n")
+        f.write("};\n")
+    return path, c
+
+
+def main():
+    out_dir = os.path.join(ROOT, "mbx_amd", "csrc", "gen")
+    os.makedirs(out_dir, exist_ok=True)
+    for name, fn in (("mbpol_2b", "poly_mbpol_2b"), ("mbpol_3b", "poly_mbpol_3b")):
+        path, c = generate(name, fn, out_dir)
+        print(path, c)
+
+
+if __name__ == "__main__":
+    main()
