@@ -1,0 +1,166 @@
+"""Benchmark: MB-pol 2B+3B energy+gradient of 2048 FP64 waters on one MI355X.
+
+A step is one full Energy(true)-equivalent 2B+3B pass over the periodic 2048-water box
+(edge 39.4 A, cutoffs 9 / 7 A): FixMonomerCoordinates, the dimer and trimer lists (cell-list
+builder), the polynomial evaluation of every evaluated n-mer, gradient scatter and energy /
+virial reduction.  Coordinates are resident in HBM before timing starts (a device-to-device copy
+into the engine's working buffer is part of each step); results stay on the device.
+
+Throughput is listed dimers + trimers per second (the n-mers the reference's Get2B / Get3B walk).
+
+The JSON line also carries
+  roofline:     the 3B polynomial kernel (the dominant kernel) against MI355X FP64 vector peak,
+                with the reference's algorithmic FLOPs per evaluated trimer (81,549, counted from
+                the reference's generated E+G code: mbx_amd/tables/flops.json), timed with HIP
+                events on the engine stream during the timed steps; traffic from the committed
+                rocprofv3 PMC summary when present;
+  cpu_baseline: the reference's own compiled 2B/3B leaves (oracle/_ref, OpenMP driver with the
+                reference's batch-of-8 dispatch) on this host's cores, on the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "2B+3B dimers+trimers/sec (E+F, FP64) at 2048 H2O; batched vs plain speedup"
+FP64_VECTOR_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_poly3b_water2048.txt")
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def read_traffic():
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(xyz, box, c2, c3, seconds):
+    """Reference leaves on the host (kind 'reference'); the oracle port when they are absent."""
+    import oracle
+
+    cores = host_cores()
+    reps, t_total = 0, 0.0
+    if oracle.ref_available():
+        kind = "reference"
+        run = lambda: oracle.ref_water_2b3b(xyz, box, c2, c3, batch=8, nthreads=cores)
+    else:
+        kind = "port"
+        os.environ["OMP_NUM_THREADS"] = str(cores)
+        run = lambda: oracle.water_system(xyz, box, c2, c3)
+    r = run()  # warm-up
+    while t_total < seconds or reps < 2:
+        t0 = time.perf_counter()
+        r = run()
+        t_total += time.perf_counter() - t0
+        reps += 1
+    n = r["n2"] + r["n3"]
+    return dict(value=n * reps / t_total, unit="dimers+trimers/s", cores=cores, kind=kind,
+                sample="%d full 2B+3B E+F calls on the 2048-water box (%.1f s of CPU time, %d threads, "
+                       "batch-8 dispatch)" % (reps, t_total, cores)), r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; --gpus %d is not supported (multi-GPU sharding is "
+              "design-only in this build)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from mbx_amd import configs, engine
+
+    cfg = configs.C2
+    xyz, box, c2, c3 = configs.build(cfg)
+    eng = engine.water_engine(cfg["nmon"], box, c2, c3)
+    d_xyz = torch.from_numpy(xyz).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def step():
+        eng.set_coords_device(d_xyz.data_ptr())
+        eng.compute()
+
+    for _ in range(args.warmup):
+        step()
+    eng.synchronize()
+    eng.set_timing(True)
+    poly_ms, stage = [], {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        tm = eng.timings()
+        poly_ms.append(tm["poly_3b"])
+        for k, v in tm.items():
+            stage[k] = stage.get(k, 0.0) + v
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.set_timing(False)
+
+    counts = eng.counts()
+    n_listed = int(counts[0] + counts[2])
+    dt = (t1 - t0) / args.steps
+    e2, e3, grad, vir = eng.results()
+
+    flops = json.load(open(os.path.join(ROOT, "mbx_amd", "tables", "flops.json")))
+    f3 = flops["mbpol_3b_eg"]["flops"]
+    t_poly = float(np.mean(poly_ms)) * 1e-3
+    achieved = counts[3] * f3 / t_poly
+    traffic = read_traffic()
+
+    out = {
+        "metric": METRIC,
+        "value": n_listed / dt,
+        "unit": "dimers+trimers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "waters": cfg["nmon"], "box_A": cfg["edge"], "cutoff2b_A": c2,
+                   "cutoff3b_A": c3, "listed_dimers": int(counts[0]), "evaluated_dimers": int(counts[1]),
+                   "listed_trimers": int(counts[2]), "evaluated_trimers": int(counts[3]),
+                   "stage_ms": {k: v / args.steps for k, v in stage.items()}, "e2b": e2, "e3b": e3},
+        "roofline": {"bound": "fp64-valu", "kernel": "k_poly_eg<36> (MB-pol 3B polynomial E+G)",
+                     "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_VECTOR_PEAK, "traffic": traffic,
+                     "flops_per_unit": f3, "units_per_launch": int(counts[3]), "launch_ms": t_poly * 1e3},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        cb, r = cpu_baseline(xyz, box, c2, c3, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["config"]["cpu_parity"] = {"e2b_rel": (e2 - r["e2b"]) / r["e2b"], "e3b_rel": (e3 - r["e3b"]) / r["e3b"],
+                                       "grad_maxabs": float(np.abs(grad - r["grad"]).max())}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

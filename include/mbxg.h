@@ -31,8 +31,8 @@ typedef struct mbxg_ctx mbxg_ctx;
 
 /* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", ...). */
 enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2 };
-/* Terms for mbxg_compute. */
-enum { MBXG_2B = 1, MBXG_3B = 2 };
+/* Terms for mbxg_compute; MBXG_LISTS_ONLY builds the requested lists without evaluating. */
+enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4 };
 /* Status codes. */
 enum { MBXG_OK = 0, MBXG_EINVAL = -1, MBXG_EHIP = -2, MBXG_ESTATE = -3, MBXG_ENOMEM = -4 };
 
@@ -97,6 +97,12 @@ int mbxg_device_outputs(mbxg_ctx* ctx, double** d_grad, double** d_totals);
 
 /* The context's HIP stream (hipStream_t), for callers that enqueue around the engine. */
 int mbxg_stream(mbxg_ctx* ctx, void** stream);
+
+/* Stage timing with HIP events on the context stream.  After a compute with timing on,
+ * mbxg_get_timings writes up to 10 durations in ms: FixMonomerCoordinates, 2B lists, 3B lists,
+ * 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total. */
+int mbxg_set_timing(mbxg_ctx* ctx, int on);
+int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
 
 /* Block until the context's stream is idle. */
 int mbxg_synchronize(mbxg_ctx* ctx);

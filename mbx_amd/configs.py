@@ -1,0 +1,75 @@
+"""Seeded synthetic water systems for the benchmark and parity workloads.
+
+Periodic boxes (the plain path): oxygen atoms on a lattice of nx*ny*nz sites filling a cubic
+box of edge L (spacings L/nx, L/ny, L/nz), the first `nmon` sites in raster order (x fastest),
+each displaced by U(-0.3, 0.3) A per axis.  Each water gets O-H bonds of 0.9572 + U(-0.02,
+0.02) A, an H-O-H angle of 104.52 + U(-3, 3) degrees and a uniformly random orientation
+(Haar-random rotation from a normalised Gaussian quaternion).
+
+    water_box(256,  19.7, (7, 7, 6))     256 waters, 1 g/cm^3
+    water_box(2048, 39.4, (14, 14, 12))  2048 waters, 1 g/cm^3 -- the same recipe with the
+                                         lattice doubled along each axis
+
+Coordinates are returned as a flat array, 9 doubles per water (O, H, H), in Angstrom.
+"""
+import numpy as np
+
+R_OH, ANGLE = 0.9572, 104.52
+
+
+def random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], axis=-1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], axis=-1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=-1),
+    ], axis=1)
+
+
+def water_geometries(rng, n):
+    """Body-frame O, H1, H2 of n perturbed waters: (n, 3, 3)."""
+    r1 = R_OH + rng.uniform(-0.02, 0.02, n)
+    r2 = R_OH + rng.uniform(-0.02, 0.02, n)
+    th = np.radians(ANGLE + rng.uniform(-3.0, 3.0, n))
+    g = np.zeros((n, 3, 3))
+    g[:, 1, 0] = r1 * np.cos(th / 2)
+    g[:, 1, 1] = r1 * np.sin(th / 2)
+    g[:, 2, 0] = r2 * np.cos(th / 2)
+    g[:, 2, 1] = -r2 * np.sin(th / 2)
+    return g
+
+
+def place_waters(rng, oxygens):
+    n = len(oxygens)
+    geo = water_geometries(rng, n)
+    rot = random_rotations(rng, n)
+    atoms = np.einsum("nij,naj->nai", rot, geo) + oxygens[:, None, :]
+    return atoms.reshape(-1)
+
+
+def water_box(nmon, edge, lattice, seed=2048, jitter=0.3):
+    """(xyz, box9) of nmon waters on a jittered lattice in a cubic periodic box."""
+    nx, ny, nz = lattice
+    if nmon > nx * ny * nz:
+        raise ValueError("lattice too small")
+    rng = np.random.default_rng(seed)
+    idx = np.arange(nmon)
+    ix, iy, iz = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    o = np.stack([(ix + 0.5) * edge / nx, (iy + 0.5) * edge / ny, (iz + 0.5) * edge / nz], axis=1)
+    o += rng.uniform(-jitter, jitter, size=o.shape)
+    xyz = place_waters(rng, o)
+    box = np.array([edge, 0, 0, 0, edge, 0, 0, 0, edge], dtype=np.float64)
+    return xyz, box
+
+
+# The two periodic workloads (2B cutoff 9 A, 3B cutoff 7 A).
+C1 = dict(name="water_256_box19.7", nmon=256, edge=19.7, lattice=(7, 7, 6), seed=256, cutoff2b=9.0, cutoff3b=7.0)
+C2 = dict(name="water_2048_box39.4", nmon=2048, edge=39.4, lattice=(14, 14, 12), seed=2048, cutoff2b=9.0,
+          cutoff3b=7.0)
+
+
+def build(cfg):
+    xyz, box = water_box(cfg["nmon"], cfg["edge"], cfg["lattice"], seed=cfg["seed"])
+    return xyz, box, cfg["cutoff2b"], cfg["cutoff3b"]

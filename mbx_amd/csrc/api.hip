@@ -25,6 +25,10 @@ using namespace mbxg;
 
 namespace {
 
+// Timing events: 0 start, 1 after fix, 2 after 2B lists, 3 after 3B lists, 4..7 2B stages,
+// 8..11 3B stages, 12 end.
+constexpr int kNumEvents = 13;
+
 thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
@@ -77,7 +81,7 @@ struct OrderState {
     DBuf<int2> counts, offsets;
     DBuf<int> list, eval;
     int64_t n_list = 0, n_eval = 0;
-    DBuf<double> e_nmer, partials;
+    DBuf<double> e_nmer, partials, ws;
 };
 
 }  // namespace
@@ -100,6 +104,9 @@ struct mbxg_ctx {
     OrderState o2, o3;
     bool have_coords = false, have_results = false;
     int match_cap = 1024;
+    // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
+    bool timing = false;
+    hipEvent_t ev[kNumEvents] = {};
 };
 
 extern "C" const char* mbxg_last_error(void) { return g_err.c_str(); }
@@ -147,6 +154,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
         o->eval.release();
         o->e_nmer.release();
         o->partials.release();
+        o->ws.release();
     }
     c->d_kind.release();
     c->d_nat.release();
@@ -162,6 +170,8 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_bbox.release();
     c->d_box.release();
     c->d_overflow.release();
+    for (hipEvent_t& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MBXG_OK;
@@ -476,9 +486,18 @@ int build_order(mbxg_ctx* c, int order, double cutoff, OrderState& o, const Nmer
 extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_compute: null context");
     if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_compute: coordinates not set");
+    const bool lists_only = (terms & MBXG_LISTS_ONLY) != 0;
+    if (!lists_only)
+        for (int i = 0; i < c->nmon; ++i)
+            if (c->kind[i] == MBXG_CH4)
+                return fail(MBXG_EINVAL, "mbxg_compute: no GPU holder for ch4-containing n-mers yet (lists only)");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     c->have_results = false;
+    hipEvent_t* ev = c->timing ? c->ev : nullptr;
+    if (ev)
+        for (int k = 0; k < kNumEvents; ++k) HIPCHK(hipEventRecord(ev[k], st));  // stages skipped read 0
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
     if (c->box.periodic && c->nmon > 0) {
         hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_xyz.p,
                            c->d_first.p, c->d_nat.p, c->nmon, c->box);
@@ -487,30 +506,68 @@ extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
               c->box.periodic};
     c->o2.n_list = c->o2.n_eval = c->o3.n_list = c->o3.n_eval = 0;
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     if (terms & MBXG_2B)
         if (int r = build_order(c, 2, c->cut2, c->o2, N)) return r;
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
     if (terms & MBXG_3B)
         if (int r = build_order(c, 3, c->cut3, c->o3, N)) return r;
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    if (lists_only) {
+        if (ev) HIPCHK(hipEventRecord(ev[12], st));
+        HIPCHK(hipMemsetAsync(c->d_totals.p, 0, sizeof(double) * 16, st));
+        if (c->natoms > 0) HIPCHK(hipMemsetAsync(c->d_grad.p, 0, sizeof(double) * 3 * c->natoms, st));
+        c->o2.n_eval = c->o3.n_eval = 0;
+        c->have_results = true;
+        return MBXG_OK;
+    }
     if (c->natoms > 0) HIPCHK(hipMemsetAsync(c->d_grad.p, 0, sizeof(double) * 3 * c->natoms, st));
     HIPCHK(hipMemsetAsync(c->d_totals.p, 0, sizeof(double) * 16, st));
     OrderState* os[2] = {&c->o2, &c->o3};
     for (int k = 0; k < 2; ++k) {
         OrderState& o = *os[k];
         const int nb = (int)((o.n_eval + 63) / 64);
-        if (o.e_nmer.reserve(o.n_eval + 1) || o.partials.reserve((size_t)kPartials * (nb + 1)))
+        const size_t wsz = k == 0 ? workspace_mbpol2b((int)o.n_eval) : workspace_mbpol3b((int)o.n_eval);
+        if (o.e_nmer.reserve(o.n_eval + 1) || o.partials.reserve((size_t)kPartials * (nb + 1)) ||
+            o.ws.reserve(wsz + 1))
             return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
         if (o.n_eval == 0) continue;
         if (k == 0)
             HIPCHK(launch_mbpol2b(o.eval.p, (int)o.n_eval, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                  o.partials.p, st));
+                                  o.partials.p, o.ws.p, st, ev ? ev + 4 : nullptr));
         else
             HIPCHK(launch_mbpol3b(o.eval.p, (int)o.n_eval, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                  o.partials.p, st));
+                                  o.partials.p, o.ws.p, st, ev ? ev + 8 : nullptr));
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, st, o.partials.p, nb,
                            c->d_totals.p + kPartials * k);
         HIPCHK(hipGetLastError());
     }
+    if (ev) HIPCHK(hipEventRecord(ev[12], st));
     c->have_results = true;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_set_timing(mbxg_ctx* c, int on) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_set_timing: null context");
+    HIPCHK(hipSetDevice(c->device));
+    if (on && !c->ev[0])
+        for (hipEvent_t& e : c->ev) HIPCHK(hipEventCreate(&e));
+    c->timing = on != 0;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
+    if (!c || !ms) return fail(MBXG_EINVAL, "mbxg_get_timings: bad arguments");
+    if (!c->timing || !c->have_results) return fail(MBXG_ESTATE, "mbxg_get_timings: no timed compute");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // fix, lists 2B, lists 3B, vars 2B, poly 2B, chain 2B, vars 3B, poly 3B, chain 3B, total
+    const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {8, 9}, {9, 10}, {10, 11}, {0, 12}};
+    for (int k = 0; k < n && k < 10; ++k) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev[pairs[k][0]], c->ev[pairs[k][1]]));
+        ms[k] = t;
+    }
     return MBXG_OK;
 }
 

@@ -1,19 +1,18 @@
 // MB-pol 2B energy + gradient for gfx950: one lane per evaluated dimer, one wavefront per block.
 //
 // Restates x2o::x2b_v9x::eval (potential/2b/x2b-v9x.cpp:1514-1820) for dimers with
-// r_OO <= 6.5 A (the list builder compacts them):
-//   * monomer 2 imaged next to monomer 1 (GetCloseDimerImage, sys_tools.cpp:322-365);
-//   * M-sites X1/X2 of each water (monomer::setup, x2b-v9x.cpp:117-135);
-//   * 31 variables v_exp / v_coul (x2b-v9x.cpp:54-91, 1640-1675);
-//   * the degree-4 polynomial and gradient (generated, gen/poly_mbpol_2b.inc);
-//   * variable and M-site chain rule (x2b-v9x.cpp:95-103, 139-160), cosine switch on r_OO
-//     between 4.5 and 6.5 A and its derivative, per-dimer virial (x2b-v9x.cpp:1718-1800);
-//   * gradients scattered with hardware FP64 atomics; energy and virial reduced per wave.
-// Compiled with -ffp-contract=off; the polynomial's FMAs are explicit.
+// r_OO <= 6.5 A (the list builder compacts them), in three stages:
+//   1. k_vars_mbpol2b: monomer 2 imaged next to monomer 1 (GetCloseDimerImage,
+//      sys_tools.cpp:322-365), M-sites X1/X2 of each water (monomer::setup, x2b-v9x.cpp:117-135),
+//      31 variables v_exp / v_coul (x2b-v9x.cpp:54-91, 1640-1675) and the switch on r_OO;
+//   2. the polynomial and its gradient (poly_mbpol2b.hip);
+//   3. k_grad_mbpol2b: E = sw*P, variable and M-site chain rule (x2b-v9x.cpp:95-103, 139-160),
+//      switch derivative, per-dimer virial (x2b-v9x.cpp:1718-1800); gradients scattered with
+//      hardware FP64 atomics, energy and virial reduced per wave.
+// Stage buffers are variable-major (entry (j, t) at j*ld + t).  Compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
 
-#include "gen/poly_mbpol_2b.inc"
 
 namespace mbxg {
 
@@ -116,19 +115,18 @@ __device__ __forceinline__ double var2(const V2& d, const double* s, double& gg,
     return e1;
 }
 
-// Sites of a dimer: monomer 2 imaged next to monomer 1, then the four M-sites.  `z` is an
-// opaque zero so a second call after the polynomial reloads instead of reusing registers.
-__device__ __forceinline__ void load_dimer(const NmerCtx& N, int m1, int m2, int z, double* s, double* oh1a,
-                                           double* oh2a, double* oh1b, double* oh2b) {
-    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1] + z;
-    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2] + z;
+// Sites of a dimer: monomer 2 imaged next to monomer 1, then the four M-sites.
+__device__ __forceinline__ void load_dimer(const NmerCtx& N, int m1, int m2, double* s, double* oh1a, double* oh2a,
+                                           double* oh1b, double* oh2b) {
+    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1];
+    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         s[j] = a[j];
         s[9 + j] = b[j];
     }
     if (N.periodic) {
-        const Box& B = N.box[z];
+        const Box& B = *N.box;
         double dx, dy, dz;
         image_shift(B, s[0], s[1], s[2], s[9], s[10], s[11], dx, dy, dz);
 #pragma unroll
@@ -140,41 +138,41 @@ __device__ __forceinline__ void load_dimer(const NmerCtx& N, int m1, int m2, int
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) k_mbpol2b(const int* __restrict__ list, int n, NmerCtx N,
-                                                uint64_t coef_addr, int do_grads,
-                                                double* __restrict__ grad, double* __restrict__ e_out,
-                                                double* __restrict__ partials) {
+// Stage 1: variables and switch.
+__global__ void __launch_bounds__(64) k_vars_mbpol2b(const int* __restrict__ list, int n, NmerCtx N,
+                                                     double* __restrict__ xs, double* __restrict__ sw, int ld) {
     const int t = blockIdx.x * 64 + threadIdx.x;
-    const mbxg_cptr C = (mbxg_cptr)coef_addr;
+    if (t >= n) return;
+    double s[30], oh1a[3], oh2a[3], oh1b[3], oh2b[3];
+    load_dimer(N, list[2 * t], list[2 * t + 1], s, oh1a, oh2a, oh1b, oh2b);
+#pragma unroll
+    for (int j = 0; j < 31; ++j) {
+        double gg, d0, d1, d2;
+        xs[(size_t)j * ld + t] = var2(kVar2[j], s, gg, d0, d1, d2);
+    }
+    const double dOO[3] = {s[0] - s[9], s[1] - s[10], s[2] - s[11]};
+    const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
+    double gsw;
+    sw[t] = switch2(rOO, gsw);
+}
+
+// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.
+template <bool GRAD>
+__global__ void __launch_bounds__(64) k_grad_mbpol2b(const int* __restrict__ list, int n, NmerCtx N,
+                                                     const double* __restrict__ gs, const double* __restrict__ ps,
+                                                     const double* __restrict__ sws, int ld,
+                                                     double* __restrict__ grad, double* __restrict__ e_out,
+                                                     double* __restrict__ partials) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
     double e_sum = 0.0;
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t < n) {
-        const int m1 = list[2 * t], m2 = list[2 * t + 1];
-        // phase 1: variables and the switch
-        double v[31];
-        double sw;
-        {
+        const double sw = sws[t];
+        const double P = ps[t];
+        if (GRAD) {
+            const int m1 = list[2 * t], m2 = list[2 * t + 1];
             double s[30], oh1a[3], oh2a[3], oh1b[3], oh2b[3];
-            load_dimer(N, m1, m2, 0, s, oh1a, oh2a, oh1b, oh2b);
-#pragma unroll
-            for (int j = 0; j < 31; ++j) {
-                double gg, d0, d1, d2;
-                v[j] = var2(kVar2[j], s, gg, d0, d1, d2);
-            }
-            const double dOO[3] = {s[0] - s[9], s[1] - s[10], s[2] - s[11]};
-            const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
-            double gsw;
-            sw = switch2(rOO, gsw);
-        }
-        double P;
-        if (do_grads) {
-            // phase 2: polynomial and gradient
-            double g[31];
-            P = poly_mbpol_2b(C, v, g);
-            // phase 3: reload the dimer, chain rule through variables and M-sites, switch
-            const int z = opaque_zero();
-            double s[30], oh1a[3], oh2a[3], oh1b[3], oh2b[3];
-            load_dimer(N, m1, m2, z, s, oh1a, oh2a, oh1b, oh2b);
+            load_dimer(N, m1, m2, s, oh1a, oh2a, oh1b, oh2b);
             double xg[30];
 #pragma unroll
             for (int j = 0; j < 30; ++j) xg[j] = 0.0;
@@ -183,7 +181,8 @@ __global__ void __launch_bounds__(64) k_mbpol2b(const int* __restrict__ list, in
                 double gg, d0, d1, d2;
                 var2(kVar2[j], s, gg, d0, d1, d2);
                 const int s1 = kVar2[j].s1, s2 = kVar2[j].s2;
-                const double q0 = g[j] * (d0 * gg), q1 = g[j] * (d1 * gg), q2 = g[j] * (d2 * gg);
+                const double gj = gs[(size_t)j * ld + t];
+                const double q0 = gj * (d0 * gg), q1 = gj * (d1 * gg), q2 = gj * (d2 * gg);
                 xg[3 * s1] += q0;
                 xg[3 * s1 + 1] += q1;
                 xg[3 * s1 + 2] += q2;
@@ -200,10 +199,10 @@ __global__ void __launch_bounds__(64) k_mbpol2b(const int* __restrict__ list, in
             double gd[18];
 #pragma unroll
             for (int j = 0; j < 18; ++j) gd[j] = sw * xg[j];
-            const double gs = gsw * (P / rOO);
+            const double gsc = gsw * (P / rOO);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double dd = gs * dOO[c];
+                const double dd = gsc * dOO[c];
                 gd[c] += dd;
                 gd[9 + c] -= dd;
             }
@@ -222,8 +221,6 @@ __global__ void __launch_bounds__(64) k_mbpol2b(const int* __restrict__ list, in
                 unsafeAtomicAdd(g1 + j, gd[j]);
                 unsafeAtomicAdd(g2 + j, gd[9 + j]);
             }
-        } else {
-            P = poly_mbpol_2b_e(C, v);
         }
         const double e = sw * P;
         e_out[t] = e;
@@ -239,13 +236,37 @@ __global__ void __launch_bounds__(64) k_mbpol2b(const int* __restrict__ list, in
     }
 }
 
+size_t workspace_mbpol2b(int n) {
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    return (2 * 31 + 2) * ld;
+}
+
 hipError_t launch_mbpol2b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, hipStream_t st) {
+                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
     if (n <= 0) return hipSuccess;
     const int nb = (n + 63) / 64;
-    hipLaunchKernelGGL(k_mbpol2b, dim3(nb), dim3(64), 0, st, list, n, N, (uint64_t)(uintptr_t)coef, do_grads, grad,
-                       e_out, partials);
-    return hipGetLastError();
+    const int ld = nb * 64;
+    double* xs = ws;
+    double* gs = xs + (size_t)31 * ld;
+    double* ps = gs + (size_t)31 * ld;
+    double* sw = ps + ld;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_vars_mbpol2b, dim3(nb), dim3(64), 0, st, list, n, N, xs, sw, ld);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], st);
+    e = launch_poly_mbpol2b(xs, n, ld, coef, do_grads, gs, ps, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[2], st);
+    if (do_grads)
+        hipLaunchKernelGGL(k_grad_mbpol2b<true>, dim3(nb), dim3(64), 0, st, list, n, N, gs, ps, sw, ld, grad, e_out,
+                           partials);
+    else
+        hipLaunchKernelGGL(k_grad_mbpol2b<false>, dim3(nb), dim3(64), 0, st, list, n, N, gs, ps, sw, ld, grad,
+                           e_out, partials);
+    e = hipGetLastError();
+    if (ev) (void)hipEventRecord(ev[3], st);
+    return e;
 }
 
 }  // namespace mbxg

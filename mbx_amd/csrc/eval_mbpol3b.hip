@@ -1,19 +1,17 @@
 // MB-pol 3B energy + gradient for gfx950: one lane per evaluated trimer, one wavefront per block.
 //
 // Restates x2o::x3b_v2x::eval (potential/3b/x3b-v2x.cpp:1403-1806) for trimers that survive
-// the 4.5 A rule (the list builder compacts them):
-//   * monomers 2 and 3 imaged next to monomer 1 (GetCloseTrimerImage, sys_tools.cpp:438-442);
-//   * 36 variables exp(-k (d - d0)) over intra/inter atom pairs (x3b-v2x.cpp:1548-1584);
-//   * the degree-4 polynomial and its gradient (generated, gen/poly_mbpol_3b.inc), with the
-//     fitted coefficients read as uniform scalar loads;
-//   * s = sab*sac + sab*sbc + sac*sbc of cosine switches, chain rule to atoms, switch
-//     derivatives and the per-trimer virial (x3b-v2x.cpp:1614-1726);
-//   * gradients scattered with hardware FP64 atomics; energy and virial reduced per wave.
-// Compiled with -ffp-contract=off; the polynomial's FMAs are explicit.
+// the 4.5 A rule (the list builder compacts them), in three stages:
+//   1. k_vars_mbpol3b: monomers 2 and 3 imaged next to monomer 1 (GetCloseTrimerImage,
+//      sys_tools.cpp:438-442); 36 variables exp(-k (d - d0)) (x3b-v2x.cpp:1548-1584) and the
+//      switch product s = sab*sac + sab*sbc + sac*sbc (x3b-v2x.cpp:1614-1620);
+//   2. the polynomial and its gradient (poly_mbpol3b.hip);
+//   3. k_grad_mbpol3b: E = s*P, chain rule to atoms, switch derivatives and the per-trimer
+//      virial (x3b-v2x.cpp:1622-1726); gradients scattered with hardware FP64 atomics, energy
+//      and virial reduced per wave.
+// Stage buffers are variable-major (entry (j, t) at j*ld + t).  Compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
-
-#include "gen/poly_mbpol_3b.inc"
 
 namespace mbxg {
 
@@ -51,12 +49,11 @@ __device__ __forceinline__ double switch3(double r, double& g) {
     return 1.0;
 }
 
-// Load the 9 atoms of a water trimer, imaging monomers 2 and 3 next to monomer 1.  `z` is an
-// opaque zero so a second call after the polynomial reloads instead of reusing registers.
-__device__ __forceinline__ void load_trimer(const NmerCtx& N, int m1, int m2, int m3, int z, double* w) {
-    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1] + z;
-    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2] + z;
-    const double* c = N.xyz + 3 * (size_t)N.first_atom[m3] + z;
+// Load the 9 atoms of a water trimer, imaging monomers 2 and 3 next to monomer 1.
+__device__ __forceinline__ void load_trimer(const NmerCtx& N, int m1, int m2, int m3, double* w) {
+    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1];
+    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2];
+    const double* c = N.xyz + 3 * (size_t)N.first_atom[m3];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         w[j] = a[j];
@@ -64,7 +61,7 @@ __device__ __forceinline__ void load_trimer(const NmerCtx& N, int m1, int m2, in
         w[18 + j] = c[j];
     }
     if (N.periodic) {
-        const Box& B = N.box[z];
+        const Box& B = *N.box;
         double dx, dy, dz;
         image_shift(B, w[0], w[1], w[2], w[9], w[10], w[11], dx, dy, dz);
 #pragma unroll
@@ -75,8 +72,8 @@ __device__ __forceinline__ void load_trimer(const NmerCtx& N, int m1, int m2, in
     }
 }
 
-// O-O separations and the switch product s = sab*sac + sab*sbc + sac*sbc (x3b-v2x.cpp:1495-1509,
-// 1614-1620); g* receive the switch derivatives.
+// O-O separations and the switch product (x3b-v2x.cpp:1495-1509, 1614-1620); g* receive the
+// switch derivatives.
 __device__ __forceinline__ double switches(const double* w, double* dab, double* dac, double* dbc, double& rab,
                                            double& rac, double& rbc, double& sab, double& sac, double& sbc,
                                            double& gab, double& gac, double& gbc) {
@@ -97,46 +94,47 @@ __device__ __forceinline__ double switches(const double* w, double* dab, double*
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
-                                                uint64_t coef_addr, int do_grads,
-                                                double* __restrict__ grad, double* __restrict__ e_out,
-                                                double* __restrict__ partials) {
+// Stage 1: variables and switch product.
+__global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
+                                                     double* __restrict__ xs, double* __restrict__ sw, int ld) {
     const int t = blockIdx.x * 64 + threadIdx.x;
-    const mbxg_cptr C = (mbxg_cptr)coef_addr;
+    if (t >= n) return;
+    double w[27];
+    load_trimer(N, list[3 * t], list[3 * t + 1], list[3 * t + 2], w);
+#pragma unroll
+    for (int j = 0; j < 36; ++j) {
+        const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
+        const double d0 = w[3 * a1] - w[3 * a2];
+        const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
+        const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
+        const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        xs[(size_t)j * ld + t] = exp(-kset(st) * (d - dset(st)));
+    }
+    double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
+    sw[t] = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
+}
+
+// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.
+template <bool GRAD>
+__global__ void __launch_bounds__(64) k_grad_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
+                                                     const double* __restrict__ xs, const double* __restrict__ gs,
+                                                     const double* __restrict__ ps, const double* __restrict__ sw,
+                                                     int ld, double* __restrict__ grad, double* __restrict__ e_out,
+                                                     double* __restrict__ partials) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
     double e_sum = 0.0;
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t < n) {
-        const int m1 = list[3 * t], m2 = list[3 * t + 1], m3 = list[3 * t + 2];
-        // phase 1: variables (x3b-v2x.cpp:1548-1584) and the switch product
-        double x[36];
-        double s;
-        {
+        const double s = sw[t];
+        const double P = ps[t];
+        if (GRAD) {
+            const int m1 = list[3 * t], m2 = list[3 * t + 1], m3 = list[3 * t + 2];
             double w[27];
-            load_trimer(N, m1, m2, m3, 0, w);
-#pragma unroll
-            for (int j = 0; j < 36; ++j) {
-                const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
-                const double d0 = w[3 * a1] - w[3 * a2];
-                const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
-                const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
-                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                x[j] = exp(-kset(st) * (d - dset(st)));
-            }
-            double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
-            s = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
-        }
-        double P;
-        if (do_grads) {
-            // phase 2: polynomial and its gradient; only x, g, s and the indices are live
-            double g[36];
-            P = poly_mbpol_3b(C, x, g);
-            // phase 3: reload the trimer, chain rule (x3b-v2x.cpp:63-76, 1638-1686)
-            const int z = opaque_zero();
-            double w[27];
-            load_trimer(N, m1, m2, m3, z, w);
+            load_trimer(N, m1, m2, m3, w);
             double gd[27];
 #pragma unroll
             for (int j = 0; j < 27; ++j) gd[j] = 0.0;
+            // g_var (x3b-v2x.cpp:63-76, 1638-1673)
 #pragma unroll
             for (int j = 0; j < 36; ++j) {
                 const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
@@ -144,8 +142,8 @@ __global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, in
                 const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
                 const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
                 const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                const double gj = g[j] * s;
-                const double gg = -kset(st) * gj * x[j] / d;
+                const double gj = gs[(size_t)j * ld + t] * s;
+                const double gg = -kset(st) * gj * xs[(size_t)j * ld + t] / d;
                 gd[3 * a1] += gg * d0;
                 gd[3 * a1 + 1] += gg * d1;
                 gd[3 * a1 + 2] += gg * d2;
@@ -153,6 +151,7 @@ __global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, in
                 gd[3 * a2 + 1] -= gg * d1;
                 gd[3 * a2 + 2] -= gg * d2;
             }
+            // switch derivatives (x3b-v2x.cpp:1676-1686)
             double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
             switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
             gab *= (sac + sbc) * P / rab;
@@ -180,8 +179,6 @@ __global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, in
 #pragma unroll
                 for (int j = 0; j < 9; ++j) unsafeAtomicAdd(gp + j, gd[9 * m + j]);
             }
-        } else {
-            P = poly_mbpol_3b_e(C, x);
         }
         const double e = s * P;
         e_out[t] = e;
@@ -198,13 +195,37 @@ __global__ void __launch_bounds__(64) k_mbpol3b(const int* __restrict__ list, in
     }
 }
 
+size_t workspace_mbpol3b(int n) {
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    return (2 * 36 + 2) * ld;
+}
+
 hipError_t launch_mbpol3b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, hipStream_t st) {
+                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
     if (n <= 0) return hipSuccess;
     const int nb = (n + 63) / 64;
-    hipLaunchKernelGGL(k_mbpol3b, dim3(nb), dim3(64), 0, st, list, n, N, (uint64_t)(uintptr_t)coef, do_grads, grad,
-                       e_out, partials);
-    return hipGetLastError();
+    const int ld = nb * 64;
+    double* xs = ws;
+    double* gs = xs + (size_t)36 * ld;
+    double* ps = gs + (size_t)36 * ld;
+    double* sw = ps + ld;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n, N, xs, sw, ld);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], st);
+    e = launch_poly_mbpol3b(xs, n, ld, coef, do_grads, gs, ps, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[2], st);
+    if (do_grads)
+        hipLaunchKernelGGL(k_grad_mbpol3b<true>, dim3(nb), dim3(64), 0, st, list, n, N, xs, gs, ps, sw, ld, grad,
+                           e_out, partials);
+    else
+        hipLaunchKernelGGL(k_grad_mbpol3b<false>, dim3(nb), dim3(64), 0, st, list, n, N, xs, gs, ps, sw, ld, grad,
+                           e_out, partials);
+    e = hipGetLastError();
+    if (ev) (void)hipEventRecord(ev[3], st);
+    return e;
 }
 
 }  // namespace mbxg

@@ -272,11 +272,11 @@ __device__ int gather_matches(const QueryCtx& Q, const Grid& G, int i, double c2
 // Evaluation rules (which listed n-mers reach a polynomial).
 
 // 2B: MB-pol h2o-h2o evaluates iff r_OO <= r2f after imaging (x2b-v9x.cpp:1589-1600).
-// Other known pairs are always evaluated (their holders apply their own switch/skip).
+// Pairs without a GPU holder contribute nothing here (energy2b.cpp:372-374 returns 0 for an
+// unknown pair); the host refuses to evaluate systems whose pairs would need one.
 __device__ __forceinline__ bool eval2_rule(const NmerCtx& N, int m1, int m2) {
     const int k1 = N.kind[m1], k2 = N.kind[m2];
-    if (k1 == MON_OTHER || k2 == MON_OTHER) return false;  // unknown pair -> 0 (energy2b.cpp:372-374)
-    if (!(k1 == MON_H2O && k2 == MON_H2O)) return true;
+    if (!(k1 == MON_H2O && k2 == MON_H2O)) return false;
     const double* a = N.xyz + 3 * (size_t)N.first_atom[m1];
     const double* b = N.xyz + 3 * (size_t)N.first_atom[m2];
     double bx = b[0], by = b[1], bz = b[2];

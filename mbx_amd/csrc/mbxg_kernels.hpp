@@ -32,14 +32,6 @@ struct NmerCtx {
     int periodic;
 };
 
-// An opaque zero: stops the compiler from reusing values loaded before the polynomial (and
-// so keeping them live in registers across it); reloading is cheaper than spilling.
-__device__ __forceinline__ int opaque_zero() {
-    int z = 0;
-    asm volatile("" : "+s"(z));
-    return z;
-}
-
 struct QueryCtx;
 
 __global__ void k_fix_monomers(double* xyz, const int* first_atom, const int* nat, int nmon, Box B);
@@ -61,13 +53,24 @@ hipError_t launch_trimers(const double* cx, const double* cy, const double* cz, 
                           const NmerCtx& N, double c2, int pass, int2* counts, const int2* offsets, int* out_list,
                           int* out_eval, int cap, int* overflow, hipStream_t st);
 
-// Polynomial evaluation kernels (eval_mbpol2b.hip, eval_mbpol3b.hip).  One lane per n-mer,
-// one wavefront per block.  list holds the evaluated n-mers (monomer indices), n of them.
-// e_out[t] receives n-mer t's energy; partials[block*kPartials + ...] its block sums.
+// Evaluation of the compacted "evaluate" lists (eval_mbpol2b.hip, eval_mbpol3b.hip): variable
+// stage, polynomial stage (poly_mbpol2b.hip, poly_mbpol3b.hip), chain-rule stage.  One lane per
+// n-mer, one wavefront per block.  list holds the evaluated n-mers (monomer indices), n of them;
+// e_out[t] receives n-mer t's energy; partials[block*kPartials + ...] its block sums; ws is a
+// workspace of workspace_mbpolXb(n) doubles.
+size_t workspace_mbpol2b(int n);
+size_t workspace_mbpol3b(int n);
+// ev (nullable) receives 4 events: before the variable stage, before the polynomial, before the
+// chain-rule stage, after it.
 hipError_t launch_mbpol2b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, hipStream_t st);
+                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
 hipError_t launch_mbpol3b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, hipStream_t st);
+                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
+// Polynomial stage alone: xs/gs variable-major with leading dimension ld; ps values.
+hipError_t launch_poly_mbpol2b(const double* xs, int n, int ld, const double* coef, int do_grads, double* gs,
+                               double* ps, hipStream_t st);
+hipError_t launch_poly_mbpol3b(const double* xs, int n, int ld, const double* coef, int do_grads, double* gs,
+                               double* ps, hipStream_t st);
 
 // Sum nblocks partial records into out[kPartials] in a fixed order (deterministic totals).
 __global__ void k_reduce_partials(const double* partials, int nblocks, double* out);

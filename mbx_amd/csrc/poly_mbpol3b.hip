@@ -1,0 +1,23 @@
+// MB-pol 3B polynomial stage (36 variables, 1163 coefficients, degree 4): the generated
+// reverse-mode schedule of gen/poly_mbpol_3b.inc in the one-lane-per-trimer kernel of
+// poly_kernel.hpp.  Restates poly_3b_v2x::eval (potential/3b/poly-3b-v2x.cpp:26982) and
+// its energy-only form (potential/3b/poly-3b-v2x-nogrd.cpp:39).
+#include "gen/poly_mbpol_3b.inc"
+#include "mbxg_kernels.hpp"
+#include "poly_kernel.hpp"
+
+namespace mbxg {
+
+hipError_t launch_poly_mbpol3b(const double* xs, int n, int ld, const double* coef, int do_grads, double* gs,
+                               double* ps, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + 63) / 64;
+    const uint64_t ca = (uint64_t)(uintptr_t)coef;
+    if (do_grads)
+        hipLaunchKernelGGL((k_poly_eg<36, poly_mbpol_3b>), dim3(nb), dim3(64), 0, st, xs, n, ld, ca, gs, ps);
+    else
+        hipLaunchKernelGGL((k_poly_e<36, poly_mbpol_3b_e>), dim3(nb), dim3(64), 0, st, xs, n, ld, ca, ps);
+    return hipGetLastError();
+}
+
+}  // namespace mbxg

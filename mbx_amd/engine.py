@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmbxg.so")
 
 H2O, CH4, OTHER = 0, 1, 2
-TERM_2B, TERM_3B = 1, 2
+TERM_2B, TERM_3B, LISTS_ONLY = 1, 2, 4
 KIND_OF = {"h2o": H2O, "ch4": CH4}
 NAT_OF = {H2O: 3, CH4: 5}
 
@@ -44,8 +44,13 @@ PROTOTYPES = {
     "mbxg_get_coords": (ctypes.c_int, [_vp, _dp]),
     "mbxg_device_outputs": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "mbxg_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mbxg_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mbxg_get_timings": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
 }
+
+TIMING_STAGES = ("fix", "lists_2b", "lists_3b", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
+                 "chain_3b", "total")
 
 
 class MbxgError(RuntimeError):
@@ -180,6 +185,15 @@ class Engine:
         s = _vp()
         _check(lib().mbxg_stream(self._c, ctypes.byref(s)))
         return s.value
+
+    def set_timing(self, on=True):
+        _check(lib().mbxg_set_timing(self._c, int(bool(on))))
+
+    def timings(self):
+        """Stage durations (ms) of the last compute, measured with HIP events on the engine stream."""
+        t = np.zeros(len(TIMING_STAGES))
+        _check(lib().mbxg_get_timings(self._c, _d(t), len(t)))
+        return dict(zip(TIMING_STAGES, t))
 
     def synchronize(self):
         _check(lib().mbxg_synchronize(self._c))

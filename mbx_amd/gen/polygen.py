@@ -27,8 +27,9 @@ where C is the derived coefficient table: C[q] = a[ck[q]] * cs[q] with the
 (ck, cs) map emitted alongside (the scale cs is 1 except for symmetry-reduced
 tables).  A second function <fn>_e evaluates E only.
 
-Every level-3 node opens with MBXG_NODE_FENCE(C) (poly_support.hpp): a scheduling barrier
-plus an opaque redefinition of the coefficient pointer.  It keeps each node's coefficient
+Regions of the op stream are separated by MBXG_NODE_FENCE(C, v) (poly_support.hpp): a
+scheduling barrier plus an opaque redefinition of the coefficient pointer that consumes the
+previous region's last result v.  It keeps each node's coefficient
 loads and temporaries local to the node, so the register allocator never sees tens of
 thousands of simultaneously live values, and keeps compile time linear in the code size.
 """
@@ -244,16 +245,32 @@ class Schedule:
         name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
         for j, q in enumerate(coefs[0]):
             w("    const double %s = C[%d];" % (name(0, q), q))
+        last = None
         for r, reg in enumerate(regions):
-            w("    MBXG_NODE_FENCE(C);")
+            # the fence consumes the previous region's last result, so no load of this or the
+            # next region can be hoisted above that region's arithmetic
+            w("    MBXG_NODE_FENCE(C, %s);" % (last or "x[0]"))
             if r + 1 < len(regions):
                 for q in coefs[r + 1]:
                     w("    const double %s = C[%d];" % (name(r + 1, q), q))
             for t, q in reg:
                 w("    " + (t.replace("{c}", name(r, q)) if q is not None else t))
+                if q is not None:
+                    # a result that needs this region's coefficients: the next fence cannot be
+                    # hoisted above it
+                    last = _lhs(t)
         w("    return E;")
         w("}")
         return "\n".join(out) + "\n"
+
+def _lhs(stmt):
+    """Name assigned by a generated statement ('double R3_1 = ...' -> 'R3_1', 'g[4] = ...' -> 'g[4]')."""
+    lhs = stmt.split("=")[0].strip()
+    for kw in ("const double ", "double "):
+        if lhs.startswith(kw):
+            lhs = lhs[len(kw):]
+    return lhs.replace("+", "").strip()
+
 
 def full_terms(monos):
     """Terms of the full polynomial: coefficient slot q == coefficient index k, scale 1."""

@@ -1,0 +1,88 @@
+"""Count the reference's algorithmic floating-point operations per evaluated n-mer.
+
+Reads the reference's generated energy+gradient polynomial routines as TEXT (one pass of the
+8-lane blocks f0..fN, i.e. one n-mer) and counts multiplications and additions/subtractions in
+their arithmetic statements; array-index arithmetic (inside [...]) is excluded.  An FMA in our
+kernels is two of these operations.  The totals define the "algorithmic FLOPs" that the
+roofline figure divides by kernel time: they describe the work the reference does, independent
+of how this project schedules it.
+
+    poly_2b_v6x  E+G   potential/2b/poly-2b-v6x.cpp:1-12934     (f0..f99)
+    poly_3b_v2x  E+G   potential/3b/poly-3b-v2x.cpp:1-26980     (f0..f130)
+    poly_2b_v6x  E     potential/2b/poly-2b-v6x-nogrd.cpp:41
+    poly_3b_v2x  E     potential/3b/poly-3b-v2x-nogrd.cpp:39
+
+Writes mbx_amd/tables/flops.json.  Usage: python mbx_amd/tables/flopcount.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def count_ops(text):
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"\[[^\]]*\]", "[]", text)  # drop index arithmetic
+    mul = add = 0
+    for stmt in text.split(";"):
+        if "=" not in stmt or "for" in stmt or "#pragma" in stmt:
+            continue
+        rhs = stmt.split("=", 1)[1]
+        if "==" in stmt or "<" in rhs:
+            continue
+        mul += rhs.count("*")
+        # binary + / -: an operator preceded by an operand character
+        add += len(re.findall(r"[\w\)\]]\s*[+-]\s*[\w\(]", rhs))
+    return mul, add
+
+
+def block_bodies(text):
+    """Bodies of the f0..fN lane-block functions."""
+    out = []
+    for m in re.finditer(r"void f\d+\([^)]*\)\s*\{", text):
+        i = m.end()
+        depth = 1
+        j = i
+        while depth:
+            depth += {"{": 1, "}": -1}.get(text[j], 0)
+            j += 1
+        out.append(text[i:j - 1])
+    return out
+
+
+def nogrd_body(text):
+    m = re.search(r"eval\s*\([^)]*\)\s*\{", text)
+    i = m.end()
+    depth = 1
+    j = i
+    while depth:
+        depth += {"{": 1, "}": -1}.get(text[j], 0)
+        j += 1
+    return text[i:j - 1]
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    res = {}
+    for key, path in (("mbpol_2b", "src/potential/2b/poly-2b-v6x.cpp"), ("mbpol_3b", "src/potential/3b/poly-3b-v2x.cpp")):
+        text = open(os.path.join(ref, path)).read()
+        mul = add = 0
+        for b in block_bodies(text):
+            m, a = count_ops(b)
+            mul += m
+            add += a
+        res[key + "_eg"] = dict(mul=mul, add=add, flops=mul + add, source=path)
+    for key, path in (("mbpol_2b", "src/potential/2b/poly-2b-v6x-nogrd.cpp"),
+                      ("mbpol_3b", "src/potential/3b/poly-3b-v2x-nogrd.cpp")):
+        text = open(os.path.join(ref, path)).read()
+        m, a = count_ops(nogrd_body(text))
+        res[key + "_e"] = dict(mul=m, add=a, flops=m + a, source=path)
+    with open(os.path.join(HERE, "flops.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    main()

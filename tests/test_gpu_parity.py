@@ -1,0 +1,157 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle on the same inputs.
+
+Bar: n-mer lists bit-exact as sets (sorted within each tuple, then sorted); 2B/3B energies within
+1e-10 relative; gradients within 1e-8 kcal/mol/A absolute; virial within 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import sorted_tuples, water_case
+
+import oracle
+from mbx_amd import configs, engine
+
+pytestmark = pytest.mark.gpu
+
+E_REL = 1e-10
+G_ABS = 1e-8
+
+FIXTURES = ["h2o_256_cubic", "h2o_2048_unittest", "h2o_2048_droplet"]
+SYNTH = {"C1": configs.C1, "C2": configs.C2}
+
+
+def _case(systems, name):
+    if name in SYNTH:
+        xyz, box, c2, c3 = configs.build(SYNTH[name])
+        return xyz, box, c2, c3
+    xyz, box, c2, c3, _ = water_case(systems, name)
+    return xyz, box, c2, c3
+
+
+def _run(xyz, box, c2, c3, terms=engine.TERM_2B | engine.TERM_3B, do_grads=True):
+    e = engine.water_engine(len(xyz) // 9, box, c2, c3)
+    e.set_coords(xyz)
+    e.compute(terms, do_grads)
+    return e
+
+
+@pytest.mark.parametrize("name", FIXTURES + list(SYNTH))
+def test_lists_bit_exact(systems, name):
+    xyz, box, c2, c3 = _case(systems, name)
+    print("lists", name, flush=True)
+    e = _run(xyz, box, c2, c3, engine.TERM_2B | engine.TERM_3B | engine.LISTS_ONLY)
+    d, t = e.lists()
+    first = oracle.fix_first(xyz, box)
+    od = oracle.clusters(2, c2, first, box)
+    ot = oracle.clusters(3, c3, first, box)
+    assert len(d) == len(od) and np.array_equal(sorted_tuples(d), sorted_tuples(od))
+    assert len(t) == len(ot) and np.array_equal(sorted_tuples(t), sorted_tuples(ot))
+
+
+@pytest.mark.parametrize("name", FIXTURES + list(SYNTH))
+def test_energy_gradient_virial(systems, name):
+    xyz, box, c2, c3 = _case(systems, name)
+    print("energy", name, flush=True)
+    e = _run(xyz, box, c2, c3)
+    e2, e3, g, v = e.results()
+    r = oracle.water_system(xyz, box, c2, c3)
+    c = e.counts()
+    assert (c[0], c[1], c[2], c[3]) == (r["n2"], r["n2_eval"], r["n3"], r["n3_eval"])
+    assert e2 == pytest.approx(r["e2b"], rel=E_REL)
+    assert e3 == pytest.approx(r["e3b"], rel=E_REL)
+    assert np.abs(g - r["grad"]).max() <= G_ABS
+    np.testing.assert_allclose(v, r["virial"], rtol=1e-8, atol=1e-8)
+    # the wrapped working coordinates equal the reference's FixMonomerCoordinates bit for bit
+    if box is not None:
+        assert np.array_equal(e.coords(), oracle.fix_monomer_coordinates(xyz, box, [3] * (len(xyz) // 9)))
+
+
+def test_per_nmer_energies(systems):
+    """Every evaluated dimer / trimer matches the oracle's energy for the same n-mer."""
+    xyz, box, c2, c3 = _case(systems, "C1")
+    e = _run(xyz, box, c2, c3)
+    r = oracle.water_system(xyz, box, c2, c3, per_nmer=True)
+    for order, key, ekey in ((2, "dimers", "e_dimer"), (3, "trimers", "e_trimer")):
+        idx, en = e.nmer_energies(order)
+        want = {tuple(sorted(t)): v for t, v in zip(r[key], r[ekey]) if v != 0.0}
+        got = {tuple(sorted(t)): v for t, v in zip(idx, en)}
+        assert set(want) <= set(got)
+        for k, v in got.items():
+            ref = want.get(k, 0.0)
+            assert v == pytest.approx(ref, rel=1e-9, abs=1e-12)
+
+
+@pytest.mark.parametrize("name", ["h2o_256_cubic", "h2o_2048_cubic", "h2o_2048_droplet"])
+def test_published_energies(systems, name):
+    """The reference's printed per-term energies (timing/*/output, 11 significant digits)."""
+    xyz, box, c2, c3, pub = water_case(systems, name)
+    e = _run(xyz, box, c2, c3)
+    e2, e3, _, _ = e.results()
+    assert e2 == pytest.approx(pub[1], rel=5e-11)
+    assert e3 == pytest.approx(pub[2], rel=5e-11)
+
+
+def test_energy_only_path(systems):
+    xyz, box, c2, c3 = _case(systems, "C1")
+    a = _run(xyz, box, c2, c3, do_grads=True).results()
+    b = _run(xyz, box, c2, c3, do_grads=False).results(grad=False)
+    assert b[0] == pytest.approx(a[0], rel=1e-12)
+    assert b[1] == pytest.approx(a[1], rel=1e-11)
+
+
+def test_repeat_is_deterministic(systems):
+    xyz, box, c2, c3 = _case(systems, "C1")
+    e = engine.water_engine(len(xyz) // 9, box, c2, c3)
+    out = []
+    for _ in range(2):
+        e.set_coords(xyz)
+        e.compute()
+        out.append(e.results()[:2] + e.lists())
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2]) and np.array_equal(out[0][3], out[1][3])
+
+
+def test_triclinic_lists(systems):
+    """Clathrate (triclinic cell, 72 CH4 + 816 H2O): lists over first atoms of mixed monomers."""
+    xyz = systems["clathrate/xyz"]
+    nat = systems["clathrate/nat"]
+    mon = systems["clathrate/mon"]
+    box = systems["clathrate/box"]
+    c2, c3 = systems["clathrate/cutoffs"]
+    e = engine.Engine()
+    e.set_topology([engine.KIND_OF[m] for m in mon], nat)
+    e.set_box(box)
+    e.set_cutoffs(c2, c3)
+    e.set_coords(xyz)
+    e.compute(engine.TERM_2B | engine.TERM_3B | engine.LISTS_ONLY)
+    d, t = e.lists()
+    fixed = oracle.fix_monomer_coordinates(xyz, box, nat)
+    fi = np.concatenate([[0], np.cumsum(nat)[:-1]])
+    first = fixed.reshape(-1, 3)[fi]
+    assert np.array_equal(sorted_tuples(d), sorted_tuples(oracle.clusters(2, c2, first, box)))
+    assert np.array_equal(sorted_tuples(t), sorted_tuples(oracle.clusters(3, c3, first, box)))
+
+
+def test_errors():
+    e = engine.Engine()
+    e.set_topology([engine.H2O] * 4)
+    with pytest.raises(engine.MbxgError):
+        e.compute()  # no coordinates yet
+    with pytest.raises(ValueError):
+        e.set_coords(np.zeros(5))
+    with pytest.raises(engine.MbxgError):
+        e.set_box([1.0, 0, 0, 0, 0, 0, 0, 0, 1.0])  # singular
+    e.set_topology([engine.CH4, engine.H2O])
+    e.set_coords(np.zeros(24))
+    with pytest.raises(engine.MbxgError):
+        e.compute()  # ch4 holders are not on the GPU yet
+
+
+def test_small_and_empty():
+    """Fewer monomers than an n-mer needs: zero energies, no lists."""
+    e = engine.water_engine(1, None, 9.0, 7.0)
+    e.set_coords(configs.water_box(1, 10.0, (1, 1, 1))[0])
+    e.compute()
+    e2, e3, g, _ = e.results()
+    assert e2 == 0.0 and e3 == 0.0 and not g.any()
+    assert list(e.counts()) == [0, 0, 0, 0]

@@ -1,0 +1,285 @@
+"""CPU tests: pin the oracle (our C restatement) against the reference's own numbers.
+
+Pins, per oracle function (the judge of every GPU parity test):
+  * polynomials   -- the reference's 21-digit KATs (unittest-poly-2b.cpp :poly_2b_v6x:struct,
+                     unittest-poly-3b.cpp poly-3b-v2x::class); the compiled reference polynomials
+                     (oracle/_ref) on random variable vectors; linearity in the coefficients and
+                     Euler's identity sum_k a_k dE/da_k = E; finite differences of the gradient.
+  * holders       -- the reference's KATs (unittest-poly-holder-2b.cpp x2b_v9x::struct,
+                     unittest-poly-holder-3b.cpp x3b-v2x::class: energy, gradients, virial); the
+                     compiled holders on random dimers/trimers; central finite differences.
+  * lists         -- the compiled AddClusters on the same points; an O(N^2)/O(N^3) minimum-image
+                     brute force; the published counts structure (every triple with >= 2 of 3
+                     distances inside the cutoff exactly once).
+  * system totals -- the reference's published per-term energies (timing/*/output, 11 digits).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, sorted_tuples, water_case
+
+import oracle
+
+KATS = json.load(open(os.path.join(GOLDEN, "kats.json")))
+ref_only = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference checkout)")
+
+
+# ---------------------------------------------------------------------------------------------
+# polynomials
+
+
+def test_poly_2b_kat():
+    k = KATS["poly_2b"]
+    t = oracle.table("mbpol_2b")
+    assert np.array_equal(np.array(k["a"]), t.coef)  # the fitted coefficients, digit for digit
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    assert e == pytest.approx(k["e_expected"][0], abs=1e-10)
+    np.testing.assert_allclose(g, k["g_expected"], atol=1e-10)
+    x2 = np.array(k["x_2"]).reshape(31, 2)
+    g2 = np.array(k["g_2_expected"]).reshape(31, 2)
+    for i in range(2):
+        e, g = oracle.poly_eval(t, x2[:, i], a=k["a"])
+        assert e == pytest.approx(k["e_2_expected"][i], abs=1e-10)
+        np.testing.assert_allclose(g, g2[:, i], atol=1e-10)
+
+
+def test_poly_3b_kat():
+    k = KATS["poly_3b"]
+    t = oracle.table("mbpol_3b")
+    assert np.array_equal(np.array(k["a"]), t.coef)
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    assert e == pytest.approx(k["e_expected"], abs=1e-10)
+    np.testing.assert_allclose(g, k["g_expected"], atol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["mbpol_2b", "mbpol_3b"])
+def test_poly_linear_and_euler(name):
+    """E is linear in a; sum_k a_k dE/da_k = E; the gradient matches central differences."""
+    t = oracle.table(name)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0.05, 0.9, t.nvars)
+    b = oracle.poly_basis(t, x)
+    e, g = oracle.poly_eval(t, x)
+    assert np.dot(t.coef, b) == pytest.approx(e, rel=1e-12)
+    a2 = rng.normal(size=t.ncoef)
+    assert oracle.poly_eval(t, x, a=a2, grad=False) == pytest.approx(np.dot(a2, b), rel=1e-10, abs=1e-10)
+    h = 1e-6
+    for j in rng.choice(t.nvars, 6, replace=False):
+        xp, xm = x.copy(), x.copy()
+        xp[j] += h
+        xm[j] -= h
+        fd = (oracle.poly_eval(t, xp, grad=False) - oracle.poly_eval(t, xm, grad=False)) / (2 * h)
+        assert fd == pytest.approx(g[j], rel=1e-6, abs=1e-6)
+
+
+@ref_only
+def test_poly_vs_compiled_reference():
+    rng = np.random.default_rng(11)
+    L = oracle.ref()
+    t2, t3 = oracle.table("mbpol_2b"), oracle.table("mbpol_3b")
+    import ctypes
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    nd = 16
+    x = rng.uniform(0.0, 1.2, (31, nd))
+    e = np.zeros(nd)
+    g = np.zeros((31, nd))
+    xa = np.ascontiguousarray(x)
+    L.ref_poly2b(nd, t2.coef.ctypes.data_as(dp), xa.ctypes.data_as(dp), e.ctypes.data_as(dp), g.ctypes.data_as(dp))
+    for i in range(nd):
+        eo, go = oracle.poly_eval(t2, x[:, i])
+        assert eo == pytest.approx(e[i], rel=1e-11, abs=1e-11)
+        np.testing.assert_allclose(go, g[:, i], rtol=1e-10, atol=1e-10)
+    x3 = np.ascontiguousarray(rng.uniform(0.0, 1.2, (36, 8)))
+    e3 = np.zeros(8)
+    g3 = np.zeros((36, 8))
+    L.ref_poly3b(t3.coef.ctypes.data_as(dp), x3.ctypes.data_as(dp), e3.ctypes.data_as(dp), g3.ctypes.data_as(dp))
+    for i in range(8):
+        eo, go = oracle.poly_eval(t3, x3[:, i])
+        assert eo == pytest.approx(e3[i], rel=1e-11, abs=1e-11)
+        np.testing.assert_allclose(go, g3[:, i], rtol=1e-10, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------------------------
+# holders
+
+
+def test_holder_2b_kat():
+    k = KATS["holder_2b"]
+    w1 = np.array(k["xyz1"]).reshape(2, 9)
+    w2 = np.array(k["xyz2"]).reshape(2, 9)
+    E, G1, G2, V = 0.0, [], [], np.zeros(9)
+    for i in range(2):
+        e, g1, g2, v, _ = oracle.mbpol2b(w1[i], w2[i])
+        E += e
+        G1.append(g1)
+        G2.append(g2)
+        V += v
+    assert E == pytest.approx(k["energy_expected"], abs=1e-9)
+    np.testing.assert_allclose(np.concatenate(G1), k["grad1_expected"], atol=1e-9)
+    np.testing.assert_allclose(np.concatenate(G2), k["grad2_expected"], atol=1e-9)
+    np.testing.assert_allclose(V, k["virial_expected"], atol=1e-8)
+
+
+def test_holder_3b_kat():
+    k = KATS["holder_3b"]
+    e, g1, g2, g3, v, ev = oracle.mbpol3b(k["xyz1"], k["xyz2"], k["xyz3"])
+    assert ev == 1
+    assert e == pytest.approx(k["energy_expected"], abs=1e-10)
+    for g, key in ((g1, "grad1_expected"), (g2, "grad2_expected"), (g3, "grad3_expected")):
+        np.testing.assert_allclose(g, k[key], atol=1e-10)
+    np.testing.assert_allclose(v, k["virial_expected"], atol=1e-10)
+
+
+def _random_water_pair(rng, r):
+    from mbx_amd import configs
+
+    o = np.array([[0.0, 0.0, 0.0], rng.normal(size=3)])
+    o[1] *= r / np.linalg.norm(o[1])
+    return configs.place_waters(rng, o).reshape(2, 9)
+
+
+def _fd5(f, x, j, h=1e-3):
+    """Five-point central difference (truncation error O(h^4)).  The energies are sums of
+    strongly cancelling monomials (~1e-11 absolute noise), so h stays large enough that the
+    round-off term noise/h does not dominate."""
+    def at(d):
+        y = x.copy()
+        y[j] += d
+        return f(y)
+    return (-at(2 * h) + 8 * at(h) - 8 * at(-h) + at(-2 * h)) / (12 * h)
+
+
+def test_holder_2b_finite_differences():
+    rng = np.random.default_rng(3)
+    for r in (2.7, 5.0, 6.2):  # inside, switching region, near the outer edge
+        w = _random_water_pair(rng, r)
+        e, g1, g2, _, ev = oracle.mbpol2b(w[0], w[1])
+        assert ev == 1
+        g = np.concatenate([g1, g2])
+        x = w.reshape(-1)
+        f = lambda y: oracle.mbpol2b(y[:9], y[9:])[0]
+        for j in rng.choice(18, 8, replace=False):
+            assert _fd5(f, x, j) == pytest.approx(g[j], abs=1e-7, rel=1e-8)
+
+
+def test_holder_3b_finite_differences():
+    from mbx_amd import configs
+
+    rng = np.random.default_rng(5)
+    o = np.array([[0.0, 0.0, 0.0], [2.8, 0.3, 0.1], [1.2, 2.6, -0.4]])
+    w = configs.place_waters(rng, o).reshape(3, 9)
+    e, g1, g2, g3, _, ev = oracle.mbpol3b(*w)
+    assert ev == 1
+    g = np.concatenate([g1, g2, g3])
+    x = w.reshape(-1)
+    f = lambda y: oracle.mbpol3b(y[:9], y[9:18], y[18:])[0]
+    for j in rng.choice(27, 10, replace=False):
+        assert _fd5(f, x, j) == pytest.approx(g[j], abs=1e-7, rel=1e-8)
+
+
+@ref_only
+def test_holders_vs_compiled_reference():
+    import ctypes
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    L = oracle.ref()
+    rng = np.random.default_rng(9)
+    for r in (2.6, 3.1, 4.9, 6.4, 6.6):
+        w = _random_water_pair(rng, r)
+        e, g1, g2, v, _ = oracle.mbpol2b(w[0], w[1])
+        a, b = np.ascontiguousarray(w[0]), np.ascontiguousarray(w[1])
+        rg1, rg2, rv = np.zeros(9), np.zeros(9), np.zeros(9)
+        re = L.ref_x2b(a.ctypes.data_as(dp), b.ctypes.data_as(dp), 1, rg1.ctypes.data_as(dp), rg2.ctypes.data_as(dp),
+                       rv.ctypes.data_as(dp))
+        assert e == pytest.approx(re, rel=1e-11, abs=1e-12)
+        np.testing.assert_allclose(np.concatenate([g1, g2]), np.concatenate([rg1, rg2]), atol=1e-10)
+        np.testing.assert_allclose(v, rv, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------------------------
+# lists
+
+
+def _min_image_brute(first, box, cutoff, nmax):
+    """Reference rule restated over minimum-image distances: every pair (triple with >= 2 of 3
+    pair distances) strictly inside the cutoff, once."""
+    n = len(first)
+    if box is not None:
+        L = np.diag(np.asarray(box).reshape(3, 3))
+        d = first[:, None, :] - first[None, :, :]
+        d -= L * np.round(d / L)
+    else:
+        d = first[:, None, :] - first[None, :, :]
+    r2 = (d * d).sum(-1)
+    inside = r2 < cutoff * cutoff
+    np.fill_diagonal(inside, False)
+    if nmax == 2:
+        i, j = np.nonzero(np.triu(inside, 1))
+        return np.stack([i, j], 1)
+    out = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                if int(inside[i, j]) + int(inside[i, k]) + int(inside[j, k]) >= 2:
+                    out.append((i, j, k))
+    return np.array(out, np.int64).reshape(-1, 3)
+
+
+def test_lists_vs_brute_force_small():
+    from mbx_amd import configs
+
+    xyz, box = configs.water_box(96, 14.4, (5, 5, 4), seed=96)
+    first = oracle.fix_first(xyz, box)
+    for nmax, c in ((2, 7.0), (3, 5.5)):
+        got = sorted_tuples(oracle.clusters(nmax, c, first, box))
+        want = sorted_tuples(_min_image_brute(first, box, c, nmax))
+        assert np.array_equal(got, want)
+    # gas phase
+    got = sorted_tuples(oracle.clusters(3, 5.0, first, None))
+    assert np.array_equal(got, sorted_tuples(_min_image_brute(first, None, 5.0, 3)))
+
+
+@ref_only
+@pytest.mark.parametrize("case", ["h2o_256_cubic", "clathrate"])
+def test_lists_vs_compiled_reference(systems, case):
+    xyz = systems[case + "/xyz"]
+    nat = systems[case + "/nat"]
+    box = systems[case + "/box"]
+    box = box if box.size else None
+    c2, c3 = systems[case + "/cutoffs"]
+    fixed = oracle.fix_monomer_coordinates(xyz, box, nat) if box is not None else xyz
+    fi = np.concatenate([[0], np.cumsum(nat)[:-1]])
+    first = fixed.reshape(-1, 3)[fi]
+    for nmax, c in ((2, c2), (3, c3)):
+        got = sorted_tuples(oracle.clusters(nmax, c, first, box))
+        want = sorted_tuples(oracle.ref_add_clusters(nmax, c, first, box))
+        assert len(got) == len(want) and np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------------------------
+# system totals
+
+
+def test_published_256_water(systems):
+    xyz, box, c2, c3, pub = water_case(systems, "h2o_256_cubic")
+    r = oracle.water_system(xyz, box, c2, c3)
+    assert r["e2b"] == pytest.approx(pub[1], rel=5e-11)
+    assert r["e3b"] == pytest.approx(pub[2], rel=5e-11)
+
+
+@ref_only
+def test_system_vs_compiled_reference():
+    from mbx_amd import configs
+
+    xyz, box, c2, c3 = configs.build(configs.C1)
+    r = oracle.water_system(xyz, box, c2, c3)
+    q = oracle.ref_water_2b3b(xyz, box, c2, c3, batch=8, nthreads=4)
+    assert r["n2"] == q["n2"] and r["n3"] == q["n3"]
+    assert r["e2b"] == pytest.approx(q["e2b"], rel=1e-11)
+    assert r["e3b"] == pytest.approx(q["e3b"], rel=1e-11)
+    assert np.abs(r["grad"] - q["grad"]).max() < 1e-9
+    np.testing.assert_allclose(r["virial"], q["virial"], rtol=1e-10, atol=1e-8)
