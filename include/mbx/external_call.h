@@ -1,0 +1,49 @@
+/*
+ * Reference-compatible extern "C" entry points for the 2B/3B path (the Fortran / Python ctypes
+ * ABI of MBX, src/bblock/external_call.cpp:47-427), implemented over bblock::System
+ * (include/mbx/system.h) and the MI355X engine.  Same names, argument meaning and one global
+ * system per process; existing callers switch by relinking against libmbxg.so.
+ *
+ * Energies cover the 2B + 3B terms; in get_energy_decomp_* the 1B, 4B, dispersion,
+ * Buckingham and electrostatic outputs are written as 0 (those terms are outside this engine).
+ */
+#ifndef MBX_EXTERNAL_CALL_H
+#define MBX_EXTERNAL_CALL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* external_call.cpp:57 */
+void initialize_system_(double* coords, int* nat_monomers, char at_names[][5], char monomers[][5], int* nmon,
+                        char json_file[20]);
+/* external_call.cpp:106 */
+void initialize_system_py_(double* coords, int* nat_monomers, char** at_names, char** monomers, int* nmon,
+                           char* json_file);
+/* external_call.cpp:131 */
+void get_energy_(double* coords, int* nat, double* energy);
+/* external_call.cpp:208 */
+void get_energy_g_(double* coords, int* nat, double* energy, double* grads);
+/* external_call.cpp:237 */
+void get_energy_pbc_(double* coords, int* nat, double* box, double* energy);
+/* external_call.cpp:318 */
+void get_energy_pbc_g_(double* coords, int* nat, double* box, double* energy, double* grads);
+/* external_call.cpp:159 */
+void get_energy_decomp_(double* coords, int* nat, double* E1b, double* E2b, double* E3b, double* E4b, double* Edisp,
+                        double* Ebuck, double* Eelec);
+/* external_call.cpp:261 */
+void get_energy_decomp_pbc_(double* coords, int* nat, double* box, double* E1b, double* E2b, double* E3b,
+                            double* E4b, double* Edisp, double* Ebuck, double* Eelec);
+/* external_call.cpp:343 */
+void get_virial_(double* virial);
+/* external_call.cpp:355 */
+void set_real_xyz_(double* coords, int* nat);
+/* external_call.cpp:376 */
+void set_box_(int* length, double* box);
+/* external_call.cpp:422 */
+void finalize_system_(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+// bblock::System -- the reference's public System API for the many-body short-range path,
+// backed by the MI355X engine (include/mbxg.h).
+//
+// Mirrors src/bblock/system.h of MBX v1.3.3 for the members a caller of the 2B/3B path uses:
+// same names, by-value arguments and return values, the same internal monomer ordering
+// (types from fewest to most monomers, input order within a type; sys_tools.cpp:44-131) and
+// CUException errors.  Energy() returns the 2B + 3B terms; the 1B, dispersion, electrostatics
+// and other terms of the reference's Energy (system.cpp:1995-2138) are outside this engine
+// and contribute nothing here.
+#pragma once
+
+#include <cstddef>
+#include <exception>
+#include <string>
+#include <vector>
+
+struct mbxg_ctx;
+
+namespace bblock {
+
+// custom_exceptions.h of the reference: a std::exception carrying a formatted message.
+class CUException : public std::exception {
+   public:
+    CUException(const char* func, const char* file, int line, const std::string& text);
+    const char* what() const noexcept override { return msg_.c_str(); }
+
+   private:
+    std::string msg_;
+};
+
+class System {
+   public:
+    System();
+    ~System();
+    System(const System&) = delete;
+    System& operator=(const System&) = delete;
+
+    // system.h:564, system.cpp:848-870
+    void AddMonomer(std::vector<double> xyz, std::vector<std::string> atoms, std::string id, size_t islocal = 1,
+                    int tag = 0);
+    // system.h:713, system.cpp:1000
+    void Initialize();
+    // system.h:770-782, system.cpp:1198-1608.  Keys read under "MBX": box, twobody_cutoff,
+    // threebody_cutoff, max_n_eval_2b, max_n_eval_3b, ignore_2b_poly, ignore_3b_poly.
+    void SetUpFromJson(char* json_file = 0);
+    void SetUpFromJson(std::string json_text);
+
+    // system.h:1057, 1088, 1100
+    double Energy(bool do_grads);
+    double TwoBodyEnergy(bool do_grads, bool use_ghost = 0);
+    double ThreeBodyEnergy(bool do_grads, bool use_ghost = 0);
+
+    // system.h:285, 450, 542, 925
+    std::vector<double> GetRealGrads();
+    std::vector<double> GetVirial();
+    std::vector<double> GetRealXyz();
+    void SetRealXyz(std::vector<double> xyz);
+    void SetPBC(std::vector<double> box = {});
+    std::vector<double> GetBox();
+
+    void Set2bCutoff(double cutoff2b);
+    void Set3bCutoff(double cutoff3b);
+    double Get2bCutoff();
+    double Get3bCutoff();
+    void SetNMaxEval2b(size_t nmax);
+    void SetNMaxEval3b(size_t nmax);
+    size_t GetMaxEval2b();
+    size_t GetMaxEval3b();
+    void Set2bIgnorePoly(std::vector<std::vector<std::string>> ignore_2b);
+    void Set3bIgnorePoly(std::vector<std::vector<std::string>> ignore_3b);
+
+    // system.h:246, system.cpp:313: n-mers (input monomer indices) with a member in [istart, iend)
+    std::vector<size_t> GetPairList(size_t nmax, double cutoff, size_t istart, size_t iend, bool use_ghost = false);
+
+    size_t GetNumMon();
+    size_t GetNumRealSites();
+    std::string GetMonId(size_t n);
+
+    // The n-mer counts of the last energy call: listed / evaluated dimers, listed / evaluated trimers.
+    std::vector<long> GetNmerCounts();
+
+   private:
+    void check_initialized(const char* func, int line);
+    double run_terms(int terms, bool do_grads, bool use_ghost);
+    void apply_box();
+
+    mbxg_ctx* ctx_ = nullptr;
+    bool initialized_ = false;
+    // input order
+    std::vector<double> in_xyz_;
+    std::vector<std::string> in_atoms_;
+    std::vector<std::string> in_mon_;
+    std::vector<size_t> in_local_;
+    std::vector<int> in_tag_;  // per atom
+    std::vector<size_t> in_nat_, in_first_;
+    int tag_counter_ = 0;
+    // internal order (OrderMonomers): internal monomer i is input monomer order_[i]
+    std::vector<size_t> order_, first_;  // first_: first real atom of internal monomer i
+    std::vector<double> xyz_;            // internal order, real atoms
+    std::vector<double> grad_;           // internal order, real atoms
+    std::vector<double> virial_;
+    std::vector<double> box_;
+    double cutoff2b_ = 100.0, cutoff3b_ = 5.0;
+    size_t max2b_ = 1024, max3b_ = 1024;
+    std::vector<std::vector<std::string>> ignore2b_, ignore3b_;
+    std::vector<long> counts_ = std::vector<long>(4, 0);
+    size_t numat_ = 0;
+};
+
+}  // namespace bblock

@@ -1,0 +1,345 @@
+// bblock::System over the mbxg engine (see include/mbx/system.h).
+//
+// Bookkeeping restated from the reference (paths relative to its src/):
+//   AddMonomer tags          bblock/system.cpp:848-870
+//   OrderMonomers            bblock/sys_tools.cpp:44-131 (types from fewest to most monomers)
+//   SetUpFromJson keys       bblock/system.cpp:1198-1581 (defaults :78, :85, :98-100)
+//   SetPBC box forms         bblock/system.cpp:712-763, tools/math_tools.cpp:83-152
+//   ResetOrderReal3N         bblock/sys_tools.cpp:859-872 (GetRealGrads / GetRealXyz)
+//   GetPairList              bblock/system.cpp:313-380
+#include "../../include/mbx/system.h"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "../../include/mbxg.h"
+#include "json_min.hpp"
+
+namespace bblock {
+
+CUException::CUException(const char* func, const char* file, int line, const std::string& text) {
+    std::ostringstream os;
+    os << "\n\n[ERROR]  Function: " << func << " in " << file << ":" << line << "\n[ERROR]  " << text << "\n";
+    msg_ = os.str();
+}
+
+#define MBX_THROW(text) throw CUException(__func__, __FILE__, __LINE__, (text))
+
+namespace {
+
+int kind_of(const std::string& id) {
+    if (id == "h2o") return MBXG_H2O;
+    if (id == "ch4") return MBXG_CH4;
+    return MBXG_OTHER;
+}
+
+void check(int rc) {
+    if (rc != MBXG_OK) throw CUException("mbxg", __FILE__, __LINE__, mbxg_last_error());
+}
+
+// BoxABCabcToBoxVec (tools/math_tools.cpp:132-152)
+std::vector<double> abc_to_vec(const std::vector<double>& box) {
+    const double A = box[0], B = box[1], C = box[2];
+    const double alpha = box[3] / 180.0 * M_PI, beta = box[4] / 180.0 * M_PI, gamma = box[5] / 180.0 * M_PI;
+    std::vector<double> out(9, 0.0);
+    out[0] = A;
+    out[3] = B * cos(gamma);
+    out[4] = B * sin(gamma);
+    out[6] = C * cos(beta);
+    const double tmp = (cos(alpha) - cos(beta) * cos(gamma)) / sin(gamma);
+    out[7] = C * tmp;
+    out[8] = C * sqrt(1.0 - cos(beta) * cos(beta) - tmp * tmp);
+    return out;
+}
+
+std::vector<std::vector<std::string>> read_ignore(const mbxjson::Value& v) {
+    std::vector<std::vector<std::string>> out;
+    if (v.kind != mbxjson::Value::Array) return out;
+    for (const auto& e : v.arr) {
+        std::vector<std::string> p;
+        for (const auto& s : e.arr) p.push_back(s.as_string());
+        std::sort(p.begin(), p.end());
+        out.push_back(p);
+    }
+    return out;
+}
+
+bool ignored(const std::vector<std::vector<std::string>>& list, std::vector<std::string> key) {
+    std::sort(key.begin(), key.end());
+    return std::find(list.begin(), list.end(), key) != list.end();
+}
+
+}  // namespace
+
+System::System() : virial_(9, 0.0) {}
+
+System::~System() {
+    if (ctx_) mbxg_destroy(ctx_);
+}
+
+void System::AddMonomer(std::vector<double> xyz, std::vector<std::string> atoms, std::string id, size_t islocal,
+                        int tag) {
+    if (initialized_)
+        MBX_THROW("The system has already been initialized. Adding a new monomer is not possible");
+    if (xyz.size() != 3 * atoms.size()) MBX_THROW("Number of coordinates is not consistent with number of atoms");
+    in_first_.push_back(in_atoms_.size());
+    in_nat_.push_back(atoms.size());
+    in_xyz_.insert(in_xyz_.end(), xyz.begin(), xyz.end());
+    in_atoms_.insert(in_atoms_.end(), atoms.begin(), atoms.end());
+    in_mon_.push_back(id);
+    in_local_.push_back(islocal);
+    for (size_t i = 0; i < atoms.size(); ++i) in_tag_.push_back(tag == 0 ? tag_counter_ + (int)i : tag + (int)i);
+    tag_counter_ += (int)atoms.size();
+}
+
+void System::Initialize() {
+    if (initialized_) MBX_THROW("The system has already been initialized. Reinitialization is not possible");
+    const size_t nmon = in_mon_.size();
+    if (nmon == 0) MBX_THROW("Cannot initialize the system with less than 3 coordinates, and current size of xyz is 0");
+    for (size_t i = 0; i < nmon; ++i) {
+        const int k = kind_of(in_mon_[i]);
+        if ((k == MBXG_H2O && in_nat_[i] != 3) || (k == MBXG_CH4 && in_nat_[i] != 5))
+            MBX_THROW("Monomer " + std::to_string(i) + " (" + in_mon_[i] + ") has " + std::to_string(in_nat_[i]) +
+                      " atoms");
+    }
+    // OrderMonomers: distinct types in order of appearance, then repeatedly the type with the
+    // fewest monomers (first such on ties), each in input order.
+    std::vector<std::pair<std::string, size_t>> types;
+    for (const auto& m : in_mon_) {
+        auto it = std::find_if(types.begin(), types.end(), [&](const auto& p) { return p.first == m; });
+        if (it == types.end())
+            types.push_back({m, 1});
+        else
+            ++it->second;
+    }
+    order_.clear();
+    while (!types.empty()) {
+        size_t best = 0;
+        for (size_t t = 1; t < types.size(); ++t)
+            if (types[t].second < types[best].second) best = t;
+        for (size_t i = 0; i < nmon; ++i)
+            if (in_mon_[i] == types[best].first) order_.push_back(i);
+        types.erase(types.begin() + best);
+    }
+    first_.assign(nmon, 0);
+    numat_ = 0;
+    std::vector<int> kind(nmon), nat(nmon), tag(nmon);
+    std::vector<unsigned char> loc(nmon);
+    for (size_t i = 0; i < nmon; ++i) {
+        const size_t k = order_[i];
+        first_[i] = numat_;
+        numat_ += in_nat_[k];
+        kind[i] = kind_of(in_mon_[k]);
+        nat[i] = (int)in_nat_[k];
+        tag[i] = in_tag_[in_first_[k]];
+        loc[i] = in_local_[k] ? 1 : 0;
+    }
+    xyz_.assign(3 * numat_, 0.0);
+    grad_.assign(3 * numat_, 0.0);
+    for (size_t i = 0; i < nmon; ++i) {
+        const size_t k = order_[i];
+        std::copy(in_xyz_.begin() + 3 * in_first_[k], in_xyz_.begin() + 3 * (in_first_[k] + in_nat_[k]),
+                  xyz_.begin() + 3 * first_[i]);
+    }
+    check(mbxg_create(&ctx_, 0));
+    check(mbxg_set_topology(ctx_, (int)nmon, kind.data(), nat.data(), tag.data(), loc.data()));
+    initialized_ = true;
+    apply_box();
+}
+
+void System::apply_box() {
+    if (!ctx_) return;
+    check(mbxg_set_box(ctx_, box_.size() == 9 ? box_.data() : nullptr));
+}
+
+void System::SetUpFromJson(char* json_file) {
+    std::string text = "{}";
+    if (json_file) {
+        std::ifstream f(json_file);
+        if (f) {
+            std::stringstream ss;
+            ss << f.rdbuf();
+            text = ss.str();
+        } else {
+            std::cerr << "**WARNING** json file " << json_file << " could not be read. Using defaults.\n";
+        }
+    }
+    SetUpFromJson(text);
+}
+
+void System::SetUpFromJson(std::string json_text) {
+    mbxjson::Value root;
+    try {
+        root = mbxjson::parse(json_text);
+    } catch (const std::exception& e) {
+        std::cerr << "**WARNING** json parse error (" << e.what() << "). Using defaults.\n";
+        root = mbxjson::parse("{}");
+    }
+    const mbxjson::Value* mbx = root.has("MBX") ? &root.at("MBX") : nullptr;
+    auto num = [&](const char* k, double dflt) { return mbx && mbx->has(k) ? mbx->at(k).as_number() : dflt; };
+    std::vector<double> box;
+    if (mbx && mbx->has("box"))
+        for (const auto& v : mbx->at("box").arr) box.push_back(v.as_number());
+    cutoff2b_ = num("twobody_cutoff", cutoff2b_);
+    cutoff3b_ = num("threebody_cutoff", cutoff3b_);
+    max2b_ = (size_t)num("max_n_eval_2b", (double)max2b_);
+    max3b_ = (size_t)num("max_n_eval_3b", (double)max3b_);
+    if (mbx && mbx->has("ignore_2b_poly")) ignore2b_ = read_ignore(mbx->at("ignore_2b_poly"));
+    if (mbx && mbx->has("ignore_3b_poly")) ignore3b_ = read_ignore(mbx->at("ignore_3b_poly"));
+    if (!initialized_ && !in_mon_.empty()) Initialize();
+    SetPBC(box);
+}
+
+void System::SetPBC(std::vector<double> box) {
+    if (box.size() != 9 && box.size() != 6 && box.size() != 0)
+        MBX_THROW("Box size of " + std::to_string(box.size()) + " is not acceptable.");
+    box_ = box.size() == 6 ? abc_to_vec(box) : box;
+    if (box_.size() == 9) {
+        const double det = box_[0] * (box_[4] * box_[8] - box_[7] * box_[5]) -
+                           box_[1] * (box_[3] * box_[8] - box_[5] * box_[6]) +
+                           box_[2] * (box_[3] * box_[7] - box_[4] * box_[6]);
+        if (std::fabs(det) < 1E-36) MBX_THROW("Matrix has no inverse. Determinant is 0.");
+    }
+    apply_box();
+}
+
+std::vector<double> System::GetBox() { return box_; }
+
+void System::check_initialized(const char* func, int line) {
+    if (!initialized_)
+        throw CUException(func, __FILE__, line, "System has not been initialized. Energy calculation not possible.");
+}
+
+double System::run_terms(int terms, bool do_grads, bool use_ghost) {
+    if (use_ghost)
+        MBX_THROW("use_ghost (local/ghost monomer decomposition) is not supported by the GPU engine");
+    std::fill(grad_.begin(), grad_.end(), 0.0);
+    std::fill(virial_.begin(), virial_.end(), 0.0);
+    const bool all_water = std::all_of(in_mon_.begin(), in_mon_.end(), [](const std::string& m) { return m == "h2o"; });
+    if (all_water && ignored(ignore2b_, {"h2o", "h2o"})) terms &= ~MBXG_2B;
+    if (all_water && ignored(ignore3b_, {"h2o", "h2o", "h2o"})) terms &= ~MBXG_3B;
+    check(mbxg_set_cutoffs(ctx_, cutoff2b_, cutoff3b_));
+    check(mbxg_set_coords(ctx_, xyz_.data()));
+    double e2 = 0.0, e3 = 0.0;
+    if (terms & (MBXG_2B | MBXG_3B)) {
+        check(mbxg_compute(ctx_, terms, do_grads ? 1 : 0));
+        check(mbxg_get_results(ctx_, &e2, &e3, do_grads ? grad_.data() : nullptr, virial_.data()));
+        int64_t c[4];
+        check(mbxg_get_counts(ctx_, c));
+        for (int k = 0; k < 4; ++k) counts_[k] = (long)c[k];
+    }
+    // SetPBC wraps the working coordinates each call (system.cpp:2015 -> sys_tools.cpp:257-320)
+    check(mbxg_get_coords(ctx_, xyz_.data()));
+    if (!do_grads) std::fill(virial_.begin(), virial_.end(), 0.0);
+    return e2 + e3;
+}
+
+double System::Energy(bool do_grads) {
+    if (!initialized_) SetUpFromJson(std::string("{}"));
+    check_initialized(__func__, __LINE__);
+    return run_terms(MBXG_2B | MBXG_3B, do_grads, false);
+}
+
+double System::TwoBodyEnergy(bool do_grads, bool use_ghost) {
+    check_initialized(__func__, __LINE__);
+    return run_terms(MBXG_2B, do_grads, use_ghost);
+}
+
+double System::ThreeBodyEnergy(bool do_grads, bool use_ghost) {
+    check_initialized(__func__, __LINE__);
+    return run_terms(MBXG_3B, do_grads, use_ghost);
+}
+
+std::vector<double> System::GetRealGrads() {
+    std::vector<double> out(3 * numat_);
+    for (size_t i = 0; i < order_.size(); ++i) {
+        const size_t k = order_[i];
+        std::copy(grad_.begin() + 3 * first_[i], grad_.begin() + 3 * (first_[i] + in_nat_[k]),
+                  out.begin() + 3 * in_first_[k]);
+    }
+    return out;
+}
+
+std::vector<double> System::GetVirial() { return virial_; }
+
+std::vector<double> System::GetRealXyz() {
+    std::vector<double> out(3 * numat_);
+    for (size_t i = 0; i < order_.size(); ++i) {
+        const size_t k = order_[i];
+        std::copy(xyz_.begin() + 3 * first_[i], xyz_.begin() + 3 * (first_[i] + in_nat_[k]),
+                  out.begin() + 3 * in_first_[k]);
+    }
+    return out;
+}
+
+void System::SetRealXyz(std::vector<double> xyz) {
+    check_initialized(__func__, __LINE__);
+    if (xyz.size() != 3 * numat_) MBX_THROW("SetRealXyz: expected " + std::to_string(3 * numat_) + " coordinates");
+    for (size_t i = 0; i < order_.size(); ++i) {
+        const size_t k = order_[i];
+        std::copy(xyz.begin() + 3 * in_first_[k], xyz.begin() + 3 * (in_first_[k] + in_nat_[k]),
+                  xyz_.begin() + 3 * first_[i]);
+    }
+}
+
+void System::Set2bCutoff(double c) { cutoff2b_ = c; }
+void System::Set3bCutoff(double c) { cutoff3b_ = c; }
+double System::Get2bCutoff() { return cutoff2b_; }
+double System::Get3bCutoff() { return cutoff3b_; }
+void System::SetNMaxEval2b(size_t n) { max2b_ = n; }
+void System::SetNMaxEval3b(size_t n) { max3b_ = n; }
+size_t System::GetMaxEval2b() { return max2b_; }
+size_t System::GetMaxEval3b() { return max3b_; }
+
+void System::Set2bIgnorePoly(std::vector<std::vector<std::string>> v) {
+    for (auto& p : v) std::sort(p.begin(), p.end());
+    ignore2b_ = v;
+}
+
+void System::Set3bIgnorePoly(std::vector<std::vector<std::string>> v) {
+    for (auto& p : v) std::sort(p.begin(), p.end());
+    ignore3b_ = v;
+}
+
+std::vector<size_t> System::GetPairList(size_t nmax, double cutoff, size_t istart, size_t iend, bool use_ghost) {
+    if (nmax != 2 && nmax != 3)
+        MBX_THROW("nmax value of " + std::to_string(nmax) + " is not acceptable. Possible values are 2 or 3.");
+    check_initialized(__func__, __LINE__);
+    if (use_ghost) MBX_THROW("use_ghost is not supported by the GPU engine");
+    check(mbxg_set_cutoffs(ctx_, nmax == 2 ? cutoff : 0.0, nmax == 3 ? cutoff : 0.0));
+    check(mbxg_set_coords(ctx_, xyz_.data()));
+    check(mbxg_compute(ctx_, (nmax == 2 ? MBXG_2B : MBXG_3B) | MBXG_LISTS_ONLY, 0));
+    int64_t c[4];
+    check(mbxg_get_counts(ctx_, c));
+    const int64_t n = nmax == 2 ? c[0] : c[2];
+    std::vector<int32_t> l((size_t)(nmax * (n > 0 ? n : 1)));
+    check(mbxg_get_lists(ctx_, nmax == 2 ? l.data() : nullptr, n, nmax == 3 ? l.data() : nullptr, n));
+    check(mbxg_get_coords(ctx_, xyz_.data()));
+    std::vector<size_t> out;
+    for (int64_t t = 0; t < n; ++t) {
+        bool in = false;
+        size_t mon[3];
+        for (size_t q = 0; q < nmax; ++q) {
+            mon[q] = order_[(size_t)l[nmax * t + q]];
+            in = in || (mon[q] >= istart && mon[q] < iend);
+        }
+        if (in)
+            for (size_t q = 0; q < nmax; ++q) out.push_back(mon[q]);
+    }
+    return out;
+}
+
+size_t System::GetNumMon() { return in_mon_.size(); }
+size_t System::GetNumRealSites() { return numat_; }
+
+std::string System::GetMonId(size_t n) {
+    if (n >= in_mon_.size()) MBX_THROW("monomer index out of range");
+    return in_mon_[n];
+}
+
+std::vector<long> System::GetNmerCounts() { return counts_; }
+
+}  // namespace bblock

@@ -78,7 +78,8 @@ def test_per_nmer_energies(systems):
         assert set(want) <= set(got)
         for k, v in got.items():
             ref = want.get(k, 0.0)
-            assert v == pytest.approx(ref, rel=1e-9, abs=1e-12)
+            # single n-mer energies are sums of strongly cancelling monomials: ~1e-12 absolute noise
+            assert v == pytest.approx(ref, rel=1e-9, abs=1e-10)
 
 
 @pytest.mark.parametrize("name", ["h2o_256_cubic", "h2o_2048_cubic", "h2o_2048_droplet"])
