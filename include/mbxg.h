@@ -63,13 +63,17 @@ int mbxg_set_cutoffs(mbxg_ctx* ctx, double cutoff2b, double cutoff3b);
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
 
 /* Coordinates of all atoms (3*natoms doubles, internal order) from host or device memory.
- * They become the context's working copy, which mbxg_compute wraps in place under PBC the way
- * System::SetPBC wraps xyz_ (sys_tools.cpp:257-320). */
+ * mbxg_compute wraps them into its working copy under PBC the way System::SetPBC wraps xyz_
+ * (sys_tools.cpp:257-320); the input copy itself is kept unchanged. */
 int mbxg_set_coords(mbxg_ctx* ctx, const double* xyz);
 int mbxg_set_coords_device(mbxg_ctx* ctx, const double* d_xyz);
 
 /* Run lists + evaluation for `terms` (MBXG_2B | MBXG_3B) on the context stream.  Results stay
- * on the device until mbxg_get_results / the device accessors. */
+ * on the device until mbxg_get_results / the device accessors.  The first compute after a new
+ * topology, box or cutoffs synchronises once to size the list buffers (with headroom); later
+ * computes enqueue everything without waiting.  A compute whose lists outgrow the buffers is
+ * detected by the next getter or mbxg_synchronize and recomputed there; if the inputs were
+ * changed in between, that call fails with MBXG_ESTATE and the caller computes again. */
 int mbxg_compute(mbxg_ctx* ctx, int terms, int do_grads);
 
 /* Wait for the context stream and copy results: energies, gradient (3*natoms, internal
@@ -99,12 +103,14 @@ int mbxg_device_outputs(mbxg_ctx* ctx, double** d_grad, double** d_totals);
 int mbxg_stream(mbxg_ctx* ctx, void** stream);
 
 /* Stage timing with HIP events on the context stream.  After a compute with timing on,
- * mbxg_get_timings writes up to 10 durations in ms: FixMonomerCoordinates, 2B lists, 3B lists,
- * 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total. */
+ * mbxg_get_timings writes up to 10 durations in ms: FixMonomerCoordinates, list count pass (grid,
+ * cloud, binning, counts, scan), list emit pass, 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total. */
 int mbxg_set_timing(mbxg_ctx* ctx, int on);
 int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
 
-/* Block until the context's stream is idle. */
+/* Block until the context's stream is idle and check that the last compute's n-mer lists fitted
+ * their buffers (recomputing it with larger buffers if not).  Call before reading the device
+ * pointers of mbxg_device_outputs; the host-side getters do this themselves. */
 int mbxg_synchronize(mbxg_ctx* ctx);
 
 #ifdef __cplusplus

@@ -1,14 +1,23 @@
 // Host runtime of the mbxg engine: context, device buffers, the 2B/3B pipeline and the C ABI
 // declared in include/mbxg.h.
 //
-// Pipeline of mbxg_compute (all on the context's stream):
-//   k_fix_monomers         FixMonomerCoordinates on the working copy (PBC only)
-//   per order (2, 3):      k_cloud -> cell binning -> k_dimers|k_trimers count pass ->
-//                          k_scan_counts -> [totals to host, grow buffers] -> emit pass
-//   k_mbpol2b, k_mbpol3b   polynomial evaluation over the compacted evaluate lists
-//   k_reduce_partials      deterministic energy / virial totals
+// Pipeline of mbxg_compute (all on the context's stream, no host synchronisation in steady
+// state):
+//   k_fix_monomers          FixMonomerCoordinates into the working copy (a copy in the gas phase)
+//   k_grid_gas | (upload)   candidate grid at the larger cutoff
+//   k_cloud, binning        cloud points with image slots, cells
+//   k_nmers pass 0          per-query counts {listed 2B, evaluated 2B, listed 3B, evaluated 3B}
+//   k_scan_counts           offsets; totals at offsets[np]
+//   k_nmers pass 1          lists written at the offsets, bounded by the buffer capacities
+//   launch_mbpol2b/3b       variables, polynomial, chain rule over the evaluate lists; n-mer
+//                           counts read on the device
+//   k_reduce_partials       deterministic energy / virial totals
+// Capacities (list buffers, match storage) come from a synchronised first call and carry
+// headroom.  Every read of results synchronises and checks the totals against them; a call that
+// exceeded them is recomputed synchronously with grown buffers, so results are always whole.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -25,8 +34,8 @@ using namespace mbxg;
 
 namespace {
 
-// Timing events: 0 start, 1 after fix, 2 after 2B lists, 3 after 3B lists, 4..7 2B stages,
-// 8..11 3B stages, 12 end.
+// Timing events: 0 start, 1 after fix, 2 after the list count pass, 3 after the emit pass,
+// 4..7 2B stages, 8..11 3B stages, 12 end.
 constexpr int kNumEvents = 13;
 
 thread_local std::string g_err;
@@ -72,17 +81,30 @@ struct DBuf {
     }
 };
 
-// Per-order neighbour-search state.
-struct OrderState {
-    Grid grid{};
-    int ncell = 0;
+// Neighbour-search and list state (both orders share one gather).
+struct ListState {
+    Grid grid{};  // host copy of the uploaded periodic grid
+    bool grid_uploaded = false;
+    DBuf<Grid> d_grid;
     DBuf<double> cx, cy, cz;
     DBuf<int> cell, cell_count, cell_start, cell_pts;
-    DBuf<int2> counts, offsets;
-    DBuf<int> list, eval;
-    int64_t n_list = 0, n_eval = 0;
+    DBuf<int4> counts, offsets;
+    DBuf<int> list2, eval2, list3, eval3;
+    DBuf<double> gscratch;        // match storage when it does not fit in LDS
+    int4 cap = {0, 0, 0, 0};      // capacities of list2, eval2, list3, eval3 in n-mers
+    int match_cap = 256;          // power of two
+    int4 tot = {0, 0, 0, 0};      // totals of the last validated compute
+    bool built = false;           // the last compute built lists
+};
+
+// Per-order evaluation buffers.
+struct EvalState {
     DBuf<double> e_nmer, partials, ws;
 };
+
+// Largest match storage kept in LDS (80 bytes per match): 512 matches = 40 KB per block.
+constexpr int kMaxLdsMatches = 512;
+constexpr int kScratchBlocks = 2048;
 
 }  // namespace
 
@@ -98,12 +120,17 @@ struct mbxg_ctx {
     // state
     Box box{};
     double cut2 = 0.0, cut3 = 0.0;
-    DBuf<double> d_xyz, d_grad, d_totals, d_coef2, d_coef3, d_bbox;
+    DBuf<double> d_in, d_xyz, d_grad, d_totals, d_coef2, d_coef3;
     DBuf<Box> d_box;
     DBuf<int> d_overflow;
-    OrderState o2, o3;
+    ListState L;
+    EvalState ev2, ev3;
     bool have_coords = false, have_results = false;
-    int match_cap = 1024;
+    bool need_sync = true;  // next compute sizes the buffers (first call, new topology/box/cutoffs)
+    bool checked = false;   // the last compute's totals were checked against the capacities
+    bool evaluated = false; // the last compute evaluated energies (not lists only)
+    bool inputs_changed = false;  // coordinates, box, cutoffs or coefficients set since the last compute
+    int last_terms = 0, last_grads = 0;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
     hipEvent_t ev[kNumEvents] = {};
@@ -125,7 +152,7 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         return fail(MBXG_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(16) ||
-        c->d_overflow.reserve(1) || c->d_bbox.reserve(8) || c->d_box.reserve(1)) {
+        c->d_overflow.reserve(1) || c->d_box.reserve(1) || c->L.d_grid.reserve(1)) {
         delete c;
         return fail(MBXG_ENOMEM, "mbxg_create: device allocation failed");
     }
@@ -140,18 +167,23 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     if (!c) return MBXG_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (OrderState* o : {&c->o2, &c->o3}) {
-        o->cx.release();
-        o->cy.release();
-        o->cz.release();
-        o->cell.release();
-        o->cell_count.release();
-        o->cell_start.release();
-        o->cell_pts.release();
-        o->counts.release();
-        o->offsets.release();
-        o->list.release();
-        o->eval.release();
+    ListState& L = c->L;
+    L.d_grid.release();
+    L.cx.release();
+    L.cy.release();
+    L.cz.release();
+    L.cell.release();
+    L.cell_count.release();
+    L.cell_start.release();
+    L.cell_pts.release();
+    L.counts.release();
+    L.offsets.release();
+    L.list2.release();
+    L.eval2.release();
+    L.list3.release();
+    L.eval3.release();
+    L.gscratch.release();
+    for (EvalState* o : {&c->ev2, &c->ev3}) {
         o->e_nmer.release();
         o->partials.release();
         o->ws.release();
@@ -162,12 +194,12 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_tag.release();
     c->d_members.release();
     c->d_islocal.release();
+    c->d_in.release();
     c->d_xyz.release();
     c->d_grad.release();
     c->d_totals.release();
     c->d_coef2.release();
     c->d_coef3.release();
-    c->d_bbox.release();
     c->d_box.release();
     c->d_overflow.release();
     for (hipEvent_t& e : c->ev)
@@ -206,7 +238,8 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     c->np = (int)c->members.size();
     const size_t nm = nmon > 0 ? nmon : 1;
     if (c->d_kind.reserve(nm) || c->d_nat.reserve(nm) || c->d_first.reserve(nm) || c->d_tag.reserve(nm) ||
-        c->d_members.reserve(nm) || c->d_islocal.reserve(nm) || c->d_xyz.reserve(3 * (size_t)acc + 3) ||
+        c->d_members.reserve(nm) || c->d_islocal.reserve(nm) || c->d_in.reserve(3 * (size_t)acc + 3) ||
+        c->d_xyz.reserve(3 * (size_t)acc + 3) ||
         c->d_grad.reserve(3 * (size_t)acc + 3))
         return fail(MBXG_ENOMEM, "mbxg_set_topology: device allocation failed");
     if (nmon > 0) {
@@ -220,6 +253,7 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
         HIPCHK(hipMemcpy(c->d_members.p, c->members.data(), sizeof(int) * c->np, hipMemcpyHostToDevice));
     c->have_coords = false;
     c->have_results = false;
+    c->need_sync = true;
     return MBXG_OK;
 }
 
@@ -249,6 +283,9 @@ extern "C" int mbxg_set_box(mbxg_ctx* c, const double* box9) {
         c->box.periodic = 0;
         HIPCHK(hipMemcpyAsync(c->d_box.p, &c->box, sizeof(Box), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->need_sync = true;
+        c->inputs_changed = true;
+        c->L.grid_uploaded = false;
         return MBXG_OK;
     }
     Box B{};
@@ -277,6 +314,9 @@ extern "C" int mbxg_set_box(mbxg_ctx* c, const double* box9) {
     c->box = B;
     HIPCHK(hipMemcpyAsync(c->d_box.p, &c->box, sizeof(Box), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->need_sync = true;
+    c->inputs_changed = true;
+    c->L.grid_uploaded = false;
     return MBXG_OK;
 }
 
@@ -284,6 +324,9 @@ extern "C" int mbxg_set_cutoffs(mbxg_ctx* c, double c2, double c3) {
     if (!c || !(c2 >= 0.0) || !(c3 >= 0.0)) return fail(MBXG_EINVAL, "mbxg_set_cutoffs: bad arguments");
     c->cut2 = c2;
     c->cut3 = c3;
+    c->need_sync = true;
+    c->inputs_changed = true;
+    c->L.grid_uploaded = false;
     return MBXG_OK;
 }
 
@@ -308,6 +351,7 @@ extern "C" int mbxg_set_coefficients(mbxg_ctx* c, const char* model, const doubl
     if (a && n != need) return fail(MBXG_EINVAL, "mbxg_set_coefficients: %s takes %d coefficients", model, need);
     HIPCHK(hipMemcpyAsync(dst, a ? a : fit, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->inputs_changed = true;
     return MBXG_OK;
 }
 
@@ -315,8 +359,9 @@ extern "C" int mbxg_set_coords(mbxg_ctx* c, const double* xyz) {
     if (!c || (!xyz && c->natoms > 0)) return fail(MBXG_EINVAL, "mbxg_set_coords: bad arguments");
     HIPCHK(hipSetDevice(c->device));
     if (c->natoms > 0)
-        HIPCHK(hipMemcpyAsync(c->d_xyz.p, xyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_in.p, xyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice, c->stream));
     c->have_coords = true;
+    c->inputs_changed = true;
     return MBXG_OK;
 }
 
@@ -324,161 +369,239 @@ extern "C" int mbxg_set_coords_device(mbxg_ctx* c, const double* d_xyz) {
     if (!c || (!d_xyz && c->natoms > 0)) return fail(MBXG_EINVAL, "mbxg_set_coords_device: bad arguments");
     HIPCHK(hipSetDevice(c->device));
     if (c->natoms > 0)
-        HIPCHK(hipMemcpyAsync(c->d_xyz.p, d_xyz, sizeof(double) * 3 * c->natoms, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(c->d_in.p, d_xyz, sizeof(double) * 3 * c->natoms, hipMemcpyDeviceToDevice,
                               c->stream));
     c->have_coords = true;
+    c->inputs_changed = true;
     return MBXG_OK;
 }
 
 namespace {
 
-// Bounding box of the cloud points (gas phase): one workgroup.
-__global__ void __launch_bounds__(256) k_bbox(const double* xyz, const int* first_atom, const int* members, int np,
-                                              double* out) {
-    __shared__ double lo[3][256], hi[3][256];
-    const int t = threadIdx.x;
-    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-    for (int p = t; p < np; p += 256) {
-        const double* x = xyz + 3 * (size_t)first_atom[members[p]];
-        for (int c = 0; c < 3; ++c) {
-            l[c] = fmin(l[c], x[c]);
-            h[c] = fmax(h[c], x[c]);
-        }
+// Candidate grid of a periodic cell: cells in fractional coordinates, each at least `cut` wide
+// along every perpendicular direction (width = V / |b x c| etc.), so every slot within the
+// cutoff of a query lies in the query's cell or a neighbour (wrapped).  With fewer than three
+// cells along a direction the neighbour range shrinks so no cell is visited twice.
+Grid periodic_grid(const Box& box, double cut) {
+    Grid G{};
+    const double* b = box.b;
+    auto cross = [](const double* u, const double* v, double* w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double bc[3], ca[3], ab[3];
+    cross(b + 3, b + 6, bc);
+    cross(b + 6, b + 0, ca);
+    cross(b + 0, b + 3, ab);
+    const double V = std::fabs(b[0] * bc[0] + b[1] * bc[1] + b[2] * bc[2]);
+    const double w[3] = {V / std::sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]),
+                         V / std::sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]),
+                         V / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2])};
+    for (int d = 0; d < 3; ++d) {
+        int n = (int)std::floor(w[d] / cut);
+        if (n < 1) n = 1;
+        if (n > kMaxCellsPerDim) n = kMaxCellsPerDim;
+        G.n[d] = n;
+        G.lo_off[d] = n >= 3 ? -1 : 0;
+        G.hi_off[d] = n >= 3 ? 1 : n - 1;
     }
-    for (int c = 0; c < 3; ++c) {
-        lo[c][t] = l[c];
-        hi[c][t] = h[c];
-    }
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s)
-            for (int c = 0; c < 3; ++c) {
-                lo[c][t] = fmin(lo[c][t], lo[c][t + s]);
-                hi[c][t] = fmax(hi[c][t], hi[c][t + s]);
-            }
-        __syncthreads();
-    }
-    if (t == 0)
-        for (int c = 0; c < 3; ++c) {
-            out[c] = lo[c][0];
-            out[3 + c] = hi[c][0];
-        }
+    G.periodic = 1;
+    return G;
 }
 
-int setup_grid(mbxg_ctx* c, double cutoff, Grid& G) {
-    G = Grid{};
-    const double cut = cutoff * (1.0 + 1e-9) + 1e-9;
-    if (c->box.periodic) {
-        const double* b = c->box.b;
-        // perpendicular widths: V / |b x c|, V / |c x a|, V / |a x b|
-        auto cross = [](const double* u, const double* v, double* w) {
-            w[0] = u[1] * v[2] - u[2] * v[1];
-            w[1] = u[2] * v[0] - u[0] * v[2];
-            w[2] = u[0] * v[1] - u[1] * v[0];
-        };
-        double bc[3], ca[3], ab[3];
-        cross(b + 3, b + 6, bc);
-        cross(b + 6, b + 0, ca);
-        cross(b + 0, b + 3, ab);
-        const double V = std::fabs(b[0] * bc[0] + b[1] * bc[1] + b[2] * bc[2]);
-        const double w[3] = {V / std::sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]),
-                             V / std::sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]),
-                             V / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2])};
-        for (int d = 0; d < 3; ++d) {
-            int n = (int)std::floor(w[d] / cut);
-            if (n < 1) n = 1;
-            if (n > 256) n = 256;
-            G.n[d] = n;
-            G.lo_off[d] = n >= 3 ? -1 : 0;
-            G.hi_off[d] = n >= 3 ? 1 : n - 1;
-        }
-        G.periodic = 1;
-    } else {
-        if (c->np == 0) {
-            G.n[0] = G.n[1] = G.n[2] = 1;
-            return 0;
-        }
-        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, c->stream, c->d_xyz.p, c->d_first.p, c->d_members.p,
-                           c->np, c->d_bbox.p);
-        double bb[6];
-        if (hipMemcpyAsync(bb, c->d_bbox.p, sizeof(bb), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            return fail(MBXG_EHIP, "bounding box readback failed");
-        for (int d = 0; d < 3; ++d) {
-            const double ext = bb[3 + d] - bb[d];
-            int n = (int)std::floor(ext / cut);
-            if (n < 1) n = 1;
-            if (n > 256) n = 256;
-            G.n[d] = n;
-            G.lo[d] = bb[d];
-            G.inv_w[d] = ext > 0 ? n / ext : 0.0;
-            G.lo_off[d] = -1;
-            G.hi_off[d] = 1;
-        }
-        G.periodic = 0;
-    }
-    return 0;
+int pow2_at_least(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
 }
 
-// Lists of one order: cloud, binning, count, scan, emit.
-int build_order(mbxg_ctx* c, int order, double cutoff, OrderState& o, const NmerCtx& N) {
-    o.n_list = o.n_eval = 0;
-    if (c->np < order || cutoff <= 0.0) return 0;
-    if (int r = setup_grid(c, cutoff, o.grid)) return r;
-    const Grid& G = o.grid;
-    o.ncell = G.n[0] * G.n[1] * G.n[2];
-    const int nslot = c->box.periodic ? 8 : 1;
-    const size_t ns = (size_t)c->np * nslot;
-    if (o.cx.reserve(ns) || o.cy.reserve(ns) || o.cz.reserve(ns) || o.cell.reserve(c->np) ||
-        o.cell_count.reserve(o.ncell) || o.cell_start.reserve(o.ncell + 1) || o.cell_pts.reserve(c->np) ||
-        o.counts.reserve(c->np) || o.offsets.reserve(c->np + 1))
-        return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
+int with_headroom(int n) { return n + n / 4 + 64; }
+
+void launch_zero_int(int* p, size_t n, hipStream_t st) {
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_int, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, p, n);
+}
+
+void launch_zero(double* p, size_t n, hipStream_t st) {
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, p, n);
+}
+
+// Dimer and trimer lists at the larger cutoff (one gather for both orders).  With sync the
+// totals and the largest match count are read back and the buffers sized from them.
+int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double cmax, bool sync, hipEvent_t* ev) {
+    ListState& L = c->L;
     hipStream_t st = c->stream;
-    HIPCHK(hipMemsetAsync(o.cx.p, 0, ns * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(o.cy.p, 0, ns * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(o.cz.p, 0, ns * sizeof(double), st));
-    const int tb = 256;
-    hipLaunchKernelGGL(k_cloud, dim3((c->np + tb - 1) / tb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p,
-                       c->d_members.p, c->np, c->box, G, o.cx.p, o.cy.p, o.cz.p, o.cell.p);
+    const int np = c->np;
+    const int nslot = c->box.periodic ? 8 : 1;
+    const size_t ns = (size_t)np * nslot;
+    const double cut = cmax * (1.0 + 1e-9) + 1e-9;
+    int ncell = kMaxCells;
+    if (c->box.periodic) {
+        const Grid G = periodic_grid(c->box, cut);
+        if (!L.grid_uploaded || std::memcmp(&G, &L.grid, sizeof G) != 0) {
+            L.grid = G;
+            HIPCHK(hipMemcpyAsync(L.d_grid.p, &L.grid, sizeof(Grid), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            L.grid_uploaded = true;
+        }
+        ncell = G.n[0] * G.n[1] * G.n[2];
+    }
+    if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.cell.reserve(np) ||
+        L.cell_count.reserve(ncell) || L.cell_start.reserve(ncell + 1) || L.cell_pts.reserve(np) ||
+        L.counts.reserve(np) || L.offsets.reserve(np + 1))
+        return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
+    const int tb = 256, pb = (np + tb - 1) / tb;
+    if (!c->box.periodic) {
+        hipLaunchKernelGGL(k_grid_gas, dim3(1), dim3(256), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, np, cut,
+                           L.d_grid.p);
+        L.grid_uploaded = false;
+    }
+    launch_zero_int(L.cell_count.p, ncell, st);
+    hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, np, c->box,
+                       L.d_grid.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p);
+    hipLaunchKernelGGL(k_bin_count, dim3(pb), dim3(tb), 0, st, L.cell.p, np, L.cell_count.p);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.d_grid.p, L.cell_start.p);
+    hipLaunchKernelGGL(k_bin_fill, dim3(pb), dim3(tb), 0, st, L.cell.p, np, L.cell_start.p, L.cell_count.p,
+                       L.cell_pts.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemsetAsync(o.cell_count.p, 0, sizeof(int) * o.ncell, st));
-    hipLaunchKernelGGL(k_bin_count, dim3((c->np + tb - 1) / tb), dim3(tb), 0, st, o.cell.p, c->np, o.cell_count.p);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, o.cell_count.p, o.ncell, o.cell_start.p);
-    hipLaunchKernelGGL(k_bin_fill, dim3((c->np + tb - 1) / tb), dim3(tb), 0, st, o.cell.p, c->np, o.cell_start.p,
-                       o.cell_count.p, o.cell_pts.p);
-    hipLaunchKernelGGL(k_bin_sort, dim3((o.ncell + tb - 1) / tb), dim3(tb), 0, st, o.cell_start.p, o.ncell,
-                       o.cell_pts.p);
-    HIPCHK(hipGetLastError());
-    const double c2 = cutoff * cutoff;
-    auto launch = order == 2 ? launch_dimers : launch_trimers;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        HIPCHK(hipMemsetAsync(c->d_overflow.p, 0, sizeof(int), st));
-        HIPCHK(launch(o.cx.p, o.cy.p, o.cz.p, o.cell.p, o.cell_start.p, o.cell_pts.p, c->np, nslot, G, N, c2, 0,
-                      o.counts.p, nullptr, nullptr, nullptr, c->match_cap, c->d_overflow.p, st));
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, o.counts.p, c->np, o.offsets.p);
+
+    auto scratch = [&]() -> double* {
+        if (L.match_cap <= kMaxLdsMatches) return nullptr;
+        const size_t need = (size_t)kScratchBlocks * L.match_cap * (nmers_lds_bytes(1) / sizeof(double));
+        return L.gscratch.reserve(need) == hipSuccess ? L.gscratch.p : nullptr;
+    };
+    const int4 none = {0, 0, 0, 0};
+    for (int attempt = 0;; ++attempt) {
+        double* gs = scratch();
+        if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
+        launch_zero_int(c->d_overflow.p, 1, st);
+        HIPCHK(launch_nmers(L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_pts.p, np, nslot, L.d_grid.p, N,
+                            c2sq, c3sq, 0, L.counts.p, nullptr, nullptr, nullptr, nullptr, nullptr, none, L.match_cap,
+                            c->d_overflow.p, gs, kScratchBlocks, st));
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p);
         HIPCHK(hipGetLastError());
-        int2 tot;
+        if (!sync) break;
+        int4 tot;
         int ovf = 0;
-        HIPCHK(hipMemcpyAsync(&tot, o.offsets.p + c->np, sizeof(int2), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (ovf > c->match_cap) {
-            if (attempt == 0 && ovf <= 16384) {
-                c->match_cap = (ovf + 255) / 256 * 256;
-                continue;
-            }
-            return fail(MBXG_EINVAL, "neighbour search: %d matches for one monomer exceed the supported 16384",
-                        ovf);
+        const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
+        if (ovf > L.match_cap) {
+            if (attempt > 0 || (size_t)want > 2 * ns + 2)
+                return fail(MBXG_EHIP, "neighbour search: match storage did not grow (%d matches)", ovf);
+            L.match_cap = want;
+            continue;  // counts were taken over clamped matches: count again
         }
-        o.n_list = tot.x;
-        o.n_eval = tot.y;
+        L.match_cap = want;  // smaller storage, more blocks per CU; still holds every match
+        L.tot = tot;
+        const int4 cap = {with_headroom(tot.x), with_headroom(tot.y), with_headroom(tot.z), with_headroom(tot.w)};
+        if (L.list2.reserve(2 * (size_t)cap.x) || L.eval2.reserve(2 * (size_t)cap.y) ||
+            L.list3.reserve(3 * (size_t)cap.z) || L.eval3.reserve(3 * (size_t)cap.w))
+            return fail(MBXG_ENOMEM, "n-mer lists: device allocation failed (%d + %d listed)", tot.x, tot.z);
+        L.cap = cap;
+        if (L.match_cap > kMaxLdsMatches && !scratch())
+            return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
         break;
     }
-    if (o.list.reserve((size_t)order * (o.n_list + 1)) || o.eval.reserve((size_t)order * (o.n_eval + 1)))
-        return fail(MBXG_ENOMEM, "n-mer lists: device allocation failed (%lld listed)", (long long)o.n_list);
-    HIPCHK(launch(o.cx.p, o.cy.p, o.cz.p, o.cell.p, o.cell_start.p, o.cell_pts.p, c->np, nslot, G, N, c2, 1,
-                  o.counts.p, o.offsets.p, o.list.p, o.eval.p, c->match_cap, c->d_overflow.p, st));
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
+    HIPCHK(launch_nmers(L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_pts.p, np, nslot, L.d_grid.p, N, c2sq,
+                        c3sq, 1, L.counts.p, L.offsets.p, L.list2.p, L.eval2.p, L.list3.p, L.eval3.p, L.cap,
+                        L.match_cap, c->d_overflow.p, L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr,
+                        kScratchBlocks, st));
     return 0;
+}
+
+// One compute: lists, then the evaluation of the requested terms.
+int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
+    hipStream_t st = c->stream;
+    ListState& L = c->L;
+    hipEvent_t* ev = c->timing ? c->ev : nullptr;
+    if (ev)
+        for (int k = 0; k < kNumEvents; ++k) HIPCHK(hipEventRecord(ev[k], st));  // stages skipped read 0
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    if (c->nmon > 0) {
+        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p, c->d_xyz.p,
+                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
+        HIPCHK(hipGetLastError());
+    }
+    NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
+              c->box.periodic};
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    const bool want2 = (terms & MBXG_2B) && c->cut2 > 0.0;
+    const bool want3 = (terms & MBXG_3B) && c->cut3 > 0.0;
+    const double c2sq = want2 ? c->cut2 * c->cut2 : 0.0;
+    const double c3sq = want3 ? c->cut3 * c->cut3 : 0.0;
+    L.built = (want2 || want3) && c->np >= 2;
+    if (L.built) {
+        if (int r = build_lists(c, N, c2sq, c3sq, std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), sync, ev))
+            return r;
+    } else {
+        L.tot = make_int4(0, 0, 0, 0);
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    if (c->natoms > 0) launch_zero(c->d_grad.p, 3 * (size_t)c->natoms, st);
+    launch_zero(c->d_totals.p, 16, st);
+    HIPCHK(hipGetLastError());
+    c->evaluated = L.built && !(terms & MBXG_LISTS_ONLY);
+    if (c->evaluated) {
+        const int* n2 = (const int*)(L.offsets.p + c->np) + 1;  // evaluated dimers
+        const int* n3 = (const int*)(L.offsets.p + c->np) + 3;  // evaluated trimers
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 ? want2 : want3)) continue;
+            EvalState& o = k == 0 ? c->ev2 : c->ev3;
+            const int cap = k == 0 ? L.cap.y : L.cap.w;
+            if (cap <= 0) continue;
+            const size_t wsz = k == 0 ? workspace_mbpol2b(cap) : workspace_mbpol3b(cap);
+            if (o.e_nmer.reserve(cap) || o.partials.reserve((size_t)kPartials * partials_records(cap)) ||
+                o.ws.reserve(wsz))
+                return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
+            if (k == 0)
+                HIPCHK(launch_mbpol2b(L.eval2.p, n2, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p,
+                                      o.partials.p, o.ws.p, st, ev ? ev + 4 : nullptr));
+            else
+                HIPCHK(launch_mbpol3b(L.eval3.p, n3, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p,
+                                      o.partials.p, o.ws.p, st, ev ? ev + 8 : nullptr));
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, o.partials.p, k == 0 ? n2 : n3, cap,
+                               c->d_totals.p + kPartials * k);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[12], st));
+    c->have_results = true;
+    c->checked = sync || !L.built;
+    if (sync) c->need_sync = false;
+    return MBXG_OK;
+}
+
+// Wait for the last compute and make sure its lists fit their buffers; if not, compute again
+// with buffers sized from a synchronised count.
+int finish(mbxg_ctx* c) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->have_results || c->checked) return MBXG_OK;
+    ListState& L = c->L;
+    int4 tot;
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tot.x > L.cap.x || tot.y > L.cap.y || tot.z > L.cap.z || tot.w > L.cap.w || ovf > L.match_cap) {
+        c->need_sync = true;
+        if (c->inputs_changed) {
+            c->have_results = false;
+            return fail(MBXG_ESTATE,
+                        "the last compute outgrew its list buffers and its inputs have changed since; compute again");
+        }
+        if (int r = run(c, c->last_terms, c->last_grads, true)) return r;
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        L.tot = tot;
+    }
+    c->checked = true;
+    return MBXG_OK;
 }
 
 }  // namespace
@@ -486,65 +609,16 @@ int build_order(mbxg_ctx* c, int order, double cutoff, OrderState& o, const Nmer
 extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_compute: null context");
     if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_compute: coordinates not set");
-    const bool lists_only = (terms & MBXG_LISTS_ONLY) != 0;
-    if (!lists_only)
+    if (!(terms & MBXG_LISTS_ONLY))
         for (int i = 0; i < c->nmon; ++i)
             if (c->kind[i] == MBXG_CH4)
                 return fail(MBXG_EINVAL, "mbxg_compute: no GPU holder for ch4-containing n-mers yet (lists only)");
     HIPCHK(hipSetDevice(c->device));
-    hipStream_t st = c->stream;
     c->have_results = false;
-    hipEvent_t* ev = c->timing ? c->ev : nullptr;
-    if (ev)
-        for (int k = 0; k < kNumEvents; ++k) HIPCHK(hipEventRecord(ev[k], st));  // stages skipped read 0
-    if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    if (c->box.periodic && c->nmon > 0) {
-        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_xyz.p,
-                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
-        HIPCHK(hipGetLastError());
-    }
-    NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
-              c->box.periodic};
-    c->o2.n_list = c->o2.n_eval = c->o3.n_list = c->o3.n_eval = 0;
-    if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    if (terms & MBXG_2B)
-        if (int r = build_order(c, 2, c->cut2, c->o2, N)) return r;
-    if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    if (terms & MBXG_3B)
-        if (int r = build_order(c, 3, c->cut3, c->o3, N)) return r;
-    if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    if (lists_only) {
-        if (ev) HIPCHK(hipEventRecord(ev[12], st));
-        HIPCHK(hipMemsetAsync(c->d_totals.p, 0, sizeof(double) * 16, st));
-        if (c->natoms > 0) HIPCHK(hipMemsetAsync(c->d_grad.p, 0, sizeof(double) * 3 * c->natoms, st));
-        c->o2.n_eval = c->o3.n_eval = 0;
-        c->have_results = true;
-        return MBXG_OK;
-    }
-    if (c->natoms > 0) HIPCHK(hipMemsetAsync(c->d_grad.p, 0, sizeof(double) * 3 * c->natoms, st));
-    HIPCHK(hipMemsetAsync(c->d_totals.p, 0, sizeof(double) * 16, st));
-    OrderState* os[2] = {&c->o2, &c->o3};
-    for (int k = 0; k < 2; ++k) {
-        OrderState& o = *os[k];
-        const int nb = (int)((o.n_eval + 63) / 64);
-        const size_t wsz = k == 0 ? workspace_mbpol2b((int)o.n_eval) : workspace_mbpol3b((int)o.n_eval);
-        if (o.e_nmer.reserve(o.n_eval + 1) || o.partials.reserve((size_t)kPartials * (nb + 1)) ||
-            o.ws.reserve(wsz + 1))
-            return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
-        if (o.n_eval == 0) continue;
-        if (k == 0)
-            HIPCHK(launch_mbpol2b(o.eval.p, (int)o.n_eval, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                  o.partials.p, o.ws.p, st, ev ? ev + 4 : nullptr));
-        else
-            HIPCHK(launch_mbpol3b(o.eval.p, (int)o.n_eval, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                  o.partials.p, o.ws.p, st, ev ? ev + 8 : nullptr));
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, st, o.partials.p, nb,
-                           c->d_totals.p + kPartials * k);
-        HIPCHK(hipGetLastError());
-    }
-    if (ev) HIPCHK(hipEventRecord(ev[12], st));
-    c->have_results = true;
-    return MBXG_OK;
+    c->last_terms = terms;
+    c->last_grads = do_grads;
+    c->inputs_changed = false;
+    return run(c, terms, do_grads, c->need_sync);
 }
 
 extern "C" int mbxg_set_timing(mbxg_ctx* c, int on) {
@@ -559,9 +633,8 @@ extern "C" int mbxg_set_timing(mbxg_ctx* c, int on) {
 extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
     if (!c || !ms) return fail(MBXG_EINVAL, "mbxg_get_timings: bad arguments");
     if (!c->timing || !c->have_results) return fail(MBXG_ESTATE, "mbxg_get_timings: no timed compute");
-    HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    // fix, lists 2B, lists 3B, vars 2B, poly 2B, chain 2B, vars 3B, poly 3B, chain 3B, total
+    if (int r = finish(c)) return r;
+    // fix, list count pass, list emit pass, vars 2B, poly 2B, chain 2B, vars 3B, poly 3B, chain 3B, total
     const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {8, 9}, {9, 10}, {10, 11}, {0, 12}};
     for (int k = 0; k < n && k < 10; ++k) {
         float t = 0.0f;
@@ -574,7 +647,7 @@ extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
 extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* grad, double* virial) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_results: null context");
     if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_results: nothing computed");
-    HIPCHK(hipSetDevice(c->device));
+    if (int r = finish(c)) return r;
     double tot[16];
     HIPCHK(hipMemcpyAsync(tot, c->d_totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     if (grad && c->natoms > 0)
@@ -596,23 +669,27 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
 
 extern "C" int mbxg_get_counts(mbxg_ctx* c, int64_t* counts) {
     if (!c || !counts) return fail(MBXG_EINVAL, "mbxg_get_counts: bad arguments");
-    counts[0] = c->o2.n_list;
-    counts[1] = c->o2.n_eval;
-    counts[2] = c->o3.n_list;
-    counts[3] = c->o3.n_eval;
+    if (int r = finish(c)) return r;
+    const int4 t = c->have_results ? c->L.tot : make_int4(0, 0, 0, 0);
+    counts[0] = t.x;
+    counts[1] = c->evaluated ? t.y : 0;
+    counts[2] = t.z;
+    counts[3] = c->evaluated ? t.w : 0;
     return MBXG_OK;
 }
 
 extern "C" int mbxg_get_lists(mbxg_ctx* c, int32_t* dimers, int64_t cap2, int32_t* trimers, int64_t cap3) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_lists: null context");
-    HIPCHK(hipSetDevice(c->device));
-    if (dimers && c->o2.n_list > 0) {
-        const int64_t n = c->o2.n_list < cap2 ? c->o2.n_list : cap2;
-        HIPCHK(hipMemcpyAsync(dimers, c->o2.list.p, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+    if (int r = finish(c)) return r;
+    if (!c->have_results) return MBXG_OK;
+    const int4 t = c->L.tot;
+    if (dimers && t.x > 0) {
+        const int64_t n = t.x < cap2 ? t.x : cap2;
+        HIPCHK(hipMemcpyAsync(dimers, c->L.list2.p, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, c->stream));
     }
-    if (trimers && c->o3.n_list > 0) {
-        const int64_t n = c->o3.n_list < cap3 ? c->o3.n_list : cap3;
-        HIPCHK(hipMemcpyAsync(trimers, c->o3.list.p, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (trimers && t.z > 0) {
+        const int64_t n = t.z < cap3 ? t.z : cap3;
+        HIPCHK(hipMemcpyAsync(trimers, c->L.list3.p, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return MBXG_OK;
@@ -620,14 +697,15 @@ extern "C" int mbxg_get_lists(mbxg_ctx* c, int32_t* dimers, int64_t cap2, int32_
 
 extern "C" int mbxg_get_nmer_energies(mbxg_ctx* c, int order, int32_t* nmers, double* energies, int64_t cap) {
     if (!c || (order != 2 && order != 3)) return fail(MBXG_EINVAL, "mbxg_get_nmer_energies: bad arguments");
-    HIPCHK(hipSetDevice(c->device));
-    OrderState& o = order == 2 ? c->o2 : c->o3;
-    const int64_t n = o.n_eval < cap ? o.n_eval : cap;
+    if (int r = finish(c)) return r;
+    if (!c->have_results || !c->evaluated) return MBXG_OK;
+    const int64_t tot = order == 2 ? c->L.tot.y : c->L.tot.w;
+    const int* list = order == 2 ? c->L.eval2.p : c->L.eval3.p;
+    const double* en = order == 2 ? c->ev2.e_nmer.p : c->ev3.e_nmer.p;
+    const int64_t n = tot < cap ? tot : cap;
     if (n > 0) {
-        if (nmers)
-            HIPCHK(hipMemcpyAsync(nmers, o.eval.p, sizeof(int32_t) * order * n, hipMemcpyDeviceToHost, c->stream));
-        if (energies)
-            HIPCHK(hipMemcpyAsync(energies, o.e_nmer.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        if (nmers) HIPCHK(hipMemcpyAsync(nmers, list, sizeof(int32_t) * order * n, hipMemcpyDeviceToHost, c->stream));
+        if (energies) HIPCHK(hipMemcpyAsync(energies, en, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return MBXG_OK;
@@ -657,7 +735,5 @@ extern "C" int mbxg_stream(mbxg_ctx* c, void** stream) {
 
 extern "C" int mbxg_synchronize(mbxg_ctx* c) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_synchronize: null context");
-    HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return MBXG_OK;
+    return finish(c);
 }

@@ -12,6 +12,7 @@
 // Stage buffers are variable-major (entry (j, t) at j*ld + t).  Compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
+#include "scatter.hpp"
 
 
 namespace mbxg {
@@ -139,10 +140,11 @@ __device__ __forceinline__ void load_dimer(const NmerCtx& N, int m1, int m2, dou
 }  // namespace
 
 // Stage 1: variables and switch.
-__global__ void __launch_bounds__(64) k_vars_mbpol2b(const int* __restrict__ list, int n, NmerCtx N,
-                                                     double* __restrict__ xs, double* __restrict__ sw, int ld) {
+__global__ void __launch_bounds__(64) k_vars_mbpol2b(const int* __restrict__ list, const int* __restrict__ n_dev,
+                                                     int cap, NmerCtx N, double* __restrict__ xs,
+                                                     double* __restrict__ sw, int ld) {
     const int t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= n) return;
+    if (t >= min(*n_dev, cap)) return;
     double s[30], oh1a[3], oh2a[3], oh1b[3], oh2b[3];
     load_dimer(N, list[2 * t], list[2 * t + 1], s, oh1a, oh2a, oh1b, oh2b);
 #pragma unroll
@@ -156,14 +158,23 @@ __global__ void __launch_bounds__(64) k_vars_mbpol2b(const int* __restrict__ lis
     sw[t] = switch2(rOO, gsw);
 }
 
-// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.
+// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.  Blocks of 128
+// dimers (two wavefronts); one partial record per wavefront.
+constexpr int kGradBlock2 = 128;
+
 template <bool GRAD>
-__global__ void __launch_bounds__(64) k_grad_mbpol2b(const int* __restrict__ list, int n, NmerCtx N,
-                                                     const double* __restrict__ gs, const double* __restrict__ ps,
-                                                     const double* __restrict__ sws, int ld,
-                                                     double* __restrict__ grad, double* __restrict__ e_out,
-                                                     double* __restrict__ partials) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
+__global__ void __launch_bounds__(kGradBlock2) k_grad_mbpol2b(const int* __restrict__ list,
+                                                              const int* __restrict__ n_dev, int cap, NmerCtx N,
+                                                              const double* __restrict__ gs,
+                                                              const double* __restrict__ ps,
+                                                              const double* __restrict__ sws, int ld,
+                                                              double* __restrict__ grad, double* __restrict__ e_out,
+                                                              double* __restrict__ partials) {
+    __shared__ GradTable<256, 9> T;
+    const int t = blockIdx.x * kGradBlock2 + threadIdx.x;
+    const int n = min(*n_dev, cap);
+    if (GRAD) table_init(T);
+    __syncthreads();
     double e_sum = 0.0;
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t < n) {
@@ -214,56 +225,59 @@ __global__ void __launch_bounds__(64) k_grad_mbpol2b(const int* __restrict__ lis
                 for (int at = 0; at < 6; ++at) acc -= s[3 * at + va[q]] * gd[3 * at + vb[q]];
                 vir[q] = acc;
             }
-            double* g1 = grad + 3 * (size_t)N.first_atom[m1];
-            double* g2 = grad + 3 * (size_t)N.first_atom[m2];
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                unsafeAtomicAdd(g1 + j, gd[j]);
-                unsafeAtomicAdd(g2 + j, gd[9 + j]);
-            }
+            table_add(T, m1, gd, grad, 3 * (size_t)N.first_atom[m1]);
+            table_add(T, m2, gd + 9, grad, 3 * (size_t)N.first_atom[m2]);
         }
         const double e = sw * P;
         e_out[t] = e;
         e_sum = e;
     }
+    if (GRAD) {
+        __syncthreads();
+        table_flush(T, N.first_atom, grad);
+    }
     e_sum = wave_sum(e_sum);
-    double* out = partials + (size_t)blockIdx.x * kPartials;
-    if (threadIdx.x == 0) out[0] = e_sum;
+    const int rec = blockIdx.x * (kGradBlock2 / 64) + threadIdx.x / 64;
+    double* out = partials + (size_t)rec * kPartials;
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (lead) out[0] = e_sum;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const double vq = wave_sum(vir[q]);
-        if (threadIdx.x == 0) out[1 + q] = vq;
+        if (lead) out[1 + q] = vq;
     }
 }
 
-size_t workspace_mbpol2b(int n) {
-    const size_t ld = ((size_t)n + 63) / 64 * 64;
+size_t workspace_mbpol2b(int cap) {
+    const size_t ld = ((size_t)cap + 63) / 64 * 64;
     return (2 * 31 + 2) * ld;
 }
 
-hipError_t launch_mbpol2b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
-    if (n <= 0) return hipSuccess;
-    const int nb = (n + 63) / 64;
+hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
+                          hipEvent_t* ev) {
+    if (cap <= 0) return hipSuccess;
+    const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
     double* xs = ws;
     double* gs = xs + (size_t)31 * ld;
     double* ps = gs + (size_t)31 * ld;
     double* sw = ps + ld;
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_vars_mbpol2b, dim3(nb), dim3(64), 0, st, list, n, N, xs, sw, ld);
+    hipLaunchKernelGGL(k_vars_mbpol2b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol2b(xs, n, ld, coef, do_grads, gs, ps, st);
+    e = launch_poly_mbpol2b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, st);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
+    const int ngb = (cap + kGradBlock2 - 1) / kGradBlock2;
     if (do_grads)
-        hipLaunchKernelGGL(k_grad_mbpol2b<true>, dim3(nb), dim3(64), 0, st, list, n, N, gs, ps, sw, ld, grad, e_out,
-                           partials);
+        hipLaunchKernelGGL(k_grad_mbpol2b<true>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps, sw,
+                           ld, grad, e_out, partials);
     else
-        hipLaunchKernelGGL(k_grad_mbpol2b<false>, dim3(nb), dim3(64), 0, st, list, n, N, gs, ps, sw, ld, grad,
-                           e_out, partials);
+        hipLaunchKernelGGL(k_grad_mbpol2b<false>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps,
+                           sw, ld, grad, e_out, partials);
     e = hipGetLastError();
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;

@@ -12,6 +12,7 @@
 // Stage buffers are variable-major (entry (j, t) at j*ld + t).  Compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
+#include "scatter.hpp"
 
 namespace mbxg {
 
@@ -95,10 +96,11 @@ __device__ __forceinline__ double switches(const double* w, double* dab, double*
 }  // namespace
 
 // Stage 1: variables and switch product.
-__global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
-                                                     double* __restrict__ xs, double* __restrict__ sw, int ld) {
+__global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ list, const int* __restrict__ n_dev,
+                                                     int cap, NmerCtx N, double* __restrict__ xs,
+                                                     double* __restrict__ sw, int ld) {
     const int t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= n) return;
+    if (t >= min(*n_dev, cap)) return;
     double w[27];
     load_trimer(N, list[3 * t], list[3 * t + 1], list[3 * t + 2], w);
 #pragma unroll
@@ -114,14 +116,24 @@ __global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ lis
     sw[t] = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
 }
 
-// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.
+// Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.  Blocks of 128
+// trimers (two wavefronts); one partial record per wavefront.
+constexpr int kGradBlock3 = 128;
+
 template <bool GRAD>
-__global__ void __launch_bounds__(64) k_grad_mbpol3b(const int* __restrict__ list, int n, NmerCtx N,
-                                                     const double* __restrict__ xs, const double* __restrict__ gs,
-                                                     const double* __restrict__ ps, const double* __restrict__ sw,
-                                                     int ld, double* __restrict__ grad, double* __restrict__ e_out,
-                                                     double* __restrict__ partials) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
+__global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restrict__ list,
+                                                              const int* __restrict__ n_dev, int cap, NmerCtx N,
+                                                              const double* __restrict__ xs,
+                                                              const double* __restrict__ gs,
+                                                              const double* __restrict__ ps,
+                                                              const double* __restrict__ sw, int ld,
+                                                              double* __restrict__ grad, double* __restrict__ e_out,
+                                                              double* __restrict__ partials) {
+    __shared__ GradTable<256, 9> T;
+    const int t = blockIdx.x * kGradBlock3 + threadIdx.x;
+    const int n = min(*n_dev, cap);
+    if (GRAD) table_init(T);
+    __syncthreads();
     double e_sum = 0.0;
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (t < n) {
@@ -172,57 +184,61 @@ __global__ void __launch_bounds__(64) k_grad_mbpol3b(const int* __restrict__ lis
                 for (int at = 0; at < 9; ++at) acc -= w[3 * at + va[q]] * gd[3 * at + vb[q]];
                 vir[q] = acc;
             }
-            const int mon[3] = {m1, m2, m3};
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                double* gp = grad + 3 * (size_t)N.first_atom[mon[m]];
-#pragma unroll
-                for (int j = 0; j < 9; ++j) unsafeAtomicAdd(gp + j, gd[9 * m + j]);
-            }
+            table_add(T, m1, gd, grad, 3 * (size_t)N.first_atom[m1]);
+            table_add(T, m2, gd + 9, grad, 3 * (size_t)N.first_atom[m2]);
+            table_add(T, m3, gd + 18, grad, 3 * (size_t)N.first_atom[m3]);
         }
         const double e = s * P;
         e_out[t] = e;
         e_sum = e;
     }
-    // per-wave partials (one wave per block)
+    if (GRAD) {
+        __syncthreads();
+        table_flush(T, N.first_atom, grad);
+    }
+    // one partial record per wavefront
     e_sum = wave_sum(e_sum);
-    double* out = partials + (size_t)blockIdx.x * kPartials;
-    if (threadIdx.x == 0) out[0] = e_sum;
+    const int rec = blockIdx.x * (kGradBlock3 / 64) + threadIdx.x / 64;
+    double* out = partials + (size_t)rec * kPartials;
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (lead) out[0] = e_sum;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const double v = wave_sum(vir[q]);
-        if (threadIdx.x == 0) out[1 + q] = v;
+        if (lead) out[1 + q] = v;
     }
 }
 
-size_t workspace_mbpol3b(int n) {
-    const size_t ld = ((size_t)n + 63) / 64 * 64;
+size_t workspace_mbpol3b(int cap) {
+    const size_t ld = ((size_t)cap + 63) / 64 * 64;
     return (2 * 36 + 2) * ld;
 }
 
-hipError_t launch_mbpol3b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
-    if (n <= 0) return hipSuccess;
-    const int nb = (n + 63) / 64;
+hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
+                          hipEvent_t* ev) {
+    if (cap <= 0) return hipSuccess;
+    const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
     double* xs = ws;
     double* gs = xs + (size_t)36 * ld;
     double* ps = gs + (size_t)36 * ld;
     double* sw = ps + ld;
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n, N, xs, sw, ld);
+    hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol3b(xs, n, ld, coef, do_grads, gs, ps, st);
+    e = launch_poly_mbpol3b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, st);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
+    const int ngb = (cap + kGradBlock3 - 1) / kGradBlock3;
     if (do_grads)
-        hipLaunchKernelGGL(k_grad_mbpol3b<true>, dim3(nb), dim3(64), 0, st, list, n, N, xs, gs, ps, sw, ld, grad,
-                           e_out, partials);
+        hipLaunchKernelGGL(k_grad_mbpol3b<true>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs, ps,
+                           sw, ld, grad, e_out, partials);
     else
-        hipLaunchKernelGGL(k_grad_mbpol3b<false>, dim3(nb), dim3(64), 0, st, list, n, N, xs, gs, ps, sw, ld, grad,
-                           e_out, partials);
+        hipLaunchKernelGGL(k_grad_mbpol3b<false>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs,
+                           ps, sw, ld, grad, e_out, partials);
     e = hipGetLastError();
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;

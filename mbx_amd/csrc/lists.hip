@@ -6,21 +6,27 @@
 // kdtree/nanoflann.hpp:166-171):
 //   * every local monomer contributes its first atom; under PBC the point is re-wrapped and
 //     receives 7 image slots (slot s of point p lives at index p + s*np);
-//   * a dimer (i, j) is listed from query i when some slot of j lies strictly inside the
+//   * a dimer (i, j) is listed from query i when some slot of j lies strictly inside the 2B
 //     cutoff of i's point and tag[i] < tag[j];
-//   * a trimer (i, j, k) is listed from query i when slots of j and k are both inside the
+//   * a trimer (i, j, k) is listed from query i when slots of j and k both lie inside the 3B
 //     cutoff, j != k != i, and either |slot_j - slot_k|^2 > c^2 or tag[i] is the smallest.
 // Every distance is computed from the same cloud coordinates with the same operation order,
-// so membership decisions are bit-identical.  The KD-tree is replaced by a grid of cells in
-// fractional coordinates (one cell >= cutoff wide perpendicular to each face); the grid only
-// proposes candidates, every candidate slot is tested exactly.
+// so membership decisions are bit-identical.  The KD-tree is replaced by a grid of cells
+// (fractional coordinates under PBC, Cartesian over the bounding box in the gas phase), at
+// least one cutoff wide; the grid only proposes candidates, every candidate slot is tested.
 //
-// The list is emitted in two passes (count, then write at scanned offsets) so its order is a
-// deterministic function of the input.  While emitting, each n-mer is also imaged
-// (GetCloseDimerImage, sys_tools.cpp:322-365) and tested against the polynomial's own
-// evaluation cutoff (x2b-v9x.cpp:1589-1600 for MB-pol 2B; x3b-v2x.cpp:1495-1515 for MB-pol
-// 3B); n-mers that pass are also written to a compacted "evaluate" list that feeds the
-// polynomial kernels, so no wavefront idles on a skipped n-mer.
+// One wavefront per query monomer gathers its matches at the larger of the two cutoffs once,
+// sorts them by slot index in LDS (so the emission order does not depend on how points landed
+// in cells), and precomputes per match the slot coordinates, the partner's first atom imaged
+// next to the query's (GetCloseDimerImage, sys_tools.cpp:322-365) and its tag / kind.  Dimers
+// and trimers are then decided from LDS only.  While emitting, each n-mer is also tested
+// against its polynomial's own evaluation cutoff (x2b-v9x.cpp:1589-1600, x3b-v2x.cpp:1495-1515);
+// those that pass go to a compacted "evaluate" list as well, so no lane of the polynomial
+// kernels idles on a skipped n-mer.
+//
+// Lists are emitted in two passes (count, exclusive scan, write at offsets); in steady state
+// the host does not wait between them: writes are bounded by the buffer capacities and the
+// host checks the totals when it next synchronises (api.hip).
 //
 // This file is compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
@@ -30,40 +36,96 @@ namespace mbxg {
 
 // ---------------------------------------------------------------------------------------------
 // FixMonomerCoordinates (sys_tools.cpp:287-319): wrap each monomer's first atom into the cell
-// and bring its other atoms next to it.  One thread per monomer.
-__global__ void k_fix_monomers(double* __restrict__ xyz, const int* __restrict__ first_atom,
-                               const int* __restrict__ nat, int nmon, Box B) {
+// and bring its other atoms next to it.  Reads the caller's coordinates, writes the working
+// copy (a plain copy in the gas phase).  One thread per monomer.
+__global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict__ out,
+                               const int* __restrict__ first_atom, const int* __restrict__ nat, int nmon, Box B) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nmon) return;
-    double* x0 = xyz + 3 * (size_t)first_atom[i];
+    const double* x = in + 3 * (size_t)first_atom[i];
+    double* y = out + 3 * (size_t)first_atom[i];
+    const int na = nat[i];
+    if (!B.periodic) {
+        for (int k = 0; k < 3 * na; ++k) y[k] = x[k];
+        return;
+    }
     double u0, v0, w0;
-    to_frac(B, x0[0], x0[1], x0[2], u0, v0, w0);
+    to_frac(B, x[0], x[1], x[2], u0, v0, w0);
     u0 -= floor(u0 + 0.5);
     v0 -= floor(v0 + 0.5);
     w0 -= floor(w0 + 0.5);
-    to_cart(B, u0, v0, w0, x0[0], x0[1], x0[2]);
-    for (int a = 1; a < nat[i]; ++a) {
-        double* xa = x0 + 3 * a;
+    to_cart(B, u0, v0, w0, y[0], y[1], y[2]);
+    for (int a = 1; a < na; ++a) {
         double u, v, w;
-        to_frac(B, xa[0], xa[1], xa[2], u, v, w);
+        to_frac(B, x[3 * a], x[3 * a + 1], x[3 * a + 2], u, v, w);
         u -= floor(u - u0 + 0.5);
         v -= floor(v - v0 + 0.5);
         w -= floor(w - w0 + 0.5);
-        to_cart(B, u, v, w, xa[0], xa[1], xa[2]);
+        to_cart(B, u, v, w, y[3 * a], y[3 * a + 1], y[3 * a + 2]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gas-phase grid: bounding box of the points and the cell layout (one workgroup).  Writes the
+// Grid the other kernels read, at most kMaxCellsPerDim cells per dimension.
+__global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz, const int* __restrict__ first_atom,
+                                                  const int* __restrict__ members, int np, double cut,
+                                                  Grid* __restrict__ G) {
+    __shared__ double lo[3][256], hi[3][256];
+    const int t = threadIdx.x;
+    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+    for (int p = t; p < np; p += 256) {
+        const double* x = xyz + 3 * (size_t)first_atom[members[p]];
+        for (int c = 0; c < 3; ++c) {
+            l[c] = fmin(l[c], x[c]);
+            h[c] = fmax(h[c], x[c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        lo[c][t] = l[c];
+        hi[c][t] = h[c];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int c = 0; c < 3; ++c) {
+                lo[c][t] = fmin(lo[c][t], lo[c][t + s]);
+                hi[c][t] = fmax(hi[c][t], hi[c][t + s]);
+            }
+        __syncthreads();
+    }
+    if (t == 0) {
+        Grid g{};
+        for (int c = 0; c < 3; ++c) {
+            const double ext = np > 0 ? hi[c][0] - lo[c][0] : 0.0;
+            int n = (int)floor(ext / cut);
+            n = n < 1 ? 1 : (n > kMaxCellsPerDim ? kMaxCellsPerDim : n);
+            g.n[c] = n;
+            g.lo[c] = np > 0 ? lo[c][0] : 0.0;
+            g.inv_w[c] = ext > 0 ? n / ext : 0.0;
+            g.lo_off[c] = n >= 2 ? -1 : 0;
+            g.hi_off[c] = n >= 2 ? 1 : 0;
+        }
+        g.periodic = 0;
+        *G = g;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // XyzToCloud (kdtree_utils.h:196-279): point p = first atom of cloud member p.  Slot 0 is the
 // re-wrapped position; slots 1..7 are the images selected from its fractional coordinates.
-// Also computes the point's grid cell.  One thread per point.
+// Every one of the 7 slots is written: each fractional coordinate admits at least one of the
+// two image directions.  Also computes the point's grid cell.  One thread per point.
 __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
-                        const int* __restrict__ members, int np, Box B, Grid G, double* __restrict__ cx,
-                        double* __restrict__ cy, double* __restrict__ cz, int* __restrict__ cell) {
+                        const int* __restrict__ members, int np, Box B, const Grid* __restrict__ Gp,
+                        double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
+                        int* __restrict__ cell) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
+    const Grid G = *Gp;
     const double* x0 = xyz + 3 * (size_t)first_atom[members[p]];
     double qx = x0[0], qy = x0[1], qz = x0[2];
+    int ca, cb, cc;
     if (B.periodic) {
         double u, v, w;
         to_frac(B, qx, qy, qz, u, v, w);
@@ -111,44 +173,40 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
                 }
             }
         }
-        cx[p] = qx;
-        cy[p] = qy;
-        cz[p] = qz;
         // grid cell from the fractional position in [-0.5, 0.5)
         to_frac(B, qx, qy, qz, u, v, w);
-        int ca = (int)floor((u + 0.5) * G.n[0]);
-        int cb = (int)floor((v + 0.5) * G.n[1]);
-        int cc = (int)floor((w + 0.5) * G.n[2]);
-        ca = ca < 0 ? 0 : (ca >= G.n[0] ? G.n[0] - 1 : ca);
-        cb = cb < 0 ? 0 : (cb >= G.n[1] ? G.n[1] - 1 : cb);
-        cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
-        cell[p] = (cc * G.n[1] + cb) * G.n[0] + ca;
+        ca = (int)floor((u + 0.5) * G.n[0]);
+        cb = (int)floor((v + 0.5) * G.n[1]);
+        cc = (int)floor((w + 0.5) * G.n[2]);
     } else {
-        cx[p] = qx;
-        cy[p] = qy;
-        cz[p] = qz;
-        int ca = (int)floor((qx - G.lo[0]) * G.inv_w[0]);
-        int cb = (int)floor((qy - G.lo[1]) * G.inv_w[1]);
-        int cc = (int)floor((qz - G.lo[2]) * G.inv_w[2]);
-        ca = ca < 0 ? 0 : (ca >= G.n[0] ? G.n[0] - 1 : ca);
-        cb = cb < 0 ? 0 : (cb >= G.n[1] ? G.n[1] - 1 : cb);
-        cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
-        cell[p] = (cc * G.n[1] + cb) * G.n[0] + ca;
+        ca = (int)floor((qx - G.lo[0]) * G.inv_w[0]);
+        cb = (int)floor((qy - G.lo[1]) * G.inv_w[1]);
+        cc = (int)floor((qz - G.lo[2]) * G.inv_w[2]);
     }
+    cx[p] = qx;
+    cy[p] = qy;
+    cz[p] = qz;
+    ca = ca < 0 ? 0 : (ca >= G.n[0] ? G.n[0] - 1 : ca);
+    cb = cb < 0 ? 0 : (cb >= G.n[1] ? G.n[1] - 1 : cb);
+    cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
+    cell[p] = (cc * G.n[1] + cb) * G.n[0] + ca;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Counting sort of points into cells: count, scan, fill, then sort each cell's members by point
-// index so the cell contents (and hence the emission order) are deterministic.
+// Binning of points into cells: count, scan, fill.  The order of points inside a cell is left
+// to the atomics; the n-mer kernel sorts each query's matches, which makes the output order
+// independent of it.  cell_count must be zero on entry (k_zero_int).
 __global__ void k_bin_count(const int* __restrict__ cell, int np, int* __restrict__ cell_count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < np) atomicAdd(&cell_count[cell[p]], 1);
 }
 
-// Single-workgroup exclusive scan of the cell counts; cell_start has ncell+1 entries.
-__global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, int ncell,
+// Single-workgroup exclusive scan of the cell counts (ncell from the grid); cell_start gets
+// ncell+1 entries and cell_count is reset to zero for use as the fill cursor.
+__global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, const Grid* __restrict__ Gp,
                                                      int* __restrict__ cell_start) {
     __shared__ int sh[1024];
+    const int ncell = Gp->n[0] * Gp->n[1] * Gp->n[2];
     const int t = threadIdx.x;
     const int per = (ncell + 1023) / 1024;
     const int beg = t * per, end = min(ncell, beg + per);
@@ -166,7 +224,7 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_coun
     for (int c = beg; c < end; ++c) {
         cell_start[c] = o;
         o += cell_count[c];
-        cell_count[c] = 0;  // reused as the fill cursor
+        cell_count[c] = 0;
     }
     if (t == 1023) cell_start[ncell] = sh[t];
 }
@@ -179,43 +237,94 @@ __global__ void k_bin_fill(const int* __restrict__ cell, int np, const int* __re
     cell_pts[cell_start[c] + atomicAdd(&cursor[c], 1)] = p;
 }
 
-__global__ void k_bin_sort(const int* __restrict__ cell_start, int ncell, int* __restrict__ cell_pts) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int beg = cell_start[c], end = cell_start[c + 1];
-    for (int a = beg + 1; a < end; ++a) {
-        const int v = cell_pts[a];
-        int b = a - 1;
-        while (b >= beg && cell_pts[b] > v) {
-            cell_pts[b + 1] = cell_pts[b];
-            --b;
+// ---------------------------------------------------------------------------------------------
+// The n-mer kernel: one wavefront (block) per query point.
+
+namespace {
+
+struct Lds {
+    int* match;  // slot index, sorted
+    int* mon;    // partner monomer
+    int* tag;    // partner tag
+    int* kind;   // partner kind
+    int* flags;  // bit0: inside 2B cutoff, bit1: inside 3B cutoff
+    int* m3;     // indices of the matches inside the 3B cutoff, in match order
+    double* sx;  // slot coordinates
+    double* sy;
+    double* sz;
+    double* ix;  // partner's first atom imaged next to the query's
+    double* iy;
+    double* iz;
+    double* r;  // |query first atom - imaged partner first atom|
+};
+
+// 7 doubles + 6 ints per match.
+constexpr int kMatchDoubles = 10;
+
+__device__ __forceinline__ Lds carve(double* d, int cap) {
+    Lds L;
+    L.sx = d;
+    L.sy = d + cap;
+    L.sz = d + 2 * cap;
+    L.ix = d + 3 * cap;
+    L.iy = d + 4 * cap;
+    L.iz = d + 5 * cap;
+    L.r = d + 6 * cap;
+    int* i = (int*)(d + 7 * cap);
+    L.match = i;
+    L.mon = i + cap;
+    L.tag = i + 2 * cap;
+    L.kind = i + 3 * cap;
+    L.flags = i + 4 * cap;
+    L.m3 = i + 5 * cap;
+    return L;
+}
+
+// Bitonic sort of a[0..n) (n a power of two) by one wavefront.
+__device__ void sort_ints(int* a, int n) {
+    const int lane = threadIdx.x;
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < n; i += 64) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int ai = a[i], aj = a[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((ai > aj) == up) {
+                        a[i] = aj;
+                        a[ixj] = ai;
+                    }
+                }
+            }
+            __syncthreads();
         }
-        cell_pts[b + 1] = v;
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Neighbour search for one query per wavefront: gathers the matching cloud slots into LDS.
-// Returns the number of matches (clamped at cap; *overflow set if exceeded).
+}  // namespace
 
-struct QueryCtx {
-    const double* cx;
-    const double* cy;
-    const double* cz;
-    const int* cell;
-    const int* cell_start;
-    const int* cell_pts;
-    int np;
-    int nslot;  // 8 under PBC, else 1
-};
+// One query: gather, sort, per-match data, dimers, trimers.  L is the block's match storage
+// (LDS, or a global scratch slice when the matches do not fit in LDS).
+__device__ void nmers_query(int q, const double* __restrict__ cx, const double* __restrict__ cy,
+                            const double* __restrict__ cz, const int* __restrict__ cell,
+                            const int* __restrict__ cell_start, const int* __restrict__ cell_pts, int np, int nslot,
+                            const Grid& G, const NmerCtx& N, double c2sq, double c3sq, int pass,
+                            int4* __restrict__ counts, const int4* __restrict__ offsets, int* __restrict__ list2,
+                            int* __restrict__ eval2, int* __restrict__ list3, int* __restrict__ eval3, int4 cap,
+                            int match_cap, int* __restrict__ overflow, const Lds& L) {
+    const int lane = threadIdx.x;
+    const int mi = N.members[q];
+    if (!N.islocal[mi]) {
+        if (pass == 0 && lane == 0) counts[q] = make_int4(0, 0, 0, 0);
+        return;
+    }
+    const double cmsq = c2sq > c3sq ? c2sq : c3sq;
 
-__device__ int gather_matches(const QueryCtx& Q, const Grid& G, int i, double c2, int* s_match, int cap,
-                              int* overflow) {
-    const int lane = threadIdx.x & 63;
-    const double qx = Q.cx[i], qy = Q.cy[i], qz = Q.cz[i];
-    const int ci = Q.cell[i];
+    // ---- gather: every candidate slot strictly inside the larger cutoff
+    const double qx = cx[q], qy = cy[q], qz = cz[q];
+    const int ci = cell[q];
     const int ia = ci % G.n[0], ib = (ci / G.n[0]) % G.n[1], ic = ci / (G.n[0] * G.n[1]);
-    int nmatch = 0;
+    int nm = 0;
     for (int oc = G.lo_off[2]; oc <= G.hi_off[2]; ++oc) {
         int cc = ic + oc;
         if (G.periodic) {
@@ -238,270 +347,283 @@ __device__ int gather_matches(const QueryCtx& Q, const Grid& G, int i, double c2
                     continue;
                 }
                 const int c = (cc * G.n[1] + cb) * G.n[0] + ca;
-                const int beg = Q.cell_start[c], end = Q.cell_start[c + 1];
-                const int total = (end - beg) * Q.nslot;
+                const int beg = cell_start[c], end = cell_start[c + 1];
+                const int total = (end - beg) * nslot;
                 for (int base = 0; base < total; base += 64) {
                     const int t = base + lane;
                     int s = -1;
                     if (t < total) {
-                        const int p = Q.cell_pts[beg + t / Q.nslot];
-                        const int slot = t % Q.nslot;
-                        const size_t si = (size_t)p + (size_t)slot * Q.np;
-                        if (dist2(qx, qy, qz, Q.cx[si], Q.cy[si], Q.cz[si]) < c2) s = (int)si;
+                        const int p = cell_pts[beg + t / nslot];
+                        const size_t si = (size_t)p + (size_t)(t % nslot) * np;
+                        if (dist2(qx, qy, qz, cx[si], cy[si], cz[si]) < cmsq) s = (int)si;
                     }
                     const unsigned long long mask = __ballot(s >= 0);
-                    const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                    if (s >= 0) {
-                        const int w = nmatch + pos;
-                        if (w < cap) s_match[w] = s;
-                    }
-                    nmatch += __popcll(mask);
+                    const int pos = nm + __popcll(mask & ((1ull << lane) - 1ull));
+                    if (s >= 0 && pos < match_cap) L.match[pos] = s;
+                    nm += __popcll(mask);
                 }
             }
         }
     }
-    if (nmatch > cap) {
-        if (lane == 0) atomicMax(overflow, nmatch);
-        nmatch = cap;
-    }
-    __builtin_amdgcn_wave_barrier();
-    return nmatch;
-}
+    if (lane == 0) atomicMax(overflow, nm);
+    if (nm > match_cap) nm = match_cap;
+    // ---- sort the matches by slot index (pad to a power of two; match_cap is one)
+    int pw = 1;
+    while (pw < nm) pw <<= 1;
+    for (int a = nm + lane; a < pw; a += 64) L.match[a] = 0x7fffffff;
+    __syncthreads();
+    sort_ints(L.match, pw);
 
-// ---------------------------------------------------------------------------------------------
-// Evaluation rules (which listed n-mers reach a polynomial).
-
-// 2B: MB-pol h2o-h2o evaluates iff r_OO <= r2f after imaging (x2b-v9x.cpp:1589-1600).
-// Pairs without a GPU holder contribute nothing here (energy2b.cpp:372-374 returns 0 for an
-// unknown pair); the host refuses to evaluate systems whose pairs would need one.
-__device__ __forceinline__ bool eval2_rule(const NmerCtx& N, int m1, int m2) {
-    const int k1 = N.kind[m1], k2 = N.kind[m2];
-    if (!(k1 == MON_H2O && k2 == MON_H2O)) return false;
-    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1];
-    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2];
-    double bx = b[0], by = b[1], bz = b[2];
-    if (N.periodic) {
-        const Box& B = *N.box;
-        double dx, dy, dz;
-        image_shift(B, a[0], a[1], a[2], bx, by, bz, dx, dy, dz);
-        image_apply(B, dx, dy, dz, bx, by, bz);
+    // ---- per-match data
+    const double* oi = N.xyz + 3 * (size_t)N.first_atom[mi];
+    const double oix = oi[0], oiy = oi[1], oiz = oi[2];
+    const int ti = N.tag[mi], ki = N.kind[mi];
+    for (int a = lane; a < nm; a += 64) {
+        const int s = L.match[a];
+        const int mj = N.members[s % np];
+        const double x = cx[s], y = cy[s], z = cz[s];
+        const double d = dist2(qx, qy, qz, x, y, z);
+        L.sx[a] = x;
+        L.sy[a] = y;
+        L.sz[a] = z;
+        L.mon[a] = mj;
+        L.tag[a] = N.tag[mj];
+        L.kind[a] = N.kind[mj];
+        L.flags[a] = (d < c2sq ? 1 : 0) | (d < c3sq ? 2 : 0);
+        const double* oj = N.xyz + 3 * (size_t)N.first_atom[mj];
+        double jx = oj[0], jy = oj[1], jz = oj[2];
+        if (N.periodic) {
+            const Box& B = *N.box;
+            double dx, dy, dz;
+            image_shift(B, oix, oiy, oiz, jx, jy, jz, dx, dy, dz);
+            image_apply(B, dx, dy, dz, jx, jy, jz);
+        }
+        L.ix[a] = jx;
+        L.iy[a] = jy;
+        L.iz[a] = jz;
+        const double d0 = oix - jx, d1 = oiy - jy, d2 = oiz - jz;
+        double rr = 0.0;
+        rr += d0 * d0;
+        rr += d1 * d1;
+        rr += d2 * d2;
+        L.r[a] = sqrt(rr);
     }
-    const double d0 = a[0] - bx, d1 = a[1] - by, d2 = a[2] - bz;
-    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    return !(r > p2::r2f);
-}
-
-// 3B: MB-pol h2o x3 is skipped when two O-O distances exceed r3f (x3b-v2x.cpp:1495-1515).
-__device__ __forceinline__ bool eval3_rule(const NmerCtx& N, int m1, int m2, int m3) {
-    const int k1 = N.kind[m1], k2 = N.kind[m2], k3 = N.kind[m3];
-    if (!(k1 == MON_H2O && k2 == MON_H2O && k3 == MON_H2O)) return false;
-    const double* a = N.xyz + 3 * (size_t)N.first_atom[m1];
-    const double* b = N.xyz + 3 * (size_t)N.first_atom[m2];
-    const double* c = N.xyz + 3 * (size_t)N.first_atom[m3];
-    double bx = b[0], by = b[1], bz = b[2], cx = c[0], cy = c[1], cz = c[2];
-    if (N.periodic) {
-        const Box& B = *N.box;
-        double dx, dy, dz;
-        image_shift(B, a[0], a[1], a[2], bx, by, bz, dx, dy, dz);
-        image_apply(B, dx, dy, dz, bx, by, bz);
-        image_shift(B, a[0], a[1], a[2], cx, cy, cz, dx, dy, dz);
-        image_apply(B, dx, dy, dz, cx, cy, cz);
-    }
-    const double ab0 = a[0] - bx, ab1 = a[1] - by, ab2 = a[2] - bz;
-    const double ac0 = a[0] - cx, ac1 = a[1] - cy, ac2 = a[2] - cz;
-    const double bc0 = bx - cx, bc1 = by - cy, bc2 = bz - cz;
-    double sab = 0.0, sac = 0.0, sbc = 0.0;
-    sab += ab0 * ab0;
-    sab += ab1 * ab1;
-    sab += ab2 * ab2;
-    sac += ac0 * ac0;
-    sac += ac1 * ac1;
-    sac += ac2 * ac2;
-    sbc += bc0 * bc0;
-    sbc += bc1 * bc1;
-    sbc += bc2 * bc2;
-    const double rab = sqrt(sab), rac = sqrt(sac), rbc = sqrt(sbc);
-    const double f = p3::r3f;
-    const bool skip = (rab > f && rac > f) || (rab > f && rbc > f) || (rac > f && rbc > f);
-    return !skip;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Dimers.  pass 0: counts[q] = {listed, evaluated}; pass 1: writes at offsets[q].
-__global__ void __launch_bounds__(64) k_dimers(QueryCtx Q, Grid G, NmerCtx N, double c2, int pass,
-                                               int2* __restrict__ counts, const int2* __restrict__ offsets,
-                                               int* __restrict__ out_list, int* __restrict__ out_eval, int cap,
-                                               int* overflow) {
-    extern __shared__ int s_match[];
-    const int q = blockIdx.x;  // cloud point index of the query
-    if (q >= Q.np) return;
-    const int lane = threadIdx.x;
-    const int mi = N.members[q];
-    if (!N.islocal[mi]) {
-        if (pass == 0 && lane == 0) counts[q] = make_int2(0, 0);
-        return;
-    }
-    const int nm = gather_matches(Q, G, q, c2, s_match, cap, overflow);
-    const int tq = N.tag[mi];
-    int base_l = pass ? offsets[q].x : 0;
-    int base_e = pass ? offsets[q].y : 0;
-    int nl = 0, ne = 0;
+    __syncthreads();
+    // compact the 3B matches (order preserved)
+    int n3 = 0;
     for (int b = 0; b < nm; b += 64) {
-        const int t = b + lane;
-        bool listed = false, ev = false;
-        int mj = 0;
-        if (t < nm) {
-            const int pj = s_match[t] % Q.np;
-            mj = N.members[pj];
-            listed = tq < N.tag[mj];
-            if (listed) ev = eval2_rule(N, mi, mj);
-        }
-        const unsigned long long ml = __ballot(listed), me = __ballot(ev);
-        const unsigned long long below = (1ull << lane) - 1ull;
-        if (pass) {
-            if (listed) {
-                const int w = base_l + nl + __popcll(ml & below);
-                out_list[2 * w] = mi;
-                out_list[2 * w + 1] = mj;
-            }
-            if (ev) {
-                const int w = base_e + ne + __popcll(me & below);
-                out_eval[2 * w] = mi;
-                out_eval[2 * w + 1] = mj;
-            }
-        }
-        nl += __popcll(ml);
-        ne += __popcll(me);
+        const int a = b + lane;
+        const bool in3 = a < nm && (L.flags[a] & 2);
+        const unsigned long long m = __ballot(in3);
+        if (in3) L.m3[n3 + __popcll(m & ((1ull << lane) - 1ull))] = a;
+        n3 += __popcll(m);
     }
-    if (pass == 0 && lane == 0) counts[q] = make_int2(nl, ne);
+    __syncthreads();
+
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int4 base = pass ? offsets[q] : make_int4(0, 0, 0, 0);
+    int nl2 = 0, ne2 = 0, nl3 = 0, ne3 = 0;
+
+    // ---- dimers (AddClusters sys_tools.cpp:558-567; evaluation rule x2b-v9x.cpp:1589-1600)
+    if (c2sq > 0.0) {
+        for (int b = 0; b < nm; b += 64) {
+            const int a = b + lane;
+            bool listed = false, ev = false;
+            int mj = 0;
+            if (a < nm && (L.flags[a] & 1)) {
+                mj = L.mon[a];
+                listed = ti < L.tag[a];
+                ev = listed && ki == MON_H2O && L.kind[a] == MON_H2O && !(L.r[a] > p2::r2f);
+            }
+            const unsigned long long ml = __ballot(listed), me = __ballot(ev);
+            if (pass) {
+                if (listed) {
+                    const int w = base.x + nl2 + __popcll(ml & below);
+                    if (w < cap.x) {
+                        list2[2 * w] = mi;
+                        list2[2 * w + 1] = mj;
+                    }
+                }
+                if (ev) {
+                    const int w = base.y + ne2 + __popcll(me & below);
+                    if (w < cap.y) {
+                        eval2[2 * w] = mi;
+                        eval2[2 * w + 1] = mj;
+                    }
+                }
+            }
+            nl2 += __popcll(ml);
+            ne2 += __popcll(me);
+        }
+    }
+
+    // ---- trimers (AddClusters sys_tools.cpp:570-592; evaluation rule x3b-v2x.cpp:1495-1515)
+    if (c3sq > 0.0) {
+        const int total = n3 * n3;
+        for (int b = 0; b < total; b += 64) {
+            const int t = b + lane;
+            bool listed = false, ev = false;
+            int mj = 0, mk = 0;
+            if (t < total) {
+                const int u = t / n3, v = t % n3;
+                if (u < v) {
+                    const int ja = L.m3[u], ka = L.m3[v];
+                    mj = L.mon[ja];
+                    mk = L.mon[ka];
+                    if (mj != mi && mk != mi && mj != mk) {
+                        const double djk = dist2(L.sx[ja], L.sy[ja], L.sz[ja], L.sx[ka], L.sy[ka], L.sz[ka]);
+                        listed = djk > c3sq || (ti < L.tag[ja] && ti < L.tag[ka]);
+                        if (listed && ki == MON_H2O && L.kind[ja] == MON_H2O && L.kind[ka] == MON_H2O) {
+                            const double e0 = L.ix[ja] - L.ix[ka], e1 = L.iy[ja] - L.iy[ka],
+                                         e2 = L.iz[ja] - L.iz[ka];
+                            double rr = 0.0;
+                            rr += e0 * e0;
+                            rr += e1 * e1;
+                            rr += e2 * e2;
+                            const double rab = L.r[ja], rac = L.r[ka], rbc = sqrt(rr);
+                            const double f = p3::r3f;
+                            ev = !((rab > f && rac > f) || (rab > f && rbc > f) || (rac > f && rbc > f));
+                        }
+                    }
+                }
+            }
+            const unsigned long long ml = __ballot(listed), me = __ballot(ev);
+            if (pass) {
+                if (listed) {
+                    const int w = base.z + nl3 + __popcll(ml & below);
+                    if (w < cap.z) {
+                        list3[3 * w] = mi;
+                        list3[3 * w + 1] = mj;
+                        list3[3 * w + 2] = mk;
+                    }
+                }
+                if (ev) {
+                    const int w = base.w + ne3 + __popcll(me & below);
+                    if (w < cap.w) {
+                        eval3[3 * w] = mi;
+                        eval3[3 * w + 1] = mj;
+                        eval3[3 * w + 2] = mk;
+                    }
+                }
+            }
+            nl3 += __popcll(ml);
+            ne3 += __popcll(me);
+        }
+    }
+    if (pass == 0 && lane == 0) counts[q] = make_int4(nl2, ne2, nl3, ne3);
 }
 
-// Trimers.  Pairs (a < b) of matches are spread over the lanes.
-__global__ void __launch_bounds__(64) k_trimers(QueryCtx Q, Grid G, NmerCtx N, double c2, int pass,
-                                                int2* __restrict__ counts, const int2* __restrict__ offsets,
-                                                int* __restrict__ out_list, int* __restrict__ out_eval, int cap,
-                                                int* overflow) {
-    extern __shared__ int s_match[];
-    const int q = blockIdx.x;
-    if (q >= Q.np) return;
-    const int lane = threadIdx.x;
-    const int mi = N.members[q];
-    if (!N.islocal[mi]) {
-        if (pass == 0 && lane == 0) counts[q] = make_int2(0, 0);
-        return;
+// pass 0: counts[q] = {listed 2B, evaluated 2B, listed 3B, evaluated 3B};
+// pass 1: writes at offsets[q], bounded by cap (capacities in the same order).
+// overflow[0] receives the largest match count of any query (the host compares it with
+// match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
+// the dynamic LDS, otherwise block b uses gscratch + b * match_cap * 10 doubles.
+__global__ void __launch_bounds__(64) k_nmers(const double* __restrict__ cx, const double* __restrict__ cy,
+                                              const double* __restrict__ cz, const int* __restrict__ cell,
+                                              const int* __restrict__ cell_start, const int* __restrict__ cell_pts,
+                                              int np, int nslot, const Grid* __restrict__ Gp, NmerCtx N, double c2sq,
+                                              double c3sq, int pass, int4* __restrict__ counts,
+                                              const int4* __restrict__ offsets, int* __restrict__ list2,
+                                              int* __restrict__ eval2, int* __restrict__ list3,
+                                              int* __restrict__ eval3, int4 cap, int match_cap,
+                                              int* __restrict__ overflow, double* __restrict__ gscratch) {
+    extern __shared__ double smem[];
+    const Grid G = *Gp;
+    double* store = gscratch ? gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles : smem;
+    const Lds L = carve(store, match_cap);
+    for (int q = blockIdx.x; q < np; q += gridDim.x) {
+        nmers_query(q, cx, cy, cz, cell, cell_start, cell_pts, np, nslot, G, N, c2sq, c3sq, pass, counts, offsets,
+                    list2, eval2, list3, eval3, cap, match_cap, overflow, L);
+        __syncthreads();
     }
-    const int nm = gather_matches(Q, G, q, c2, s_match, cap, overflow);
-    const int tq = N.tag[mi];
-    const long npair = (long)nm * (nm - 1) / 2;
-    int base_l = pass ? offsets[q].x : 0;
-    int base_e = pass ? offsets[q].y : 0;
-    int nl = 0, ne = 0;
-    for (long b = 0; b < npair; b += 64) {
-        const long t = b + lane;
-        bool listed = false, ev = false;
-        int mj = 0, mk = 0;
-        if (t < npair) {
-            // invert t -> (a, c) with a < c, enumerating row-major over a
-            // t = a*nm - a*(a+1)/2 + (c - a - 1)
-            int a = (int)((2.0 * nm - 1.0 - sqrt((2.0 * nm - 1.0) * (2.0 * nm - 1.0) - 8.0 * (double)t)) * 0.5);
-            if (a < 0) a = 0;
-            while (a > 0 && (long)a * nm - (long)a * (a + 1) / 2 > t) --a;
-            while ((long)(a + 1) * nm - (long)(a + 1) * (a + 2) / 2 <= t) ++a;
-            const int c = (int)(t - ((long)a * nm - (long)a * (a + 1) / 2)) + a + 1;
-            const int sa = s_match[a], sc = s_match[c];
-            const int pj = sa % Q.np, pk = sc % Q.np;
-            if (pj != q && pk != q && pj != pk) {
-                mj = N.members[pj];
-                mk = N.members[pk];
-                const double djk = dist2(Q.cx[sa], Q.cy[sa], Q.cz[sa], Q.cx[sc], Q.cy[sc], Q.cz[sc]);
-                listed = djk > c2 || (tq < N.tag[mj] && tq < N.tag[mk]);
-                if (listed) ev = eval3_rule(N, mi, mj, mk);
-            }
-        }
-        const unsigned long long ml = __ballot(listed), me = __ballot(ev);
-        const unsigned long long below = (1ull << lane) - 1ull;
-        if (pass) {
-            if (listed) {
-                const int w = base_l + nl + __popcll(ml & below);
-                out_list[3 * w] = mi;
-                out_list[3 * w + 1] = mj;
-                out_list[3 * w + 2] = mk;
-            }
-            if (ev) {
-                const int w = base_e + ne + __popcll(me & below);
-                out_eval[3 * w] = mi;
-                out_eval[3 * w + 1] = mj;
-                out_eval[3 * w + 2] = mk;
-            }
-        }
-        nl += __popcll(ml);
-        ne += __popcll(me);
-    }
-    if (pass == 0 && lane == 0) counts[q] = make_int2(nl, ne);
 }
 
 // Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).
-__global__ void __launch_bounds__(1024) k_scan_counts(const int2* __restrict__ counts, int n,
-                                                      int2* __restrict__ offsets) {
-    __shared__ int sx[1024], sy[1024];
+__global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ counts, int n,
+                                                      int4* __restrict__ offsets) {
+    __shared__ int4 sh[1024];
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int beg = t * per, end = min(n, beg + per);
-    int ax = 0, ay = 0;
+    int4 acc = make_int4(0, 0, 0, 0);
     for (int i = beg; i < end; ++i) {
-        ax += counts[i].x;
-        ay += counts[i].y;
+        const int4 c = counts[i];
+        acc.x += c.x;
+        acc.y += c.y;
+        acc.z += c.z;
+        acc.w += c.w;
     }
-    sx[t] = ax;
-    sy[t] = ay;
+    sh[t] = acc;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int vx = t >= off ? sx[t - off] : 0;
-        int vy = t >= off ? sy[t - off] : 0;
+        const int4 v = t >= off ? sh[t - off] : make_int4(0, 0, 0, 0);
         __syncthreads();
-        sx[t] += vx;
-        sy[t] += vy;
+        sh[t].x += v.x;
+        sh[t].y += v.y;
+        sh[t].z += v.z;
+        sh[t].w += v.w;
         __syncthreads();
     }
-    int ox = sx[t] - ax, oy = sy[t] - ay;
+    int4 o = make_int4(sh[t].x - acc.x, sh[t].y - acc.y, sh[t].z - acc.z, sh[t].w - acc.w);
     for (int i = beg; i < end; ++i) {
-        offsets[i] = make_int2(ox, oy);
-        ox += counts[i].x;
-        oy += counts[i].y;
+        offsets[i] = o;
+        const int4 c = counts[i];
+        o.x += c.x;
+        o.y += c.y;
+        o.z += c.z;
+        o.w += c.w;
     }
-    if (t == 1023) offsets[n] = make_int2(sx[t], sy[t]);
+    if (t == 1023) offsets[n] = sh[t];
 }
 
-// Deterministic sum of per-block partial records (kPartials doubles each): one wavefront,
-// lane-strided accumulation then a fixed butterfly.
-__global__ void __launch_bounds__(64) k_reduce_partials(const double* __restrict__ partials, int nblocks,
-                                                        double* __restrict__ out) {
-    const int lane = threadIdx.x;
-    for (int q = 0; q < kPartials; ++q) {
-        double acc = 0.0;
-        for (int b = lane; b < nblocks; b += 64) acc += partials[(size_t)b * kPartials + q];
-        acc = wave_sum(acc);
-        if (lane == 0) out[q] = acc;
+// Zero fills (graph-capturable replacements for memset nodes).
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+__global__ void k_zero_int(int* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Deterministic sum of per-wave partial records (kPartials doubles each) into out[kPartials]:
+// one workgroup, fixed strided accumulation and a fixed tree.  The record count comes from the
+// evaluated n-mer count on the device (ceil(n / 64), n clamped to the capacity).
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partials,
+                                                         const int* __restrict__ n_dev, int n_cap,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[kPartials][256];
+    const int t = threadIdx.x;
+    int n = *n_dev;
+    n = n < n_cap ? n : n_cap;
+    const int nb = (n + 63) / 64;
+    double acc[kPartials];
+    for (int q = 0; q < kPartials; ++q) acc[q] = 0.0;
+    for (int b = t; b < nb; b += 256)
+        for (int q = 0; q < kPartials; ++q) acc[q] += partials[(size_t)b * kPartials + q];
+    for (int q = 0; q < kPartials; ++q) sh[q][t] = acc[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < kPartials; ++q) sh[q][t] += sh[q][t + s];
+        __syncthreads();
     }
+    if (t < kPartials) out[t] = sh[t][0];
 }
 
-hipError_t launch_dimers(const double* cx, const double* cy, const double* cz, const int* cell,
-                         const int* cell_start, const int* cell_pts, int np, int nslot, const Grid& G,
-                         const NmerCtx& N, double c2, int pass, int2* counts, const int2* offsets, int* out_list,
-                         int* out_eval, int cap, int* overflow, hipStream_t st) {
-    QueryCtx Q{cx, cy, cz, cell, cell_start, cell_pts, np, nslot};
-    hipLaunchKernelGGL(k_dimers, dim3(np), dim3(64), cap * sizeof(int), st, Q, G, N, c2, pass, counts, offsets,
-                       out_list, out_eval, cap, overflow);
-    return hipGetLastError();
-}
+size_t nmers_lds_bytes(int match_cap) { return (size_t)match_cap * kMatchDoubles * sizeof(double); }
 
-hipError_t launch_trimers(const double* cx, const double* cy, const double* cz, const int* cell,
-                          const int* cell_start, const int* cell_pts, int np, int nslot, const Grid& G,
-                          const NmerCtx& N, double c2, int pass, int2* counts, const int2* offsets, int* out_list,
-                          int* out_eval, int cap, int* overflow, hipStream_t st) {
-    QueryCtx Q{cx, cy, cz, cell, cell_start, cell_pts, np, nslot};
-    hipLaunchKernelGGL(k_trimers, dim3(np), dim3(64), cap * sizeof(int), st, Q, G, N, c2, pass, counts, offsets,
-                       out_list, out_eval, cap, overflow);
+hipError_t launch_nmers(const double* cx, const double* cy, const double* cz, const int* cell, const int* cell_start,
+                        const int* cell_pts, int np, int nslot, const Grid* G, const NmerCtx& N, double c2sq,
+                        double c3sq, int pass, int4* counts, const int4* offsets, int* list2, int* eval2,
+                        int* list3, int* eval3, int4 cap, int match_cap, int* overflow, double* gscratch,
+                        int nblocks, hipStream_t st) {
+    if (np <= 0) return hipSuccess;
+    const int nb = gscratch ? (nblocks < np ? nblocks : np) : np;
+    const size_t lds = gscratch ? 0 : nmers_lds_bytes(match_cap);
+    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, cx, cy, cz, cell, cell_start, cell_pts, np, nslot, G, N,
+                       c2sq, c3sq, pass, counts, offsets, list2, eval2, list3, eval3, cap, match_cap, overflow,
+                       gscratch);
     return hipGetLastError();
 }
 

@@ -32,47 +32,60 @@ struct NmerCtx {
     int periodic;
 };
 
-struct QueryCtx;
+// Cell grids: at most kMaxCellsPerDim cells per dimension (the gas-phase grid is sized on the
+// device; the cell buffers are allocated for the maximum).
+constexpr int kMaxCellsPerDim = 64;
+constexpr int kMaxCells = kMaxCellsPerDim * kMaxCellsPerDim * kMaxCellsPerDim;
 
-__global__ void k_fix_monomers(double* xyz, const int* first_atom, const int* nat, int nmon, Box B);
-__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, int np, Box B, Grid G,
+__global__ void k_fix_monomers(const double* in, double* out, const int* first_atom, const int* nat, int nmon, Box B);
+__global__ void k_grid_gas(const double* xyz, const int* first_atom, const int* members, int np, double cut, Grid* G);
+__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, int np, Box B, const Grid* G,
                         double* cx, double* cy, double* cz, int* cell);
 __global__ void k_bin_count(const int* cell, int np, int* cell_count);
-__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
+__global__ void k_scan_cells(int* cell_count, const Grid* G, int* cell_start);
 __global__ void k_bin_fill(const int* cell, int np, const int* cell_start, int* cursor, int* cell_pts);
-__global__ void k_bin_sort(const int* cell_start, int ncell, int* cell_pts);
-__global__ void k_scan_counts(const int2* counts, int n, int2* offsets);
+__global__ void k_scan_counts(const int4* counts, int n, int4* offsets);
+__global__ void k_zero(double* p, size_t n);
+__global__ void k_zero_int(int* p, size_t n);
 
-// Launch helpers (defined in lists.hip, which owns QueryCtx).
-hipError_t launch_dimers(const double* cx, const double* cy, const double* cz, const int* cell,
-                         const int* cell_start, const int* cell_pts, int np, int nslot, const Grid& G,
-                         const NmerCtx& N, double c2, int pass, int2* counts, const int2* offsets, int* out_list,
-                         int* out_eval, int cap, int* overflow, hipStream_t st);
-hipError_t launch_trimers(const double* cx, const double* cy, const double* cz, const int* cell,
-                          const int* cell_start, const int* cell_pts, int np, int nslot, const Grid& G,
-                          const NmerCtx& N, double c2, int pass, int2* counts, const int2* offsets, int* out_list,
-                          int* out_eval, int cap, int* overflow, hipStream_t st);
+// The n-mer kernel (lists.hip): one wavefront per query point; pass 0 counts, pass 1 writes.
+// match_cap (a power of two) bounds the matches one query may hold.  With gscratch == nullptr
+// they live in LDS (nmers_lds_bytes(match_cap) per block, one block per query); otherwise
+// nblocks blocks stride over the queries, each with nmers_lds_bytes(match_cap) bytes of
+// gscratch.  overflow[0] receives the largest match count seen (must be zeroed before pass 0).
+size_t nmers_lds_bytes(int match_cap);
+hipError_t launch_nmers(const double* cx, const double* cy, const double* cz, const int* cell, const int* cell_start,
+                        const int* cell_pts, int np, int nslot, const Grid* G, const NmerCtx& N, double c2sq,
+                        double c3sq, int pass, int4* counts, const int4* offsets, int* list2, int* eval2,
+                        int* list3, int* eval3, int4 cap, int match_cap, int* overflow, double* gscratch,
+                        int nblocks, hipStream_t st);
 
 // Evaluation of the compacted "evaluate" lists (eval_mbpol2b.hip, eval_mbpol3b.hip): variable
 // stage, polynomial stage (poly_mbpol2b.hip, poly_mbpol3b.hip), chain-rule stage.  One lane per
-// n-mer, one wavefront per block.  list holds the evaluated n-mers (monomer indices), n of them;
-// e_out[t] receives n-mer t's energy; partials[block*kPartials + ...] its block sums; ws is a
-// workspace of workspace_mbpolXb(n) doubles.
-size_t workspace_mbpol2b(int n);
-size_t workspace_mbpol3b(int n);
-// ev (nullable) receives 4 events: before the variable stage, before the polynomial, before the
-// chain-rule stage, after it.
-hipError_t launch_mbpol2b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
-hipError_t launch_mbpol3b(const int* list, int n, const NmerCtx& N, const double* coef, int do_grads, double* grad,
-                          double* e_out, double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
-// Polynomial stage alone: xs/gs variable-major with leading dimension ld; ps values.
-hipError_t launch_poly_mbpol2b(const double* xs, int n, int ld, const double* coef, int do_grads, double* gs,
-                               double* ps, hipStream_t st);
-hipError_t launch_poly_mbpol3b(const double* xs, int n, int ld, const double* coef, int do_grads, double* gs,
-                               double* ps, hipStream_t st);
+// n-mer.  The evaluated count n is read on the device from n_dev (clamped to cap): kernels are
+// launched for cap n-mers and lanes past n retire at once, so the host never waits for the
+// list builder.  e_out[t] receives n-mer t's energy; partials one kPartials record per 64
+// n-mers (partials_records(cap) records); ws is a workspace of workspace_mbpolXb(cap)
+// doubles.  ev (nullable) receives 4 events: before the variable stage, before the
+// polynomial, before the chain-rule stage, after it.
+inline size_t partials_records(int cap) { return ((size_t)cap + 127) / 128 * 2; }
+size_t workspace_mbpol2b(int cap);
+size_t workspace_mbpol3b(int cap);
+hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
+                          hipEvent_t* ev);
+hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
+                          hipEvent_t* ev);
+// Polynomial stage alone: xs/gs variable-major with leading dimension ld (= cap rounded up
+// to 64); ps values; n read from n_dev as above.
+hipError_t launch_poly_mbpol2b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
+                               double* gs, double* ps, hipStream_t st);
+hipError_t launch_poly_mbpol3b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
+                               double* gs, double* ps, hipStream_t st);
 
-// Sum nblocks partial records into out[kPartials] in a fixed order (deterministic totals).
-__global__ void k_reduce_partials(const double* partials, int nblocks, double* out);
+// Sum the partial records of min(*n_dev, n_cap) n-mers into out[kPartials] in a fixed order
+// (deterministic totals).
+__global__ void k_reduce_partials(const double* partials, const int* n_dev, int n_cap, double* out);
 
 }  // namespace mbxg

@@ -22,9 +22,10 @@ typedef double (*poly_e_fn)(mbxg_cptr, const double*);
 // Energy and gradient.  Two waves per SIMD: the polynomial needs ~220 VGPRs.
 template <int NV, poly_eg_fn FN>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-    k_poly_eg(const double* __restrict__ xs, int n, int ld, uint64_t coef_addr, double* __restrict__ gs,
-              double* __restrict__ ps) {
+    k_poly_eg(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld, uint64_t coef_addr,
+              double* __restrict__ gs, double* __restrict__ ps) {
     const int t = blockIdx.x * 64 + threadIdx.x;
+    const int n = min(*n_dev, cap);
     if (t >= n) return;
     const mbxg_cptr C = (mbxg_cptr)coef_addr;
     double x[NV], g[NV];
@@ -38,9 +39,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 
 // Energy only.
 template <int NV, poly_e_fn FN>
-__global__ void __launch_bounds__(64) k_poly_e(const double* __restrict__ xs, int n, int ld, uint64_t coef_addr,
-                                               double* __restrict__ ps) {
+__global__ void __launch_bounds__(64) k_poly_e(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap,
+                                               int ld, uint64_t coef_addr, double* __restrict__ ps) {
     const int t = blockIdx.x * 64 + threadIdx.x;
+    const int n = min(*n_dev, cap);
     if (t >= n) return;
     const mbxg_cptr C = (mbxg_cptr)coef_addr;
     double x[NV];

@@ -49,7 +49,7 @@ PROTOTYPES = {
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
 }
 
-TIMING_STAGES = ("fix", "lists_2b", "lists_3b", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
+TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
                  "chain_3b", "total")
 
 

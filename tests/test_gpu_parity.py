@@ -156,3 +156,59 @@ def test_small_and_empty():
     e2, e3, g, _ = e.results()
     assert e2 == 0.0 and e3 == 0.0 and not g.any()
     assert list(e.counts()) == [0, 0, 0, 0]
+
+
+def _gas(nmon, edge, lattice, seed):
+    xyz, _ = configs.water_box(nmon, edge, lattice, seed=seed)
+    return xyz
+
+
+def _check_against_oracle(e, xyz, box, c2, c3):
+    e2, e3, g, _ = e.results()
+    r = oracle.water_system(xyz, box, c2, c3)
+    c = e.counts()
+    assert (c[0], c[1], c[2], c[3]) == (r["n2"], r["n2_eval"], r["n3"], r["n3_eval"])
+    assert e2 == pytest.approx(r["e2b"], rel=E_REL)
+    assert e3 == pytest.approx(r["e3b"], rel=E_REL)
+    assert np.abs(g - r["grad"]).max() <= G_ABS
+
+
+def test_lists_outgrow_buffers():
+    """A second compute with many more n-mers than the first sized the buffers for: the getter
+    detects it and recomputes, and the results are those of the dense system."""
+    sparse = _gas(216, 36.0, (6, 6, 6), seed=5)  # ~6 A apart
+    dense = _gas(216, 18.0, (6, 6, 6), seed=6)   # ~3 A apart
+    print("regrowth", flush=True)
+    e = engine.water_engine(216, None, 9.0, 6.0)
+    e.set_coords(sparse)
+    e.compute()
+    _check_against_oracle(e, sparse, None, 9.0, 6.0)
+    e.set_coords(dense)
+    e.compute()  # enqueued with the sparse system's capacities
+    _check_against_oracle(e, dense, None, 9.0, 6.0)
+    # outgrowing again with the inputs changed before the read: refused, then fine on recompute
+    e2 = engine.water_engine(216, None, 9.0, 6.0)
+    e2.set_coords(sparse)
+    e2.compute()
+    e2.synchronize()
+    e2.set_coords(dense)
+    e2.compute()
+    e2.set_coords(dense)
+    with pytest.raises(engine.MbxgError):
+        e2.results()
+    e2.compute()
+    _check_against_oracle(e2, dense, None, 9.0, 6.0)
+
+
+def test_large_cutoff_gas_cluster():
+    """Every monomer within the 2B cutoff of every other (the reference's gas-phase default of
+    100 A): more matches per query than LDS holds, so the builder uses its global match store."""
+    xyz = _gas(640, 28.0, (9, 9, 8), seed=11)
+    print("large cutoff", flush=True)
+    e = _run(xyz, None, 100.0, 4.5)
+    d, t = e.lists()
+    first = oracle.fix_first(xyz, None)
+    assert np.array_equal(sorted_tuples(d), sorted_tuples(oracle.clusters(2, 100.0, first, None)))
+    assert np.array_equal(sorted_tuples(t), sorted_tuples(oracle.clusters(3, 4.5, first, None)))
+    assert len(d) == 640 * 639 // 2
+    _check_against_oracle(e, xyz, None, 100.0, 4.5)
