@@ -74,6 +74,67 @@ def cpu_baseline(xyz, box, c2, c3, seconds):
                        "batch-8 dispatch)" % (reps, t_total, cores)), r
 
 
+def batched_leg(steps, warmup):
+    """512 gas-phase clusters (16..512 waters): one batched compute over all of them against one
+    plain compute per cluster (each in its own warmed-up context), coordinates resident."""
+    import numpy as np
+    import torch
+
+    from mbx_amd import configs, engine
+
+    cfg = configs.C3
+    parts, sizes = configs.cluster_batch(cfg["nconf"], cfg["nmin"], cfg["nmax"], cfg["seed"])
+    c2, c3 = cfg["cutoff2b"], cfg["cutoff3b"]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    big = engine.water_engine(int(off[-1]), None, c2, c3)
+    big.set_batch(off)
+    big.set_coords(np.concatenate(parts))
+    small = []
+    for x in parts:
+        e = engine.water_engine(len(x) // 9, None, c2, c3)
+        e.set_coords(x)
+        small.append(e)
+
+    def run_batched():
+        big.compute()
+
+    def run_plain():
+        for e in small:
+            e.compute()
+
+    def timed(fn, sync):
+        for _ in range(warmup):
+            fn()
+        sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        sync()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    big.compute()
+    schedule = big.autotune(reps=2)
+    t_b = timed(run_batched, big.synchronize)
+    t_p = timed(run_plain, lambda: [e.synchronize() for e in small])
+    c = big.counts()
+    n_listed = int(c[0] + c[2])
+    assert n_listed == sum(int(e.counts()[0] + e.counts()[2]) for e in small)
+    e2, e3, _ = big.batch_results()
+    dev = max(abs(e2[k] - small[k].results(grad=False)[0]) / max(abs(e2[k]), 1e-300) for k in range(len(small)))
+    for e in small:
+        e.close()
+    big.close()
+    return {"workload": cfg["name"], "configs": int(cfg["nconf"]), "waters": int(off[-1]),
+            "cutoff2b_A": c2, "cutoff3b_A": c3, "listed_nmers": n_listed,
+            "evaluated_dimers": int(c[1]), "evaluated_trimers": int(c[3]),
+            "ms_batched": t_b * 1e3, "ms_plain": t_p * 1e3,
+            "batched_nmers_per_s": n_listed / t_b, "plain_nmers_per_s": n_listed / t_p,
+            "speedup": t_p / t_b, "max_rel_dev_e2b_vs_plain": dev, "schedule": schedule,
+            "plain": "one warmed-up context per cluster, one compute each, no host wait until the end"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +142,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; --gpus %d is not supported (multi-GPU sharding is "
@@ -89,6 +153,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "copies": 5}))
+        return
 
     from mbx_amd import configs, engine
 
@@ -105,19 +178,24 @@ def main():
     for _ in range(args.warmup):
         step()
     eng.synchronize()
-    eng.set_timing(True)
-    poly_ms, stage = [], {}
+    schedule = eng.autotune(reps=3)  # polynomial-stage schedule for this workload
+    # throughput: K steps back to back, no host wait in between
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # stage breakdown: K more steps with HIP events around every stage
+    eng.set_timing(True)
+    poly_ms, stage = [], {}
     for _ in range(args.steps):
         step()
         tm = eng.timings()
         poly_ms.append(tm["poly_3b"])
         for k, v in tm.items():
             stage[k] = stage.get(k, 0.0) + v
-    eng.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     eng.set_timing(False)
 
     counts = eng.counts()
@@ -147,13 +225,16 @@ def main():
         "config": {"workload": cfg["name"], "waters": cfg["nmon"], "box_A": cfg["edge"], "cutoff2b_A": c2,
                    "cutoff3b_A": c3, "listed_dimers": int(counts[0]), "evaluated_dimers": int(counts[1]),
                    "listed_trimers": int(counts[2]), "evaluated_trimers": int(counts[3]),
-                   "stage_ms": {k: v / args.steps for k, v in stage.items()}, "e2b": e2, "e3b": e3},
+                   "stage_ms": {k: v / args.steps for k, v in stage.items()}, "e2b": e2, "e3b": e3,
+                   "schedule": schedule},
         "roofline": {"bound": "fp64-valu", "kernel": "k_poly_eg<36> (MB-pol 3B polynomial E+G)",
                      "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK, "traffic": traffic,
                      "flops_per_unit": f3, "units_per_launch": int(counts[3]), "launch_ms": t_poly * 1e3},
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        out["config"]["batched"] = batched_leg(args.steps, args.warmup)
     if not args.no_cpu:
         cb, r = cpu_baseline(xyz, box, c2, c3, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -58,6 +58,15 @@ int mbxg_set_box(mbxg_ctx* ctx, const double* box9);
 /* 2B / 3B cutoffs (System::Set2bCutoff / Set3bCutoff, system.h:826,834).  0 disables a term. */
 int mbxg_set_cutoffs(mbxg_ctx* ctx, double cutoff2b, double cutoff3b);
 
+/* Batches: split the monomers (in topology order) into nconf independent configurations,
+ * configuration c holding monomers [mon_offsets[c], mon_offsets[c+1]) (mon_offsets[0] = 0,
+ * mon_offsets[nconf] = nmon).  Lists never cross configurations; one compute evaluates all of
+ * them in the same launches.  Configurations share the box (or are all gas phase).  nconf = 1
+ * (mon_offsets may be NULL) is the single-system default that mbxg_set_topology restores.
+ * Replaces the reference's per-configuration System calls (system.cpp:1995-2138) for many
+ * configurations or path-integral beads at once. */
+int mbxg_set_batch(mbxg_ctx* ctx, int nconf, const int* mon_offsets);
+
 /* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163).
  * a == NULL restores the fitted values. */
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
@@ -80,6 +89,10 @@ int mbxg_compute(mbxg_ctx* ctx, int terms, int do_grads);
  * order; may be NULL), virial (9; may be NULL; the reference's ordering with [3]=[1],
  * [6]=[2], [7]=[5]). */
 int mbxg_get_results(mbxg_ctx* ctx, double* e2b, double* e3b, double* grad, double* virial);
+
+/* Per-configuration results of the last compute: e2b[nconf], e3b[nconf], virial[9*nconf]
+ * (each may be NULL).  The gradient of every atom is in mbxg_get_results' array. */
+int mbxg_get_batch_results(mbxg_ctx* ctx, double* e2b, double* e3b, double* virial);
 
 /* counts[0..3] = listed dimers, evaluated dimers, listed trimers, evaluated trimers. */
 int mbxg_get_counts(mbxg_ctx* ctx, int64_t* counts);
@@ -107,6 +120,16 @@ int mbxg_stream(mbxg_ctx* ctx, void** stream);
  * cloud, binning, counts, scan), list emit pass, 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total. */
 int mbxg_set_timing(mbxg_ctx* ctx, int on);
 int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
+
+/* Schedule parameters.  "split_2b" / "split_3b": wavefronts (1, 2, 4 or 8) that share each
+ * block of 64 dimers / trimers in the polynomial stage, each evaluating 1/split of the
+ * polynomial (defaults 4 and 2).  mbxg_autotune times every split of both orders on the
+ * current inputs (the polynomial stage, `reps` timed repetitions after one warm-up), keeps the
+ * fastest and recomputes once with it.  No counterpart in the reference (its kernels have no
+ * launch shape). */
+int mbxg_set_tuning(mbxg_ctx* ctx, const char* key, int value);
+int mbxg_get_tuning(mbxg_ctx* ctx, const char* key, int* value);
+int mbxg_autotune(mbxg_ctx* ctx, int terms, int do_grads, int reps);
 
 /* Block until the context's stream is idle and check that the last compute's n-mer lists fitted
  * their buffers (recomputing it with larger buffers if not).  Call before reading the device

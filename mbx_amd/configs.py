@@ -73,3 +73,44 @@ C2 = dict(name="water_2048_box39.4", nmon=2048, edge=39.4, lattice=(14, 14, 12),
 def build(cfg):
     xyz, box = water_box(cfg["nmon"], cfg["edge"], cfg["lattice"], seed=cfg["seed"])
     return xyz, box, cfg["cutoff2b"], cfg["cutoff3b"]
+
+
+# Gas-phase clusters for batches: oxygen atoms by random sequential addition (O-O >= 2.6 A)
+# inside a sphere sized for 1 g/cm^3 (0.03343 waters per A^3), random orientations.
+WATER_DENSITY = 0.03343
+
+
+def water_cluster(rng, n, dmin=2.6):
+    """(9n,) coordinates of an n-water gas-phase cluster."""
+    radius = (3.0 * n / (4.0 * np.pi * WATER_DENSITY)) ** (1.0 / 3.0)
+    pts = np.zeros((n, 3))
+    k = 0
+    d2 = dmin * dmin
+    while k < n:
+        cand = rng.uniform(-radius, radius, size=(256, 3))
+        cand = cand[np.einsum("ij,ij->i", cand, cand) <= radius * radius]
+        # clear of every point placed before this batch (vectorised), then of those accepted in it
+        if k > 0:
+            dd = ((cand[:, None, :] - pts[None, :k, :]) ** 2).sum(-1)
+            cand = cand[dd.min(axis=1) >= d2]
+        k0 = k
+        for c in cand:
+            if k == n:
+                break
+            if k == k0 or np.min(((pts[k0:k] - c) ** 2).sum(-1)) >= d2:
+                pts[k] = c
+                k += 1
+    return place_waters(rng, pts)
+
+
+def cluster_batch(nconf, nmin=16, nmax=512, seed=512):
+    """nconf clusters with log-uniform sizes in [nmin, nmax]: (list of coordinate arrays, sizes)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax + 1), nconf))).astype(int)
+    sizes = np.clip(sizes, nmin, nmax)
+    return [water_cluster(rng, int(n)) for n in sizes], sizes
+
+
+# The batched workload: 512 gas-phase clusters of 16..512 waters, cutoffs 9 / 7 A.
+C3 = dict(name="water_clusters_512x16-512_gas", nconf=512, nmin=16, nmax=512, seed=512, cutoff2b=9.0,
+          cutoff3b=7.0)

@@ -83,11 +83,14 @@ struct DBuf {
 
 // Neighbour-search and list state (both orders share one gather).
 struct ListState {
-    Grid grid{};  // host copy of the uploaded periodic grid
-    bool grid_uploaded = false;
-    DBuf<Grid> d_grid;
-    DBuf<double> cx, cy, cz;
-    DBuf<int> cell, cell_count, cell_start, cell_pts;
+    // cell layout over all configurations (rebuilt when topology, batch, box or cutoff change)
+    bool layout_ok = false;
+    double layout_cut = -1.0;
+    int ncell_total = 0;
+    DBuf<Grid> d_grids;           // one per configuration
+    DBuf<int> d_cell_base, d_cell_bound;
+    DBuf<double> cx, cy, cz, bx, by, bz;
+    DBuf<int> cell, cell_count, cell_start, cell_slot;
     DBuf<int4> counts, offsets;
     DBuf<int> list2, eval2, list3, eval3;
     DBuf<double> gscratch;        // match storage when it does not fit in LDS
@@ -99,7 +102,7 @@ struct ListState {
 
 // Per-order evaluation buffers.
 struct EvalState {
-    DBuf<double> e_nmer, partials, ws;
+    DBuf<double> e_nmer, vir_nmer, partials, ws;
 };
 
 // Largest match storage kept in LDS (80 bytes per match): 512 matches = 40 KB per block.
@@ -111,6 +114,8 @@ constexpr int kScratchBlocks = 2048;
 struct mbxg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;               // the 2B evaluation runs here, beside the 3B one
+    hipEvent_t fork = nullptr, join = nullptr;   // stream2 waits for `fork`, stream waits for `join`
     // topology
     int nmon = 0, natoms = 0, np = 0;
     std::vector<int> kind, nat, first_atom, tag, members;
@@ -123,6 +128,12 @@ struct mbxg_ctx {
     DBuf<double> d_in, d_xyz, d_grad, d_totals, d_coef2, d_coef3;
     DBuf<Box> d_box;
     DBuf<int> d_overflow;
+    // configurations (mbxg_set_batch): monomers [mon_off[c], mon_off[c+1]) form configuration c;
+    // its cloud points are [pt_off[c], pt_off[c+1])
+    int nconf = 1;
+    std::vector<int> mon_off, pt_off;
+    DBuf<int> d_pconf, d_pt_off;
+    DBuf<double> d_conf_tot;  // per configuration: 2B record, then 3B record (kPartials each)
     ListState L;
     EvalState ev2, ev3;
     bool have_coords = false, have_results = false;
@@ -131,6 +142,8 @@ struct mbxg_ctx {
     bool evaluated = false; // the last compute evaluated energies (not lists only)
     bool inputs_changed = false;  // coordinates, box, cutoffs or coefficients set since the last compute
     int last_terms = 0, last_grads = 0;
+    // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
+    int split2 = 4, split3 = 2;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
     hipEvent_t ev[kNumEvents] = {};
@@ -147,12 +160,15 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
     mbxg_ctx* c = new mbxg_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete c;
-        return fail(MBXG_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        return fail(MBXG_EHIP, "mbxg_create: stream/event creation failed: %s", hipGetErrorString(e));
     }
     if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(16) ||
-        c->d_overflow.reserve(1) || c->d_box.reserve(1) || c->L.d_grid.reserve(1)) {
+            c->d_overflow.reserve(1) || c->d_box.reserve(1)) {
         delete c;
         return fail(MBXG_ENOMEM, "mbxg_create: device allocation failed");
     }
@@ -168,14 +184,19 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     ListState& L = c->L;
-    L.d_grid.release();
+    L.d_grids.release();
+    L.d_cell_base.release();
+    L.d_cell_bound.release();
     L.cx.release();
     L.cy.release();
     L.cz.release();
+    L.bx.release();
+    L.by.release();
+    L.bz.release();
     L.cell.release();
     L.cell_count.release();
     L.cell_start.release();
-    L.cell_pts.release();
+    L.cell_slot.release();
     L.counts.release();
     L.offsets.release();
     L.list2.release();
@@ -183,8 +204,12 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.list3.release();
     L.eval3.release();
     L.gscratch.release();
+    c->d_pconf.release();
+    c->d_pt_off.release();
+    c->d_conf_tot.release();
     for (EvalState* o : {&c->ev2, &c->ev3}) {
         o->e_nmer.release();
+        o->vir_nmer.release();
         o->partials.release();
         o->ws.release();
     }
@@ -204,10 +229,53 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_overflow.release();
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c->fork);
+    (void)hipEventDestroy(c->join);
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MBXG_OK;
 }
+
+namespace {
+
+// Split the monomers into configurations (CSR offsets over monomers; nullptr = one).  Cloud
+// points keep monomer order, so each configuration's points are a contiguous range.
+int set_configs(mbxg_ctx* c, int nconf, const int* mon_off) {
+    std::vector<int> off;
+    if (!mon_off || nconf <= 1) {
+        nconf = 1;
+        off = {0, c->nmon};
+    } else {
+        off.assign(mon_off, mon_off + nconf + 1);
+        if (off[0] != 0 || off[nconf] != c->nmon)
+            return fail(MBXG_EINVAL, "mbxg_set_batch: offsets must run from 0 to the monomer count %d", c->nmon);
+        for (int k = 0; k < nconf; ++k)
+            if (off[k + 1] < off[k]) return fail(MBXG_EINVAL, "mbxg_set_batch: offsets must not decrease");
+    }
+    std::vector<int> pt_off(nconf + 1, 0), pconf(c->np > 0 ? c->np : 1, 0);
+    int k = 0;
+    for (int p = 0; p < c->np; ++p) {
+        while (c->members[p] >= off[k + 1]) ++k;
+        pconf[p] = k;
+        ++pt_off[k + 1];
+    }
+    for (int q = 0; q < nconf; ++q) pt_off[q + 1] += pt_off[q];
+    if (c->d_pconf.reserve(pconf.size()) || c->d_pt_off.reserve(nconf + 1) ||
+        c->d_conf_tot.reserve((size_t)2 * kPartials * nconf))
+        return fail(MBXG_ENOMEM, "configurations: device allocation failed");
+    HIPCHK(hipMemcpy(c->d_pconf.p, pconf.data(), sizeof(int) * pconf.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_pt_off.p, pt_off.data(), sizeof(int) * (nconf + 1), hipMemcpyHostToDevice));
+    c->nconf = nconf;
+    c->mon_off = off;
+    c->pt_off = pt_off;
+    c->L.layout_ok = false;
+    c->need_sync = true;
+    c->have_results = false;
+    return MBXG_OK;
+}
+
+}  // namespace
 
 extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const int* nat, const int* tag,
                                  const unsigned char* islocal) {
@@ -251,10 +319,17 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     }
     if (c->np > 0)
         HIPCHK(hipMemcpy(c->d_members.p, c->members.data(), sizeof(int) * c->np, hipMemcpyHostToDevice));
+    if (int r = set_configs(c, 1, nullptr)) return r;
     c->have_coords = false;
     c->have_results = false;
     c->need_sync = true;
     return MBXG_OK;
+}
+
+extern "C" int mbxg_set_batch(mbxg_ctx* c, int nconf, const int* mon_offsets) {
+    if (!c || nconf < 1 || (nconf > 1 && !mon_offsets)) return fail(MBXG_EINVAL, "mbxg_set_batch: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    return set_configs(c, nconf, mon_offsets);
 }
 
 // InvertUnitCell (tools/math_tools.cpp:37-64), same operation order.
@@ -285,7 +360,7 @@ extern "C" int mbxg_set_box(mbxg_ctx* c, const double* box9) {
         HIPCHK(hipStreamSynchronize(c->stream));
         c->need_sync = true;
         c->inputs_changed = true;
-        c->L.grid_uploaded = false;
+        c->L.layout_ok = false;
         return MBXG_OK;
     }
     Box B{};
@@ -316,7 +391,7 @@ extern "C" int mbxg_set_box(mbxg_ctx* c, const double* box9) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->need_sync = true;
     c->inputs_changed = true;
-    c->L.grid_uploaded = false;
+    c->L.layout_ok = false;
     return MBXG_OK;
 }
 
@@ -326,7 +401,7 @@ extern "C" int mbxg_set_cutoffs(mbxg_ctx* c, double c2, double c3) {
     c->cut3 = c3;
     c->need_sync = true;
     c->inputs_changed = true;
-    c->L.grid_uploaded = false;
+    c->L.layout_ok = false;
     return MBXG_OK;
 }
 
@@ -378,10 +453,9 @@ extern "C" int mbxg_set_coords_device(mbxg_ctx* c, const double* d_xyz) {
 
 namespace {
 
-// Candidate grid of a periodic cell: cells in fractional coordinates, each at least `cut` wide
-// along every perpendicular direction (width = V / |b x c| etc.), so every slot within the
-// cutoff of a query lies in the query's cell or a neighbour (wrapped).  With fewer than three
-// cells along a direction the neighbour range shrinks so no cell is visited twice.
+// Candidate grid of a periodic cell over the slots' extended fractional range [-1, 1]^3: n
+// cells per direction, each at least `cut` wide along the perpendicular (the cell's
+// perpendicular width is 2 V / (n |b x c|) etc.).
 Grid periodic_grid(const Box& box, double cut) {
     Grid G{};
     const double* b = box.b;
@@ -399,12 +473,12 @@ Grid periodic_grid(const Box& box, double cut) {
                          V / std::sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]),
                          V / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2])};
     for (int d = 0; d < 3; ++d) {
-        int n = (int)std::floor(w[d] / cut);
+        int n = (int)std::floor(2.0 * w[d] / cut);
         if (n < 1) n = 1;
         if (n > kMaxCellsPerDim) n = kMaxCellsPerDim;
         G.n[d] = n;
-        G.lo_off[d] = n >= 3 ? -1 : 0;
-        G.hi_off[d] = n >= 3 ? 1 : n - 1;
+        G.lo[d] = -1.0;
+        G.inv_w[d] = 0.5 * n;
     }
     G.periodic = 1;
     return G;
@@ -430,42 +504,74 @@ void launch_zero(double* p, size_t n, hipStream_t st) {
 
 // Dimer and trimer lists at the larger cutoff (one gather for both orders).  With sync the
 // totals and the largest match count are read back and the buffers sized from them.
+// Cell layout of all configurations: under PBC every configuration gets the same extended
+// grid; in the gas phase each configuration's grid is sized on the device within a cell bound
+// of 4 cells per point (at least 64, at most 64^3).
+int prepare_layout(mbxg_ctx* c, double cut) {
+    ListState& L = c->L;
+    const int nconf = c->nconf;
+    std::vector<int> bound(nconf), base(nconf);
+    std::vector<Grid> grids;
+    int total = 0;
+    if (c->box.periodic) {
+        const Grid G = periodic_grid(c->box, cut);
+        grids.assign(nconf, G);
+        for (int k = 0; k < nconf; ++k) bound[k] = G.n[0] * G.n[1] * G.n[2];
+    } else {
+        for (int k = 0; k < nconf; ++k) {
+            const int npk = c->pt_off[k + 1] - c->pt_off[k];
+            bound[k] = std::min(kMaxCells, std::max(64, 4 * npk));
+        }
+    }
+    for (int k = 0; k < nconf; ++k) {
+        base[k] = total;
+        total += bound[k];
+    }
+    if (L.d_grids.reserve(nconf) || L.d_cell_base.reserve(nconf) || L.d_cell_bound.reserve(nconf) ||
+        L.cell_count.reserve(total) || L.cell_start.reserve(total + 1))
+        return fail(MBXG_ENOMEM, "cell layout: device allocation failed");
+    hipStream_t st = c->stream;
+    HIPCHK(hipMemcpyAsync(L.d_cell_base.p, base.data(), sizeof(int) * nconf, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(L.d_cell_bound.p, bound.data(), sizeof(int) * nconf, hipMemcpyHostToDevice, st));
+    if (!grids.empty())
+        HIPCHK(hipMemcpyAsync(L.d_grids.p, grids.data(), sizeof(Grid) * nconf, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    L.ncell_total = total;
+    L.layout_cut = cut;
+    L.layout_ok = true;
+    return MBXG_OK;
+}
+
+// Dimer and trimer lists at the larger cutoff (one gather for both orders).  With sync the
+// totals and the largest match count are read back and the buffers sized from them.
 int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double cmax, bool sync, hipEvent_t* ev) {
     ListState& L = c->L;
     hipStream_t st = c->stream;
     const int np = c->np;
     const int nslot = c->box.periodic ? 8 : 1;
-    const size_t ns = (size_t)np * nslot;
+    const int ns = np * nslot;
     const double cut = cmax * (1.0 + 1e-9) + 1e-9;
-    int ncell = kMaxCells;
-    if (c->box.periodic) {
-        const Grid G = periodic_grid(c->box, cut);
-        if (!L.grid_uploaded || std::memcmp(&G, &L.grid, sizeof G) != 0) {
-            L.grid = G;
-            HIPCHK(hipMemcpyAsync(L.d_grid.p, &L.grid, sizeof(Grid), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));
-            L.grid_uploaded = true;
-        }
-        ncell = G.n[0] * G.n[1] * G.n[2];
-    }
-    if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.cell.reserve(np) ||
-        L.cell_count.reserve(ncell) || L.cell_start.reserve(ncell + 1) || L.cell_pts.reserve(np) ||
-        L.counts.reserve(np) || L.offsets.reserve(np + 1))
+    if (!L.layout_ok || L.layout_cut != cut)
+        if (int r = prepare_layout(c, cut)) return r;
+    if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.bx.reserve(ns) || L.by.reserve(ns) ||
+        L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.counts.reserve(np) ||
+        L.offsets.reserve(np + 1))
         return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
-    const int tb = 256, pb = (np + tb - 1) / tb;
-    if (!c->box.periodic) {
-        hipLaunchKernelGGL(k_grid_gas, dim3(1), dim3(256), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, np, cut,
-                           L.d_grid.p);
-        L.grid_uploaded = false;
-    }
-    launch_zero_int(L.cell_count.p, ncell, st);
+    const int tb = 256, pb = (np + tb - 1) / tb, sb = (ns + tb - 1) / tb;
+    if (!c->box.periodic)
+        hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p,
+                           c->d_pt_off.p, L.d_cell_bound.p, cut, L.d_grids.p);
+    launch_zero_int(L.cell_count.p, L.ncell_total, st);
     hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, np, c->box,
-                       L.d_grid.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p);
-    hipLaunchKernelGGL(k_bin_count, dim3(pb), dim3(tb), 0, st, L.cell.p, np, L.cell_count.p);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.d_grid.p, L.cell_start.p);
-    hipLaunchKernelGGL(k_bin_fill, dim3(pb), dim3(tb), 0, st, L.cell.p, np, L.cell_start.p, L.cell_count.p,
-                       L.cell_pts.p);
+                       L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p);
+    hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p);
+    hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_start.p, L.cell_count.p, L.cx.p,
+                       L.cy.p, L.cz.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
     HIPCHK(hipGetLastError());
+    NmersArgs A{L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p, L.d_grids.p,
+                c->d_pconf.p, L.d_cell_base.p, np, c2sq, c3sq, L.counts.p, L.offsets.p, nullptr, nullptr, nullptr,
+                nullptr, c->d_overflow.p};
 
     auto scratch = [&]() -> double* {
         if (L.match_cap <= kMaxLdsMatches) return nullptr;
@@ -477,9 +583,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         double* gs = scratch();
         if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
         launch_zero_int(c->d_overflow.p, 1, st);
-        HIPCHK(launch_nmers(L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_pts.p, np, nslot, L.d_grid.p, N,
-                            c2sq, c3sq, 0, L.counts.p, nullptr, nullptr, nullptr, nullptr, nullptr, none, L.match_cap,
-                            c->d_overflow.p, gs, kScratchBlocks, st));
+        HIPCHK(launch_nmers(A, N, 0, none, L.match_cap, gs, kScratchBlocks, st));
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p);
         HIPCHK(hipGetLastError());
         if (!sync) break;
@@ -490,7 +594,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         HIPCHK(hipStreamSynchronize(st));
         const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
         if (ovf > L.match_cap) {
-            if (attempt > 0 || (size_t)want > 2 * ns + 2)
+            if (attempt > 0 || want > 2 * ns + 2)
                 return fail(MBXG_EHIP, "neighbour search: match storage did not grow (%d matches)", ovf);
             L.match_cap = want;
             continue;  // counts were taken over clamped matches: count again
@@ -507,9 +611,11 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         break;
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    HIPCHK(launch_nmers(L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_pts.p, np, nslot, L.d_grid.p, N, c2sq,
-                        c3sq, 1, L.counts.p, L.offsets.p, L.list2.p, L.eval2.p, L.list3.p, L.eval3.p, L.cap,
-                        L.match_cap, c->d_overflow.p, L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr,
+    A.list2 = L.list2.p;
+    A.eval2 = L.eval2.p;
+    A.list3 = L.list3.p;
+    A.eval3 = L.eval3.p;
+    HIPCHK(launch_nmers(A, N, 1, L.cap, L.match_cap, L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr,
                         kScratchBlocks, st));
     return 0;
 }
@@ -544,30 +650,46 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     if (c->natoms > 0) launch_zero(c->d_grad.p, 3 * (size_t)c->natoms, st);
     launch_zero(c->d_totals.p, 16, st);
+    if (c->nconf > 1) launch_zero(c->d_conf_tot.p, (size_t)2 * kPartials * c->nconf, st);
     HIPCHK(hipGetLastError());
     c->evaluated = L.built && !(terms & MBXG_LISTS_ONLY);
     if (c->evaluated) {
+        // the 2B chain runs on stream2, concurrently with the 3B chain: its small blocks fill
+        // the machine while the last 3B blocks drain
+        HIPCHK(hipEventRecord(c->fork, st));
+        HIPCHK(hipStreamWaitEvent(c->stream2, c->fork, 0));
         const int* n2 = (const int*)(L.offsets.p + c->np) + 1;  // evaluated dimers
         const int* n3 = (const int*)(L.offsets.p + c->np) + 3;  // evaluated trimers
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 1; k >= 0; --k) {
             if (!(k == 0 ? want2 : want3)) continue;
+            // with stage timing on, both chains run on the main stream so each stage is timed alone
+            hipStream_t sk = (k == 0 && !ev) ? c->stream2 : st;
             EvalState& o = k == 0 ? c->ev2 : c->ev3;
             const int cap = k == 0 ? L.cap.y : L.cap.w;
             if (cap <= 0) continue;
             const size_t wsz = k == 0 ? workspace_mbpol2b(cap) : workspace_mbpol3b(cap);
+            const int ld = (cap + 63) / 64 * 64;
+            const bool batch = c->nconf > 1;
             if (o.e_nmer.reserve(cap) || o.partials.reserve((size_t)kPartials * partials_records(cap)) ||
-                o.ws.reserve(wsz))
+                o.ws.reserve(wsz) || (batch && o.vir_nmer.reserve((size_t)6 * ld)))
                 return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
+            double* vir = batch ? o.vir_nmer.p : nullptr;
             if (k == 0)
-                HIPCHK(launch_mbpol2b(L.eval2.p, n2, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                      o.partials.p, o.ws.p, st, ev ? ev + 4 : nullptr));
+                HIPCHK(launch_mbpol2b(L.eval2.p, n2, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
+                                      o.partials.p, o.ws.p, c->split2, sk, ev ? ev + 4 : nullptr));
             else
-                HIPCHK(launch_mbpol3b(L.eval3.p, n3, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p,
-                                      o.partials.p, o.ws.p, st, ev ? ev + 8 : nullptr));
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, o.partials.p, k == 0 ? n2 : n3, cap,
+                HIPCHK(launch_mbpol3b(L.eval3.p, n3, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
+                                      o.partials.p, o.ws.p, c->split3, sk, ev ? ev + 8 : nullptr));
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, k == 0 ? n2 : n3, cap,
                                c->d_totals.p + kPartials * k);
+            if (batch)
+                hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf), dim3(256), 0, sk, o.e_nmer.p, vir, ld,
+                                   L.offsets.p, c->d_pt_off.p, k == 0 ? 1 : 3, cap,
+                                   c->d_conf_tot.p + (size_t)k * kPartials * c->nconf);
             HIPCHK(hipGetLastError());
         }
+        HIPCHK(hipEventRecord(c->join, c->stream2));
+        HIPCHK(hipStreamWaitEvent(st, c->join, 0));
     }
     if (ev) HIPCHK(hipEventRecord(ev[12], st));
     c->have_results = true;
@@ -644,6 +766,74 @@ extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
     return MBXG_OK;
 }
 
+namespace {
+
+bool valid_split(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
+
+}  // namespace
+
+extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
+    if (!c || !key) return fail(MBXG_EINVAL, "mbxg_set_tuning: bad arguments");
+    const std::string k(key);
+    if ((k == "split_2b" || k == "split_3b") && !valid_split(value))
+        return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
+    if (k == "split_2b")
+        c->split2 = value;
+    else if (k == "split_3b")
+        c->split3 = value;
+    else
+        return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
+    if (!c || !key || !value) return fail(MBXG_EINVAL, "mbxg_get_tuning: bad arguments");
+    const std::string k(key);
+    if (k == "split_2b")
+        *value = c->split2;
+    else if (k == "split_3b")
+        *value = c->split3;
+    else
+        return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
+    return MBXG_OK;
+}
+
+// Autotuner: time the polynomial stage of each order for every split on the current inputs
+// (HIP events, stages run one at a time), keep the fastest, then recompute once so the results
+// are those of the chosen schedule.
+extern "C" int mbxg_autotune(mbxg_ctx* c, int terms, int do_grads, int reps) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_autotune: null context");
+    if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_autotune: coordinates not set");
+    if (reps < 1) reps = 1;
+    const bool was_timing = c->timing;
+    if (int r = mbxg_set_timing(c, 1)) return r;
+    const int cand[4] = {1, 2, 4, 8};
+    double best[2] = {1e300, 1e300};
+    int pick[2] = {c->split2, c->split3};
+    for (int i = 0; i < 4; ++i) {
+        c->split2 = c->split3 = cand[i];
+        double t[2] = {1e300, 1e300};
+        for (int r = 0; r < reps + 1; ++r) {  // first repetition warms up
+            if (int e = mbxg_compute(c, terms, do_grads)) return e;
+            double ms[10];
+            if (int e = mbxg_get_timings(c, ms, 10)) return e;
+            if (r > 0) {
+                t[0] = std::min(t[0], ms[4]);
+                t[1] = std::min(t[1], ms[7]);
+            }
+        }
+        for (int k = 0; k < 2; ++k)
+            if (t[k] < best[k]) {
+                best[k] = t[k];
+                pick[k] = cand[i];
+            }
+    }
+    c->split2 = pick[0];
+    c->split3 = pick[1];
+    if (int r = mbxg_set_timing(c, was_timing ? 1 : 0)) return r;
+    return mbxg_compute(c, terms, do_grads);
+}
+
 extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* grad, double* virial) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_results: null context");
     if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_results: nothing computed");
@@ -663,6 +853,38 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
         virial[3] = virial[1];
         virial[6] = virial[2];
         virial[7] = virial[5];
+    }
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_batch_results(mbxg_ctx* c, double* e2b, double* e3b, double* virial) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_get_batch_results: null context");
+    if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_batch_results: nothing computed");
+    if (int r = finish(c)) return r;
+    const int nc = c->nconf;
+    std::vector<double> tot((size_t)2 * kPartials * nc, 0.0);
+    if (nc > 1) {
+        HIPCHK(hipMemcpyAsync(tot.data(), c->d_conf_tot.p, sizeof(double) * tot.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(tot.data(), c->d_totals.p, sizeof(double) * 2 * kPartials, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double* t2 = tot.data();
+    const double* t3 = tot.data() + (size_t)kPartials * nc;
+    const int slot[6] = {0, 1, 2, 4, 5, 8};
+    for (int k = 0; k < nc; ++k) {
+        if (e2b) e2b[k] = t2[k * kPartials];
+        if (e3b) e3b[k] = t3[k * kPartials];
+        if (virial) {
+            double* v = virial + 9 * (size_t)k;
+            for (int q = 0; q < 9; ++q) v[q] = 0.0;
+            for (int q = 0; q < 6; ++q) v[slot[q]] = t2[k * kPartials + 1 + q] + t3[k * kPartials + 1 + q];
+            v[3] = v[1];
+            v[6] = v[2];
+            v[7] = v[5];
+        }
     }
     return MBXG_OK;
 }

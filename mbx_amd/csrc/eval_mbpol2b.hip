@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(kGradBlock2) k_grad_mbpol2b(const int* __restr
                                                               const double* __restrict__ ps,
                                                               const double* __restrict__ sws, int ld,
                                                               double* __restrict__ grad, double* __restrict__ e_out,
+                                                              double* __restrict__ vir_out,
                                                               double* __restrict__ partials) {
     __shared__ GradTable<256, 9> T;
     const int t = blockIdx.x * kGradBlock2 + threadIdx.x;
@@ -231,6 +232,9 @@ __global__ void __launch_bounds__(kGradBlock2) k_grad_mbpol2b(const int* __restr
         const double e = sw * P;
         e_out[t] = e;
         e_sum = e;
+        if (vir_out)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) vir_out[(size_t)q * ld + t] = vir[q];
     }
     if (GRAD) {
         __syncthreads();
@@ -254,8 +258,8 @@ size_t workspace_mbpol2b(int cap) {
 }
 
 hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
-                          hipEvent_t* ev) {
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev) {
     if (cap <= 0) return hipSuccess;
     const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
@@ -268,16 +272,16 @@ hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const Nmer
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol2b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, st);
+    e = launch_poly_mbpol2b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
     const int ngb = (cap + kGradBlock2 - 1) / kGradBlock2;
     if (do_grads)
         hipLaunchKernelGGL(k_grad_mbpol2b<true>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps, sw,
-                           ld, grad, e_out, partials);
+                           ld, grad, e_out, vir_out, partials);
     else
         hipLaunchKernelGGL(k_grad_mbpol2b<false>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps,
-                           sw, ld, grad, e_out, partials);
+                           sw, ld, grad, e_out, vir_out, partials);
     e = hipGetLastError();
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;

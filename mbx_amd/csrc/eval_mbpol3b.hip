@@ -128,6 +128,7 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
                                                               const double* __restrict__ ps,
                                                               const double* __restrict__ sw, int ld,
                                                               double* __restrict__ grad, double* __restrict__ e_out,
+                                                              double* __restrict__ vir_out,
                                                               double* __restrict__ partials) {
     __shared__ GradTable<256, 9> T;
     const int t = blockIdx.x * kGradBlock3 + threadIdx.x;
@@ -191,6 +192,9 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
         const double e = s * P;
         e_out[t] = e;
         e_sum = e;
+        if (vir_out)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) vir_out[(size_t)q * ld + t] = vir[q];
     }
     if (GRAD) {
         __syncthreads();
@@ -215,8 +219,8 @@ size_t workspace_mbpol3b(int cap) {
 }
 
 hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
-                          hipEvent_t* ev) {
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev) {
     if (cap <= 0) return hipSuccess;
     const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
@@ -229,16 +233,16 @@ hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const Nmer
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol3b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, st);
+    e = launch_poly_mbpol3b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
     const int ngb = (cap + kGradBlock3 - 1) / kGradBlock3;
     if (do_grads)
         hipLaunchKernelGGL(k_grad_mbpol3b<true>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs, ps,
-                           sw, ld, grad, e_out, partials);
+                           sw, ld, grad, e_out, vir_out, partials);
     else
         hipLaunchKernelGGL(k_grad_mbpol3b<false>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs,
-                           ps, sw, ld, grad, e_out, partials);
+                           ps, sw, ld, grad, e_out, vir_out, partials);
     e = hipGetLastError();
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;

@@ -66,15 +66,19 @@ __global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// Gas-phase grid: bounding box of the points and the cell layout (one workgroup).  Writes the
-// Grid the other kernels read, at most kMaxCellsPerDim cells per dimension.
+// Gas-phase grids: one workgroup per configuration computes the bounding box of its points
+// [pt_off[c], pt_off[c+1]) and a grid of cells at least `cut` wide, with at most
+// min(kMaxCellsPerDim^3, cell_bound[c]) cells (coarser cells stay correct: every cell is still
+// at least one cutoff wide).
 __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz, const int* __restrict__ first_atom,
-                                                  const int* __restrict__ members, int np, double cut,
-                                                  Grid* __restrict__ G) {
+                                                  const int* __restrict__ members, const int* __restrict__ pt_off,
+                                                  const int* __restrict__ cell_bound, double cut,
+                                                  Grid* __restrict__ grids) {
     __shared__ double lo[3][256], hi[3][256];
     const int t = threadIdx.x;
+    const int c0 = pt_off[blockIdx.x], c1 = pt_off[blockIdx.x + 1];
     double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-    for (int p = t; p < np; p += 256) {
+    for (int p = c0 + t; p < c1; p += 256) {
         const double* x = xyz + 3 * (size_t)first_atom[members[p]];
         for (int c = 0; c < 3; ++c) {
             l[c] = fmin(l[c], x[c]);
@@ -95,19 +99,25 @@ __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz
         __syncthreads();
     }
     if (t == 0) {
+        const bool any = c1 > c0;
         Grid g{};
+        double ext[3];
         for (int c = 0; c < 3; ++c) {
-            const double ext = np > 0 ? hi[c][0] - lo[c][0] : 0.0;
-            int n = (int)floor(ext / cut);
-            n = n < 1 ? 1 : (n > kMaxCellsPerDim ? kMaxCellsPerDim : n);
-            g.n[c] = n;
-            g.lo[c] = np > 0 ? lo[c][0] : 0.0;
-            g.inv_w[c] = ext > 0 ? n / ext : 0.0;
-            g.lo_off[c] = n >= 2 ? -1 : 0;
-            g.hi_off[c] = n >= 2 ? 1 : 0;
+            ext[c] = any ? hi[c][0] - lo[c][0] : 0.0;
+            int n = (int)floor(ext[c] / cut);
+            g.n[c] = n < 1 ? 1 : (n > kMaxCellsPerDim ? kMaxCellsPerDim : n);
+        }
+        const long bound = cell_bound[blockIdx.x];
+        while ((long)g.n[0] * g.n[1] * g.n[2] > bound) {
+            const int d = g.n[0] >= g.n[1] && g.n[0] >= g.n[2] ? 0 : (g.n[1] >= g.n[2] ? 1 : 2);
+            g.n[d] -= 1;
+        }
+        for (int c = 0; c < 3; ++c) {
+            g.lo[c] = any ? lo[c][0] : 0.0;
+            g.inv_w[c] = ext[c] > 0 ? g.n[c] / ext[c] : 0.0;
         }
         g.periodic = 0;
-        *G = g;
+        grids[blockIdx.x] = g;
     }
 }
 
@@ -115,17 +125,33 @@ __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz
 // XyzToCloud (kdtree_utils.h:196-279): point p = first atom of cloud member p.  Slot 0 is the
 // re-wrapped position; slots 1..7 are the images selected from its fractional coordinates.
 // Every one of the 7 slots is written: each fractional coordinate admits at least one of the
-// two image directions.  Also computes the point's grid cell.  One thread per point.
+// two image directions.
+//
+// Every slot is also given a cell of its configuration's grid.  Under PBC the grid spans the
+// extended fractional range [-1, 1]^3 that the slots occupy (a slot's fractional position is
+// its point's plus the image shift), so a query only visits slots near it instead of all 8
+// slots of the points near it.  Cell index = cell_base[config] + local cell.  One thread per
+// point.
 __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
-                        const int* __restrict__ members, int np, Box B, const Grid* __restrict__ Gp,
-                        double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
-                        int* __restrict__ cell) {
+                        const int* __restrict__ members, int np, Box B, const Grid* __restrict__ grids,
+                        const int* __restrict__ pconf, const int* __restrict__ cell_base, double* __restrict__ cx,
+                        double* __restrict__ cy, double* __restrict__ cz, int* __restrict__ cell) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
-    const Grid G = *Gp;
+    const int conf = pconf[p];
+    const Grid G = grids[conf];
+    const int base = cell_base[conf];
+    auto cell_of = [&](double a, double b, double c) {
+        int ca = (int)floor((a - G.lo[0]) * G.inv_w[0]);
+        int cb = (int)floor((b - G.lo[1]) * G.inv_w[1]);
+        int cc = (int)floor((c - G.lo[2]) * G.inv_w[2]);
+        ca = ca < 0 ? 0 : (ca >= G.n[0] ? G.n[0] - 1 : ca);
+        cb = cb < 0 ? 0 : (cb >= G.n[1] ? G.n[1] - 1 : cb);
+        cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
+        return base + (cc * G.n[1] + cb) * G.n[0] + ca;
+    };
     const double* x0 = xyz + 3 * (size_t)first_atom[members[p]];
     double qx = x0[0], qy = x0[1], qz = x0[2];
-    int ca, cb, cc;
     if (B.periodic) {
         double u, v, w;
         to_frac(B, qx, qy, qz, u, v, w);
@@ -137,6 +163,7 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
         const double uvw0 = B.f00 * qx - B.f01 * qy + B.f02 * qz;
         const double uvw1 = B.f11 * qy + B.f12 * qz;
         const double uvw2 = B.f22 * qz;
+        to_frac(B, qx, qy, qz, u, v, w);  // cell placement only
         const int ind[3] = {0, -1, 1};
         for (int m2 = 0; m2 < 3; ++m2) {
             const int m = ind[m2];
@@ -170,43 +197,34 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
                     cx[s] = qx + sx;
                     cy[s] = qy + sy;
                     cz[s] = qz + sz;
+                    cell[s] = cell_of(u + m, v + n, w + l);
                 }
             }
         }
-        // grid cell from the fractional position in [-0.5, 0.5)
-        to_frac(B, qx, qy, qz, u, v, w);
-        ca = (int)floor((u + 0.5) * G.n[0]);
-        cb = (int)floor((v + 0.5) * G.n[1]);
-        cc = (int)floor((w + 0.5) * G.n[2]);
+        cell[p] = cell_of(u, v, w);
     } else {
-        ca = (int)floor((qx - G.lo[0]) * G.inv_w[0]);
-        cb = (int)floor((qy - G.lo[1]) * G.inv_w[1]);
-        cc = (int)floor((qz - G.lo[2]) * G.inv_w[2]);
+        cell[p] = cell_of(qx, qy, qz);
     }
     cx[p] = qx;
     cy[p] = qy;
     cz[p] = qz;
-    ca = ca < 0 ? 0 : (ca >= G.n[0] ? G.n[0] - 1 : ca);
-    cb = cb < 0 ? 0 : (cb >= G.n[1] ? G.n[1] - 1 : cb);
-    cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
-    cell[p] = (cc * G.n[1] + cb) * G.n[0] + ca;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Binning of points into cells: count, scan, fill.  The order of points inside a cell is left
-// to the atomics; the n-mer kernel sorts each query's matches, which makes the output order
-// independent of it.  cell_count must be zero on entry (k_zero_int).
-__global__ void k_bin_count(const int* __restrict__ cell, int np, int* __restrict__ cell_count) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < np) atomicAdd(&cell_count[cell[p]], 1);
+// Binning of slots into cells: count, scan, fill.  The fill also writes the slots' coordinates
+// in cell order, so the gather reads them contiguously.  The order of slots inside a cell is
+// left to the atomics; the n-mer kernel sorts each query's matches, which makes the output
+// order independent of it.  cell_count must be zero on entry (k_zero_int).
+__global__ void k_bin_count(const int* __restrict__ cell, int ns, int* __restrict__ cell_count) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) atomicAdd(&cell_count[cell[s]], 1);
 }
 
-// Single-workgroup exclusive scan of the cell counts (ncell from the grid); cell_start gets
-// ncell+1 entries and cell_count is reset to zero for use as the fill cursor.
-__global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, const Grid* __restrict__ Gp,
+// Single-workgroup exclusive scan of ncell cell counts; cell_start gets ncell+1 entries and
+// cell_count is reset to zero for use as the fill cursor.
+__global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, int ncell,
                                                      int* __restrict__ cell_start) {
     __shared__ int sh[1024];
-    const int ncell = Gp->n[0] * Gp->n[1] * Gp->n[2];
     const int t = threadIdx.x;
     const int per = (ncell + 1023) / 1024;
     const int beg = t * per, end = min(ncell, beg + per);
@@ -229,12 +247,18 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_coun
     if (t == 1023) cell_start[ncell] = sh[t];
 }
 
-__global__ void k_bin_fill(const int* __restrict__ cell, int np, const int* __restrict__ cell_start,
-                           int* __restrict__ cursor, int* __restrict__ cell_pts) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int c = cell[p];
-    cell_pts[cell_start[c] + atomicAdd(&cursor[c], 1)] = p;
+__global__ void k_bin_fill(const int* __restrict__ cell, int ns, const int* __restrict__ cell_start,
+                           int* __restrict__ cursor, const double* __restrict__ cx, const double* __restrict__ cy,
+                           const double* __restrict__ cz, int* __restrict__ cell_slot, double* __restrict__ bx,
+                           double* __restrict__ by, double* __restrict__ bz) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int c = cell[s];
+    const int at = cell_start[c] + atomicAdd(&cursor[c], 1);
+    cell_slot[at] = s;
+    bx[at] = cx[s];
+    by[at] = cy[s];
+    bz[at] = cz[s];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -307,11 +331,13 @@ __device__ void sort_ints(int* a, int n) {
 // (LDS, or a global scratch slice when the matches do not fit in LDS).
 __device__ void nmers_query(int q, const double* __restrict__ cx, const double* __restrict__ cy,
                             const double* __restrict__ cz, const int* __restrict__ cell,
-                            const int* __restrict__ cell_start, const int* __restrict__ cell_pts, int np, int nslot,
-                            const Grid& G, const NmerCtx& N, double c2sq, double c3sq, int pass,
-                            int4* __restrict__ counts, const int4* __restrict__ offsets, int* __restrict__ list2,
-                            int* __restrict__ eval2, int* __restrict__ list3, int* __restrict__ eval3, int4 cap,
-                            int match_cap, int* __restrict__ overflow, const Lds& L) {
+                            const int* __restrict__ cell_start, const int* __restrict__ cell_slot,
+                            const double* __restrict__ bx, const double* __restrict__ by,
+                            const double* __restrict__ bz, int np, const Grid& G, int base, const NmerCtx& N,
+                            double c2sq, double c3sq, int pass, int4* __restrict__ counts,
+                            const int4* __restrict__ offsets, int* __restrict__ list2, int* __restrict__ eval2,
+                            int* __restrict__ list3, int* __restrict__ eval3, int4 cap, int match_cap,
+                            int* __restrict__ overflow, const Lds& L) {
     const int lane = threadIdx.x;
     const int mi = N.members[q];
     if (!N.islocal[mi]) {
@@ -320,48 +346,26 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
     }
     const double cmsq = c2sq > c3sq ? c2sq : c3sq;
 
-    // ---- gather: every candidate slot strictly inside the larger cutoff
+    // ---- gather: every slot strictly inside the larger cutoff, from the 27 cells around the
+    // query's (clamped at the grid's faces)
     const double qx = cx[q], qy = cy[q], qz = cz[q];
-    const int ci = cell[q];
+    const int ci = cell[q] - base;
     const int ia = ci % G.n[0], ib = (ci / G.n[0]) % G.n[1], ic = ci / (G.n[0] * G.n[1]);
     int nm = 0;
-    for (int oc = G.lo_off[2]; oc <= G.hi_off[2]; ++oc) {
-        int cc = ic + oc;
-        if (G.periodic) {
-            cc = (cc + G.n[2]) % G.n[2];
-        } else if (cc < 0 || cc >= G.n[2]) {
-            continue;
-        }
-        for (int ob = G.lo_off[1]; ob <= G.hi_off[1]; ++ob) {
-            int cb = ib + ob;
-            if (G.periodic) {
-                cb = (cb + G.n[1]) % G.n[1];
-            } else if (cb < 0 || cb >= G.n[1]) {
-                continue;
-            }
-            for (int oa = G.lo_off[0]; oa <= G.hi_off[0]; ++oa) {
-                int ca = ia + oa;
-                if (G.periodic) {
-                    ca = (ca + G.n[0]) % G.n[0];
-                } else if (ca < 0 || ca >= G.n[0]) {
-                    continue;
-                }
-                const int c = (cc * G.n[1] + cb) * G.n[0] + ca;
-                const int beg = cell_start[c], end = cell_start[c + 1];
-                const int total = (end - beg) * nslot;
-                for (int base = 0; base < total; base += 64) {
-                    const int t = base + lane;
-                    int s = -1;
-                    if (t < total) {
-                        const int p = cell_pts[beg + t / nslot];
-                        const size_t si = (size_t)p + (size_t)(t % nslot) * np;
-                        if (dist2(qx, qy, qz, cx[si], cy[si], cz[si]) < cmsq) s = (int)si;
-                    }
-                    const unsigned long long mask = __ballot(s >= 0);
-                    const int pos = nm + __popcll(mask & ((1ull << lane) - 1ull));
-                    if (s >= 0 && pos < match_cap) L.match[pos] = s;
-                    nm += __popcll(mask);
-                }
+    for (int cc = max(ic - 1, 0); cc <= min(ic + 1, G.n[2] - 1); ++cc) {
+        for (int cb = max(ib - 1, 0); cb <= min(ib + 1, G.n[1] - 1); ++cb) {
+            const int row = base + (cc * G.n[1] + cb) * G.n[0];
+            // the (up to) three cells along a are contiguous in cell order
+            const int beg = cell_start[row + max(ia - 1, 0)];
+            const int end = cell_start[row + min(ia + 1, G.n[0] - 1) + 1];
+            for (int i0 = beg; i0 < end; i0 += 64) {
+                const int i = i0 + lane;
+                int s = -1;
+                if (i < end && dist2(qx, qy, qz, bx[i], by[i], bz[i]) < cmsq) s = cell_slot[i];
+                const unsigned long long mask = __ballot(s >= 0);
+                const int pos = nm + __popcll(mask & ((1ull << lane) - 1ull));
+                if (s >= 0 && pos < match_cap) L.match[pos] = s;
+                nm += __popcll(mask);
             }
         }
     }
@@ -421,7 +425,7 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
     __syncthreads();
 
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int4 base = pass ? offsets[q] : make_int4(0, 0, 0, 0);
+    const int4 at = pass ? offsets[q] : make_int4(0, 0, 0, 0);
     int nl2 = 0, ne2 = 0, nl3 = 0, ne3 = 0;
 
     // ---- dimers (AddClusters sys_tools.cpp:558-567; evaluation rule x2b-v9x.cpp:1589-1600)
@@ -438,14 +442,14 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
             const unsigned long long ml = __ballot(listed), me = __ballot(ev);
             if (pass) {
                 if (listed) {
-                    const int w = base.x + nl2 + __popcll(ml & below);
+                    const int w = at.x + nl2 + __popcll(ml & below);
                     if (w < cap.x) {
                         list2[2 * w] = mi;
                         list2[2 * w + 1] = mj;
                     }
                 }
                 if (ev) {
-                    const int w = base.y + ne2 + __popcll(me & below);
+                    const int w = at.y + ne2 + __popcll(me & below);
                     if (w < cap.y) {
                         eval2[2 * w] = mi;
                         eval2[2 * w + 1] = mj;
@@ -490,7 +494,7 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
             const unsigned long long ml = __ballot(listed), me = __ballot(ev);
             if (pass) {
                 if (listed) {
-                    const int w = base.z + nl3 + __popcll(ml & below);
+                    const int w = at.z + nl3 + __popcll(ml & below);
                     if (w < cap.z) {
                         list3[3 * w] = mi;
                         list3[3 * w + 1] = mj;
@@ -498,7 +502,7 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
                     }
                 }
                 if (ev) {
-                    const int w = base.w + ne3 + __popcll(me & below);
+                    const int w = at.w + ne3 + __popcll(me & below);
                     if (w < cap.w) {
                         eval3[3 * w] = mi;
                         eval3[3 * w + 1] = mj;
@@ -517,23 +521,18 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
 // pass 1: writes at offsets[q], bounded by cap (capacities in the same order).
 // overflow[0] receives the largest match count of any query (the host compares it with
 // match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
-// the dynamic LDS, otherwise block b uses gscratch + b * match_cap * 10 doubles.
-__global__ void __launch_bounds__(64) k_nmers(const double* __restrict__ cx, const double* __restrict__ cy,
-                                              const double* __restrict__ cz, const int* __restrict__ cell,
-                                              const int* __restrict__ cell_start, const int* __restrict__ cell_pts,
-                                              int np, int nslot, const Grid* __restrict__ Gp, NmerCtx N, double c2sq,
-                                              double c3sq, int pass, int4* __restrict__ counts,
-                                              const int4* __restrict__ offsets, int* __restrict__ list2,
-                                              int* __restrict__ eval2, int* __restrict__ list3,
-                                              int* __restrict__ eval3, int4 cap, int match_cap,
-                                              int* __restrict__ overflow, double* __restrict__ gscratch) {
+// the dynamic LDS, otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.
+__global__ void __launch_bounds__(64) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int match_cap,
+                                              double* __restrict__ gscratch) {
     extern __shared__ double smem[];
-    const Grid G = *Gp;
     double* store = gscratch ? gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles : smem;
     const Lds L = carve(store, match_cap);
-    for (int q = blockIdx.x; q < np; q += gridDim.x) {
-        nmers_query(q, cx, cy, cz, cell, cell_start, cell_pts, np, nslot, G, N, c2sq, c3sq, pass, counts, offsets,
-                    list2, eval2, list3, eval3, cap, match_cap, overflow, L);
+    for (int q = blockIdx.x; q < A.np; q += gridDim.x) {
+        const int conf = A.pconf[q];
+        const Grid G = A.grids[conf];
+        nmers_query(q, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
+                    A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.offsets, A.list2, A.eval2, A.list3,
+                    A.eval3, cap, match_cap, A.overflow, L);
         __syncthreads();
     }
 }
@@ -611,19 +610,42 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* __restric
     if (t < kPartials) out[t] = sh[t][0];
 }
 
+__global__ void __launch_bounds__(256) k_reduce_configs(const double* __restrict__ e_out,
+                                                        const double* __restrict__ vir_out, int ld,
+                                                        const int4* __restrict__ offsets,
+                                                        const int* __restrict__ pt_off, int comp, int cap,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[kPartials][256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int4 o0 = offsets[pt_off[c]], o1 = offsets[pt_off[c + 1]];
+    int beg = comp == 1 ? o0.y : o0.w, end = comp == 1 ? o1.y : o1.w;
+    beg = min(beg, cap);
+    end = min(end, cap);
+    double acc[kPartials];
+    for (int q = 0; q < kPartials; ++q) acc[q] = 0.0;
+    for (int i = beg + t; i < end; i += 256) {
+        acc[0] += e_out[i];
+        if (vir_out)
+            for (int q = 0; q < 6; ++q) acc[1 + q] += vir_out[(size_t)q * ld + i];
+    }
+    for (int q = 0; q < kPartials; ++q) sh[q][t] = acc[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < kPartials; ++q) sh[q][t] += sh[q][t + s];
+        __syncthreads();
+    }
+    if (t < kPartials) out[(size_t)c * kPartials + t] = sh[t][0];
+}
+
 size_t nmers_lds_bytes(int match_cap) { return (size_t)match_cap * kMatchDoubles * sizeof(double); }
 
-hipError_t launch_nmers(const double* cx, const double* cy, const double* cz, const int* cell, const int* cell_start,
-                        const int* cell_pts, int np, int nslot, const Grid* G, const NmerCtx& N, double c2sq,
-                        double c3sq, int pass, int4* counts, const int4* offsets, int* list2, int* eval2,
-                        int* list3, int* eval3, int4 cap, int match_cap, int* overflow, double* gscratch,
+hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int match_cap, double* gscratch,
                         int nblocks, hipStream_t st) {
-    if (np <= 0) return hipSuccess;
-    const int nb = gscratch ? (nblocks < np ? nblocks : np) : np;
+    if (A.np <= 0) return hipSuccess;
+    const int nb = gscratch ? (nblocks < A.np ? nblocks : A.np) : A.np;
     const size_t lds = gscratch ? 0 : nmers_lds_bytes(match_cap);
-    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, cx, cy, cz, cell, cell_start, cell_pts, np, nslot, G, N,
-                       c2sq, c3sq, pass, counts, offsets, list2, eval2, list3, eval3, cap, match_cap, overflow,
-                       gscratch);
+    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, A, N, pass, cap, match_cap, gscratch);
     return hipGetLastError();
 }
 

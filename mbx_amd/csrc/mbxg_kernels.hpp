@@ -7,12 +7,12 @@
 
 namespace mbxg {
 
-// Candidate grid for the neighbour search (cells in fractional coordinates under PBC,
-// Cartesian cells over the bounding box in the gas phase).
+// Candidate grid of one configuration.  Cell of a position y (fractional coordinates under
+// PBC, where the grid spans the slots' extended range [-1, 1]^3; Cartesian in the gas phase,
+// over the bounding box): floor((y - lo) * inv_w) per dimension, clamped to [0, n).  Every cell
+// is at least one cutoff wide, so a query's matches lie in the 27 cells around its own.
 struct Grid {
     int n[3];
-    int lo_off[3];
-    int hi_off[3];
     int periodic;
     double lo[3];
     double inv_w[3];
@@ -38,54 +38,90 @@ constexpr int kMaxCellsPerDim = 64;
 constexpr int kMaxCells = kMaxCellsPerDim * kMaxCellsPerDim * kMaxCellsPerDim;
 
 __global__ void k_fix_monomers(const double* in, double* out, const int* first_atom, const int* nat, int nmon, Box B);
-__global__ void k_grid_gas(const double* xyz, const int* first_atom, const int* members, int np, double cut, Grid* G);
-__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, int np, Box B, const Grid* G,
-                        double* cx, double* cy, double* cz, int* cell);
-__global__ void k_bin_count(const int* cell, int np, int* cell_count);
-__global__ void k_scan_cells(int* cell_count, const Grid* G, int* cell_start);
-__global__ void k_bin_fill(const int* cell, int np, const int* cell_start, int* cursor, int* cell_pts);
+__global__ void k_grid_gas(const double* xyz, const int* first_atom, const int* members, const int* pt_off,
+                           const int* cell_bound, double cut, Grid* grids);
+__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, int np, Box B, const Grid* grids,
+                        const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell);
+__global__ void k_bin_count(const int* cell, int ns, int* cell_count);
+__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
+__global__ void k_bin_fill(const int* cell, int ns, const int* cell_start, int* cursor, const double* cx,
+                           const double* cy, const double* cz, int* cell_slot, double* bx, double* by, double* bz);
 __global__ void k_scan_counts(const int4* counts, int n, int4* offsets);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_int(int* p, size_t n);
 
 // The n-mer kernel (lists.hip): one wavefront per query point; pass 0 counts, pass 1 writes.
+// Points are grouped by configuration (pconf[p]); a configuration's cells start at
+// cell_base[conf] and its grid is grids[conf], so lists never cross configurations.  Slots in
+// cell order: cell_slot (slot index), bx/by/bz (coordinates), cell c holding
+// [cell_start[c], cell_start[c+1]).
+struct NmersArgs {
+    const double* cx;
+    const double* cy;
+    const double* cz;
+    const int* cell;
+    const int* cell_start;
+    const int* cell_slot;
+    const double* bx;
+    const double* by;
+    const double* bz;
+    const Grid* grids;
+    const int* pconf;
+    const int* cell_base;
+    int np;
+    double c2sq, c3sq;
+    int4* counts;
+    const int4* offsets;
+    int* list2;
+    int* eval2;
+    int* list3;
+    int* eval3;
+    int* overflow;  // largest match count seen (zeroed before pass 0)
+};
 // match_cap (a power of two) bounds the matches one query may hold.  With gscratch == nullptr
 // they live in LDS (nmers_lds_bytes(match_cap) per block, one block per query); otherwise
 // nblocks blocks stride over the queries, each with nmers_lds_bytes(match_cap) bytes of
-// gscratch.  overflow[0] receives the largest match count seen (must be zeroed before pass 0).
+// gscratch.
 size_t nmers_lds_bytes(int match_cap);
-hipError_t launch_nmers(const double* cx, const double* cy, const double* cz, const int* cell, const int* cell_start,
-                        const int* cell_pts, int np, int nslot, const Grid* G, const NmerCtx& N, double c2sq,
-                        double c3sq, int pass, int4* counts, const int4* offsets, int* list2, int* eval2,
-                        int* list3, int* eval3, int4 cap, int match_cap, int* overflow, double* gscratch,
+hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int match_cap, double* gscratch,
                         int nblocks, hipStream_t st);
 
 // Evaluation of the compacted "evaluate" lists (eval_mbpol2b.hip, eval_mbpol3b.hip): variable
 // stage, polynomial stage (poly_mbpol2b.hip, poly_mbpol3b.hip), chain-rule stage.  One lane per
 // n-mer.  The evaluated count n is read on the device from n_dev (clamped to cap): kernels are
 // launched for cap n-mers and lanes past n retire at once, so the host never waits for the
-// list builder.  e_out[t] receives n-mer t's energy; partials one kPartials record per 64
-// n-mers (partials_records(cap) records); ws is a workspace of workspace_mbpolXb(cap)
-// doubles.  ev (nullable) receives 4 events: before the variable stage, before the
+// list builder.  e_out[t] receives n-mer t's energy and, when vir_out is not null,
+// vir_out[q * ld + t] (q = xx xy xz yy yz zz, ld = cap rounded up to 64) its virial; partials
+// one kPartials record per 64 n-mers (partials_records(cap) records); ws is a workspace of
+// workspace_mbpolXb(cap) doubles.  ev (nullable) receives 4 events: before the variable stage, before the
 // polynomial, before the chain-rule stage, after it.
 inline size_t partials_records(int cap) { return ((size_t)cap + 127) / 128 * 2; }
 size_t workspace_mbpol2b(int cap);
 size_t workspace_mbpol3b(int cap);
 hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
-                          hipEvent_t* ev);
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev);
 hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* partials, double* ws, hipStream_t st,
-                          hipEvent_t* ev);
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev);
 // Polynomial stage alone: xs/gs variable-major with leading dimension ld (= cap rounded up
-// to 64); ps values; n read from n_dev as above.
+// to 64); ps values; n read from n_dev as above; `split` wavefronts (1, 2, 4 or 8) share each
+// 64 n-mers (poly_kernel.hpp).
 hipError_t launch_poly_mbpol2b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
-                               double* gs, double* ps, hipStream_t st);
+                               double* gs, double* ps, int split, hipStream_t st);
 hipError_t launch_poly_mbpol3b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
-                               double* gs, double* ps, hipStream_t st);
+                               double* gs, double* ps, int split, hipStream_t st);
 
 // Sum the partial records of min(*n_dev, n_cap) n-mers into out[kPartials] in a fixed order
 // (deterministic totals).
 __global__ void k_reduce_partials(const double* partials, const int* n_dev, int n_cap, double* out);
+
+// Per-configuration totals: block c sums e_out / vir_out (leading dimension ld) over the
+// evaluated n-mers of configuration c, [offsets[pt_off[c]].comp, offsets[pt_off[c+1]].comp)
+// clamped to cap, in a fixed order, into out[c * kPartials ...] (energy, 6 virial components;
+// zeros when vir_out is null).  comp selects the int4 component (1: evaluated dimers,
+// 3: evaluated trimers).
+__global__ void k_reduce_configs(const double* e_out, const double* vir_out, int ld, const int4* offsets,
+                                 const int* pt_off, int comp, int cap, double* out);
 
 }  // namespace mbxg

@@ -9,15 +9,9 @@
 namespace mbxg {
 
 hipError_t launch_poly_mbpol3b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
-                               double* gs, double* ps, hipStream_t st) {
-    if (cap <= 0) return hipSuccess;
-    const int nb = (cap + 63) / 64;
-    const uint64_t ca = (uint64_t)(uintptr_t)coef;
-    if (do_grads)
-        hipLaunchKernelGGL((k_poly_eg<36, poly_mbpol_3b>), dim3(nb), dim3(64), 0, st, xs, n_dev, cap, ld, ca, gs, ps);
-    else
-        hipLaunchKernelGGL((k_poly_e<36, poly_mbpol_3b_e>), dim3(nb), dim3(64), 0, st, xs, n_dev, cap, ld, ca, ps);
-    return hipGetLastError();
+                               double* gs, double* ps, int P, hipStream_t st) {
+    return launch_poly<36, 8, poly_mbpol_3b_parts, poly_mbpol_3b_e_parts>(xs, n_dev, cap, ld, coef, do_grads, gs,
+                                                                                ps, P, st);
 }
 
 }  // namespace mbxg

@@ -47,6 +47,11 @@ PROTOTYPES = {
     "mbxg_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_timings": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
+    "mbxg_set_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+    "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
+    "mbxg_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "mbxg_get_batch_results": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
 }
 
 TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
@@ -112,6 +117,33 @@ class Engine:
                                        loc.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)) if loc is not None else None))
         self.nmon = n
         self.natoms = int(nats.sum())
+        self.nconf = 1
+
+    def set_batch(self, mon_offsets):
+        """Split the monomers into configurations: configuration c = monomers
+        [mon_offsets[c], mon_offsets[c+1])."""
+        off = np.ascontiguousarray(mon_offsets, np.int32)
+        _check(lib().mbxg_set_batch(self._c, len(off) - 1, off.ctypes.data_as(_ip)))
+        self.nconf = len(off) - 1
+
+    def batch_results(self):
+        n = getattr(self, "nconf", 1)
+        e2, e3, v = np.zeros(n), np.zeros(n), np.zeros(9 * n)
+        _check(lib().mbxg_get_batch_results(self._c, _d(e2), _d(e3), _d(v)))
+        return e2, e3, v.reshape(n, 9)
+
+    def set_tuning(self, key, value):
+        _check(lib().mbxg_set_tuning(self._c, key.encode(), int(value)))
+
+    def tuning(self, key):
+        v = ctypes.c_int()
+        _check(lib().mbxg_get_tuning(self._c, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, terms=TERM_2B | TERM_3B, do_grads=True, reps=3):
+        """Pick the fastest polynomial schedule for the current inputs (see mbxg_autotune)."""
+        _check(lib().mbxg_autotune(self._c, int(terms), int(bool(do_grads)), int(reps)))
+        return {"split_2b": self.tuning("split_2b"), "split_3b": self.tuning("split_3b")}
 
     def set_box(self, box):
         if box is None or len(box) == 0:

@@ -146,27 +146,62 @@ class Schedule:
                     deg2_const=len(self.const2), level1_nodes=len(self.nodes1), fp64_ops_eg=ops)
 
     # ---------------------------------------------------------------------------------------
-    def ops(self, grad=True):
+    def partition(self, nparts):
+        """Split the schedule into nparts pieces of near-equal op count.  Items are level-2
+        subtrees, each level-1 node's own terms ('x1'), and the degree-0/1 terms ('z');
+        everything is linear in the pieces, so value and gradient are sums over parts.
+        Longest-processing-time assignment; returns {item: part}."""
+        items = []
+        for n1 in self.nodes1:
+            for n2 in self.children1.get(n1, []):
+                c = 3 + 2 * len(self.leaves2.get(n2, [])) + (1 if n2 in self.const2 else 0)
+                for n3 in self.children2.get(n2, []):
+                    c += 3 + 2 * len(self.leaves3[n3]) + (1 if n3 in self.const3 else 0)
+                items.append((c, ("n2", n2)))
+            items.append((2 + 2 * len(self.leaves1.get(n1, [])) + (1 if n1 in self.const1 else 0), ("x1", n1)))
+        items.append((1 + 2 * len(self.leaves0), ("z",)))
+        load = [0] * nparts
+        assign = {}
+        for c, it in sorted(items, key=lambda t: (-t[0], t[1])):
+            k = min(range(nparts), key=lambda j: (load[j], j))
+            assign[it] = k
+            load[k] += c
+        self.part_load = load
+        return assign
+
+    def ops(self, grad=True, part=None, assign=None):
         """Linear op list.  Each op is (text template, coefficient slot or None); the template
         contains '{c}' where the coefficient value goes.  Temporaries get unique names, so the
-        whole body lives at function scope."""
+        whole body lives at function scope.  With `part`, only the terms assigned to that part
+        (see partition) are emitted and E starts from 0."""
         o = []
         w = lambda t, q=None: o.append((t, q))
-        w("double E = %s;" % ("{c}" if self.const0 else "0.0"), self.const0[0] if self.const0 else None)
-        for q, l in self.leaves0:
-            w("E = __fma_rn({c}, x[%d], E);" % l, q)
-            if grad:
-                w("g[%d] += {c};" % l, q)
+        mine = lambda it: part is None or assign[it] == part
+        if part is not None:
+            w("double E = E0;")
+        if mine(("z",)):
+            if part is None:
+                w("double E = %s;" % ("{c}" if self.const0 else "0.0"), self.const0[0] if self.const0 else None)
+            elif self.const0:
+                w("E += {c};", self.const0[0])
+            for q, l in self.leaves0:
+                w("E = __fma_rn({c}, x[%d], E);" % l, q)
+                if grad:
+                    w("g[%d] += {c};" % l, q)
         i1 = i2 = i3 = 0
         for n1 in self.nodes1:
             p = n1[0]
+            kids = [n2 for n2 in self.children1.get(n1, []) if mine(("n2", n2))]
+            own = mine(("x1", n1))
+            if not kids and not own:
+                continue
             R1 = "R1_%d" % i1
             i1 += 1
-            if n1 in self.const1:
+            if own and n1 in self.const1:
                 w("double %s = {c};" % R1, self.const1[n1])
             else:
                 w("double %s = 0.0;" % R1)
-            for n2 in self.children1.get(n1, []):
+            for n2 in kids:
                 qv = n2[1]
                 P2, R2 = "P2_%d" % i2, "R2_%d" % i2
                 i2 += 1
@@ -202,7 +237,7 @@ class Schedule:
                 if grad:
                     w("g[%d] = __fma_rn(x[%d], %s, g[%d]);" % (qv, p, R2, qv))
                 w("%s = __fma_rn(x[%d], %s, %s);" % (R1, qv, R2, R1))
-            for q, l in self.leaves1.get(n1, []):
+            for q, l in (self.leaves1.get(n1, []) if own else []):
                 w("%s = __fma_rn({c}, x[%d], %s);" % (R1, l, R1), q)
                 if grad:
                     w("g[%d] = __fma_rn({c}, x[%d], g[%d]);" % (l, p, l), q)
@@ -211,7 +246,7 @@ class Schedule:
             w("E = __fma_rn(x[%d], %s, E);" % (p, R1))
         return o
 
-    def emit(self, fn, grad=True, region_coefs=12, region_ops=64):
+    def emit(self, fn, grad=True, region_coefs=12, region_ops=64, part=None, assign=None):
         """C++ source of the evaluation function.
 
         The op list is cut into regions of at most `region_coefs` distinct coefficients and
@@ -219,7 +254,7 @@ class Schedule:
         values) during region r-1, and a fence separates regions, so scalar-load latency is
         covered by the previous region's FMAs while only two regions' coefficients are live.
         """
-        ops = self.ops(grad)
+        ops = self.ops(grad, part, assign)
         regions = [[]]
         coefs = [[]]
         nar = 0
@@ -237,11 +272,13 @@ class Schedule:
             nar += arith
         out = []
         w = out.append
+        e0 = ", double E0" if part is not None else ""
         if grad:
-            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x, double* g) {" % fn)
-            w("    for (int i = 0; i < %d; ++i) g[i] = 0.0;" % self.nvars)
+            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x, double* g%s) {" % (fn, e0))
+            if part is None:
+                w("    for (int i = 0; i < %d; ++i) g[i] = 0.0;" % self.nvars)
         else:
-            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x) {" % fn)
+            w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x%s) {" % (fn, e0))
         name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
         for j, q in enumerate(coefs[0]):
             w("    const double %s = C[%d];" % (name(0, q), q))
@@ -249,7 +286,7 @@ class Schedule:
         for r, reg in enumerate(regions):
             # the fence consumes the previous region's last result, so no load of this or the
             # next region can be hoisted above that region's arithmetic
-            w("    MBXG_NODE_FENCE(C, %s);" % (last or "x[0]"))
+            w("    MBXG_NODE_FENCE(C, %s);" % (last or ("E0" if part is not None else "x[0]")))
             if r + 1 < len(regions):
                 for q in coefs[r + 1]:
                     w("    const double %s = C[%d];" % (name(r + 1, q), q))
@@ -277,6 +314,13 @@ def full_terms(monos):
     return [(k, m) for k, m in monos], None
 
 
+# The polynomial is cut into NPARTS pieces so that 1, 2, 4 or 8 wavefronts can share one
+# block of 64 n-mers (each wavefront evaluating NPARTS / P pieces).  A piece takes the running
+# value E0 and returns it with its own terms added; its first fence consumes E0, so no
+# arithmetic of the previous piece can sink into it.
+NPARTS = 8
+
+
 def generate(name, fn, out_dir):
     nvars, coef, monos = load_table(name)
     terms, _ = full_terms(monos)
@@ -287,7 +331,24 @@ def generate(name, fn, out_dir):
            "// Coefficient slots index the fitted coefficient vector directly (%d slots)." % len(coef),
            "#pragma once",
            "#include \"poly_support.hpp\"", ""]
-    src = "\n".join(hdr) + s.emit(fn, grad=True) + "\n" + s.emit(fn + "_e", grad=False)
+    assign = s.partition(NPARTS)
+    hdr.insert(2, "// %d parts (op-count balance: %s); <fn>_parts(w, P, ...) evaluates parts w, w+P, ... "
+                  "into g and returns their share of the value." % (NPARTS, ", ".join(str(v) for v in s.part_load)))
+    body = []
+    for grad, suffix in ((True, ""), (False, "_e")):
+        for k in range(NPARTS):
+            body.append(s.emit("%s%s_p%d" % (fn, suffix, k), grad=grad, part=k, assign=assign))
+        args, call = (("mbxg_cptr C, const double* x, double* g", "C, x, g, E") if grad
+                      else ("mbxg_cptr C, const double* x", "C, x, E"))
+        # wavefront w of P evaluates pieces w, w + P, ...: a straight sequence of uniform
+        # branches (a switch over the pieces inside a loop costs the register allocator dearly)
+        d = ["__device__ __forceinline__ double %s%s_parts(int w, int P, %s) {" % (fn, suffix, args),
+             "    double E = 0.0;"]
+        for k in range(NPARTS):
+            d.append("    if (%d %% P == w) E = %s%s_p%d(%s);" % (k, fn, suffix, k, call))
+        d += ["    return E;", "}", ""]
+        body.append("\n".join(d))
+    src = "\n".join(hdr) + "\n".join(body)
     path = os.path.join(out_dir, "poly_%s.inc" % name)
     with open(path, "w") as f:
         f.write(src)

@@ -212,3 +212,39 @@ def test_large_cutoff_gas_cluster():
     assert np.array_equal(sorted_tuples(t), sorted_tuples(oracle.clusters(3, 4.5, first, None)))
     assert len(d) == 640 * 639 // 2
     _check_against_oracle(e, xyz, None, 100.0, 4.5)
+
+
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+def test_polynomial_splits(systems, split):
+    """Every schedule of the polynomial stage (1, 2, 4 or 8 wavefronts per 64 n-mers) gives the
+    oracle's energies and gradients, with and without gradients."""
+    xyz, box, c2, c3 = _case(systems, "C1")
+    print("split", split, flush=True)
+    e = engine.water_engine(len(xyz) // 9, box, c2, c3)
+    e.set_tuning("split_2b", split)
+    e.set_tuning("split_3b", split)
+    e.set_coords(xyz)
+    e.compute()
+    e2, e3, g, v = e.results()
+    r = oracle.water_system(xyz, box, c2, c3)
+    assert e2 == pytest.approx(r["e2b"], rel=E_REL)
+    assert e3 == pytest.approx(r["e3b"], rel=E_REL)
+    assert np.abs(g - r["grad"]).max() <= G_ABS
+    e.compute(do_grads=False)
+    f2, f3, _, _ = e.results(grad=False)
+    assert f2 == pytest.approx(r["e2b"], rel=1e-10)
+    assert f3 == pytest.approx(r["e3b"], rel=1e-10)
+
+
+def test_autotune_keeps_results(systems):
+    xyz, box, c2, c3 = _case(systems, "C1")
+    e = _run(xyz, box, c2, c3)
+    before = e.results()
+    picked = e.autotune(reps=1)
+    assert picked["split_2b"] in (1, 2, 4, 8) and picked["split_3b"] in (1, 2, 4, 8)
+    after = e.results()
+    assert after[0] == pytest.approx(before[0], rel=1e-12)
+    assert after[1] == pytest.approx(before[1], rel=1e-12)
+    assert np.abs(after[2] - before[2]).max() <= 1e-10
+    with pytest.raises(engine.MbxgError):
+        e.set_tuning("split_3b", 3)
