@@ -121,6 +121,13 @@ int mbxg_stream(mbxg_ctx* ctx, void** stream);
 int mbxg_set_timing(mbxg_ctx* ctx, int on);
 int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
 
+/* HIP graph mode (off by default).  When on, a compute in steady state (buffers already sized by
+ * an earlier compute of the same terms, stage timing off) replays a captured HIP graph of the
+ * whole pipeline instead of launching its ~25 kernels one by one; the graph is captured on first
+ * use and recaptured after anything that changes sizes, box, cutoffs or schedule.  Results are
+ * read and checked exactly as in eager mode.  No counterpart in the reference. */
+int mbxg_set_graph(mbxg_ctx* ctx, int on);
+
 /* Schedule parameters.  "split_2b" / "split_3b": wavefronts (1, 2, 4 or 8) that share each
  * block of 64 dimers / trimers in the polynomial stage, each evaluating 1/split of the
  * polynomial (defaults 4 and 2).  mbxg_autotune times every split of both orders on the

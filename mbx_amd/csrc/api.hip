@@ -142,6 +142,11 @@ struct mbxg_ctx {
     bool evaluated = false; // the last compute evaluated energies (not lists only)
     bool inputs_changed = false;  // coordinates, box, cutoffs or coefficients set since the last compute
     int last_terms = 0, last_grads = 0;
+    // HIP graph replay of the steady-state pipeline (mbxg_set_graph)
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int g_terms = -1, g_grads = -1;       // what gexec computes
+    int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
@@ -229,6 +234,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_overflow.release();
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     (void)hipEventDestroy(c->fork);
     (void)hipEventDestroy(c->join);
     (void)hipStreamDestroy(c->stream2);
@@ -620,6 +626,13 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     return 0;
 }
 
+void drop_graph(mbxg_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_terms = c->g_grads = -1;
+    c->eager_terms = c->eager_grads = -1;
+}
+
 // One compute: lists, then the evaluation of the requested terms.
 int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     hipStream_t st = c->stream;
@@ -694,7 +707,10 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev) HIPCHK(hipEventRecord(ev[12], st));
     c->have_results = true;
     c->checked = sync || !L.built;
-    if (sync) c->need_sync = false;
+    if (sync) {
+        c->need_sync = false;
+        drop_graph(c);  // buffers may have moved
+    }
     return MBXG_OK;
 }
 
@@ -740,7 +756,55 @@ extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     c->last_terms = terms;
     c->last_grads = do_grads;
     c->inputs_changed = false;
-    return run(c, terms, do_grads, c->need_sync);
+    const bool steady = !c->need_sync && !c->timing;
+    if (!(c->use_graph && steady && c->eager_terms == terms && c->eager_grads == do_grads)) {
+        const bool sync = c->need_sync;
+        if (int r = run(c, terms, do_grads, sync)) return r;
+        if (!c->timing) {
+            c->eager_terms = terms;
+            c->eager_grads = do_grads;
+        }
+        return MBXG_OK;
+    }
+    // Steady state with the buffers already sized by an eager compute of the same terms:
+    // capture the pipeline once, then replay it (no launches from the host but one).
+    hipStream_t st = c->stream;
+    if (!c->gexec || c->g_terms != terms || c->g_grads != do_grads) {
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int r = run(c, terms, do_grads, false);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (r) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess) return fail(MBXG_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(MBXG_EHIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        c->g_terms = terms;
+        c->g_grads = do_grads;
+    }
+    HIPCHK(hipGraphLaunch(c->gexec, st));
+    c->have_results = true;
+    c->checked = !c->L.built;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_set_graph(mbxg_ctx* c, int on) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_set_graph: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->use_graph = on != 0;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_terms = c->g_grads = -1;
+    return MBXG_OK;
 }
 
 extern "C" int mbxg_set_timing(mbxg_ctx* c, int on) {
@@ -783,6 +847,9 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
         c->split3 = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_terms = c->g_grads = -1;
     return MBXG_OK;
 }
 

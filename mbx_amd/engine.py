@@ -49,6 +49,7 @@ PROTOTYPES = {
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
     "mbxg_set_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+    "mbxg_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
     "mbxg_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mbxg_get_batch_results": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
@@ -131,6 +132,10 @@ class Engine:
         e2, e3, v = np.zeros(n), np.zeros(n), np.zeros(9 * n)
         _check(lib().mbxg_get_batch_results(self._c, _d(e2), _d(e3), _d(v)))
         return e2, e3, v.reshape(n, 9)
+
+    def set_graph(self, on=True):
+        """Replay the steady-state pipeline as one captured HIP graph (see mbxg_set_graph)."""
+        _check(lib().mbxg_set_graph(self._c, int(bool(on))))
 
     def set_tuning(self, key, value):
         _check(lib().mbxg_set_tuning(self._c, key.encode(), int(value)))

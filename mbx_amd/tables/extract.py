@@ -6,7 +6,7 @@ kernel generator and the oracle read.  Nothing here imports or executes
 reference code: the generated energy-only polynomial routines are parsed as
 straight-line arithmetic and expanded symbolically.
 
-What is extracted, per polynomial:
+What is extracted, per polynomial (MB-pol 2B/3B and the MB-nrg CH4 families):
   * ``coef``  -- the fitted linear coefficients ``thefit[]`` of the holder
     (e.g. src/potential/2b/x2b-v9x.cpp:164-1318, src/potential/3b/x3b-v2x.cpp:80-1244);
   * ``monomials`` -- the expansion  E(x) = sum_k a_k * sum_{m in S_k} prod_{i in m} x_i
@@ -14,6 +14,8 @@ What is extracted, per polynomial:
     src/potential/3b/poly-3b-v2x-nogrd.cpp:39).  Every monomial of those
     routines belongs to exactly one coefficient with weight 1; the script
     checks this and aborts otherwise.
+and per MB-nrg holder (``<name>.holder.json``): its variables as data (form, k, d, sites),
+its lone-pair sites and switching radii.
 
 Usage:  python mbx_amd/tables/extract.py [/root/reference]
 """
@@ -29,7 +31,61 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 POLYS = [
     ("mbpol_2b", "src/potential/2b/poly-2b-v6x-nogrd.cpp", "src/potential/2b/x2b-v9x.cpp", 31, 1153),
     ("mbpol_3b", "src/potential/3b/poly-3b-v2x-nogrd.cpp", "src/potential/3b/x3b-v2x.cpp", 36, 1163),
+    # MB-nrg CH4 families (energy-only routines; coefficients of the holders' single parameter set)
+    ("ch4h2o_2b", "src/potential/2b/poly_2b_A1B2Z2_C1D4_deg3_exp0_v1.cpp",
+     "src/potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp", 38, 603),
+    ("ch4ch4_2b", "src/potential/2b/poly_2b_A1B4_A1B4_deg4_exp0_v1.cpp",
+     "src/potential/2b/x2b_A1B4_A1B4_deg4_exp0_v1x.cpp", 45, 1345),
+    ("ch4h2o2_3b", "src/potential/3b/poly_3b_A1B4_C1D2_C1D2_deg3_nograd_v1.cpp",
+     "src/potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp", 55, 1254),
 ]
+
+# Holders of the MB-nrg families: argument order of the monomers, and how the variables are
+# built (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:821-949, x2b_A1B4_A1B4_deg4_exp0_v1x.cpp:1470-1600,
+# mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1336-1480).
+HOLDERS = [
+    ("ch4h2o_2b", "src/potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp", ["h2o", "ch4"], [3, 5]),
+    ("ch4ch4_2b", "src/potential/2b/x2b_A1B4_A1B4_deg4_exp0_v1x.cpp", ["ch4", "ch4"], [5, 5]),
+    ("ch4h2o2_3b", "src/potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp", ["ch4", "h2o", "h2o"], [5, 3, 3]),
+]
+
+
+def parse_holder(text, mons, nats):
+    """Variables of a generated MB-nrg holder, as data: sites (real atoms by position in the
+    concatenated coordinates, plus the water's two lone-pair sites when the holder builds them),
+    and for each variable its form (exp0: exp(k (d - r)); coul0: exp(k (d - r)) / r; exp:
+    exp(-k r)), k, d and its two sites; the switching radii."""
+    src = re.sub(r"//[^\n]*", "", text)
+    ev = src[src.index("::eval("):]
+    ev = ev[:ev.index("::eval(", 10)] if "::eval(" in ev[10:] else ev
+    sites = {}
+    for name, off in re.findall(r"const double\*\s*(?:coords_)?([A-Z]_\d+_[a-z])\s*=\s*(?:xcrd|xyz\.data\(\))\s*\+\s*(\d+);", ev):
+        sites[name] = int(off) // 3
+    nreal = sum(nats)
+    vsite = None
+    m = re.search(r"m\.setup\(([A-Z]_\d+_[a-z]),\s*(\w+),\s*(\w+),\s*([A-Z]_\d+_[a-z]),\s*([A-Z]_\d+_[a-z])\)", ev)
+    if m:
+        w = {k: float(v) for k, v in re.findall(r"double\s+(w12|w13|wcross)\s*=\s*([-+0-9.eE]+);", ev)}
+        sites[m.group(4)] = nreal
+        sites[m.group(5)] = nreal + 1
+        vsite = {"oxygen_site": sites[m.group(1)], "in_plane": w[m.group(2)], "out_of_plane": w[m.group(3)]}
+    # parameters of the holder's (single) parameter set
+    ctor = src[:src.index("::f_switch")]
+    par = {k: float(v) for k, v in re.findall(r"\b(m_\w+)\s*=\s*([-+0-9.eE]+)\s*;", ctor)}
+    vars_ = {}
+    for idx, form, args in re.findall(r"(?:v|xs)\[(\d+)\]\s*=\s*(?:vr|vs)\[\d+\]\.(v_exp0|v_coul0|v_exp)\(([^)]*)\);", ev):
+        a = [t.strip() for t in args.split(",")]
+        if form == "v_exp":
+            d, k, s1, s2 = 0.0, par[a[0]], a[1], a[2]
+        else:
+            d, k, s1, s2 = par[a[0]], par[a[1]], a[2], a[3]
+        s1, s2 = s1.replace("coords_", ""), s2.replace("coords_", "")
+        vars_[int(idx)] = [form[2:], k, d, sites[s1], sites[s2]]
+    nv = max(vars_) + 1
+    assert sorted(vars_) == list(range(nv))
+    sw = {"inner": par.get("m_r2i", par.get("m_ri")), "outer": par.get("m_r2f", par.get("m_ro"))}
+    return {"monomers": mons, "atoms": nats, "nreal": nreal, "nsites": nreal + (2 if vsite else 0),
+            "vsite": vsite, "switch": sw, "vars": [vars_[i] for i in range(nv)]}
 
 
 class Poly:
@@ -126,8 +182,45 @@ def expand(text):
     return result
 
 
+def expand_tarray(text):
+    """Routines split into helpers fmgN(x, a, t) that fill a temporary array t[]
+    (src/potential/2b/poly_2b_A1B4_A1B4_deg4_exp0_v1.cpp:44982): eval() calls them in order
+    and returns an expression of t[]."""
+    src = re.sub(r"//[^\n]*", "", text)
+    m = re.search(r"double\s+\w+::eval\s*\(\s*const double a\[\d+\]\s*,\s*const double x\[\d+\]\s*\)\s*\{", src)
+    body = src[m.end():src.index("}", m.end())]
+    calls = re.findall(r"(fmg\d+)\s*\(", body)
+    ret = re.search(r"return\s*(.*?);", body, re.S).group(1)
+    T = {}
+
+    class _T:
+        def __getitem__(self, i):
+            return T[int(i)]
+
+    env = {"a": _Coef(), "x": _Var(), "t": _T()}
+    for fn in calls:
+        fm = re.search(r"void\s+%s\s*\([^)]*\)\s*\{" % fn, src)
+        i = j = fm.end()
+        depth = 1
+        while depth:
+            depth += src[j] == "{"
+            depth -= src[j] == "}"
+            j += 1
+        for stmt in " ".join(src[i:j - 1].split()).split(";"):
+            stmt = stmt.strip()
+            if stmt:
+                lhs, expr = stmt.split("=", 1)
+                idx = int(re.match(r"t\[(\d+)\]", lhs.strip()).group(1))
+                T[idx] = eval(compile(ast.parse(expr.strip(), mode="eval"), "<poly>", "eval"), env)
+    return eval(compile(ast.parse(" ".join(ret.split()), mode="eval"), "<poly>", "eval"), env)
+
+
 def parse_thefit(text, ncoef):
+    """Fitted linear coefficients: ``thefit[] = {...}`` (MB-pol holders) or the first
+    ``coefficients = std::vector<double>{...}`` (MB-nrg holders, one parameter set each)."""
     m = re.search(r"thefit\[\]\s*=\s*\{(.*?)\};", text, re.S)
+    if not m:
+        m = re.search(r"coefficients\s*=\s*std::vector<double>\s*\{(.*?)\};", text, re.S)
     body = re.sub(r"//[^\n]*", "", m.group(1))
     vals = [float(v) for v in body.replace(",", " ").split()]
     if len(vals) != ncoef:
@@ -166,8 +259,16 @@ def write_table(path, name, nvars, coef, poly):
 
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    import json
+    for name, holder_path, mons, nats in HOLDERS:
+        h = parse_holder(open(os.path.join(ref, holder_path)).read(), mons, nats)
+        h["name"] = name
+        with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
+            json.dump(h, f, indent=1)
+        print("%s holder: %d variables, %d sites" % (name, len(h["vars"]), h["nsites"]))
     for name, poly_path, holder_path, nvars, ncoef in POLYS:
-        poly = expand(open(os.path.join(ref, poly_path)).read())
+        text = open(os.path.join(ref, poly_path)).read()
+        poly = expand_tarray(text) if "fmg1(" in text else expand(text)
         coef = parse_thefit(open(os.path.join(ref, holder_path)).read(), ncoef)
         n = write_table(os.path.join(HERE, name + ".tbl"), name, nvars, coef, poly)
         print("%s: %d monomials, %d coefficients" % (name, n, len(coef)))

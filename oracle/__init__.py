@@ -67,7 +67,48 @@ class Table:
                           _i(self.mono_deg), _i(self.mono_idx))
 
 
+class _OrcHolder(ctypes.Structure):
+    _fields_ = [("nmon", ctypes.c_int), ("nat", ctypes.c_int * 3), ("nreal", ctypes.c_int),
+                ("nsites", ctypes.c_int), ("vsite", ctypes.c_int), ("osite", ctypes.c_int),
+                ("in_plane", ctypes.c_double), ("out_of_plane", ctypes.c_double), ("r_in", ctypes.c_double),
+                ("r_out", ctypes.c_double), ("nvars", ctypes.c_int), ("form", _ip), ("k", _dp), ("d", _dp),
+                ("s1", _ip), ("s2", _ip)]
+
+
+FORMS = {"exp0": 0, "coul0": 1, "exp": 2}
+NRG = ("ch4h2o_2b", "ch4ch4_2b", "ch4h2o2_3b")
+
+
+class Holder:
+    """An MB-nrg holder description (mbx_amd/tables/<name>.holder.json)."""
+
+    def __init__(self, name):
+        import json
+
+        h = json.load(open(os.path.join(TABLES, name + ".holder.json")))
+        self.name, self.desc = name, h
+        v = h["vars"]
+        self.form = np.array([FORMS[t[0]] for t in v], np.int32)
+        self.k = np.array([t[1] for t in v], np.float64)
+        self.d = np.array([t[2] for t in v], np.float64)
+        self.s1 = np.array([t[3] for t in v], np.int32)
+        self.s2 = np.array([t[4] for t in v], np.int32)
+        nat = (ctypes.c_int * 3)(*(h["atoms"] + [0] * (3 - len(h["atoms"]))))
+        vs = h["vsite"] or {"oxygen_site": 0, "in_plane": 0.0, "out_of_plane": 0.0}
+        self.c = _OrcHolder(len(h["monomers"]), nat, h["nreal"], h["nsites"], int(h["vsite"] is not None),
+                            vs["oxygen_site"], vs["in_plane"], vs["out_of_plane"], h["switch"]["inner"],
+                            h["switch"]["outer"], len(v), _i(self.form), _d(self.k), _d(self.d), _i(self.s1),
+                            _i(self.s2))
+
+
 _TABLES = {}
+_HOLDERS = {}
+
+
+def holder(name):
+    if name not in _HOLDERS:
+        _HOLDERS[name] = Holder(name)
+    return _HOLDERS[name]
 
 
 def table(name):
@@ -109,6 +150,11 @@ def lib():
                                    _ip, ctypes.c_int, _ip, ctypes.c_long]
         L.orc_water_system.argtypes = [P, P, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                        _dp, _dp, _lp, _ip, ctypes.c_long, _ip, ctypes.c_long, _dp, _dp]
+        H = ctypes.POINTER(_OrcHolder)
+        L.orc_nrg_holder.restype = ctypes.c_double
+        L.orc_nrg_holder.argtypes = [H, P, _dp, ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp, _ip]
+        L.orc_system.argtypes = [P, P, ctypes.POINTER(H), ctypes.POINTER(P), ctypes.c_int, _ip, _ip, _ip, _dp, _dp,
+                                 ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp, _lp]
         _LIB = L
     return _LIB
 
@@ -232,6 +278,79 @@ def water_system(xyz, box, cutoff2b, cutoff3b, lists=False, per_nmer=False):
         res["e_dimer"] = ed[:cap2]
         res["e_trimer"] = et[:cap3]
     return res
+
+
+def nrg(name, mons, a=None):
+    """One n-mer through an MB-nrg holder: mons in the holder's monomer order (already imaged).
+    Returns (E, [grad per monomer], virial(9), evaluated)."""
+    h = holder(name)
+    tab = table(name)
+    ms = [_arr(m).reshape(-1).copy() for m in mons]
+    gs = [np.zeros_like(m) for m in ms]
+    vir = np.zeros(9)
+    ev = ctypes.c_int()
+    mp = (_dp * 3)(*[_d(m) for m in ms], *([None] * (3 - len(ms))))
+    gp = (_dp * 3)(*[_d(g) for g in gs], *([None] * (3 - len(gs))))
+    aa = _arr(a) if a is not None else None
+    e = lib().orc_nrg_holder(ctypes.byref(h.c), ctypes.byref(tab.c), _d(aa), mp, gp, _d(vir), ctypes.byref(ev))
+    return e, gs, vir, ev.value
+
+
+KIND_NAT = {0: 3, 1: 5}
+
+
+def system(xyz, kinds, box, cutoff2b, cutoff3b, tags=None, families=NRG):
+    """2B+3B energy/gradient of a mixed H2O (kind 0) / CH4 (kind 1) system: 3 doubles per atom,
+    monomers in the given order; tags default to each monomer's first-atom index."""
+    kinds = np.ascontiguousarray(kinds, np.int32)
+    nat = np.array([KIND_NAT[int(k)] for k in kinds], np.int32)
+    first = np.concatenate([[0], np.cumsum(nat)[:-1]]).astype(np.int32)
+    tag = first.copy() if tags is None else np.ascontiguousarray(tags, np.int32)
+    xyz = _arr(xyz).reshape(-1).copy()
+    b = _arr(box) if box is not None and len(box) else None
+    e2, e3 = ctypes.c_double(), ctypes.c_double()
+    grad = np.zeros(len(xyz))
+    vir = np.zeros(9)
+    counts = np.zeros(4, np.int64)
+    H = ctypes.POINTER(_OrcHolder)
+    P = ctypes.POINTER(_OrcPoly)
+    hs = (H * 3)(*[ctypes.pointer(holder(n).c) if n in families else H() for n in NRG])
+    ps = (P * 3)(*[ctypes.pointer(table(n).c) if n in families else P() for n in NRG])
+    rc = lib().orc_system(ctypes.byref(table("mbpol_2b").c), ctypes.byref(table("mbpol_3b").c), hs, ps, len(kinds),
+                          _i(kinds), _i(nat), _i(tag), _d(xyz), _d(b), cutoff2b, cutoff3b, ctypes.byref(e2),
+                          ctypes.byref(e3), _d(grad), _d(vir), counts.ctypes.data_as(_lp))
+    if rc:
+        raise ValueError("orc_system failed (%d)" % rc)
+    return dict(e2b=e2.value, e3b=e3.value, grad=grad, virial=vir, xyz=xyz, n2=int(counts[0]),
+                n2_eval=int(counts[1]), n3=int(counts[2]), n3_eval=int(counts[3]))
+
+
+def abs_table(name):
+    """The table `name` with every coefficient replaced by its magnitude: evaluated through the
+    holders it gives sum_n sw_n sum_k |a_k| M_k(x_n), the scale of the cancelling terms behind
+    an energy (its floating-point conditioning)."""
+    key = name + "|abs"
+    if key not in _TABLES:
+        t = table(name)
+        a = Table.__new__(Table)
+        a.__dict__.update(t.__dict__)
+        a.coef = np.abs(t.coef)
+        a.c = _OrcPoly(t.nvars, t.ncoef, t.nmono, _d(a.coef), _i(t.mono_k), _i(t.mono_deg), _i(t.mono_idx))
+        _TABLES[key] = a
+    return _TABLES[key]
+
+
+def water_conditioning(xyz, box, cutoff2b, cutoff3b):
+    """(E2_abs, E3_abs): the 2B / 3B energies with |coefficients| (see abs_table)."""
+    xyz = _arr(xyz).reshape(-1).copy()
+    nmon = len(xyz) // 9
+    b = _arr(box) if box is not None and len(box) else None
+    e2, e3 = ctypes.c_double(), ctypes.c_double()
+    grad, vir, counts = np.zeros(9 * nmon), np.zeros(9), np.zeros(4, np.int64)
+    lib().orc_water_system(ctypes.byref(abs_table("mbpol_2b").c), ctypes.byref(abs_table("mbpol_3b").c), nmon,
+                           _d(xyz), _d(b), cutoff2b, cutoff3b, ctypes.byref(e2), ctypes.byref(e3), _d(grad),
+                           _d(vir), counts.ctypes.data_as(_lp), None, 0, None, 0, None, None)
+    return e2.value, e3.value
 
 
 def fix_first(xyz, box):

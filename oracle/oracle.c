@@ -133,6 +133,42 @@ static void msite_grads(const double* oh1, const double* oh2, const double* g1, 
     }
 }
 
+/* The same lone-pair construction with explicit gammas: the MB-nrg CH4-H2O holder builds its
+ * Z sites with MB-pol's values (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:119-160, 866-872). */
+static void msite_setup_w(const double* ohh, double ipg, double oopg, double* oh1, double* oh2, double* x1,
+                          double* x2) {
+    for (int i = 0; i < 3; ++i) {
+        oh1[i] = ohh[i + 3] - ohh[i];
+        oh2[i] = ohh[i + 6] - ohh[i];
+    }
+    const double v[3] = {oh1[1] * oh2[2] - oh1[2] * oh2[1], oh1[2] * oh2[0] - oh1[0] * oh2[2],
+                         oh1[0] * oh2[1] - oh1[1] * oh2[0]};
+    for (int i = 0; i < 3; ++i) {
+        const double in_plane = ohh[i] + 0.5 * ipg * (oh1[i] + oh2[i]);
+        const double out_of_plane = oopg * v[i];
+        x1[i] = in_plane + out_of_plane;
+        x2[i] = in_plane - out_of_plane;
+    }
+}
+
+static void msite_grads_w(const double* oh1, const double* oh2, double ipg, double oopg, const double* g1,
+                          const double* g2, double* grd) {
+    const double gm[3] = {g1[0] - g2[0], g1[1] - g2[1], g1[2] - g2[2]};
+    const double t1[3] = {oh2[1] * gm[2] - oh2[2] * gm[1], oh2[2] * gm[0] - oh2[0] * gm[2],
+                          oh2[0] * gm[1] - oh2[1] * gm[0]};
+    const double t2[3] = {oh1[1] * gm[2] - oh1[2] * gm[1], oh1[2] * gm[0] - oh1[0] * gm[2],
+                          oh1[0] * gm[1] - oh1[1] * gm[0]};
+    for (int i = 0; i < 3; ++i) {
+        const double gsum = g1[i] + g2[i];
+        const double in_plane = 0.5 * ipg * gsum;
+        const double gh1 = in_plane + oopg * t1[i];
+        const double gh2 = in_plane - oopg * t2[i];
+        grd[i + 0] += gsum - (gh1 + gh2);
+        grd[i + 3] += gh1;
+        grd[i + 6] += gh2;
+    }
+}
+
 /* f_switch (x2b-v9x.cpp:1349-1362). */
 static double switch2(double r, double* g) {
     if (r > r2f) {
@@ -717,6 +753,264 @@ int orc_water_system(const orc_poly* p2, const orc_poly* p3, int nmon, double* x
     free(fi);
     free(loc);
     free(tag);
+    free(first);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* MB-nrg holders (see oracle.h). */
+
+/* Variable and its dr derivative factor: value, and gg with d(value)/d(p1) = gg * (p1 - p2)
+ * (tools/variable.cpp:49-129). */
+static double nrg_var(int form, double k, double d, const double* p1, const double* p2, double* gvec) {
+    gvec[0] = p1[0] - p2[0];
+    gvec[1] = p1[1] - p2[1];
+    gvec[2] = p1[2] - p2[2];
+    const double rsq = gvec[0] * gvec[0] + gvec[1] * gvec[1] + gvec[2] * gvec[2];
+    const double r = sqrt(rsq);
+    double val, gg;
+    if (form == 0) { /* v_exp0 */
+        val = exp(k * (d - r));
+        gg = -k * val / r;
+    } else if (form == 1) { /* v_coul0 */
+        const double e1 = exp(k * (d - r));
+        const double rinv = 1.0 / r;
+        val = e1 * rinv;
+        gg = -(k + rinv) * val * rinv;
+    } else { /* v_exp */
+        val = exp(k * (-r));
+        gg = -k * val / r;
+    }
+    gvec[0] *= gg;
+    gvec[1] *= gg;
+    gvec[2] *= gg;
+    return val;
+}
+
+/* f_switch of the MB-nrg holders (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:804-817,
+ * mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1319-1333). */
+static double nrg_switch(double ri, double ro, double r, double* g) {
+    if (r > ro) {
+        *g = 0.0;
+        return 0.0;
+    } else if (r > ri) {
+        const double t1 = M_PI / (ro - ri);
+        const double x = (r - ri) * t1;
+        *g = -sin(x) * t1 / 2.0;
+        return (1.0 + cos(x)) / 2.0;
+    }
+    *g = 0.0;
+    return 1.0;
+}
+
+double orc_nrg_holder(const orc_holder* h, const orc_poly* p, const double* a, const double* const* mons,
+                      double* const* grads, double* vir, int* evaluated) {
+    double xs[3 * 16], xg[3 * 16];
+    int off[4] = {0, 0, 0, 0};
+    for (int m = 0; m < h->nmon; ++m) off[m + 1] = off[m] + h->nat[m];
+    for (int m = 0; m < h->nmon; ++m) memcpy(xs + 3 * off[m], mons[m], sizeof(double) * 3 * h->nat[m]);
+    *evaluated = 0;
+    /* first-atom distances */
+    double dd[3][3], r[3];
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    const int npair = h->nmon == 2 ? 1 : 3;
+    for (int q = 0; q < npair; ++q) {
+        const double* u = xs + 3 * off[pa[q]];
+        const double* v = xs + 3 * off[pb[q]];
+        dd[q][0] = u[0] - v[0];
+        dd[q][1] = u[1] - v[1];
+        dd[q][2] = u[2] - v[2];
+        r[q] = sqrt(dd[q][0] * dd[q][0] + dd[q][1] * dd[q][1] + dd[q][2] * dd[q][2]);
+    }
+    if (h->nmon == 2 ? r[0] > h->r_out : (r[0] > h->r_out && r[1] > h->r_out && r[2] > h->r_out)) return 0.0;
+    *evaluated = 1;
+    double oh1[3], oh2[3];
+    if (h->vsite) msite_setup_w(xs + 3 * h->osite, h->in_plane, h->out_of_plane, oh1, oh2, xs + 3 * h->nreal,
+                                xs + 3 * h->nreal + 3);
+    double v[64], gvv[64][3];
+    for (int j = 0; j < h->nvars; ++j)
+        v[j] = nrg_var(h->form[j], h->k[j], h->d[j], xs + 3 * h->s1[j], xs + 3 * h->s2[j], gvv[j]);
+    double g[64];
+    const double P = orc_poly_eval(p, a, v, g);
+    double sw, s[3], gs[3];
+    if (h->nmon == 2) {
+        s[0] = nrg_switch(h->r_in, h->r_out, r[0], &gs[0]);
+        sw = s[0];
+    } else {
+        for (int q = 0; q < 3; ++q) s[q] = nrg_switch(h->r_in, h->r_out, r[q], &gs[q]);
+        sw = s[0] * s[1] + s[0] * s[2] + s[1] * s[2];
+        for (int j = 0; j < h->nvars; ++j) g[j] *= sw; /* 3B scales the variable gradients first */
+    }
+    memset(xg, 0, sizeof(xg));
+    for (int j = 0; j < h->nvars; ++j)
+        for (int c = 0; c < 3; ++c) {
+            const double t = g[j] * gvv[j][c];
+            xg[3 * h->s1[j] + c] += t;
+            xg[3 * h->s2[j] + c] -= t;
+        }
+    if (h->vsite) msite_grads_w(oh1, oh2, h->in_plane, h->out_of_plane, xg + 3 * h->nreal, xg + 3 * h->nreal + 3,
+                                xg + 3 * h->osite);
+    double E;
+    if (h->nmon == 2) {
+        for (int i = 0; i < 3 * h->nreal; ++i) xg[i] *= sw;
+        const double gsw = gs[0] * (P / r[0]);
+        E = P * sw;
+        for (int c = 0; c < 3; ++c) {
+            const double t = gsw * dd[0][c];
+            xg[c] += t;
+            xg[3 * off[1] + c] -= t;
+        }
+    } else {
+        const double g12 = gs[0] * ((1.0 * s[1] + 1.0 * s[2]) * P / r[0]);
+        const double g13 = gs[1] * ((s[0] * 1.0 + 1.0 * s[2]) * P / r[1]);
+        const double g23 = gs[2] * ((s[0] * 1.0 + s[1] * 1.0) * P / r[2]);
+        E = P * sw;
+        for (int c = 0; c < 3; ++c) {
+            xg[c] += 0.0 + (g12 * dd[0][c]) + (g13 * dd[1][c]);
+            xg[3 * off[1] + c] += 0.0 - (g12 * dd[0][c]) + (g23 * dd[2][c]);
+            xg[3 * off[2] + c] += 0.0 - (g13 * dd[1][c]) - (g23 * dd[2][c]);
+        }
+    }
+    for (int m = 0; m < h->nmon; ++m)
+        if (grads && grads[m])
+            for (int i = 0; i < 3 * h->nat[m]; ++i) grads[m][i] += xg[3 * off[m] + i];
+    if (vir) {
+        const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2}, slot[6] = {0, 1, 2, 4, 5, 8};
+        for (int q = 0; q < 6; ++q) {
+            double acc = 0.0;
+            for (int at = 0; at < h->nreal; ++at) acc += -xs[3 * at + va[q]] * xg[3 * at + vb[q]];
+            vir[slot[q]] += acc;
+        }
+        vir[3] = vir[1];
+        vir[6] = vir[2];
+        vir[7] = vir[5];
+    }
+    return E;
+}
+
+/* Order of an n-mer's monomers for its holder (energy2b.cpp:243-254, 323-325;
+ * energy3b.cpp:185-216).  Returns the family (0 mbpol 2B, 1 ch4h2o 2B, 2 ch4ch4 2B, 3 mbpol 3B,
+ * 4 ch4h2o2 3B) or -1 when the n-mer has no term; perm[m] = list position of holder slot m. */
+static int nmer_family(int order, const int* kinds, int* perm) {
+    if (order == 2) {
+        perm[0] = 0;
+        perm[1] = 1;
+        if (kinds[0] == 0 && kinds[1] == 0) return 0;
+        if (kinds[0] == 1 && kinds[1] == 1) return 2;
+        if (kinds[0] == 0 && kinds[1] == 1) return 1;
+        if (kinds[0] == 1 && kinds[1] == 0) {
+            perm[0] = 1;
+            perm[1] = 0;
+            return 1;
+        }
+        return -1;
+    }
+    /* the exchange network of e3b::get_3b_energy on the names "ch4" < "h2o" */
+    int k[3] = {kinds[0], kinds[1], kinds[2]}, ix[3] = {0, 1, 2}, t;
+#define SWP(i, j) (t = k[i], k[i] = k[j], k[j] = t, t = ix[i], ix[i] = ix[j], ix[j] = t)
+    if (k[2] < k[0] && k[2] < k[1]) SWP(0, 2);
+    if (k[0] > k[1] && k[0] > k[2])
+        SWP(0, 2);
+    else if (k[1] > k[0] && k[1] > k[2])
+        SWP(1, 2);
+    if (k[0] > k[1]) SWP(0, 1);
+#undef SWP
+    perm[0] = ix[0];
+    perm[1] = ix[1];
+    perm[2] = ix[2];
+    if (k[0] == 0 && k[1] == 0 && k[2] == 0) return 3;
+    if (k[0] == 1 && k[1] == 0 && k[2] == 0) return 4;
+    return -1;
+}
+
+int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* hol, const orc_poly* const* hp,
+               int nmon, const int* kind, const int* nat, const int* tag, double* xyz, const double* box,
+               double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
+               long* counts) {
+    double bi[9];
+    const int pbc = box != NULL;
+    int* fi = (int*)malloc(sizeof(int) * nmon);
+    int* loc = (int*)malloc(sizeof(int) * nmon);
+    double* first = (double*)malloc(sizeof(double) * 3 * nmon);
+    int natoms = 0;
+    for (int i = 0; i < nmon; ++i) {
+        fi[i] = natoms;
+        natoms += nat[i];
+        loc[i] = 1;
+    }
+    if (pbc) {
+        if (orc_invert_unit_cell(box, bi)) return -1;
+        orc_fix_monomer_coordinates(xyz, box, bi, nmon, nat, fi);
+    }
+    for (int i = 0; i < nmon; ++i) memcpy(first + 3 * i, xyz + 3 * fi[i], 3 * sizeof(double));
+    memset(grad, 0, sizeof(double) * 3 * natoms);
+    memset(virial, 0, sizeof(double) * 9);
+    *e2b = 0.0;
+    *e3b = 0.0;
+    for (int order = 2; order <= 3; ++order) {
+        const double cut = order == 2 ? cutoff2b : cutoff3b;
+        const long n = orc_clusters(order, cut, nmon, pbc, box, bi, first, loc, tag, 0, NULL, 0);
+        int* list = (int*)malloc(sizeof(int) * order * (n > 0 ? n : 1));
+        orc_clusters(order, cut, nmon, pbc, box, bi, first, loc, tag, 0, list, n);
+        long nev = 0;
+        double etot = 0.0;
+#pragma omp parallel reduction(+ : nev, etot)
+        {
+            double* gl = (double*)calloc(3 * (size_t)natoms, sizeof(double));
+            double vl[9] = {0};
+#pragma omp for schedule(dynamic, 64)
+            for (long t = 0; t < n; ++t) {
+                const int* id = list + order * t;
+                double w[3][15], g[3][15], v[9] = {0};
+                int kd[3], perm[3];
+                for (int m = 0; m < order; ++m) {
+                    kd[m] = kind[id[m]];
+                    memcpy(w[m], xyz + 3 * fi[id[m]], sizeof(double) * 3 * nat[id[m]]);
+                    memset(g[m], 0, sizeof(g[m]));
+                }
+                if (pbc)
+                    for (int m = 1; m < order; ++m) orc_close_image(box, bi, w[0], nat[id[m]], w[m]);
+                const int fam = nmer_family(order, kd, perm);
+                if (fam < 0 || (fam != 0 && fam != 3 && !hol[fam == 4 ? 2 : fam - 1])) continue;
+                int ev = 0;
+                double e;
+                if (fam == 0) {
+                    e = orc_mbpol2b(p2, NULL, w[0], w[1], g[0], g[1], v, &ev);
+                } else if (fam == 3) {
+                    e = orc_mbpol3b(p3, NULL, w[perm[0]], w[perm[1]], w[perm[2]], g[perm[0]], g[perm[1]], g[perm[2]],
+                                    v, &ev);
+                } else {
+                    const int h = fam == 4 ? 2 : fam - 1;
+                    const double* ms[3] = {w[perm[0]], w[perm[1]], order == 3 ? w[perm[2]] : NULL};
+                    double* gs[3] = {g[perm[0]], g[perm[1]], order == 3 ? g[perm[2]] : NULL};
+                    e = orc_nrg_holder(hol[h], hp[h], NULL, ms, gs, v, &ev);
+                }
+                nev += ev;
+                etot += e;
+                for (int m = 0; m < order; ++m)
+                    for (int j = 0; j < 3 * nat[id[m]]; ++j) gl[3 * (size_t)fi[id[m]] + j] += g[m][j];
+                for (int q = 0; q < 9; ++q) vl[q] += v[q];
+            }
+#pragma omp critical(orc_reduce_sys)
+            {
+                for (long j = 0; j < 3L * natoms; ++j) grad[j] += gl[j];
+                for (int q = 0; q < 9; ++q) virial[q] += vl[q];
+            }
+            free(gl);
+        }
+        if (order == 2) {
+            *e2b = etot;
+            counts[0] = n;
+            counts[1] = nev;
+        } else {
+            *e3b = etot;
+            counts[2] = n;
+            counts[3] = nev;
+        }
+        free(list);
+    }
+    free(fi);
+    free(loc);
     free(first);
     return 0;
 }

@@ -51,6 +51,38 @@ double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const d
 double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const double* w2, const double* w3,
                    double* g1, double* g2, double* g3, double* vir, int* evaluated);
 
+/* An MB-nrg holder as data (mbx_amd/tables/<name>.holder.json, extracted from the holder's
+ * text): monomers in the holder's argument order, sites = real atoms in that order (+ the
+ * water's two lone-pair sites when vsite), variables (form 0: exp(k (d - r)), 1:
+ * exp(k (d - r)) / r, 2: exp(-k r)) over site pairs, switching radii. */
+typedef struct {
+    int nmon;
+    int nat[3];
+    int nreal, nsites;
+    int vsite, osite;
+    double in_plane, out_of_plane;
+    double r_in, r_out;
+    int nvars;
+    const int* form;
+    const double* k;
+    const double* d;
+    const int* s1;
+    const int* s2;
+} orc_holder;
+
+/* One n-mer through an MB-nrg holder: the CH4-H2O 2B (potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:954-1220),
+ * CH4-CH4 2B (x2b_A1B4_A1B4_deg4_exp0_v1x.cpp:1600-1790) and CH4-H2O-H2O 3B
+ * (potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1481-1790) share this structure, with the
+ * variable forms of tools/variable.cpp:10-129.  mons[m] / grads[m] (+=, may be NULL): the
+ * n-mer's monomers in the holder's order, already imaged.  vir (+=, 9) may be NULL.
+ * 2B: skipped if the first-atom distance exceeds r_out, E = sw(r12) P.  3B: skipped if all
+ * three first-atom distances exceed r_out, E = (s12 s13 + s12 s23 + s13 s23) P.
+ * Pinned by: unittest-energy2b.cpp:314-420 and unittest-energy3b.cpp:472-540 KATs, the
+ * compiled reference holders (CH4-H2O 2B, CH4-H2O-H2O 3B) on random n-mers, central finite
+ * differences, and the published per-term energies of the reference's CH4/H2O systems. */
+double orc_nrg_holder(const orc_holder* h, const orc_poly* p, const double* a, const double* const* mons,
+                      double* const* grads, double* vir, int* evaluated);
+
 /* InvertUnitCell (tools/math_tools.cpp:37-64).  Returns 0, or -1 if singular. */
 int orc_invert_unit_cell(const double* box, double* box_inv);
 
@@ -84,6 +116,18 @@ int orc_water_system(const orc_poly* p2, const orc_poly* p3, int nmon, double* x
                      double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
                      long* counts, int* dimers, long cap2, int* trimers, long cap3, double* e_dimer,
                      double* e_trimer);
+
+/* The same 2B+3B orchestration for a mixed H2O/CH4 system (kind 0 = h2o, 1 = ch4; monomers in
+ * the caller's order with their first-atom input tags), dispatched like e2b::get_2b_energy
+ * (potential/2b/energy2b.cpp:206-411: names sorted, ch4-h2o through the (h2o, ch4) holder)
+ * and e3b::get_3b_energy (potential/3b/energy3b.cpp:139-310: names sorted by the same
+ * exchange network; ch4-ch4-h2o and ch4x3 have no 3B term).  hol[0..2] / hp[0..2]: the
+ * ch4h2o_2b, ch4ch4_2b and ch4h2o2_3b holders and polynomials (NULL skips that family).
+ * xyz/grad: 3 doubles per atom.  counts[4] as orc_water_system. */
+int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* hol, const orc_poly* const* hp,
+               int nmon, const int* kind, const int* nat, const int* tag, double* xyz, const double* box,
+               double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
+               long* counts);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,8 @@
 //   systools::FixMonomerCoordinates  bblock/sys_tools.cpp:257
 //   systools::GetCloseDimerImage     bblock/sys_tools.cpp:322
 //   InvertUnitCell              tools/math_tools.cpp:37
+//   x2b_A1B2Z2_C1D4_v1x::eval   potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:954 (E+G; water, methane)
+//   mbnrg_A1B4_C1D2_C1D2_deg3_v1::eval  potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1481 (E+G)
 // The 2B+3B driver at the bottom restates the OpenMP orchestration of
 // System::Get2B / System::Get3B (bblock/system.cpp:2425-2725, 2769-3073) over
 // those leaves for a pure-water system; it is the reference-faithful CPU
@@ -30,6 +32,8 @@
 #include "potential/3b/poly-3b-v2x.h"
 #include "potential/3b/x3b-v2x.h"
 #include "tools/math_tools.h"
+#include "potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.h"
+#include "potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.h"
 
 extern "C" {
 
@@ -244,6 +248,25 @@ int ref_water_2b3b(size_t nmon, double* xyz, const double* box9, double cutoff2b
     *e2b = ref_nmer_term(2, cutoff2b, nmon, v, pbc, box, box_inv, batch, nthreads, grad, virial, &counts[0]);
     *e3b = ref_nmer_term(3, cutoff3b, nmon, v, pbc, box, box_inv, batch, nthreads, grad, virial, &counts[1]);
     return 0;
+}
+
+// MB-nrg CH4-H2O 2B holder, n dimers: w[9n] waters, c[15n] methanes (+= g, vir).
+double ref_x2b_ch4h2o(const double* w, const double* c, size_t n, double* gw, double* gc, double* vir) {
+    x2b_A1B2Z2_C1D4_deg3_exp0::x2b_A1B2Z2_C1D4_v1x pot("h2o", "ch4");
+    std::vector<double> v(vir, vir + 9);
+    const double e = pot.eval(w, c, gw, gc, n, &v);
+    std::copy(v.begin(), v.end(), vir);
+    return e;
+}
+
+// MB-nrg CH4-H2O-H2O 3B holder, n trimers: c[15n], w1[9n], w2[9n] (+= g, vir).
+double ref_x3b_ch4h2o2(const double* c, const double* w1, const double* w2, size_t n, double* gc, double* g1,
+                       double* g2, double* vir) {
+    mbnrg_A1B4_C1D2_C1D2_deg3::mbnrg_A1B4_C1D2_C1D2_deg3_v1 pot("ch4", "h2o", "h2o");
+    std::vector<double> v(vir, vir + 9);
+    const double e = pot.eval(c, w1, w2, gc, g1, g2, n, &v);
+    std::copy(v.begin(), v.end(), vir);
+    return e;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     poly_3b    src/unittests/unittest-poly-3b.cpp         TEST_CASE("poly-3b-v2x::class")
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
+    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases (see CASES).
 
 Usage:  python tests/golden/make_kats.py [/root/reference]
 """
@@ -23,11 +24,24 @@ CASES = {
     "poly_3b": ("src/unittests/unittest-poly-3b.cpp", 'poly-3b-v2x::class'),
     "holder_2b": ("src/unittests/unittest-poly-holder-2b.cpp", 'x2b_v9x::struct'),
     "holder_3b": ("src/unittests/unittest-poly-holder-3b.cpp", 'x3b-v2x::class'),
+    # MB-nrg CH4 families: polynomials, holders, and the dispatching energy routines
+    "poly_ch4h2o_2b": ("src/unittests/unittest-poly-2b.cpp", ':poly_2b_A1B2Z2_C1D4_deg3_exp0_v1x:struct'),
+    "poly_ch4ch4_2b": ("src/unittests/unittest-poly-2b.cpp", 'poly_2b_A1B4_A1B4_deg4_exp0_v1x:struct'),
+    "poly_ch4h2o2_3b": ("src/unittests/unittest-poly-3b.cpp", 'poly_3b_A1B4_C1D2_C1D2_deg3_v1.h'),
+    "holder_ch4h2o_2b": ("src/unittests/unittest-poly-holder-2b.cpp", 'x2b_A1B2Z2_C1D4_v1x::struct'),
+    "holder_ch4ch4_2b": ("src/unittests/unittest-poly-holder-2b.cpp", 'x2b_A1B4_A1B4_v1x::struct'),
+    "holder_ch4h2o2_3b": ("src/unittests/unittest-poly-holder-3b.cpp", 'mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1::class'),
+    "energy_ch4ch4_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-ch4'),
+    "energy_ch4h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-h2o'),
+    "energy_ch4h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:ch4-h2o-h2o'),
 }
 
 
 def block(text, name):
-    i = text.index('TEST_CASE("%s")' % name)
+    if name.startswith("SECTION:"):
+        i = text.index('SECTION("%s")' % name[len("SECTION:"):])
+    else:
+        i = text.index('TEST_CASE("%s")' % name)
     j = text.index("{", i) + 1
     depth = 1
     k = j
@@ -54,7 +68,8 @@ def main():
     res = {}
     for key, (path, name) in CASES.items():
         res[key] = values(block(open(os.path.join(ref, path)).read(), name))
-        res[key]["_source"] = '%s TEST_CASE("%s")' % (path, name)
+        res[key]["_source"] = ('%s SECTION("%s")' % (path, name[8:]) if name.startswith("SECTION:")
+                               else '%s TEST_CASE("%s")' % (path, name))
     with open(os.path.join(HERE, "kats.json"), "w") as f:
         json.dump(res, f)
     for k, v in res.items():

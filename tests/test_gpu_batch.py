@@ -29,6 +29,15 @@ def _batch_engine(parts, box, c2, c3):
     return e, off
 
 
+def close(got, want, scale):
+    """Energies are sums of n-mer energies, each a sum of strongly cancelling monomials; a small
+    cluster's total can be many orders below its terms (a 100-water cluster: E3B = -0.14 from
+    sum |a_k M_k| = 1e7).  No FP64 evaluation order is better than ~eps * scale there, so the
+    bar is 1e-10 relative to the total or 1e-17 * scale (a tenth of one rounding of the
+    terms' magnitude), whichever is looser; scale from oracle.water_conditioning."""
+    return abs(got - want) <= max(E_REL * abs(want), 1e-17 * scale, 1e-13)
+
+
 def _check(parts, box, c2, c3):
     e, off = _batch_engine(parts, box, c2, c3)
     e2, e3, vir = e.batch_results()
@@ -37,8 +46,9 @@ def _check(parts, box, c2, c3):
     want_d, want_t = [], []
     for k, x in enumerate(parts):
         r = oracle.water_system(x, box, c2, c3)
-        assert e2[k] == pytest.approx(r["e2b"], rel=E_REL, abs=1e-12), k
-        assert e3[k] == pytest.approx(r["e3b"], rel=E_REL, abs=1e-12), k
+        s2, s3 = oracle.water_conditioning(x, box, c2, c3)
+        assert close(e2[k], r["e2b"], s2), (k, e2[k], r["e2b"], s2)
+        assert close(e3[k], r["e3b"], s3), (k, e3[k], r["e3b"], s3)
         gk = g[9 * off[k]:9 * off[k + 1]]
         assert np.abs(gk - r["grad"]).max() <= G_ABS, k
         np.testing.assert_allclose(vir[k], r["virial"], rtol=1e-8, atol=1e-8)
@@ -76,14 +86,17 @@ def test_batch_equals_plain():
         p.set_coords(x)
         p.compute()
         pe2, pe3, pg, pv = p.results()
-        assert e2[k] == pytest.approx(pe2, rel=1e-12, abs=1e-13)
-        assert e3[k] == pytest.approx(pe3, rel=1e-12, abs=1e-13)
+        # same kernels, different summation order of the per-n-mer energies
+        s2 = np.abs(p.nmer_energies(2)[1]).sum()
+        s3 = np.abs(p.nmer_energies(3)[1]).sum()
+        assert abs(e2[k] - pe2) <= max(1e-12 * abs(pe2), 1e-14 * s2, 1e-14)
+        assert abs(e3[k] - pe3) <= max(1e-12 * abs(pe3), 1e-14 * s3, 1e-14)
         assert np.abs(g[9 * off[k]:9 * off[k + 1]] - pg).max() <= 1e-10
         np.testing.assert_allclose(vir[k], pv, rtol=1e-10, atol=1e-10)
     # totals of the batch = sum over configurations
     t2, t3, _, _ = e.results()
-    assert t2 == pytest.approx(e2.sum(), rel=1e-12)
-    assert t3 == pytest.approx(e3.sum(), rel=1e-12)
+    assert t2 == pytest.approx(e2.sum(), rel=1e-11)
+    assert t3 == pytest.approx(e3.sum(), rel=1e-11)
 
 
 def test_batch_reset_to_single():

@@ -243,8 +243,39 @@ def test_autotune_keeps_results(systems):
     picked = e.autotune(reps=1)
     assert picked["split_2b"] in (1, 2, 4, 8) and picked["split_3b"] in (1, 2, 4, 8)
     after = e.results()
-    assert after[0] == pytest.approx(before[0], rel=1e-12)
-    assert after[1] == pytest.approx(before[1], rel=1e-12)
-    assert np.abs(after[2] - before[2]).max() <= 1e-10
+    # the chosen schedule may differ from the default one: same values up to rounding order
+    assert after[0] == pytest.approx(before[0], rel=E_REL)
+    assert after[1] == pytest.approx(before[1], rel=E_REL)
+    assert np.abs(after[2] - before[2]).max() <= G_ABS
     with pytest.raises(engine.MbxgError):
         e.set_tuning("split_3b", 3)
+
+
+def test_graph_replay(systems):
+    """Graph mode: every replay (new coordinates each time, eager work and host waits between
+    replays) equals an eager compute on the same coordinates."""
+    xyz, box, c2, c3 = _case(systems, "C1")
+    rng = np.random.default_rng(11)
+    frames = [xyz + rng.normal(0.0, 0.02, xyz.shape) for _ in range(4)]
+    g = engine.water_engine(len(xyz) // 9, box, c2, c3)
+    g.set_graph(True)
+    ref = engine.water_engine(len(xyz) // 9, box, c2, c3)
+    for i, x in enumerate(frames + frames[:2]):
+        print("graph replay", i, flush=True)
+        g.set_coords(x)
+        g.compute()
+        a = g.results()
+        ref.set_coords(x)
+        ref.compute()
+        b = ref.results()
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.abs(a[2] - b[2]).max() <= 1e-12
+        np.testing.assert_allclose(a[3], b[3], rtol=1e-12, atol=1e-12)
+        assert np.array_equal(g.counts(), ref.counts())
+    # a denser frame outgrows the captured buffers: detected, recomputed eagerly, recaptured
+    dense = configs.water_box(len(xyz) // 9, 18.0, (7, 7, 6), seed=5)[0]
+    g.set_coords(dense)
+    g.compute()
+    a = g.results()
+    r = oracle.water_system(dense, box, c2, c3)
+    assert a[0] == pytest.approx(r["e2b"], rel=E_REL) and a[1] == pytest.approx(r["e3b"], rel=E_REL)
