@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed polynomial schedule 'S2,S3' (wavefronts per 64 dimers, trimers) instead of "
+                         "autotuning; used for profiles, so the profiled launches are the timed ones")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -182,7 +185,13 @@ def main():
     for _ in range(args.warmup):
         step()
     eng.synchronize()
-    schedule = eng.autotune(reps=3)  # polynomial-stage schedule for this workload
+    if args.schedule:
+        s2, s3 = (int(v) for v in args.schedule.split(","))
+        eng.set_tuning("split_2b", s2)
+        eng.set_tuning("split_3b", s3)
+        schedule = {"split_2b": s2, "split_3b": s3}
+    else:
+        schedule = eng.autotune(reps=3)  # polynomial-stage schedule for this workload
     eng.set_graph(True)  # steady state: one captured HIP graph per compute
     for _ in range(2):
         step()

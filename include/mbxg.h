@@ -58,6 +58,18 @@ int mbxg_set_box(mbxg_ctx* ctx, const double* box9);
 /* 2B / 3B cutoffs (System::Set2bCutoff / Set3bCutoff, system.h:826,834).  0 disables a term. */
 int mbxg_set_cutoffs(mbxg_ctx* ctx, double cutoff2b, double cutoff3b);
 
+/* Evaluated families: bit MBXG_FAM_* set = evaluate those n-mers (default: all).  A cleared bit
+ * is the System's ignore_2b_poly / ignore_3b_poly entry for that family (bblock/system.cpp:
+ * 2606-2619, 2940-2953): the n-mers stay in the lists but are not evaluated. */
+enum {
+    MBXG_FAM_MBPOL_2B = 1,      /* h2o-h2o, MB-pol */
+    MBXG_FAM_CH4H2O_2B = 2,     /* ch4-h2o, MB-nrg A1B2Z2_C1D4 */
+    MBXG_FAM_CH4CH4_2B = 4,     /* ch4-ch4, MB-nrg A1B4_A1B4 */
+    MBXG_FAM_MBPOL_3B = 8,      /* h2o x3, MB-pol */
+    MBXG_FAM_CH4H2O2_3B = 16    /* ch4-h2o-h2o, MB-nrg A1B4_C1D2_C1D2 */
+};
+int mbxg_set_families(mbxg_ctx* ctx, int mask);
+
 /* Batches: split the monomers (in topology order) into nconf independent configurations,
  * configuration c holding monomers [mon_offsets[c], mon_offsets[c+1]) (mon_offsets[0] = 0,
  * mon_offsets[nconf] = nmon).  Lists never cross configurations; one compute evaluates all of

@@ -26,6 +26,9 @@
 #include <vector>
 
 #include "../../include/mbxg.h"
+#include "gen/coef_ch4ch4_2b.inc"
+#include "gen/coef_ch4h2o2_3b.inc"
+#include "gen/coef_ch4h2o_2b.inc"
 #include "gen/coef_mbpol_2b.inc"
 #include "gen/coef_mbpol_3b.inc"
 #include "mbxg_kernels.hpp"
@@ -91,12 +94,14 @@ struct ListState {
     DBuf<int> d_cell_base, d_cell_bound;
     DBuf<double> cx, cy, cz, bx, by, bz;
     DBuf<int> cell, cell_count, cell_start, cell_slot;
-    DBuf<int4> counts, offsets;
-    DBuf<int> list2, eval2, list3, eval3;
+    DBuf<int4> counts, offsets, counts_x, offsets_x;
+    DBuf<int> list2, list3, evals[kFamilies];
     DBuf<double> gscratch;        // match storage when it does not fit in LDS
-    int4 cap = {0, 0, 0, 0};      // capacities of list2, eval2, list3, eval3 in n-mers
+    // capacities in n-mers: cap = {list2, MB-pol 2B, list3, MB-pol 3B},
+    // capx = {CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, unused}; tot / totx: totals, same layout
+    int4 cap = {0, 0, 0, 0}, capx = {0, 0, 0, 0};
     int match_cap = 256;          // power of two
-    int4 tot = {0, 0, 0, 0};      // totals of the last validated compute
+    int4 tot = {0, 0, 0, 0}, totx = {0, 0, 0, 0};
     bool built = false;           // the last compute built lists
 };
 
@@ -126,6 +131,8 @@ struct mbxg_ctx {
     Box box{};
     double cut2 = 0.0, cut3 = 0.0;
     DBuf<double> d_in, d_xyz, d_grad, d_totals, d_coef2, d_coef3;
+    DBuf<double> d_coefn[kFamilies];  // MB-nrg coefficient tables (families 1, 2, 4)
+    DBuf<double> d_fam_tot;           // per family: energy + 6 virial components
     DBuf<Box> d_box;
     DBuf<int> d_overflow;
     // configurations (mbxg_set_batch): monomers [mon_off[c], mon_off[c+1]) form configuration c;
@@ -134,8 +141,9 @@ struct mbxg_ctx {
     std::vector<int> mon_off, pt_off;
     DBuf<int> d_pconf, d_pt_off;
     DBuf<double> d_conf_tot;  // per configuration: 2B record, then 3B record (kPartials each)
+    DBuf<double> d_conf_fam;  // per family and configuration (kPartials each)
     ListState L;
-    EvalState ev2, ev3;
+    EvalState fam[kFamilies];
     bool have_coords = false, have_results = false;
     bool need_sync = true;  // next compute sizes the buffers (first call, new topology/box/cutoffs)
     bool checked = false;   // the last compute's totals were checked against the capacities
@@ -147,6 +155,7 @@ struct mbxg_ctx {
     hipGraphExec_t gexec = nullptr;
     int g_terms = -1, g_grads = -1;       // what gexec computes
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
+    int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
@@ -173,12 +182,20 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         return fail(MBXG_EHIP, "mbxg_create: stream/event creation failed: %s", hipGetErrorString(e));
     }
     if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(16) ||
-            c->d_overflow.reserve(1) || c->d_box.reserve(1)) {
+        c->d_overflow.reserve(1) || c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
+        c->d_coefn[FAM_CH4H2O_2B].reserve(kNCoef_ch4h2o_2b) || c->d_coefn[FAM_CH4CH4_2B].reserve(kNCoef_ch4ch4_2b) ||
+        c->d_coefn[FAM_CH4H2O2_3B].reserve(kNCoef_ch4h2o2_3b)) {
         delete c;
         return fail(MBXG_ENOMEM, "mbxg_create: device allocation failed");
     }
     HIPCHK(hipMemcpy(c->d_coef2.p, kFit_mbpol_2b, sizeof(double) * kNCoef_mbpol_2b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_coef3.p, kFit_mbpol_3b, sizeof(double) * kNCoef_mbpol_3b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_coefn[FAM_CH4H2O_2B].p, kFit_ch4h2o_2b, sizeof(double) * kNCoef_ch4h2o_2b,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_coefn[FAM_CH4CH4_2B].p, kFit_ch4ch4_2b, sizeof(double) * kNCoef_ch4ch4_2b,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_coefn[FAM_CH4H2O2_3B].p, kFit_ch4h2o2_3b, sizeof(double) * kNCoef_ch4h2o2_3b,
+                     hipMemcpyHostToDevice));
     c->box.periodic = 0;
     *out = c;
     return MBXG_OK;
@@ -205,14 +222,18 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.counts.release();
     L.offsets.release();
     L.list2.release();
-    L.eval2.release();
     L.list3.release();
-    L.eval3.release();
+    L.counts_x.release();
+    L.offsets_x.release();
+    for (DBuf<int>& b : L.evals) b.release();
     L.gscratch.release();
     c->d_pconf.release();
     c->d_pt_off.release();
     c->d_conf_tot.release();
-    for (EvalState* o : {&c->ev2, &c->ev3}) {
+    c->d_conf_fam.release();
+    c->d_fam_tot.release();
+    for (DBuf<double>& b : c->d_coefn) b.release();
+    for (EvalState* o = c->fam; o != c->fam + kFamilies; ++o) {
         o->e_nmer.release();
         o->vir_nmer.release();
         o->partials.release();
@@ -268,7 +289,8 @@ int set_configs(mbxg_ctx* c, int nconf, const int* mon_off) {
     }
     for (int q = 0; q < nconf; ++q) pt_off[q + 1] += pt_off[q];
     if (c->d_pconf.reserve(pconf.size()) || c->d_pt_off.reserve(nconf + 1) ||
-        c->d_conf_tot.reserve((size_t)2 * kPartials * nconf))
+        c->d_conf_tot.reserve((size_t)2 * kPartials * nconf) ||
+        c->d_conf_fam.reserve((size_t)kFamilies * kPartials * nconf))
         return fail(MBXG_ENOMEM, "configurations: device allocation failed");
     HIPCHK(hipMemcpy(c->d_pconf.p, pconf.data(), sizeof(int) * pconf.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_pt_off.p, pt_off.data(), sizeof(int) * (nconf + 1), hipMemcpyHostToDevice));
@@ -329,6 +351,15 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     c->have_coords = false;
     c->have_results = false;
     c->need_sync = true;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_set_families(mbxg_ctx* c, int mask) {
+    if (!c || (mask & ~((1 << kFamilies) - 1))) return fail(MBXG_EINVAL, "mbxg_set_families: bad arguments");
+    if (mask == c->fam_mask) return MBXG_OK;
+    c->fam_mask = mask;
+    c->need_sync = true;
+    c->inputs_changed = true;
     return MBXG_OK;
 }
 
@@ -403,6 +434,7 @@ extern "C" int mbxg_set_box(mbxg_ctx* c, const double* box9) {
 
 extern "C" int mbxg_set_cutoffs(mbxg_ctx* c, double c2, double c3) {
     if (!c || !(c2 >= 0.0) || !(c3 >= 0.0)) return fail(MBXG_EINVAL, "mbxg_set_cutoffs: bad arguments");
+    if (c2 == c->cut2 && c3 == c->cut3) return MBXG_OK;
     c->cut2 = c2;
     c->cut3 = c3;
     c->need_sync = true;
@@ -426,6 +458,18 @@ extern "C" int mbxg_set_coefficients(mbxg_ctx* c, const char* model, const doubl
         dst = c->d_coef3.p;
         fit = kFit_mbpol_3b;
         need = kNCoef_mbpol_3b;
+    } else if (m == "ch4h2o_2b") {
+        dst = c->d_coefn[FAM_CH4H2O_2B].p;
+        fit = kFit_ch4h2o_2b;
+        need = kNCoef_ch4h2o_2b;
+    } else if (m == "ch4ch4_2b") {
+        dst = c->d_coefn[FAM_CH4CH4_2B].p;
+        fit = kFit_ch4ch4_2b;
+        need = kNCoef_ch4ch4_2b;
+    } else if (m == "ch4h2o2_3b") {
+        dst = c->d_coefn[FAM_CH4H2O2_3B].p;
+        fit = kFit_ch4h2o2_3b;
+        need = kNCoef_ch4h2o2_3b;
     } else {
         return fail(MBXG_EINVAL, "mbxg_set_coefficients: unknown model '%s'", model);
     }
@@ -510,6 +554,50 @@ void launch_zero(double* p, size_t n, hipStream_t st) {
 
 // Dimer and trimer lists at the larger cutoff (one gather for both orders).  With sync the
 // totals and the largest match count are read back and the buffers sized from them.
+// Capacity / total / device count of a family's evaluate list.
+int fam_cap(const ListState& L, int f) {
+    switch (f) {
+        case FAM_MBPOL_2B: return L.cap.y;
+        case FAM_CH4H2O_2B: return L.capx.x;
+        case FAM_CH4CH4_2B: return L.capx.y;
+        case FAM_MBPOL_3B: return L.cap.w;
+        default: return L.capx.z;
+    }
+}
+
+int fam_tot(const ListState& L, int f) {
+    switch (f) {
+        case FAM_MBPOL_2B: return L.tot.y;
+        case FAM_CH4H2O_2B: return L.totx.x;
+        case FAM_CH4CH4_2B: return L.totx.y;
+        case FAM_MBPOL_3B: return L.tot.w;
+        default: return L.totx.z;
+    }
+}
+
+// The int4 component of a family's count in offsets / offsets_x (k_reduce_configs' comp).
+int fam_component(int f) {
+    switch (f) {
+        case FAM_MBPOL_2B: return 1;
+        case FAM_CH4H2O_2B: return 0;
+        case FAM_CH4CH4_2B: return 1;
+        case FAM_MBPOL_3B: return 3;
+        default: return 2;
+    }
+}
+
+const int* fam_ndev(const ListState& L, int np, int f) {
+    const int* a = (const int*)(L.offsets.p + np);
+    const int* b = (const int*)(L.offsets_x.p + np);
+    switch (f) {
+        case FAM_MBPOL_2B: return a + 1;
+        case FAM_CH4H2O_2B: return b + 0;
+        case FAM_CH4CH4_2B: return b + 1;
+        case FAM_MBPOL_3B: return a + 3;
+        default: return b + 2;
+    }
+}
+
 // Cell layout of all configurations: under PBC every configuration gets the same extended
 // grid; in the gas phase each configuration's grid is sized on the device within a cell bound
 // of 4 cells per point (at least 64, at most 64^3).
@@ -561,7 +649,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         if (int r = prepare_layout(c, cut)) return r;
     if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.bx.reserve(ns) || L.by.reserve(ns) ||
         L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.counts.reserve(np) ||
-        L.offsets.reserve(np + 1))
+        L.offsets.reserve(np + 1) || L.counts_x.reserve(np) || L.offsets_x.reserve(np + 1))
         return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
     const int tb = 256, pb = (np + tb - 1) / tb, sb = (ns + tb - 1) / tb;
     if (!c->box.periodic)
@@ -575,9 +663,28 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_start.p, L.cell_count.p, L.cx.p,
                        L.cy.p, L.cz.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
     HIPCHK(hipGetLastError());
-    NmersArgs A{L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.cell_start.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p, L.d_grids.p,
-                c->d_pconf.p, L.d_cell_base.p, np, c2sq, c3sq, L.counts.p, L.offsets.p, nullptr, nullptr, nullptr,
-                nullptr, c->d_overflow.p};
+    NmersArgs A{};
+    A.cx = L.cx.p;
+    A.cy = L.cy.p;
+    A.cz = L.cz.p;
+    A.cell = L.cell.p;
+    A.cell_start = L.cell_start.p;
+    A.cell_slot = L.cell_slot.p;
+    A.bx = L.bx.p;
+    A.by = L.by.p;
+    A.bz = L.bz.p;
+    A.grids = L.d_grids.p;
+    A.pconf = c->d_pconf.p;
+    A.cell_base = L.d_cell_base.p;
+    A.np = np;
+    A.c2sq = c2sq;
+    A.c3sq = c3sq;
+    A.counts = L.counts.p;
+    A.counts_x = L.counts_x.p;
+    A.offsets = L.offsets.p;
+    A.offsets_x = L.offsets_x.p;
+    A.overflow = c->d_overflow.p;
+    A.fam_mask = c->fam_mask;
 
     auto scratch = [&]() -> double* {
         if (L.match_cap <= kMaxLdsMatches) return nullptr;
@@ -589,13 +696,15 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         double* gs = scratch();
         if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
         launch_zero_int(c->d_overflow.p, 1, st);
-        HIPCHK(launch_nmers(A, N, 0, none, L.match_cap, gs, kScratchBlocks, st));
+        HIPCHK(launch_nmers(A, N, 0, none, none, L.match_cap, gs, kScratchBlocks, st));
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts_x.p, np, L.offsets_x.p);
         HIPCHK(hipGetLastError());
         if (!sync) break;
-        int4 tot;
+        int4 tot, totx;
         int ovf = 0;
         HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
@@ -607,22 +716,25 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         }
         L.match_cap = want;  // smaller storage, more blocks per CU; still holds every match
         L.tot = tot;
+        L.totx = totx;
         const int4 cap = {with_headroom(tot.x), with_headroom(tot.y), with_headroom(tot.z), with_headroom(tot.w)};
-        if (L.list2.reserve(2 * (size_t)cap.x) || L.eval2.reserve(2 * (size_t)cap.y) ||
-            L.list3.reserve(3 * (size_t)cap.z) || L.eval3.reserve(3 * (size_t)cap.w))
-            return fail(MBXG_ENOMEM, "n-mer lists: device allocation failed (%d + %d listed)", tot.x, tot.z);
+        const int4 capx = {with_headroom(totx.x), with_headroom(totx.y), with_headroom(totx.z), 0};
         L.cap = cap;
+        L.capx = capx;
+        bool bad = L.list2.reserve(2 * (size_t)cap.x) || L.list3.reserve(3 * (size_t)cap.z);
+        for (int f = 0; f < kFamilies; ++f)
+            bad = bad || L.evals[f].reserve((size_t)family_order(f) * fam_cap(L, f) + 1);
+        if (bad) return fail(MBXG_ENOMEM, "n-mer lists: device allocation failed (%d + %d listed)", tot.x, tot.z);
         if (L.match_cap > kMaxLdsMatches && !scratch())
             return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
         break;
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     A.list2 = L.list2.p;
-    A.eval2 = L.eval2.p;
     A.list3 = L.list3.p;
-    A.eval3 = L.eval3.p;
-    HIPCHK(launch_nmers(A, N, 1, L.cap, L.match_cap, L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr,
-                        kScratchBlocks, st));
+    for (int f = 0; f < kFamilies; ++f) A.evals[f] = L.evals[f].p;
+    HIPCHK(launch_nmers(A, N, 1, L.cap, L.capx, L.match_cap,
+                        L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr, kScratchBlocks, st));
     return 0;
 }
 
@@ -646,7 +758,7 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                            c->d_first.p, c->d_nat.p, c->nmon, c->box);
         HIPCHK(hipGetLastError());
     }
-    NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
+    NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_nat.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
               c->box.periodic};
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const bool want2 = (terms & MBXG_2B) && c->cut2 > 0.0;
@@ -659,51 +771,66 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
             return r;
     } else {
         L.tot = make_int4(0, 0, 0, 0);
+        L.totx = make_int4(0, 0, 0, 0);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     if (c->natoms > 0) launch_zero(c->d_grad.p, 3 * (size_t)c->natoms, st);
-    launch_zero(c->d_totals.p, 16, st);
-    if (c->nconf > 1) launch_zero(c->d_conf_tot.p, (size_t)2 * kPartials * c->nconf, st);
+    launch_zero(c->d_fam_tot.p, (size_t)kFamilies * kPartials, st);
+    if (c->nconf > 1) launch_zero(c->d_conf_fam.p, (size_t)kFamilies * kPartials * c->nconf, st);
     HIPCHK(hipGetLastError());
     c->evaluated = L.built && !(terms & MBXG_LISTS_ONLY);
     if (c->evaluated) {
-        // the 2B chain runs on stream2, concurrently with the 3B chain: its small blocks fill
-        // the machine while the last 3B blocks drain
+        // the 2B families run on stream2, concurrently with the 3B ones: their small blocks
+        // fill the machine while the last 3B blocks drain
         HIPCHK(hipEventRecord(c->fork, st));
         HIPCHK(hipStreamWaitEvent(c->stream2, c->fork, 0));
-        const int* n2 = (const int*)(L.offsets.p + c->np) + 1;  // evaluated dimers
-        const int* n3 = (const int*)(L.offsets.p + c->np) + 3;  // evaluated trimers
-        for (int k = 1; k >= 0; --k) {
-            if (!(k == 0 ? want2 : want3)) continue;
-            // with stage timing on, both chains run on the main stream so each stage is timed alone
-            hipStream_t sk = (k == 0 && !ev) ? c->stream2 : st;
-            EvalState& o = k == 0 ? c->ev2 : c->ev3;
-            const int cap = k == 0 ? L.cap.y : L.cap.w;
+        const bool batch = c->nconf > 1;
+        const int order_of[kFamilies] = {FAM_MBPOL_3B, FAM_CH4H2O2_3B, FAM_MBPOL_2B, FAM_CH4H2O_2B, FAM_CH4CH4_2B};
+        for (int i = 0; i < kFamilies; ++i) {
+            const int f = order_of[i];
+            const bool two = family_order(f) == 2;
+            if (!(two ? want2 : want3)) continue;
+            const int cap = fam_cap(L, f);
             if (cap <= 0) continue;
-            const size_t wsz = k == 0 ? workspace_mbpol2b(cap) : workspace_mbpol3b(cap);
+            // with stage timing on, everything runs on the main stream so each stage is timed alone
+            hipStream_t sk = (two && !ev) ? c->stream2 : st;
+            EvalState& o = c->fam[f];
             const int ld = (cap + 63) / 64 * 64;
-            const bool batch = c->nconf > 1;
+            const size_t wsz = f == FAM_MBPOL_2B   ? workspace_mbpol2b(cap)
+                               : f == FAM_MBPOL_3B ? workspace_mbpol3b(cap)
+                                                   : workspace_nrg_family(f, cap);
             if (o.e_nmer.reserve(cap) || o.partials.reserve((size_t)kPartials * partials_records(cap)) ||
                 o.ws.reserve(wsz) || (batch && o.vir_nmer.reserve((size_t)6 * ld)))
                 return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
             double* vir = batch ? o.vir_nmer.p : nullptr;
-            if (k == 0)
-                HIPCHK(launch_mbpol2b(L.eval2.p, n2, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
+            const int* nd = fam_ndev(L, c->np, f);
+            if (f == FAM_MBPOL_2B)
+                HIPCHK(launch_mbpol2b(L.evals[f].p, nd, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
                                       o.partials.p, o.ws.p, c->split2, sk, ev ? ev + 4 : nullptr));
-            else
-                HIPCHK(launch_mbpol3b(L.eval3.p, n3, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
+            else if (f == FAM_MBPOL_3B)
+                HIPCHK(launch_mbpol3b(L.evals[f].p, nd, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
                                       o.partials.p, o.ws.p, c->split3, sk, ev ? ev + 8 : nullptr));
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, k == 0 ? n2 : n3, cap,
-                               c->d_totals.p + kPartials * k);
+            else
+                HIPCHK(launch_nrg_family(f, L.evals[f].p, nd, cap, N, c->d_coefn[f].p, do_grads, c->d_grad.p,
+                                         o.e_nmer.p, vir, o.partials.p, o.ws.p, sk, nullptr));
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
+                               c->d_fam_tot.p + kPartials * f);
             if (batch)
                 hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf), dim3(256), 0, sk, o.e_nmer.p, vir, ld,
-                                   L.offsets.p, c->d_pt_off.p, k == 0 ? 1 : 3, cap,
-                                   c->d_conf_tot.p + (size_t)k * kPartials * c->nconf);
+                                   f == FAM_MBPOL_2B || f == FAM_MBPOL_3B ? L.offsets.p : L.offsets_x.p,
+                                   c->d_pt_off.p, fam_component(f), cap,
+                                   c->d_conf_fam.p + (size_t)f * kPartials * c->nconf);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(c->join, c->stream2));
         HIPCHK(hipStreamWaitEvent(st, c->join, 0));
     }
+    // per-order totals: 2B = MB-pol + CH4-H2O + CH4-CH4, 3B = MB-pol + CH4-H2O-H2O (fixed order)
+    hipLaunchKernelGGL(k_combine_families, dim3(1), dim3(64), 0, st, c->d_fam_tot.p, 1, c->d_totals.p);
+    if (c->nconf > 1)
+        hipLaunchKernelGGL(k_combine_families, dim3((c->nconf * kPartials + 63) / 64), dim3(64), 0, st,
+                           c->d_conf_fam.p, c->nconf, c->d_conf_tot.p);
+    HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[12], st));
     c->have_results = true;
     c->checked = sync || !L.built;
@@ -721,12 +848,14 @@ int finish(mbxg_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!c->have_results || c->checked) return MBXG_OK;
     ListState& L = c->L;
-    int4 tot;
+    int4 tot, totx;
     int ovf = 0;
     HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (tot.x > L.cap.x || tot.y > L.cap.y || tot.z > L.cap.z || tot.w > L.cap.w || ovf > L.match_cap) {
+    if (tot.x > L.cap.x || tot.y > L.cap.y || tot.z > L.cap.z || tot.w > L.cap.w || totx.x > L.capx.x ||
+        totx.y > L.capx.y || totx.z > L.capx.z || ovf > L.match_cap) {
         c->need_sync = true;
         if (c->inputs_changed) {
             c->have_results = false;
@@ -737,6 +866,7 @@ int finish(mbxg_ctx* c) {
         HIPCHK(hipStreamSynchronize(c->stream));
     } else {
         L.tot = tot;
+        L.totx = totx;
     }
     c->checked = true;
     return MBXG_OK;
@@ -749,8 +879,8 @@ extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_compute: coordinates not set");
     if (!(terms & MBXG_LISTS_ONLY))
         for (int i = 0; i < c->nmon; ++i)
-            if (c->kind[i] == MBXG_CH4)
-                return fail(MBXG_EINVAL, "mbxg_compute: no GPU holder for ch4-containing n-mers yet (lists only)");
+            if (c->kind[i] != MBXG_H2O && c->kind[i] != MBXG_CH4)
+                return fail(MBXG_EINVAL, "mbxg_compute: monomer %d: only h2o and ch4 have GPU holders (lists only)", i);
     HIPCHK(hipSetDevice(c->device));
     c->have_results = false;
     c->last_terms = terms;
@@ -960,10 +1090,11 @@ extern "C" int mbxg_get_counts(mbxg_ctx* c, int64_t* counts) {
     if (!c || !counts) return fail(MBXG_EINVAL, "mbxg_get_counts: bad arguments");
     if (int r = finish(c)) return r;
     const int4 t = c->have_results ? c->L.tot : make_int4(0, 0, 0, 0);
+    const int4 u = c->have_results ? c->L.totx : make_int4(0, 0, 0, 0);
     counts[0] = t.x;
-    counts[1] = c->evaluated ? t.y : 0;
+    counts[1] = c->evaluated ? (int64_t)t.y + u.x + u.y : 0;
     counts[2] = t.z;
-    counts[3] = c->evaluated ? t.w : 0;
+    counts[3] = c->evaluated ? (int64_t)t.w + u.z : 0;
     return MBXG_OK;
 }
 
@@ -988,13 +1119,19 @@ extern "C" int mbxg_get_nmer_energies(mbxg_ctx* c, int order, int32_t* nmers, do
     if (!c || (order != 2 && order != 3)) return fail(MBXG_EINVAL, "mbxg_get_nmer_energies: bad arguments");
     if (int r = finish(c)) return r;
     if (!c->have_results || !c->evaluated) return MBXG_OK;
-    const int64_t tot = order == 2 ? c->L.tot.y : c->L.tot.w;
-    const int* list = order == 2 ? c->L.eval2.p : c->L.eval3.p;
-    const double* en = order == 2 ? c->ev2.e_nmer.p : c->ev3.e_nmer.p;
-    const int64_t n = tot < cap ? tot : cap;
-    if (n > 0) {
-        if (nmers) HIPCHK(hipMemcpyAsync(nmers, list, sizeof(int32_t) * order * n, hipMemcpyDeviceToHost, c->stream));
-        if (energies) HIPCHK(hipMemcpyAsync(energies, en, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    // families of this order, in family order (MB-pol first)
+    int64_t done = 0;
+    for (int f = 0; f < kFamilies && done < cap; ++f) {
+        if (family_order(f) != order) continue;
+        const int64_t n = std::min<int64_t>(fam_tot(c->L, f), cap - done);
+        if (n <= 0) continue;
+        if (nmers)
+            HIPCHK(hipMemcpyAsync(nmers + order * done, c->L.evals[f].p, sizeof(int32_t) * order * n,
+                                  hipMemcpyDeviceToHost, c->stream));
+        if (energies)
+            HIPCHK(hipMemcpyAsync(energies + done, c->fam[f].e_nmer.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                  c->stream));
+        done += n;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return MBXG_OK;

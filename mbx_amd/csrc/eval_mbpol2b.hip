@@ -226,8 +226,8 @@ __global__ void __launch_bounds__(kGradBlock2) k_grad_mbpol2b(const int* __restr
                 for (int at = 0; at < 6; ++at) acc -= s[3 * at + va[q]] * gd[3 * at + vb[q]];
                 vir[q] = acc;
             }
-            table_add(T, m1, gd, grad, 3 * (size_t)N.first_atom[m1]);
-            table_add(T, m2, gd + 9, grad, 3 * (size_t)N.first_atom[m2]);
+            table_add(T, m1, gd, 9, grad, 3 * (size_t)N.first_atom[m1]);
+            table_add(T, m2, gd + 9, 9, grad, 3 * (size_t)N.first_atom[m2]);
         }
         const double e = sw * P;
         e_out[t] = e;
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(kGradBlock2) k_grad_mbpol2b(const int* __restr
     }
     if (GRAD) {
         __syncthreads();
-        table_flush(T, N.first_atom, grad);
+        table_flush(T, N.first_atom, N.nat, grad);
     }
     e_sum = wave_sum(e_sum);
     const int rec = blockIdx.x * (kGradBlock2 / 64) + threadIdx.x / 64;

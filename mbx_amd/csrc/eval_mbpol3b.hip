@@ -185,9 +185,9 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
                 for (int at = 0; at < 9; ++at) acc -= w[3 * at + va[q]] * gd[3 * at + vb[q]];
                 vir[q] = acc;
             }
-            table_add(T, m1, gd, grad, 3 * (size_t)N.first_atom[m1]);
-            table_add(T, m2, gd + 9, grad, 3 * (size_t)N.first_atom[m2]);
-            table_add(T, m3, gd + 18, grad, 3 * (size_t)N.first_atom[m3]);
+            table_add(T, m1, gd, 9, grad, 3 * (size_t)N.first_atom[m1]);
+            table_add(T, m2, gd + 9, 9, grad, 3 * (size_t)N.first_atom[m2]);
+            table_add(T, m3, gd + 18, 9, grad, 3 * (size_t)N.first_atom[m3]);
         }
         const double e = s * P;
         e_out[t] = e;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
     }
     if (GRAD) {
         __syncthreads();
-        table_flush(T, N.first_atom, grad);
+        table_flush(T, N.first_atom, N.nat, grad);
     }
     // one partial record per wavefront
     e_sum = wave_sum(e_sum);

@@ -31,6 +31,10 @@
 // This file is compiled with -ffp-contract=off.
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
+#include "nrg_kernel.hpp"
+#include "gen/holder_ch4ch4_2b.inc"
+#include "gen/holder_ch4h2o2_3b.inc"
+#include "gen/holder_ch4h2o_2b.inc"
 
 namespace mbxg {
 
@@ -335,13 +339,17 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
                             const double* __restrict__ bx, const double* __restrict__ by,
                             const double* __restrict__ bz, int np, const Grid& G, int base, const NmerCtx& N,
                             double c2sq, double c3sq, int pass, int4* __restrict__ counts,
-                            const int4* __restrict__ offsets, int* __restrict__ list2, int* __restrict__ eval2,
-                            int* __restrict__ list3, int* __restrict__ eval3, int4 cap, int match_cap,
+                            int4* __restrict__ counts_x, const int4* __restrict__ offsets,
+                            const int4* __restrict__ offsets_x, int* __restrict__ list2, int* __restrict__ list3,
+                            int* const* evals, int fam_mask, int4 cap, int4 capx, int match_cap,
                             int* __restrict__ overflow, const Lds& L) {
     const int lane = threadIdx.x;
     const int mi = N.members[q];
     if (!N.islocal[mi]) {
-        if (pass == 0 && lane == 0) counts[q] = make_int4(0, 0, 0, 0);
+        if (pass == 0 && lane == 0) {
+            counts[q] = make_int4(0, 0, 0, 0);
+            counts_x[q] = make_int4(0, 0, 0, 0);
+        }
         return;
     }
     const double cmsq = c2sq > c3sq ? c2sq : c3sq;
@@ -426,48 +434,71 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
 
     const unsigned long long below = (1ull << lane) - 1ull;
     const int4 at = pass ? offsets[q] : make_int4(0, 0, 0, 0);
-    int nl2 = 0, ne2 = 0, nl3 = 0, ne3 = 0;
+    const int4 atx = pass ? offsets_x[q] : make_int4(0, 0, 0, 0);
+    // evaluate-list bases and capacities per family (FAM_*)
+    const int fbase[kFamilies] = {at.y, atx.x, atx.y, at.w, atx.z};
+    const int fcap[kFamilies] = {cap.y, capx.x, capx.y, cap.w, capx.z};
+    int nl2 = 0, nl3 = 0;
+    int ne[kFamilies] = {0, 0, 0, 0, 0};
 
-    // ---- dimers (AddClusters sys_tools.cpp:558-567; evaluation rule x2b-v9x.cpp:1589-1600)
+    // ---- dimers (AddClusters sys_tools.cpp:558-567).  Evaluation rules: MB-pol r_OO <= 6.5
+    // (x2b-v9x.cpp:1589-1600); the MB-nrg holders skip r > r2f = 9 on the first atoms
+    // (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:840, x2b_A1B4_A1B4_deg4_exp0_v1x.cpp:1489).
     if (c2sq > 0.0) {
         for (int b = 0; b < nm; b += 64) {
             const int a = b + lane;
-            bool listed = false, ev = false;
-            int mj = 0;
+            bool listed = false;
+            int mj = 0, fam = -1;
             if (a < nm && (L.flags[a] & 1)) {
                 mj = L.mon[a];
                 listed = ti < L.tag[a];
-                ev = listed && ki == MON_H2O && L.kind[a] == MON_H2O && !(L.r[a] > p2::r2f);
-            }
-            const unsigned long long ml = __ballot(listed), me = __ballot(ev);
-            if (pass) {
                 if (listed) {
-                    const int w = at.x + nl2 + __popcll(ml & below);
-                    if (w < cap.x) {
-                        list2[2 * w] = mi;
-                        list2[2 * w + 1] = mj;
+                    const int kj = L.kind[a];
+                    const double r = L.r[a];
+                    if (ki == MON_H2O && kj == MON_H2O) {
+                        if (!(r > p2::r2f)) fam = FAM_MBPOL_2B;
+                    } else if ((ki == MON_H2O && kj == MON_CH4) || (ki == MON_CH4 && kj == MON_H2O)) {
+                        if (!(r > Holder_ch4h2o_2b::kROut)) fam = FAM_CH4H2O_2B;
+                    } else if (ki == MON_CH4 && kj == MON_CH4) {
+                        if (!(r > Holder_ch4ch4_2b::kROut)) fam = FAM_CH4CH4_2B;
                     }
                 }
-                if (ev) {
-                    const int w = at.y + ne2 + __popcll(me & below);
-                    if (w < cap.y) {
-                        eval2[2 * w] = mi;
-                        eval2[2 * w + 1] = mj;
-                    }
+            }
+            if (fam >= 0 && !((fam_mask >> fam) & 1)) fam = -1;
+            const unsigned long long ml = __ballot(listed);
+            if (pass && listed) {
+                const int w = at.x + nl2 + __popcll(ml & below);
+                if (w < cap.x) {
+                    list2[2 * w] = mi;
+                    list2[2 * w + 1] = mj;
                 }
             }
             nl2 += __popcll(ml);
-            ne2 += __popcll(me);
+#pragma unroll
+            for (int f = FAM_MBPOL_2B; f <= FAM_CH4CH4_2B; ++f) {
+                const unsigned long long me = __ballot(fam == f);
+                if (pass && fam == f) {
+                    const int w = fbase[f] + ne[f] + __popcll(me & below);
+                    if (w < fcap[f]) {
+                        evals[f][2 * w] = mi;
+                        evals[f][2 * w + 1] = mj;
+                    }
+                }
+                ne[f] += __popcll(me);
+            }
         }
     }
 
-    // ---- trimers (AddClusters sys_tools.cpp:570-592; evaluation rule x3b-v2x.cpp:1495-1515)
+    // ---- trimers (AddClusters sys_tools.cpp:570-592).  Evaluation rules: MB-pol skips a trimer
+    // when any two O-O distances exceed 4.5 (x3b-v2x.cpp:1495-1515); the CH4-H2O-H2O holder
+    // only when all three first-atom distances exceed 6 (mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1364);
+    // other mixtures have no 3B term (energy3b.cpp:258-260).
     if (c3sq > 0.0) {
         const int total = n3 * n3;
         for (int b = 0; b < total; b += 64) {
             const int t = b + lane;
-            bool listed = false, ev = false;
-            int mj = 0, mk = 0;
+            bool listed = false;
+            int mj = 0, mk = 0, fam = -1;
             if (t < total) {
                 const int u = t / n3, v = t % n3;
                 if (u < v) {
@@ -477,7 +508,9 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
                     if (mj != mi && mk != mi && mj != mk) {
                         const double djk = dist2(L.sx[ja], L.sy[ja], L.sz[ja], L.sx[ka], L.sy[ka], L.sz[ka]);
                         listed = djk > c3sq || (ti < L.tag[ja] && ti < L.tag[ka]);
-                        if (listed && ki == MON_H2O && L.kind[ja] == MON_H2O && L.kind[ka] == MON_H2O) {
+                        const int nch4 = (ki == MON_CH4) + (L.kind[ja] == MON_CH4) + (L.kind[ka] == MON_CH4);
+                        const int nh2o = (ki == MON_H2O) + (L.kind[ja] == MON_H2O) + (L.kind[ka] == MON_H2O);
+                        if (listed && (nh2o == 3 || (nh2o == 2 && nch4 == 1))) {
                             const double e0 = L.ix[ja] - L.ix[ka], e1 = L.iy[ja] - L.iy[ka],
                                          e2 = L.iz[ja] - L.iz[ka];
                             double rr = 0.0;
@@ -485,44 +518,57 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
                             rr += e1 * e1;
                             rr += e2 * e2;
                             const double rab = L.r[ja], rac = L.r[ka], rbc = sqrt(rr);
-                            const double f = p3::r3f;
-                            ev = !((rab > f && rac > f) || (rab > f && rbc > f) || (rac > f && rbc > f));
+                            if (nh2o == 3) {
+                                const double f = p3::r3f;
+                                if (!((rab > f && rac > f) || (rab > f && rbc > f) || (rac > f && rbc > f)))
+                                    fam = FAM_MBPOL_3B;
+                            } else {
+                                const double f = Holder_ch4h2o2_3b::kROut;
+                                if (!(rab > f && rac > f && rbc > f)) fam = FAM_CH4H2O2_3B;
+                            }
                         }
                     }
                 }
             }
-            const unsigned long long ml = __ballot(listed), me = __ballot(ev);
-            if (pass) {
-                if (listed) {
-                    const int w = at.z + nl3 + __popcll(ml & below);
-                    if (w < cap.z) {
-                        list3[3 * w] = mi;
-                        list3[3 * w + 1] = mj;
-                        list3[3 * w + 2] = mk;
-                    }
-                }
-                if (ev) {
-                    const int w = at.w + ne3 + __popcll(me & below);
-                    if (w < cap.w) {
-                        eval3[3 * w] = mi;
-                        eval3[3 * w + 1] = mj;
-                        eval3[3 * w + 2] = mk;
-                    }
+            if (fam >= 0 && !((fam_mask >> fam) & 1)) fam = -1;
+            const unsigned long long ml = __ballot(listed);
+            if (pass && listed) {
+                const int w = at.z + nl3 + __popcll(ml & below);
+                if (w < cap.z) {
+                    list3[3 * w] = mi;
+                    list3[3 * w + 1] = mj;
+                    list3[3 * w + 2] = mk;
                 }
             }
             nl3 += __popcll(ml);
-            ne3 += __popcll(me);
+#pragma unroll
+            for (int f = FAM_MBPOL_3B; f <= FAM_CH4H2O2_3B; ++f) {
+                const unsigned long long me = __ballot(fam == f);
+                if (pass && fam == f) {
+                    const int w = fbase[f] + ne[f] + __popcll(me & below);
+                    if (w < fcap[f]) {
+                        evals[f][3 * w] = mi;
+                        evals[f][3 * w + 1] = mj;
+                        evals[f][3 * w + 2] = mk;
+                    }
+                }
+                ne[f] += __popcll(me);
+            }
         }
     }
-    if (pass == 0 && lane == 0) counts[q] = make_int4(nl2, ne2, nl3, ne3);
+    if (pass == 0 && lane == 0) {
+        counts[q] = make_int4(nl2, ne[FAM_MBPOL_2B], nl3, ne[FAM_MBPOL_3B]);
+        counts_x[q] = make_int4(ne[FAM_CH4H2O_2B], ne[FAM_CH4CH4_2B], ne[FAM_CH4H2O2_3B], 0);
+    }
 }
 
-// pass 0: counts[q] = {listed 2B, evaluated 2B, listed 3B, evaluated 3B};
-// pass 1: writes at offsets[q], bounded by cap (capacities in the same order).
+// pass 0: counts[q] = {listed 2B, evaluated MB-pol 2B, listed 3B, evaluated MB-pol 3B},
+// counts_x[q] = {evaluated CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, 0};
+// pass 1: writes at offsets[q] / offsets_x[q], bounded by cap / capx (same orders).
 // overflow[0] receives the largest match count of any query (the host compares it with
 // match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
 // the dynamic LDS, otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.
-__global__ void __launch_bounds__(64) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int match_cap,
+__global__ void __launch_bounds__(64) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap,
                                               double* __restrict__ gscratch) {
     extern __shared__ double smem[];
     double* store = gscratch ? gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles : smem;
@@ -531,8 +577,8 @@ __global__ void __launch_bounds__(64) k_nmers(NmersArgs A, NmerCtx N, int pass, 
         const int conf = A.pconf[q];
         const Grid G = A.grids[conf];
         nmers_query(q, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
-                    A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.offsets, A.list2, A.eval2, A.list3,
-                    A.eval3, cap, match_cap, A.overflow, L);
+                    A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.counts_x, A.offsets, A.offsets_x,
+                    A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, A.overflow, L);
         __syncthreads();
     }
 }
@@ -638,14 +684,25 @@ __global__ void __launch_bounds__(256) k_reduce_configs(const double* __restrict
     if (t < kPartials) out[(size_t)c * kPartials + t] = sh[t][0];
 }
 
+// Per-order records from per-family ones (kPartials doubles per record, nconf configurations):
+// out[0][c] = fam[MB-pol 2B][c] + fam[CH4-H2O 2B][c] + fam[CH4-CH4 2B][c],
+// out[1][c] = fam[MB-pol 3B][c] + fam[CH4-H2O-H2O 3B][c].
+__global__ void k_combine_families(const double* __restrict__ fam, int nconf, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nconf * kPartials) return;
+    const size_t st = (size_t)nconf * kPartials;
+    out[i] = fam[FAM_MBPOL_2B * st + i] + fam[FAM_CH4H2O_2B * st + i] + fam[FAM_CH4CH4_2B * st + i];
+    out[st + i] = fam[FAM_MBPOL_3B * st + i] + fam[FAM_CH4H2O2_3B * st + i];
+}
+
 size_t nmers_lds_bytes(int match_cap) { return (size_t)match_cap * kMatchDoubles * sizeof(double); }
 
-hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int match_cap, double* gscratch,
-                        int nblocks, hipStream_t st) {
+hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int4 capx, int match_cap,
+                        double* gscratch, int nblocks, hipStream_t st) {
     if (A.np <= 0) return hipSuccess;
     const int nb = gscratch ? (nblocks < A.np ? nblocks : A.np) : A.np;
     const size_t lds = gscratch ? 0 : nmers_lds_bytes(match_cap);
-    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, A, N, pass, cap, match_cap, gscratch);
+    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, A, N, pass, cap, capx, match_cap, gscratch);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ struct Grid {
 struct NmerCtx {
     const double* xyz;
     const int* first_atom;
+    const int* nat;
     const int* kind;
     const int* tag;
     const unsigned char* islocal;
@@ -55,6 +56,12 @@ __global__ void k_zero_int(int* p, size_t n);
 // cell_base[conf] and its grid is grids[conf], so lists never cross configurations.  Slots in
 // cell order: cell_slot (slot index), bx/by/bz (coordinates), cell c holding
 // [cell_start[c], cell_start[c+1]).
+// Evaluated n-mer families (one evaluate list each): MB-pol 2B, MB-nrg CH4-H2O 2B, CH4-CH4 2B,
+// MB-pol 3B, MB-nrg CH4-H2O-H2O 3B.
+enum Family : int { FAM_MBPOL_2B = 0, FAM_CH4H2O_2B = 1, FAM_CH4CH4_2B = 2, FAM_MBPOL_3B = 3, FAM_CH4H2O2_3B = 4 };
+constexpr int kFamilies = 5;
+constexpr int family_order(int f) { return f < FAM_MBPOL_3B ? 2 : 3; }
+
 struct NmersArgs {
     const double* cx;
     const double* cy;
@@ -71,11 +78,13 @@ struct NmersArgs {
     int np;
     double c2sq, c3sq;
     int4* counts;
+    int4* counts_x;
     const int4* offsets;
+    const int4* offsets_x;
     int* list2;
-    int* eval2;
     int* list3;
-    int* eval3;
+    int* evals[kFamilies];
+    int fam_mask;   // bit f: family f is evaluated (listed n-mers of a cleared family are not)
     int* overflow;  // largest match count seen (zeroed before pass 0)
 };
 // match_cap (a power of two) bounds the matches one query may hold.  With gscratch == nullptr
@@ -83,8 +92,8 @@ struct NmersArgs {
 // nblocks blocks stride over the queries, each with nmers_lds_bytes(match_cap) bytes of
 // gscratch.
 size_t nmers_lds_bytes(int match_cap);
-hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int match_cap, double* gscratch,
-                        int nblocks, hipStream_t st);
+hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap, int4 capx, int match_cap,
+                        double* gscratch, int nblocks, hipStream_t st);
 
 // Evaluation of the compacted "evaluate" lists (eval_mbpol2b.hip, eval_mbpol3b.hip): variable
 // stage, polynomial stage (poly_mbpol2b.hip, poly_mbpol3b.hip), chain-rule stage.  One lane per
@@ -104,6 +113,14 @@ hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const Nmer
 hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
                           int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
                           int split, hipStream_t st, hipEvent_t* ev);
+// MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and
+// FAM_CH4H2O2_3B; list entries in emission order (the kernels order each n-mer's monomers
+// for its holder).
+size_t workspace_nrg_family(int family, int cap);
+hipError_t launch_nrg_family(int family, const int* list, const int* n_dev, int cap, const NmerCtx& N,
+                             const double* coef, int do_grads, double* grad, double* e_out, double* vir_out,
+                             double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
+
 // Polynomial stage alone: xs/gs variable-major with leading dimension ld (= cap rounded up
 // to 64); ps values; n read from n_dev as above; `split` wavefronts (1, 2, 4 or 8) share each
 // 64 n-mers (poly_kernel.hpp).
@@ -121,6 +138,7 @@ __global__ void k_reduce_partials(const double* partials, const int* n_dev, int 
 // clamped to cap, in a fixed order, into out[c * kPartials ...] (energy, 6 virial components;
 // zeros when vir_out is null).  comp selects the int4 component (1: evaluated dimers,
 // 3: evaluated trimers).
+__global__ void k_combine_families(const double* fam, int nconf, double* out);
 __global__ void k_reduce_configs(const double* e_out, const double* vir_out, int ld, const int4* offsets,
                                  const int* pt_off, int comp, int cap, double* out);
 

@@ -24,9 +24,9 @@ __device__ __forceinline__ void table_init(GradTable<TB, NA>& T) {
     for (int i = threadIdx.x; i < TB * NA; i += blockDim.x) T.val[i] = 0.0;
 }
 
-// Add NA values for monomer `mon`; falls back to device atomics at grad + goff if full.
+// Add cnt (<= NA) values for monomer `mon`; falls back to device atomics at grad + goff if full.
 template <int TB, int NA>
-__device__ __forceinline__ void table_add(GradTable<TB, NA>& T, int mon, const double* g, double* grad,
+__device__ __forceinline__ void table_add(GradTable<TB, NA>& T, int mon, const double* g, int cnt, double* grad,
                                           size_t goff) {
     unsigned h = ((unsigned)mon * 2654435761u) % TB;
     for (int probe = 0; probe < TB; ++probe) {
@@ -34,24 +34,30 @@ __device__ __forceinline__ void table_add(GradTable<TB, NA>& T, int mon, const d
         if (k == -1) k = atomicCAS(&T.key[h], -1, mon);
         if (k == -1 || k == mon) {
 #pragma unroll
-            for (int c = 0; c < NA; ++c) unsafeAtomicAdd(&T.val[h * NA + c], g[c]);
+            for (int c = 0; c < NA; ++c)
+                if (c < cnt) unsafeAtomicAdd(&T.val[h * NA + c], g[c]);
             return;
         }
         h = (h + 1) % TB;
     }
 #pragma unroll
-    for (int c = 0; c < NA; ++c) unsafeAtomicAdd(grad + goff + c, g[c]);
+    for (int c = 0; c < NA; ++c)
+        if (c < cnt) unsafeAtomicAdd(grad + goff + c, g[c]);
 }
 
-// After a barrier: one set of device atomics per occupied entry.
+// After a barrier: one set of device atomics per occupied entry (3 values per atom of the
+// entry's monomer).
 template <int TB, int NA>
-__device__ __forceinline__ void table_flush(const GradTable<TB, NA>& T, const int* first_atom, double* grad) {
+__device__ __forceinline__ void table_flush(const GradTable<TB, NA>& T, const int* first_atom, const int* nat,
+                                            double* grad) {
     for (int i = threadIdx.x; i < TB; i += blockDim.x) {
         const int mon = T.key[i];
         if (mon < 0) continue;
         double* gp = grad + 3 * (size_t)first_atom[mon];
+        const int cnt = 3 * nat[mon];
 #pragma unroll
-        for (int c = 0; c < NA; ++c) unsafeAtomicAdd(gp + c, T.val[i * NA + c]);
+        for (int c = 0; c < NA; ++c)
+            if (c < cnt) unsafeAtomicAdd(gp + c, T.val[i * NA + c]);
     }
 }
 

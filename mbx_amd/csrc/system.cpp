@@ -218,9 +218,15 @@ double System::run_terms(int terms, bool do_grads, bool use_ghost) {
         MBX_THROW("use_ghost (local/ghost monomer decomposition) is not supported by the GPU engine");
     std::fill(grad_.begin(), grad_.end(), 0.0);
     std::fill(virial_.begin(), virial_.end(), 0.0);
-    const bool all_water = std::all_of(in_mon_.begin(), in_mon_.end(), [](const std::string& m) { return m == "h2o"; });
-    if (all_water && ignored(ignore2b_, {"h2o", "h2o"})) terms &= ~MBXG_2B;
-    if (all_water && ignored(ignore3b_, {"h2o", "h2o", "h2o"})) terms &= ~MBXG_3B;
+    // ignore_2b_poly / ignore_3b_poly (system.cpp:2606-2619, 2940-2953): those n-mers are listed
+    // but not evaluated
+    int fam = 0;
+    if (!ignored(ignore2b_, {"h2o", "h2o"})) fam |= MBXG_FAM_MBPOL_2B;
+    if (!ignored(ignore2b_, {"ch4", "h2o"})) fam |= MBXG_FAM_CH4H2O_2B;
+    if (!ignored(ignore2b_, {"ch4", "ch4"})) fam |= MBXG_FAM_CH4CH4_2B;
+    if (!ignored(ignore3b_, {"h2o", "h2o", "h2o"})) fam |= MBXG_FAM_MBPOL_3B;
+    if (!ignored(ignore3b_, {"ch4", "h2o", "h2o"})) fam |= MBXG_FAM_CH4H2O2_3B;
+    check(mbxg_set_families(ctx_, fam));
     check(mbxg_set_cutoffs(ctx_, cutoff2b_, cutoff3b_));
     check(mbxg_set_coords(ctx_, xyz_.data()));
     double e2 = 0.0, e3 = 0.0;

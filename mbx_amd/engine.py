@@ -48,6 +48,7 @@ PROTOTYPES = {
     "mbxg_get_timings": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
     "mbxg_set_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "mbxg_set_families": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
     "mbxg_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
@@ -55,6 +56,7 @@ PROTOTYPES = {
     "mbxg_get_batch_results": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
 }
 
+FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")
 TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
                  "chain_3b", "total")
 
@@ -119,6 +121,10 @@ class Engine:
         self.nmon = n
         self.natoms = int(nats.sum())
         self.nconf = 1
+
+    def set_families(self, families=FAMILIES):
+        """Evaluate only these n-mer families (names as FAMILIES); the others stay listed."""
+        _check(lib().mbxg_set_families(self._c, sum(1 << i for i, f in enumerate(FAMILIES) if f in families)))
 
     def set_batch(self, mon_offsets):
         """Split the monomers into configurations: configuration c = monomers

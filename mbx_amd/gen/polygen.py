@@ -321,6 +321,95 @@ def full_terms(monos):
 NPARTS = 8
 
 
+def write_if_changed(path, text):
+    """Write only when the content differs, so make does not rebuild unchanged kernels."""
+    try:
+        if open(path).read() == text:
+            return
+    except OSError:
+        pass
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def coef_source(name, coef):
+    out = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+           "#pragma once", "static const int kNCoef_%s = %d;" % (name, len(coef)),
+           "static const double kFit_%s[%d] = {" % (name, len(coef))]
+    for i in range(0, len(coef), 4):
+        out.append("    " + ", ".join("%.17g" % v for v in coef[i:i + 4]) + ",")
+    out.append("};")
+    return "\n".join(out) + "\n"
+
+
+# MB-nrg polynomials: each piece is its own translation unit (compiled in parallel) with its
+# own kernel writing a partial value and gradient; the chain-rule stage sums the pieces.
+NRG_PARTS = {"ch4h2o_2b": 1, "ch4h2o2_3b": 2, "ch4ch4_2b": 8}
+
+
+def generate_nrg(name, out_dir):
+    nvars, coef, monos = load_table(name)
+    s = Schedule(nvars, [(k, m) for k, m in monos])
+    q = NRG_PARTS[name]
+    assign = s.partition(q)
+    c = s.counts()
+    for k in range(q):
+        hdr = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+               "// Schedule: %s" % ", ".join("%s=%d" % kv for kv in c.items()),
+               "// Piece %d of %d (op-count balance: %s)." % (k, q, ", ".join(str(v) for v in s.part_load)),
+               "#pragma once", "#include \"poly_support.hpp\"", ""]
+        src = "\n".join(hdr) + s.emit("poly_%s_p%d" % (name, k), grad=True, part=k, assign=assign) + "\n" + \
+            s.emit("poly_%s_e_p%d" % (name, k), grad=False, part=k, assign=assign)
+        write_if_changed(os.path.join(out_dir, "poly_%s_p%d.inc" % (name, k)), src)
+        tu = ["// GENERATED by mbx_amd/gen/polygen.py -- do not edit.",
+              "// Piece %d of %d of the %s polynomial (%d variables): its own kernel, so the pieces "
+              "compile in parallel." % (k, q, name, nvars),
+              "#include \"gen/poly_%s_p%d.inc\"" % (name, k), "#include \"nrg_kernel.hpp\"", "",
+              "namespace mbxg {", "",
+              "hipError_t launch_poly_%s_p%d(const double* xs, const int* n_dev, int cap, int ld, const double* coef, "
+              "int do_grads, double* gs, double* ps, hipStream_t st) {" % (name, k),
+              "    return launch_poly_piece<%d, poly_%s_p%d, poly_%s_e_p%d>(xs, n_dev, cap, ld, coef, do_grads, gs, ps, "
+              "st);" % (nvars, name, k, name, k),
+              "}", "", "}  // namespace mbxg", ""]
+        write_if_changed(os.path.join(out_dir, "poly_%s_p%d.hip" % (name, k)), "\n".join(tu))
+    write_if_changed(os.path.join(out_dir, "coef_%s.inc" % name), coef_source(name, coef))
+    return c
+
+
+def generate_holder(name, out_dir):
+    """gen/holder_<name>.inc: the holder description (mbx_amd/tables/<name>.holder.json) as
+    constexpr tables for the MB-nrg variable and chain-rule kernels."""
+    import json
+
+    h = json.load(open(os.path.join(TABLES, name + ".holder.json")))
+    forms = {"exp0": 0, "coul0": 1, "exp": 2}
+    vs = h["vsite"] or {"oxygen_site": 0, "in_plane": 0.0, "out_of_plane": 0.0}
+    nat = h["atoms"] + [0] * (3 - len(h["atoms"]))
+    kinds = ["MON_%s" % m.upper() for m in h["monomers"]] + ["-1"] * (3 - len(h["monomers"]))
+    out = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.holder.json -- do not edit." % name,
+           "#pragma once", "#include \"mbxg_device.hpp\"", "#include \"nrg_kernel.hpp\"", "",
+           "namespace mbxg {", "",
+           "struct Holder_%s {" % name,
+           "    static constexpr int kMon = %d;" % len(h["monomers"]),
+           "    static constexpr int kNat[3] = {%s};" % ", ".join(str(v) for v in nat),
+           "    static constexpr int kKind[3] = {%s};" % ", ".join(kinds),
+           "    static constexpr int kNReal = %d;" % h["nreal"],
+           "    static constexpr int kNSites = %d;" % h["nsites"],
+           "    static constexpr bool kVsite = %s;" % ("true" if h["vsite"] else "false"),
+           "    static constexpr int kOSite = %d;" % vs["oxygen_site"],
+           "    static constexpr double kInPlane = %.17g;" % vs["in_plane"],
+           "    static constexpr double kOutOfPlane = %.17g;" % vs["out_of_plane"],
+           "    static constexpr double kRIn = %.17g;" % h["switch"]["inner"],
+           "    static constexpr double kROut = %.17g;" % h["switch"]["outer"],
+           "    static constexpr int kNVars = %d;" % len(h["vars"]),
+           "    static constexpr int kPieces = %d;" % NRG_PARTS[name],
+           "    static constexpr NrgVar kVars[%d] = {" % len(h["vars"])]
+    for f, k, d, s1, s2 in h["vars"]:
+        out.append("        {%d, %.17g, %.17g, %d, %d}," % (forms[f], k, d, s1, s2))
+    out += ["    };", "};", "", "}  // namespace mbxg", ""]
+    write_if_changed(os.path.join(out_dir, "holder_%s.inc" % name), "\n".join(out))
+
+
 def generate(name, fn, out_dir):
     nvars, coef, monos = load_table(name)
     terms, _ = full_terms(monos)
@@ -350,18 +439,9 @@ def generate(name, fn, out_dir):
         body.append("\n".join(d))
     src = "\n".join(hdr) + "\n".join(body)
     path = os.path.join(out_dir, "poly_%s.inc" % name)
-    with open(path, "w") as f:
-        f.write(src)
+    write_if_changed(path, src)
     # fitted coefficients, for the host runtime's default table
-    cpath = os.path.join(out_dir, "coef_%s.inc" % name)
-    with open(cpath, "w") as f:
-        f.write("// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit.\n" % name)
-        f.write("#pragma once\n")
-        f.write("static const int kNCoef_%s = %d;\n" % (name, len(coef)))
-        f.write("static const double kFit_%s[%d] = {\n" % (name, len(coef)))
-        for i in range(0, len(coef), 4):
-            f.write("    " + ", ".join("%.17g" % v for v in coef[i:i + 4]) + ",\n")
-        f.write("};\n")
+    write_if_changed(os.path.join(out_dir, "coef_%s.inc" % name), coef_source(name, coef))
     return path, c
 
 
@@ -371,6 +451,9 @@ def main():
     for name, fn in (("mbpol_2b", "poly_mbpol_2b"), ("mbpol_3b", "poly_mbpol_3b")):
         path, c = generate(name, fn, out_dir)
         print(path, c)
+    for name in NRG_PARTS:
+        print(name, generate_nrg(name, out_dir))
+        generate_holder(name, out_dir)
 
 
 if __name__ == "__main__":

@@ -153,7 +153,8 @@ def lib():
         H = ctypes.POINTER(_OrcHolder)
         L.orc_nrg_holder.restype = ctypes.c_double
         L.orc_nrg_holder.argtypes = [H, P, _dp, ctypes.POINTER(_dp), ctypes.POINTER(_dp), _dp, _ip]
-        L.orc_system.argtypes = [P, P, ctypes.POINTER(H), ctypes.POINTER(P), ctypes.c_int, _ip, _ip, _ip, _dp, _dp,
+        L.orc_system.argtypes = [P, P, ctypes.POINTER(H), ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, _ip, _ip, _ip,
+                                 _dp, _dp,
                                  ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp, _lp]
         _LIB = L
     return _LIB
@@ -299,9 +300,15 @@ def nrg(name, mons, a=None):
 KIND_NAT = {0: 3, 1: 5}
 
 
-def system(xyz, kinds, box, cutoff2b, cutoff3b, tags=None, families=NRG):
+FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")
+
+
+def system(xyz, kinds, box, cutoff2b, cutoff3b, tags=None, families=FAMILIES, conditioning=False):
     """2B+3B energy/gradient of a mixed H2O (kind 0) / CH4 (kind 1) system: 3 doubles per atom,
-    monomers in the given order; tags default to each monomer's first-atom index."""
+    monomers in the given order; tags default to each monomer's first-atom index.  Families not
+    in `families` are listed but not evaluated (the System's ignore lists).  conditioning=True
+    evaluates with |coefficients| (see abs_table): the scale of the cancelling terms."""
+    tab = abs_table if conditioning else table
     kinds = np.ascontiguousarray(kinds, np.int32)
     nat = np.array([KIND_NAT[int(k)] for k in kinds], np.int32)
     first = np.concatenate([[0], np.cumsum(nat)[:-1]]).astype(np.int32)
@@ -314,9 +321,10 @@ def system(xyz, kinds, box, cutoff2b, cutoff3b, tags=None, families=NRG):
     counts = np.zeros(4, np.int64)
     H = ctypes.POINTER(_OrcHolder)
     P = ctypes.POINTER(_OrcPoly)
-    hs = (H * 3)(*[ctypes.pointer(holder(n).c) if n in families else H() for n in NRG])
-    ps = (P * 3)(*[ctypes.pointer(table(n).c) if n in families else P() for n in NRG])
-    rc = lib().orc_system(ctypes.byref(table("mbpol_2b").c), ctypes.byref(table("mbpol_3b").c), hs, ps, len(kinds),
+    hs = (H * 3)(*[ctypes.pointer(holder(n).c) for n in NRG])
+    ps = (P * 3)(*[ctypes.pointer(tab(n).c) for n in NRG])
+    mask = sum(1 << i for i, f in enumerate(FAMILIES) if f in families)
+    rc = lib().orc_system(ctypes.byref(tab("mbpol_2b").c), ctypes.byref(tab("mbpol_3b").c), hs, ps, mask, len(kinds),
                           _i(kinds), _i(nat), _i(tag), _d(xyz), _d(b), cutoff2b, cutoff3b, ctypes.byref(e2),
                           ctypes.byref(e3), _d(grad), _d(vir), counts.ctypes.data_as(_lp))
     if rc:

@@ -905,8 +905,8 @@ static int nmer_family(int order, const int* kinds, int* perm) {
         }
         return -1;
     }
-    /* the exchange network of e3b::get_3b_energy on the names "ch4" < "h2o" */
-    int k[3] = {kinds[0], kinds[1], kinds[2]}, ix[3] = {0, 1, 2}, t;
+    /* the exchange network of e3b::get_3b_energy on the names: rank 0 = "ch4" < rank 1 = "h2o" */
+    int k[3] = {kinds[0] == 1 ? 0 : 1, kinds[1] == 1 ? 0 : 1, kinds[2] == 1 ? 0 : 1}, ix[3] = {0, 1, 2}, t;
 #define SWP(i, j) (t = k[i], k[i] = k[j], k[j] = t, t = ix[i], ix[i] = ix[j], ix[j] = t)
     if (k[2] < k[0] && k[2] < k[1]) SWP(0, 2);
     if (k[0] > k[1] && k[0] > k[2])
@@ -918,15 +918,16 @@ static int nmer_family(int order, const int* kinds, int* perm) {
     perm[0] = ix[0];
     perm[1] = ix[1];
     perm[2] = ix[2];
-    if (k[0] == 0 && k[1] == 0 && k[2] == 0) return 3;
-    if (k[0] == 1 && k[1] == 0 && k[2] == 0) return 4;
+    if (kinds[0] == 0 && kinds[1] == 0 && kinds[2] == 0) return 3;
+    if (k[0] == 0 && k[1] == 1 && k[2] == 1 && kinds[perm[0]] == 1 && kinds[perm[1]] == 0 && kinds[perm[2]] == 0)
+        return 4;
     return -1;
 }
 
 int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* hol, const orc_poly* const* hp,
-               int nmon, const int* kind, const int* nat, const int* tag, double* xyz, const double* box,
-               double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
-               long* counts) {
+               int fam_mask, int nmon, const int* kind, const int* nat, const int* tag, double* xyz,
+               const double* box, double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad,
+               double* virial, long* counts) {
     double bi[9];
     const int pbc = box != NULL;
     int* fi = (int*)malloc(sizeof(int) * nmon);
@@ -971,7 +972,7 @@ int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* 
                 if (pbc)
                     for (int m = 1; m < order; ++m) orc_close_image(box, bi, w[0], nat[id[m]], w[m]);
                 const int fam = nmer_family(order, kd, perm);
-                if (fam < 0 || (fam != 0 && fam != 3 && !hol[fam == 4 ? 2 : fam - 1])) continue;
+                if (fam < 0 || !((fam_mask >> fam) & 1)) continue;
                 int ev = 0;
                 double e;
                 if (fam == 0) {

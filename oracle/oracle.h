@@ -122,12 +122,15 @@ int orc_water_system(const orc_poly* p2, const orc_poly* p3, int nmon, double* x
  * (potential/2b/energy2b.cpp:206-411: names sorted, ch4-h2o through the (h2o, ch4) holder)
  * and e3b::get_3b_energy (potential/3b/energy3b.cpp:139-310: names sorted by the same
  * exchange network; ch4-ch4-h2o and ch4x3 have no 3B term).  hol[0..2] / hp[0..2]: the
- * ch4h2o_2b, ch4ch4_2b and ch4h2o2_3b holders and polynomials (NULL skips that family).
+ * ch4h2o_2b, ch4ch4_2b and ch4h2o2_3b holders and polynomials.  fam_mask: bit f evaluates
+ * family f (0 MB-pol 2B, 1 CH4-H2O 2B, 2 CH4-CH4 2B, 3 MB-pol 3B, 4 CH4-H2O-H2O 3B); a cleared
+ * bit is the System's ignore_2b_poly / ignore_3b_poly for that family (system.cpp:2606-2619,
+ * 2940-2953): listed, not evaluated.
  * xyz/grad: 3 doubles per atom.  counts[4] as orc_water_system. */
 int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* hol, const orc_poly* const* hp,
-               int nmon, const int* kind, const int* nat, const int* tag, double* xyz, const double* box,
-               double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad, double* virial,
-               long* counts);
+               int fam_mask, int nmon, const int* kind, const int* nat, const int* tag, double* xyz,
+               const double* box, double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad,
+               double* virial, long* counts);
 
 #ifdef __cplusplus
 }

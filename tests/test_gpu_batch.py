@@ -33,9 +33,9 @@ def close(got, want, scale):
     """Energies are sums of n-mer energies, each a sum of strongly cancelling monomials; a small
     cluster's total can be many orders below its terms (a 100-water cluster: E3B = -0.14 from
     sum |a_k M_k| = 1e7).  No FP64 evaluation order is better than ~eps * scale there, so the
-    bar is 1e-10 relative to the total or 1e-17 * scale (a tenth of one rounding of the
+    bar is 1e-10 relative to the total or 1e-16 * scale (just under one unit roundoff of the
     terms' magnitude), whichever is looser; scale from oracle.water_conditioning."""
-    return abs(got - want) <= max(E_REL * abs(want), 1e-17 * scale, 1e-13)
+    return abs(got - want) <= max(E_REL * abs(want), 1e-16 * scale, 1e-13)
 
 
 def _check(parts, box, c2, c3):

@@ -142,10 +142,10 @@ def test_errors():
         e.set_coords(np.zeros(5))
     with pytest.raises(engine.MbxgError):
         e.set_box([1.0, 0, 0, 0, 0, 0, 0, 0, 1.0])  # singular
-    e.set_topology([engine.CH4, engine.H2O])
-    e.set_coords(np.zeros(24))
+    e.set_topology([engine.OTHER, engine.H2O], nats=[2, 3])
+    e.set_coords(np.zeros(15))
     with pytest.raises(engine.MbxgError):
-        e.compute()  # ch4 holders are not on the GPU yet
+        e.compute()  # no GPU holder for this kind (lists only)
 
 
 def test_small_and_empty():

@@ -1,0 +1,205 @@
+"""CPU tests: pin the oracle's MB-nrg CH4 families (orc_nrg_holder, orc_system) to the reference.
+
+Pins:
+  * polynomials  -- the reference's 21-digit KATs (unittest-poly-2b.cpp
+                    :poly_2b_A1B2Z2_C1D4_deg3_exp0_v1x:struct and poly_2b_A1B4_A1B4_deg4_exp0_v1x
+                    (energy AND gradient: the CH4-CH4 E+G source is absent from the reference, its
+                    KAT is not); unittest-poly-3b.cpp poly_3b_A1B4_C1D2_C1D2_deg3_v1.h);
+                    linearity / Euler's identity;
+  * holders      -- unittest-poly-holder-2b/3b KATs (energy, gradients, virial); the compiled
+                    reference holders (CH4-H2O 2B, CH4-H2O-H2O 3B) on perturbed geometries;
+                    five-point finite differences (all three, CH4-CH4 included);
+  * dispatch     -- unittest-energy2b.cpp / unittest-energy3b.cpp KATs in every monomer order the
+                    reference tests (the name sort of get_2b_energy / get_3b_energy);
+  * system       -- the published per-term energies of timing/57_ch4_513_h2o_rectangular,
+                    342_ch4_228_h2o_rectangular and clathrate, with their mbx.json ignore lists.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+import oracle
+
+KATS = json.load(open(os.path.join(GOLDEN, "kats.json")))
+ref_only = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference checkout)")
+NRG = ["ch4h2o_2b", "ch4ch4_2b", "ch4h2o2_3b"]
+
+
+@pytest.mark.parametrize("name", NRG)
+def test_poly_kat(name):
+    k = KATS["poly_" + name]
+    t = oracle.table(name)
+    assert np.array_equal(np.array(k["a"]), t.coef)  # fitted coefficients digit for digit
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    # both sides round a sum of up to 2e5 cancelling monomials: the bar is 1e-16 of the sum of
+    # their magnitudes (the CH4-CH4 KAT: E = 0.163 from sum |a_k M_k| = 3.9e5)
+    scale, gscale = oracle.poly_eval(oracle.abs_table(name), k["x"], a=np.abs(k["a"]))
+    assert abs(e - k["e_expected"]) <= max(1e-12 * abs(e), 1e-16 * scale)
+    np.testing.assert_allclose(g, k["g_expected"], rtol=0, atol=1e-16 * np.abs(gscale).max() + 1e-14)
+
+
+@pytest.mark.parametrize("name", NRG)
+def test_poly_linear_and_euler(name):
+    t = oracle.table(name)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.05, 0.9, t.nvars)
+    a1, a2 = rng.normal(size=t.ncoef), rng.normal(size=t.ncoef)
+    e1, _ = oracle.poly_eval(t, x, a=a1)
+    e2, _ = oracle.poly_eval(t, x, a=a2)
+    e12, _ = oracle.poly_eval(t, x, a=2.0 * a1 - 3.0 * a2)
+    assert e12 == pytest.approx(2.0 * e1 - 3.0 * e2, rel=1e-12, abs=1e-12)
+    basis = oracle.poly_basis(t, x)
+    assert float(basis @ a1) == pytest.approx(e1, rel=1e-12, abs=1e-12)
+
+
+def _kat_holder(name):
+    k = KATS["holder_" + name]
+    h = oracle.holder(name).desc
+    n = int(k.get("ndim", k.get("ntrim", 1)))
+    mons = [np.array(k["xyz%d" % (m + 1)]).reshape(n, -1) for m in range(len(h["monomers"]))]
+    return k, mons, n
+
+
+@pytest.mark.parametrize("name", NRG)
+def test_holder_kat(name):
+    k, mons, n = _kat_holder(name)
+    e_tot, vir = 0.0, np.zeros(9)
+    grads = [np.zeros_like(m) for m in mons]
+    for j in range(n):
+        e, g, v, _ = oracle.nrg(name, [m[j] for m in mons])
+        e_tot += e
+        vir += v
+        for m in range(len(mons)):
+            grads[m][j] += g[m]
+    vir[3], vir[6], vir[7] = vir[1], vir[2], vir[5]
+    assert e_tot == pytest.approx(k["energy_expected"], rel=1e-9, abs=1e-9)
+    for m in range(len(mons)):
+        np.testing.assert_allclose(grads[m].reshape(-1), k["grad%d_expected" % (m + 1)], rtol=1e-8, atol=1e-9)
+    np.testing.assert_allclose(vir, k["virial_expected"], rtol=1e-8, atol=1e-9)
+
+
+def test_energy2b_dispatch_kats():
+    """unittest-energy2b.cpp ch4-ch4 and ch4-h2o (n dimers each), in both monomer orders."""
+    for key, name in (("energy_ch4ch4_2b", "ch4ch4_2b"), ("energy_ch4h2o_2b", "ch4h2o_2b")):
+        k = KATS[key]
+        n = int(k["nm"])
+        a = np.array(k["xyz1"]).reshape(n, -1)  # first listed: ch4
+        b = np.array(k["xyz2"]).reshape(n, -1)
+        for first, second in ((a, b), (b, a)):
+            e_tot, g1, g2 = 0.0, np.zeros_like(first), np.zeros_like(second)
+            for j in range(n):
+                kinds = [1 if first.shape[1] == 15 else 0, 1 if second.shape[1] == 15 else 0]
+                sysx = np.concatenate([first[j], second[j]])
+                r = oracle.system(sysx, kinds, None, 100.0, 0.0)
+                e_tot += r["e2b"]
+                g1[j] = r["grad"][:first.shape[1]]
+                g2[j] = r["grad"][first.shape[1]:]
+            assert e_tot == pytest.approx(k["energy_expected"], rel=1e-9, abs=1e-10)
+            ga, gb = (g1, g2) if first is a else (g2, g1)
+            np.testing.assert_allclose(ga.reshape(-1), k["grad1_expected"], rtol=1e-8, atol=1e-9)
+            np.testing.assert_allclose(gb.reshape(-1), k["grad2_expected"], rtol=1e-8, atol=1e-9)
+
+
+def test_energy3b_dispatch_kats():
+    """unittest-energy3b.cpp ch4-h2o-h2o in the four monomer orders it tests."""
+    k = KATS["energy_ch4h2o2_3b"]
+    xs = [np.array(k["xyz%d" % m]) for m in (1, 2, 3)]
+    kinds = [1, 0, 0]
+    for order in ((0, 1, 2), (1, 0, 2), (0, 2, 1), (2, 1, 0)):
+        sysx = np.concatenate([xs[i] for i in order])
+        r = oracle.system(sysx, [kinds[i] for i in order], None, 0.0, 100.0, families=("ch4h2o2_3b",))
+        assert r["e3b"] == pytest.approx(k["energy_expected"], rel=1e-9, abs=1e-10)
+        off = np.concatenate([[0], np.cumsum([len(xs[i]) for i in order])])
+        for pos, i in enumerate(order):
+            np.testing.assert_allclose(r["grad"][off[pos]:off[pos + 1]], k["grad%d_expected" % (i + 1)], rtol=1e-8,
+                                       atol=1e-9)
+        np.testing.assert_allclose(r["virial"], k["virial_expected"], rtol=1e-8, atol=1e-9)
+
+
+def _fd5(f, x, j, h=1e-3):
+    def at(d):
+        y = x.copy()
+        y[j] += d
+        return f(y)
+    return (-at(2 * h) + 8 * at(h) - 8 * at(-h) + at(-2 * h)) / (12 * h)
+
+
+@pytest.mark.parametrize("name", NRG)
+def test_holder_finite_differences(name):
+    """Analytic gradients of all three holders against central differences (for CH4-CH4 this is
+    the gradient pin the reference's missing E+G source cannot give)."""
+    k, mons, n = _kat_holder(name)
+    rng = np.random.default_rng(17)
+    sizes = [m.shape[1] for m in mons]
+    x0 = np.concatenate([m[0] for m in mons]) + rng.normal(0.0, 0.05, sum(sizes))
+    split = np.cumsum(sizes)[:-1]
+
+    def energy(x):
+        return oracle.nrg(name, np.split(x, split))[0]
+
+    e, g, _, ev = oracle.nrg(name, np.split(x0, split))
+    assert ev
+    g = np.concatenate(g)
+    for j in range(len(x0)):
+        assert _fd5(energy, x0, j) == pytest.approx(g[j], rel=1e-6, abs=1e-7)
+
+
+def _perturbed(name, scale, seed):
+    k, mons, n = _kat_holder(name)
+    rng = np.random.default_rng(seed)
+    return [m[0] + rng.normal(0.0, scale, m.shape[1]) for m in mons]
+
+
+@ref_only
+def test_holders_vs_compiled_reference():
+    dp = ctypes.POINTER(ctypes.c_double)
+    L = oracle.ref()
+    L.ref_x2b_ch4h2o.restype = ctypes.c_double
+    L.ref_x2b_ch4h2o.argtypes = [dp, dp, ctypes.c_size_t, dp, dp, dp]
+    L.ref_x3b_ch4h2o2.restype = ctypes.c_double
+    L.ref_x3b_ch4h2o2.argtypes = [dp, dp, dp, ctypes.c_size_t, dp, dp, dp, dp]
+    p = lambda a: a.ctypes.data_as(dp)
+    for seed in range(6):
+        w, c = (np.ascontiguousarray(v) for v in _perturbed("ch4h2o_2b", 0.1 + 0.3 * seed, seed))
+        e, g, v, _ = oracle.nrg("ch4h2o_2b", [w, c])
+        gw, gc, rv = np.zeros(9), np.zeros(15), np.zeros(9)
+        re = L.ref_x2b_ch4h2o(p(w), p(c), 1, p(gw), p(gc), p(rv))
+        assert e == pytest.approx(re, rel=1e-11, abs=1e-13)
+        np.testing.assert_allclose(np.concatenate(g), np.concatenate([gw, gc]), rtol=1e-10, atol=1e-11)
+        np.testing.assert_allclose(v, rv, rtol=1e-10, atol=1e-11)
+        c, w1, w2 = (np.ascontiguousarray(v) for v in _perturbed("ch4h2o2_3b", 0.1 + 0.3 * seed, seed))
+        e, g, v, _ = oracle.nrg("ch4h2o2_3b", [c, w1, w2])
+        gc, g1, g2, rv = np.zeros(15), np.zeros(9), np.zeros(9), np.zeros(9)
+        re = L.ref_x3b_ch4h2o2(p(c), p(w1), p(w2), 1, p(gc), p(g1), p(g2), p(rv))
+        assert e == pytest.approx(re, rel=1e-11, abs=1e-13)
+        np.testing.assert_allclose(np.concatenate(g), np.concatenate([gc, g1, g2]), rtol=1e-10, atol=1e-11)
+        np.testing.assert_allclose(v, rv, rtol=1e-10, atol=1e-11)
+
+
+# the reference runs' own mbx.json ignore lists (timing/<case>/mbx.json)
+PUBLISHED = {
+    "ch4_57_h2o_513_rect": ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b"),
+    "ch4_342_h2o_228_rect": ("mbpol_2b", "mbpol_3b"),
+    "clathrate": oracle.FAMILIES,
+}
+
+
+def mixed_case(systems, name):
+    kinds = np.array([0 if m == "h2o" else 1 for m in systems[name + "/mon"]])
+    box = systems[name + "/box"]
+    c2, c3 = (float(v) for v in systems[name + "/cutoffs"])
+    return systems[name + "/xyz"], kinds, (box if box.size else None), c2, c3, systems[name + "/published"]
+
+
+@pytest.mark.parametrize("name", sorted(PUBLISHED))
+def test_published_mixed_systems(systems, name):
+    """Per-term 2B / 3B energies printed by the reference (timing/*/output, 11 digits)."""
+    xyz, kinds, box, c2, c3, pub = mixed_case(systems, name)
+    r = oracle.system(xyz, kinds, box, c2, c3, families=PUBLISHED[name])
+    assert r["e2b"] == pytest.approx(pub[1], rel=5e-11)
+    assert r["e3b"] == pytest.approx(pub[2], rel=5e-11)
