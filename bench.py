@@ -74,33 +74,36 @@ def cpu_baseline(xyz, box, c2, c3, seconds):
                        "batch-8 dispatch)" % (reps, t_total, cores)), r
 
 
-def batched_leg(steps, warmup):
-    """512 gas-phase clusters (16..512 waters): one batched compute over all of them against one
-    plain compute per cluster (each in its own warmed-up context), coordinates resident."""
+def batched_vs_plain(parts, kinds, box, c2, c3, steps, warmup):
+    """One batched compute over every configuration (one context, mbxg_set_batch) against one
+    plain compute per configuration (one warmed-up context each), coordinates resident, HIP
+    graph mode on both sides, no host wait until the end."""
     import numpy as np
     import torch
 
-    from mbx_amd import configs, engine
+    from mbx_amd import engine
 
-    cfg = configs.C3
-    parts, sizes = configs.cluster_batch(cfg["nconf"], cfg["nmin"], cfg["nmax"], cfg["seed"])
-    c2, c3 = cfg["cutoff2b"], cfg["cutoff3b"]
+    sizes = [len(k) for k in kinds]
     off = np.concatenate([[0], np.cumsum(sizes)])
-    big = engine.water_engine(int(off[-1]), None, c2, c3)
+    big = engine.Engine()
+    big.set_topology(np.concatenate(kinds))
+    big.set_box(box)
+    big.set_cutoffs(c2, c3)
     big.set_batch(off)
     big.set_coords(np.concatenate(parts))
     small = []
-    for x in parts:
-        e = engine.water_engine(len(x) // 9, None, c2, c3)
+    for x, k in zip(parts, kinds):
+        e = engine.Engine()
+        e.set_topology(k)
+        e.set_box(box)
+        e.set_cutoffs(c2, c3)
         e.set_coords(x)
         small.append(e)
-
-    def run_batched():
-        big.compute()
-
-    def run_plain():
-        for e in small:
-            e.compute()
+    big.compute()
+    schedule = big.autotune(reps=2)
+    big.set_graph(True)
+    for e in small:
+        e.set_graph(True)
 
     def timed(fn, sync):
         for _ in range(warmup):
@@ -114,12 +117,12 @@ def batched_leg(steps, warmup):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    big.compute()
-    schedule = big.autotune(reps=2)
-    big.set_graph(True)
-    for e in small:
-        e.set_graph(True)
-    t_b = timed(run_batched, big.synchronize)
+    t_b = timed(big.compute, big.synchronize)
+
+    def run_plain():
+        for e in small:
+            e.compute()
+
     t_p = timed(run_plain, lambda: [e.synchronize() for e in small])
     c = big.counts()
     n_listed = int(c[0] + c[2])
@@ -129,14 +132,40 @@ def batched_leg(steps, warmup):
     for e in small:
         e.close()
     big.close()
-    return {"workload": cfg["name"], "configs": int(cfg["nconf"]), "waters": int(off[-1]),
-            "cutoff2b_A": c2, "cutoff3b_A": c3, "listed_nmers": n_listed,
-            "evaluated_dimers": int(c[1]), "evaluated_trimers": int(c[3]),
-            "ms_batched": t_b * 1e3, "ms_plain": t_p * 1e3,
-            "batched_nmers_per_s": n_listed / t_b, "plain_nmers_per_s": n_listed / t_p,
-            "speedup": t_p / t_b, "max_rel_dev_e2b_vs_plain": dev, "schedule": schedule,
-            "plain": "one warmed-up context per cluster (HIP graph mode), one compute each, no host wait until the end",
-            "batched": "one context over all clusters (HIP graph mode), one compute"}
+    return {"configs": len(parts), "molecules": int(off[-1]), "cutoff2b_A": c2, "cutoff3b_A": c3,
+            "listed_nmers": n_listed, "evaluated_dimers": int(c[1]), "evaluated_trimers": int(c[3]),
+            "ms_batched": t_b * 1e3, "ms_plain": t_p * 1e3, "batched_nmers_per_s": n_listed / t_b,
+            "plain_nmers_per_s": n_listed / t_p, "speedup": t_p / t_b, "max_rel_dev_e2b_vs_plain": dev,
+            "schedule": schedule,
+            "plain": "one warmed-up context per configuration (HIP graph mode), one compute each, no host wait "
+                     "until the end",
+            "batched": "one context over all configurations (HIP graph mode), one compute"}
+
+
+def batched_leg(steps, warmup):
+    """512 gas-phase water clusters (16..512 waters), cutoffs 9 / 7 A."""
+    import numpy as np
+
+    from mbx_amd import configs
+
+    cfg = configs.C3
+    parts, sizes = configs.cluster_batch(cfg["nconf"], cfg["nmin"], cfg["nmax"], cfg["seed"])
+    out = {"workload": cfg["name"]}
+    out.update(batched_vs_plain(parts, [np.zeros(n, int) for n in sizes], None, cfg["cutoff2b"], cfg["cutoff3b"],
+                                steps, warmup))
+    return out
+
+
+def pimd_leg(steps, warmup):
+    """32 path-integral beads of a periodic 57 CH4 + 513 H2O box (all five n-mer families)."""
+    from mbx_amd import configs
+
+    cfg = configs.C5
+    xyz, kinds, box = configs.mixed_box(cfg["n_ch4"], cfg["n_h2o"], seed=cfg["seed"])
+    parts = configs.beads(xyz, cfg["nbeads"])
+    out = {"workload": cfg["name"]}
+    out.update(batched_vs_plain(parts, [kinds] * len(parts), box, cfg["cutoff2b"], cfg["cutoff3b"], steps, warmup))
+    return out
 
 
 def main():
@@ -146,7 +175,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain legs")
+    ap.add_argument("--solo", action="store_true",
+                    help="profiling: only the stage-timed steps (each kernel alone on the stream), so a "
+                         "profiler's per-kernel averages are the launches the roofline is computed from")
+    ap.add_argument("--pimd-only", action="store_true",
+                    help="only run the batched-vs-plain leg of the 32-bead CH4/H2O workload (profiling)")
     ap.add_argument("--schedule", default=None,
                     help="fixed polynomial schedule 'S2,S3' (wavefronts per 64 dimers, trimers) instead of "
                          "autotuning; used for profiles, so the profiled launches are the timed ones")
@@ -172,6 +206,10 @@ def main():
 
     from mbx_amd import configs, engine
 
+    if args.pimd_only:
+        print(json.dumps({"pimd_batched": pimd_leg(args.steps, args.warmup)}))
+        return
+
     cfg = configs.C2
     xyz, box, c2, c3 = configs.build(cfg)
     eng = engine.water_engine(cfg["nmon"], box, c2, c3)
@@ -182,24 +220,25 @@ def main():
         eng.set_coords_device(d_xyz.data_ptr())
         eng.compute()
 
-    for _ in range(args.warmup):
-        step()
-    eng.synchronize()
-    if args.schedule:
+    if args.schedule:  # fixed schedule, set before the first launch
         s2, s3 = (int(v) for v in args.schedule.split(","))
         eng.set_tuning("split_2b", s2)
         eng.set_tuning("split_3b", s3)
         schedule = {"split_2b": s2, "split_3b": s3}
-    else:
-        schedule = eng.autotune(reps=3)  # polynomial-stage schedule for this workload
-    eng.set_graph(True)  # steady state: one captured HIP graph per compute
-    for _ in range(2):
+    for _ in range(args.warmup):
         step()
     eng.synchronize()
+    if not args.schedule:
+        schedule = eng.autotune(reps=3)  # polynomial-stage schedule for this workload
+    if not args.solo:
+        eng.set_graph(True)  # steady state: one captured HIP graph per compute
+        for _ in range(2):
+            step()
+        eng.synchronize()
     # throughput: K steps back to back, no host wait in between
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(0 if args.solo else args.steps):
         step()
     eng.synchronize()
     torch.cuda.synchronize()
@@ -218,7 +257,7 @@ def main():
 
     counts = eng.counts()
     n_listed = int(counts[0] + counts[2])
-    dt = (t1 - t0) / args.steps
+    dt = (t1 - t0) / args.steps if not args.solo else stage["total"] / 1e3 / args.steps
     e2, e3, grad, vir = eng.results()
 
     flops = json.load(open(os.path.join(ROOT, "mbx_amd", "tables", "flops.json")))
@@ -253,6 +292,7 @@ def main():
     }
     if not args.no_batch:
         out["config"]["batched"] = batched_leg(args.steps, args.warmup)
+        out["config"]["pimd_batched"] = pimd_leg(args.steps, args.warmup)
     if not args.no_cpu:
         cb, r = cpu_baseline(xyz, box, c2, c3, args.cpu_seconds)
         out["cpu_baseline"] = cb

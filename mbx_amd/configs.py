@@ -114,3 +114,39 @@ def cluster_batch(nconf, nmin=16, nmax=512, seed=512):
 # The batched workload: 512 gas-phase clusters of 16..512 waters, cutoffs 9 / 7 A.
 C3 = dict(name="water_clusters_512x16-512_gas", nconf=512, nmin=16, nmax=512, seed=512, cutoff2b=9.0,
           cutoff3b=7.0)
+
+
+# Path-integral beads of a mixed CH4 / H2O box: molecule centres on an 8 x 8 x 9 lattice over
+# the 21.6 x 21.6 x 30 A cell (U(+-0.2) A jitter), 57 of them methanes (C-H 1.09 A, ideal
+# tetrahedron, random orientation), the rest waters; each bead displaces every atom of the
+# centroid by N(0, 0.05 A).
+def methane(rng, center):
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float) / np.sqrt(3.0) * 1.09
+    rot = random_rotations(rng, 1)[0]
+    return np.concatenate([center, (tet @ rot.T + center).reshape(-1)])
+
+
+def mixed_box(n_ch4=57, n_h2o=513, cell=(21.6, 21.6, 30.0), lattice=(8, 8, 9), seed=57, jitter=0.2):
+    """(xyz, kinds, box9) of the centroid system; kinds 0 = h2o, 1 = ch4, methanes first."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = lattice
+    n = n_ch4 + n_h2o
+    sites = rng.permutation(nx * ny * nz)[:n]
+    ix, iy, iz = sites % nx, (sites // nx) % ny, sites // (nx * ny)
+    c = np.stack([(ix + 0.5) * cell[0] / nx, (iy + 0.5) * cell[1] / ny, (iz + 0.5) * cell[2] / nz], axis=1)
+    c += rng.uniform(-jitter, jitter, size=c.shape)
+    parts = [methane(rng, c[i]) for i in range(n_ch4)]
+    parts.append(place_waters(rng, c[n_ch4:]))
+    kinds = np.array([1] * n_ch4 + [0] * n_h2o)
+    box = np.array([cell[0], 0, 0, 0, cell[1], 0, 0, 0, cell[2]], dtype=np.float64)
+    return np.concatenate(parts), kinds, box
+
+
+def beads(xyz, nbeads, sigma=0.05, seed=32):
+    rng = np.random.default_rng(seed)
+    return [xyz + rng.normal(0.0, sigma, xyz.shape) for _ in range(nbeads)]
+
+
+# The batched path-integral workload: 32 beads of 57 CH4 + 513 H2O, cutoffs 9 / 7 A.
+C5 = dict(name="pimd_32beads_57ch4_513h2o_box21.6x21.6x30", nbeads=32, n_ch4=57, n_h2o=513, cutoff2b=9.0,
+          cutoff3b=7.0, seed=57)

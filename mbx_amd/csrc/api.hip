@@ -113,6 +113,8 @@ struct EvalState {
 // Largest match storage kept in LDS (80 bytes per match): 512 matches = 40 KB per block.
 constexpr int kMaxLdsMatches = 512;
 constexpr int kScratchBlocks = 2048;
+// slices per configuration in the per-configuration reductions (blocks = configurations x slices)
+constexpr int kConfSlices = 16;
 
 }  // namespace
 
@@ -142,6 +144,7 @@ struct mbxg_ctx {
     DBuf<int> d_pconf, d_pt_off;
     DBuf<double> d_conf_tot;  // per configuration: 2B record, then 3B record (kPartials each)
     DBuf<double> d_conf_fam;  // per family and configuration (kPartials each)
+    DBuf<double> d_conf_part; // per family, configuration and slice (kConfSlices per configuration)
     ListState L;
     EvalState fam[kFamilies];
     bool have_coords = false, have_results = false;
@@ -231,6 +234,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_pt_off.release();
     c->d_conf_tot.release();
     c->d_conf_fam.release();
+    c->d_conf_part.release();
     c->d_fam_tot.release();
     for (DBuf<double>& b : c->d_coefn) b.release();
     for (EvalState* o = c->fam; o != c->fam + kFamilies; ++o) {
@@ -290,7 +294,8 @@ int set_configs(mbxg_ctx* c, int nconf, const int* mon_off) {
     for (int q = 0; q < nconf; ++q) pt_off[q + 1] += pt_off[q];
     if (c->d_pconf.reserve(pconf.size()) || c->d_pt_off.reserve(nconf + 1) ||
         c->d_conf_tot.reserve((size_t)2 * kPartials * nconf) ||
-        c->d_conf_fam.reserve((size_t)kFamilies * kPartials * nconf))
+        c->d_conf_fam.reserve((size_t)kFamilies * kPartials * nconf) ||
+        c->d_conf_part.reserve((size_t)kFamilies * kConfSlices * kPartials * nconf))
         return fail(MBXG_ENOMEM, "configurations: device allocation failed");
     HIPCHK(hipMemcpy(c->d_pconf.p, pconf.data(), sizeof(int) * pconf.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_pt_off.p, pt_off.data(), sizeof(int) * (nconf + 1), hipMemcpyHostToDevice));
@@ -815,11 +820,14 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                                          o.e_nmer.p, vir, o.partials.p, o.ws.p, sk, nullptr));
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
                                c->d_fam_tot.p + kPartials * f);
-            if (batch)
-                hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf), dim3(256), 0, sk, o.e_nmer.p, vir, ld,
-                                   f == FAM_MBPOL_2B || f == FAM_MBPOL_3B ? L.offsets.p : L.offsets_x.p,
-                                   c->d_pt_off.p, fam_component(f), cap,
-                                   c->d_conf_fam.p + (size_t)f * kPartials * c->nconf);
+            if (batch) {
+                double* part = c->d_conf_part.p + (size_t)f * kConfSlices * kPartials * c->nconf;
+                hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf, kConfSlices), dim3(256), 0, sk, o.e_nmer.p, vir,
+                                   ld, f == FAM_MBPOL_2B || f == FAM_MBPOL_3B ? L.offsets.p : L.offsets_x.p,
+                                   c->d_pt_off.p, fam_component(f), cap, part);
+                hipLaunchKernelGGL(k_reduce_slices, dim3((c->nconf * kPartials + 255) / 256), dim3(256), 0, sk, part,
+                                   c->nconf, kConfSlices, c->d_conf_fam.p + (size_t)f * kPartials * c->nconf);
+            }
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(c->join, c->stream2));

@@ -19,9 +19,22 @@
 #include "nrg_kernel.hpp"
 #include "scatter.hpp"
 
+#include <utility>
+
 namespace mbxg {
 
 namespace {
+
+// Compile-time loop: f(std::integral_constant<int, J>) for J = 0..N-1, so every variable's site
+// indices are constants and the site arrays stay in registers.
+template <class F, int... J>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // Holder slot s -> position of its monomer in the listed n-mer.
 template <class H>
@@ -143,14 +156,14 @@ __device__ __forceinline__ void holder_switches(const double* x, double d[3][3],
     }
 }
 
-// Variable j's value from its site pair (tools/variable.cpp:49-129); gg with
-// d(value)/d(site s1) = gg * (x[s1] - x[s2]).
-__device__ __forceinline__ double nrg_value(const NrgVar& V, const double* x, double* dvec, double& gg) {
-    const double* p1 = x + 3 * V.s1;
-    const double* p2 = x + 3 * V.s2;
-    dvec[0] = p1[0] - p2[0];
-    dvec[1] = p1[1] - p2[1];
-    dvec[2] = p1[2] - p2[2];
+// Variable J's value from its site pair (tools/variable.cpp:49-129); gg with
+// d(value)/d(site s1) = gg * (x[s1] - x[s2]).  Everything about the variable is a constant.
+template <class H, int J>
+__device__ __forceinline__ double nrg_value(const double* x, double* dvec, double& gg) {
+    constexpr NrgVar V = H::kVars[J];
+    dvec[0] = x[3 * V.s1] - x[3 * V.s2];
+    dvec[1] = x[3 * V.s1 + 1] - x[3 * V.s2 + 1];
+    dvec[2] = x[3 * V.s1 + 2] - x[3 * V.s2 + 2];
     const double r = sqrt(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2]);
     double val;
     if (V.form == 0) {
@@ -179,11 +192,11 @@ __global__ void __launch_bounds__(64) k_vars_nrg(const int* __restrict__ list, c
     double x[3 * H::kNSites], oh1[3], oh2[3];
     int mon[3];
     holder_sites<H>(N, list + H::kMon * (size_t)t, x, mon, oh1, oh2);
-#pragma unroll
-    for (int j = 0; j < H::kNVars; ++j) {
+    static_for<H::kNVars>([&](auto J) {
+        constexpr int j = decltype(J)::value;
         double dv[3], gg;
-        xs[(size_t)j * ld + t] = nrg_value(H::kVars[j], x, dv, gg);
-    }
+        xs[(size_t)j * ld + t] = nrg_value<H, j>(x, dv, gg);
+    });
     double d[3][3], r[3], s[3], gs[3];
     holder_switches<H>(x, d, r, s, gs);
     sw[t] = H::kMon == 2 ? s[0] : s[0] * s[1] + s[0] * s[2] + s[1] * s[2];
@@ -224,22 +237,22 @@ __global__ void __launch_bounds__(kGradBlockNrg) k_grad_nrg(const int* __restric
             double xg[3 * H::kNSites];
 #pragma unroll
             for (int i = 0; i < 3 * H::kNSites; ++i) xg[i] = 0.0;
-#pragma unroll
-            for (int j = 0; j < H::kNVars; ++j) {
+            static_for<H::kNVars>([&](auto J) {
+                constexpr int j = decltype(J)::value;
                 double gj = 0.0;
 #pragma unroll
                 for (int q = 0; q < H::kPieces; ++q) gj += gs[((size_t)q * H::kNVars + j) * ld + t];
                 if (H::kMon == 3) gj *= sw;  // the 3B holder scales the polynomial gradient first
                 double dv[3], gg;
-                nrg_value(H::kVars[j], x, dv, gg);
-                const int a1 = H::kVars[j].s1, a2 = H::kVars[j].s2;
+                nrg_value<H, j>(x, dv, gg);
+                constexpr int a1 = H::kVars[j].s1, a2 = H::kVars[j].s2;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double tc = gj * (dv[c] * gg);
                     xg[3 * a1 + c] += tc;
                     xg[3 * a2 + c] -= tc;
                 }
-            }
+            });
             if (H::kVsite) {
                 // monomer::grads (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:139-160)
                 const double* g1 = xg + 3 * H::kNReal;

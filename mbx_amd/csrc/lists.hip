@@ -660,13 +660,14 @@ __global__ void __launch_bounds__(256) k_reduce_configs(const double* __restrict
                                                         const double* __restrict__ vir_out, int ld,
                                                         const int4* __restrict__ offsets,
                                                         const int* __restrict__ pt_off, int comp, int cap,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ part) {
     __shared__ double sh[kPartials][256];
-    const int c = blockIdx.x, t = threadIdx.x;
+    const int c = blockIdx.x, sl = blockIdx.y, nsl = gridDim.y, t = threadIdx.x;
     const int4 o0 = offsets[pt_off[c]], o1 = offsets[pt_off[c + 1]];
-    int beg = comp == 1 ? o0.y : o0.w, end = comp == 1 ? o1.y : o1.w;
-    beg = min(beg, cap);
-    end = min(end, cap);
+    auto pick = [comp](const int4& o) { return comp == 0 ? o.x : comp == 1 ? o.y : comp == 2 ? o.z : o.w; };
+    const int cb = min(pick(o0), cap), ce = min(pick(o1), cap);
+    const int chunk = (ce - cb + nsl - 1) / nsl;
+    const int beg = cb + sl * chunk, end = min(ce, beg + chunk);
     double acc[kPartials];
     for (int q = 0; q < kPartials; ++q) acc[q] = 0.0;
     for (int i = beg + t; i < end; i += 256) {
@@ -676,12 +677,22 @@ __global__ void __launch_bounds__(256) k_reduce_configs(const double* __restrict
     }
     for (int q = 0; q < kPartials; ++q) sh[q][t] = acc[q];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s)
-            for (int q = 0; q < kPartials; ++q) sh[q][t] += sh[q][t + s];
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (t < s2)
+            for (int q = 0; q < kPartials; ++q) sh[q][t] += sh[q][t + s2];
         __syncthreads();
     }
-    if (t < kPartials) out[(size_t)c * kPartials + t] = sh[t][0];
+    if (t < kPartials) part[((size_t)c * nsl + sl) * kPartials + t] = sh[t][0];
+}
+
+// Second level: out[c * kPartials + q] = sum over slices s (in order) of part[c][s][q].
+__global__ void k_reduce_slices(const double* __restrict__ part, int nconf, int nsl, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nconf * kPartials) return;
+    const int c = i / kPartials, q = i % kPartials;
+    double acc = 0.0;
+    for (int s = 0; s < nsl; ++s) acc += part[((size_t)c * nsl + s) * kPartials + q];
+    out[i] = acc;
 }
 
 // Per-order records from per-family ones (kPartials doubles per record, nconf configurations):

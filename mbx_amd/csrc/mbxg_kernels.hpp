@@ -133,11 +133,15 @@ hipError_t launch_poly_mbpol3b(const double* xs, const int* n_dev, int cap, int 
 // (deterministic totals).
 __global__ void k_reduce_partials(const double* partials, const int* n_dev, int n_cap, double* out);
 
-// Per-configuration totals: block c sums e_out / vir_out (leading dimension ld) over the
-// evaluated n-mers of configuration c, [offsets[pt_off[c]].comp, offsets[pt_off[c+1]].comp)
-// clamped to cap, in a fixed order, into out[c * kPartials ...] (energy, 6 virial components;
-// zeros when vir_out is null).  comp selects the int4 component (1: evaluated dimers,
-// 3: evaluated trimers).
+// Per-configuration totals in two fixed-order levels: block (c, s) of a (nconf, S) grid sums
+// slice s of configuration c's evaluated n-mers, [offsets[pt_off[c]].comp,
+// offsets[pt_off[c+1]].comp) clamped to cap, from e_out / vir_out (leading dimension ld) into
+// part[c][s][kPartials] (energy, 6 virial components; zeros when vir_out is null); comp selects
+// the int4 component (0..3) holding the family's count.  k_reduce_slices then adds the S
+// slices of each configuration in order.
+__global__ void k_reduce_configs(const double* e_out, const double* vir_out, int ld, const int4* offsets,
+                                 const int* pt_off, int comp, int cap, double* part);
+__global__ void k_reduce_slices(const double* part, int nconf, int nsl, double* out);
 __global__ void k_combine_families(const double* fam, int nconf, double* out);
 __global__ void k_reduce_configs(const double* e_out, const double* vir_out, int ld, const int4* offsets,
                                  const int* pt_off, int comp, int cap, double* out);

@@ -109,3 +109,28 @@ def test_batch_reset_to_single():
     e2, e3, _, _ = e.results()
     assert e2 == pytest.approx(r["e2b"], rel=E_REL)
     assert e3 == pytest.approx(r["e3b"], rel=E_REL)
+
+
+def test_batch_pimd_mixed_beads():
+    """Beads of a periodic CH4 / H2O box in one batch: every bead's 2B / 3B energies and
+    gradients (all five families) equal the oracle's for that bead alone."""
+    xyz, kinds, box = configs.mixed_box()
+    parts = configs.beads(xyz, 2)
+    print("pimd beads", flush=True)
+    n = len(kinds)
+    e = engine.Engine()
+    e.set_topology(np.concatenate([kinds, kinds]))
+    e.set_box(box)
+    e.set_cutoffs(9.0, 7.0)
+    e.set_batch([0, n, 2 * n])
+    e.set_coords(np.concatenate(parts))
+    e.compute()
+    e2, e3, vir = e.batch_results()
+    _, _, g, _ = e.results()
+    na = len(xyz)
+    for k, x in enumerate(parts):
+        r = oracle.system(x, kinds, box, 9.0, 7.0, tags=np.arange(n))
+        assert e2[k] == pytest.approx(r["e2b"], rel=E_REL)
+        assert e3[k] == pytest.approx(r["e3b"], rel=E_REL)
+        assert np.abs(g[k * na:(k + 1) * na] - r["grad"]).max() <= G_ABS
+        np.testing.assert_allclose(vir[k], r["virial"], rtol=1e-8, atol=1e-8)
