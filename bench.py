@@ -168,6 +168,64 @@ def pimd_leg(steps, warmup):
     return out
 
 
+def differentiation_leg(steps, warmup):
+    """Energy + force loss over 4096 water dimers and 4096 water trimers, differentiated with
+    respect to coordinates, linear coefficients and nonlinear parameters (mbx_amd.autograd).
+    A step is the forward of both families, the loss and the full backward."""
+    import numpy as np
+    import torch
+
+    from mbx_amd import autograd as ad
+    from mbx_amd import configs
+
+    cfg = configs.C4
+    xd, xt = configs.build_c4(cfg)
+    g = torch.Generator(device="cpu").manual_seed(cfg["seed"])
+    leaves, refs = {}, {}
+    for fam, x in (("mbpol_2b", xd), ("mbpol_3b", xt)):
+        coef, theta = ad.defaults(fam)
+        xx = torch.from_numpy(x).to("cuda")
+        with torch.no_grad():
+            e, f = ad.energy_forces(xx, coef, theta, fam)
+        refs[fam] = (e + 0.1 * torch.randn(e.shape, generator=g, dtype=torch.float64).cuda(),
+                     f + 0.1 * torch.randn(f.shape, generator=g, dtype=torch.float64).cuda())
+        leaves[fam] = [xx.requires_grad_(), coef.requires_grad_(), theta.requires_grad_()]
+
+    def step():
+        for v in leaves.values():
+            for t in v:
+                t.grad = None
+        loss = 0.0
+        for fam, (x, coef, theta) in leaves.items():
+            e, f = ad.energy_forces(x, coef, theta, fam)
+            loss = loss + ad.energy_force_loss(e, f, *refs[fam])
+        loss.backward()
+        return loss
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    # forward alone, for the split
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        for _ in range(steps):
+            for fam, (x, coef, theta) in leaves.items():
+                ad.energy_forces(x, coef, theta, fam)
+    torch.cuda.synchronize()
+    dtf = (time.perf_counter() - t0) / steps
+    n = cfg["ndimers"] + cfg["ntrimers"]
+    return {"workload": cfg["name"], "dimers": cfg["ndimers"], "trimers": cfg["ntrimers"],
+            "ms_per_step": dt * 1e3, "ms_forward": dtf * 1e3, "nmers_per_s": n / dt, "loss": float(loss),
+            "step": "forward (E, F) of both families + energy/force MSE loss + backward to coordinates, "
+                    "1153 + 1163 coefficients and 2 x 10 nonlinear parameters"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +237,8 @@ def main():
     ap.add_argument("--solo", action="store_true",
                     help="profiling: only the stage-timed steps (each kernel alone on the stream), so a "
                          "profiler's per-kernel averages are the launches the roofline is computed from")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="only run the differentiation leg (4096 dimers + 4096 trimers, loss and backward)")
     ap.add_argument("--pimd-only", action="store_true",
                     help="only run the batched-vs-plain leg of the 32-bead CH4/H2O workload (profiling)")
     ap.add_argument("--schedule", default=None,
@@ -208,6 +268,9 @@ def main():
 
     if args.pimd_only:
         print(json.dumps({"pimd_batched": pimd_leg(args.steps, args.warmup)}))
+        return
+    if args.diff_only:
+        print(json.dumps({"differentiation": differentiation_leg(args.steps, args.warmup)}))
         return
 
     cfg = configs.C2
@@ -293,6 +356,7 @@ def main():
     if not args.no_batch:
         out["config"]["batched"] = batched_leg(args.steps, args.warmup)
         out["config"]["pimd_batched"] = pimd_leg(args.steps, args.warmup)
+        out["config"]["differentiation"] = differentiation_leg(args.steps, args.warmup)
     if not args.no_cpu:
         cb, r = cpu_baseline(xyz, box, c2, c3, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -155,6 +155,34 @@ int mbxg_autotune(mbxg_ctx* ctx, int terms, int do_grads, int reps);
  * pointers of mbxg_device_outputs; the host-side getters do this themselves. */
 int mbxg_synchronize(mbxg_ctx* ctx);
 
+/* ---- Differentiable evaluation of independent n-mers ---------------------------------------
+ * The reference has no differentiable interface: x2b_v9x::eval (potential/2b/x2b-v9x.cpp:1514)
+ * and x3b_v2x::eval (potential/3b/x3b-v2x.cpp:1403) fix the coefficients and nonlinear
+ * parameters at compile time and return E and dE/dx.  These entry points take both as inputs and
+ * back the torch.autograd.Function of the Python plugin (mbx_amd/autograd.py).
+ *
+ * Family: MBXG_AD_MBPOL_2B (water dimers, 6 atoms = O H H O H H, 1153 coefficients) or
+ * MBXG_AD_MBPOL_3B (water trimers, 9 atoms, 1163 coefficients); 10 nonlinear parameters each
+ * (2B: k_HH_intra k_OH_intra k_HH_coul k_OH_coul k_OO_coul k_XH_main k_XO_main k_XX_main
+ * in_plane_gamma out_of_plane_gamma; 3B: kHH_intra kOH_intra kHH kOH kOO dHH_intra dOH_intra
+ * dHH dOH dOO).  Coordinates are n-mer-local (no periodic imaging), row-major [n][3*atoms].
+ * All array arguments of forward/backward are device pointers; work is enqueued on `stream`
+ * (a hipStream_t; NULL = the default stream) and nothing is allocated: the caller passes a
+ * device workspace of at least mbxg_ad_workspace_bytes(family, n) bytes. */
+enum { MBXG_AD_MBPOL_2B = 0, MBXG_AD_MBPOL_3B = 1 };
+int mbxg_ad_shape(int family, int* n_atoms, int* n_coef, int* n_theta);
+/* Host arrays: the reference's fitted coefficients and nonlinear parameters. */
+int mbxg_ad_defaults(int family, double* coef, double* theta);
+size_t mbxg_ad_workspace_bytes(int family, int n);
+/* energy[n] = E_n and grad[n][3*atoms] = dE_n/dx_n. */
+int mbxg_ad_forward(int family, int n, const double* xyz, const double* coef, const double* theta, double* energy,
+                    double* grad, void* workspace, size_t workspace_bytes, void* stream);
+/* With phi = sum_n g_energy[n] E_n + g_grad[n] . dE_n/dx_n: d_xyz = dphi/dx ([n][3*atoms]),
+ * d_coef = dphi/da, d_theta = dphi/dtheta.  g_grad may be NULL (phi = sum g_energy E). */
+int mbxg_ad_backward(int family, int n, const double* xyz, const double* coef, const double* theta,
+                     const double* g_energy, const double* g_grad, double* d_xyz, double* d_coef, double* d_theta,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,3 +150,45 @@ def beads(xyz, nbeads, sigma=0.05, seed=32):
 # The batched path-integral workload: 32 beads of 57 CH4 + 513 H2O, cutoffs 9 / 7 A.
 C5 = dict(name="pimd_32beads_57ch4_513h2o_box21.6x21.6x30", nbeads=32, n_ch4=57, n_h2o=513, cutoff2b=9.0,
           cutoff3b=7.0, seed=57)
+
+
+# Independent water dimers and trimers for the differentiable path: dimer O-O separations
+# U(2.4, 9) A; trimer O-O separations each U(2.5, 7) A, the three drawn together and redrawn
+# until they form a triangle; random orientations (molecules and the whole n-mer).
+C4 = dict(name="water_4096_dimers_4096_trimers", ndimers=4096, ntrimers=4096, seed=4096, dimer_range=(2.4, 9.0),
+          trimer_range=(2.5, 7.0))
+
+
+def water_dimers(n, seed, r_range=(2.4, 9.0)):
+    """(n, 18): O H H O H H per dimer."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(r_range[0], r_range[1], n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = np.zeros((n, 2, 3))
+    o[:, 1] = r[:, None] * u
+    return place_waters(rng, o.reshape(-1, 3)).reshape(n, 18)
+
+
+def water_trimers(n, seed, r_range=(2.5, 7.0)):
+    """(n, 27): O H H per water, three waters per trimer."""
+    rng = np.random.default_rng(seed)
+    d = np.empty((0, 3))
+    while len(d) < n:
+        c = rng.uniform(r_range[0], r_range[1], size=(2 * n, 3))  # r_ab, r_ac, r_bc
+        ok = (c[:, 0] < c[:, 1] + c[:, 2]) & (c[:, 1] < c[:, 0] + c[:, 2]) & (c[:, 2] < c[:, 0] + c[:, 1])
+        d = np.concatenate([d, c[ok]])
+    d = d[:n]
+    ab, ac, bc = d.T
+    cosa = (ab * ab + ac * ac - bc * bc) / (2 * ab * ac)
+    o = np.zeros((n, 3, 3))
+    o[:, 1, 0] = ab
+    o[:, 2, 0] = ac * cosa
+    o[:, 2, 1] = ac * np.sqrt(np.clip(1 - cosa * cosa, 0.0, None))
+    o = np.einsum("nij,naj->nai", random_rotations(rng, n), o)
+    return place_waters(rng, o.reshape(-1, 3)).reshape(n, 27)
+
+
+def build_c4(cfg=C4):
+    return (water_dimers(cfg["ndimers"], cfg["seed"], cfg["dimer_range"]),
+            water_trimers(cfg["ntrimers"], cfg["seed"] + 1, cfg["trimer_range"]))

@@ -166,6 +166,11 @@ struct mbxg_ctx {
     hipEvent_t ev[kNumEvents] = {};
 };
 
+int mbxg::report_error(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+
 extern "C" const char* mbxg_last_error(void) { return g_err.c_str(); }
 
 extern "C" int mbxg_create(mbxg_ctx** out, int device) {

@@ -19,22 +19,9 @@
 #include "nrg_kernel.hpp"
 #include "scatter.hpp"
 
-#include <utility>
-
 namespace mbxg {
 
 namespace {
-
-// Compile-time loop: f(std::integral_constant<int, J>) for J = 0..N-1, so every variable's site
-// indices are constants and the site arrays stay in registers.
-template <class F, int... J>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, J...>) {
-    (f(std::integral_constant<int, J>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
 
 // Holder slot s -> position of its monomer in the listed n-mer.
 template <class H>

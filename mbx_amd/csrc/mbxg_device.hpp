@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace mbxg {
 
 // Monomer kinds known to the GPU path (the reference keys them by name, energy2b.cpp:243-254).
@@ -108,6 +110,17 @@ constexpr double dOO = 4.816312044947604e-08;
 // Per-block partial sums written by the evaluation kernels: energy, then the six distinct
 // virial components (xx, xy, xz, yy, yz, zz).
 constexpr int kPartials = 7;
+
+// Compile-time loop: f(std::integral_constant<int, J>) for J = 0..N-1, so table entries indexed
+// by J are constants and per-lane arrays indexed through them stay in registers.
+template <class F, int... J>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

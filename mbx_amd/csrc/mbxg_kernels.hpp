@@ -33,6 +33,9 @@ struct NmerCtx {
     int periodic;
 };
 
+// Records the message returned by mbxg_last_error and returns code (api.hip).
+int report_error(int code, const char* msg);
+
 // Cell grids: at most kMaxCellsPerDim cells per dimension (the gas-phase grid is sized on the
 // device; the cell buffers are allocated for the maximum).
 constexpr int kMaxCellsPerDim = 64;

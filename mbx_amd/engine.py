@@ -54,6 +54,12 @@ PROTOTYPES = {
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
     "mbxg_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mbxg_get_batch_results": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "mbxg_ad_shape": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
+    "mbxg_ad_defaults": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+    "mbxg_ad_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    "mbxg_ad_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "mbxg_ad_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        ctypes.c_size_t, _vp]),
 }
 
 FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")

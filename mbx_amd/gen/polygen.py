@@ -35,6 +35,7 @@ thousands of simultaneously live values, and keeps compile time linear in the co
 """
 import collections
 import os
+import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -445,12 +446,256 @@ def generate(name, fn, out_dir):
     return path, c
 
 
+# ------------------------------------------------------------------------------------------
+# Differentiation: the polynomial in dual numbers, and the coefficient basis.
+#
+# The adjoint of an energy+force loss needs, per n-mer and along the tangent direction xd of
+# the variables, (i) P, dP = grad P . xd, grad P and d(grad P) = H xd (forward-over-reverse:
+# the reverse-mode trie schedule above, every value carried with its tangent), and (ii) the
+# coefficient gradient sum_n [alpha_n M_k(x_n) + sw_n dM_k(x_n; xd_n)], where M_k is the
+# basis function of coefficient k (the sum of its monomials) and dM_k its tangent.
+
+_DECL = re.compile(r"^(const )?double (\w+) = (.*);$")
+_FMA = re.compile(r"^(\S+) = __fma_rn\((.*)\);$")
+_ADD = re.compile(r"^(\S+) \+= (.*);$")
+
+
+def _tan(a):
+    """Tangent of an operand of a generated statement (None for a constant)."""
+    a = a.strip()
+    if a == "{c}" or re.match(r"^[0-9.]+$", a):
+        return None
+    m = re.match(r"^(x|g)\[(\d+)\]$", a)
+    if m:
+        return "%sd[%s]" % m.groups()
+    if a == "E":
+        return "dE"
+    assert re.match(r"^\w+$", a), a
+    return "d" + a
+
+
+def _tan_product(u, v):
+    """Tangent of u * v."""
+    du, dv = _tan(u), _tan(v)
+    if du and dv:
+        return "__fma_rn(%s, %s, %s * %s)" % (du, v, u, dv)
+    if du:
+        return "%s * %s" % (du, v)
+    if dv:
+        return "%s * %s" % (u, dv)
+    return None
+
+
+def dual_stmt(t):
+    """The statement followed by the statement(s) that carry its tangent."""
+    m = _DECL.match(t)
+    if m:
+        const, name, expr = m.groups()
+        if " * " in expr:
+            u, v = expr.split(" * ")
+            d = _tan_product(u, v)
+        else:
+            d = _tan(expr)
+        return [t, "%sdouble d%s = %s;" % (const or "", name, d or "0.0")]
+    m = _FMA.match(t)
+    if m:
+        target, args = m.groups()
+        u, v, acc = [s.strip() for s in args.split(",")]
+        assert acc == target, t
+        d = _tan(target)
+        for p, q in ((u, _tan(v)), (_tan(u), v)):
+            if p is not None and q is not None:
+                d = "__fma_rn(%s, %s, %s)" % (p, q, d)
+        return [t, "%s = %s;" % (_tan(target), d)] if d != _tan(target) else [t]
+    m = _ADD.match(t)
+    if m:
+        target, a = m.groups()
+        da = _tan(a)
+        return [t, "%s += %s;" % (_tan(target), da)] if da else [t]
+    raise ValueError("no tangent rule for: " + t)
+
+
+def emit_dual(s, fn, part, assign, region_coefs=12, region_ops=96):
+    """Dual-number version of Schedule.emit for one part: value, tangent, gradient and gradient
+    tangent accumulate into E, dE, g, gd (passed in, not cleared)."""
+    ops = []
+    for t, q in s.ops(True, part, assign):
+        if t == "double E = E0;":
+            continue
+        ops += [(d, q if "{c}" in d else None) for d in dual_stmt(t)]
+    regions, coefs, nar = [[]], [[]], 0
+    for t, q in ops:
+        cur = coefs[-1]
+        if q is not None and q not in cur and (len(cur) >= region_coefs or nar >= region_ops):
+            regions.append([])
+            coefs.append([])
+            nar = 0
+            cur = coefs[-1]
+        if q is not None and q not in cur:
+            cur.append(q)
+        regions[-1].append((t, q))
+        nar += 1
+    out = ["__device__ __forceinline__ void %s(mbxg_cptr C, const double* x, const double* xd, double* g, "
+           "double* gd, double& E, double& dE) {" % fn]
+    w = out.append
+    name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
+    for q in coefs[0]:
+        w("    const double %s = C[%d];" % (name(0, q), q))
+    last = "E"
+    for r, reg in enumerate(regions):
+        w("    MBXG_NODE_FENCE(C, %s);" % last)
+        if r + 1 < len(regions):
+            for q in coefs[r + 1]:
+                w("    const double %s = C[%d];" % (name(r + 1, q), q))
+        for t, q in reg:
+            w("    " + (t.replace("{c}", name(r, q)) if q is not None else t))
+            if q is not None:
+                last = _lhs(t)
+    w("}")
+    return "\n".join(out) + "\n"
+
+
+def basis_slices(monos, ncoef, nslices):
+    """Contiguous coefficient ranges of near-equal cost (one per slice)."""
+    cost = [0] * ncoef
+    for k, m in monos:
+        cost[k] += 2 * len(m) + 1
+    total = sum(cost)
+    bounds, acc, k0 = [], 0, 0
+    for k in range(ncoef):
+        acc += cost[k]
+        if acc >= total * (len(bounds) + 1) / nslices and len(bounds) < nslices - 1:
+            bounds.append((k0, k + 1))
+            k0 = k + 1
+    bounds.append((k0, ncoef))
+    return bounds
+
+
+def emit_basis(fn, monos, c0, c1):
+    """Slice function: for coefficients c0..c1-1, each lane's alpha * M_k + sw * dM_k goes to
+    the block's LDS tile red[64][65] (row = coefficient within a chunk of 64, column = lane);
+    after every chunk the column sums (over the wave's 64 n-mers) go to out[k]."""
+    by = collections.defaultdict(list)
+    for k, m in monos:
+        if c0 <= k < c1:
+            by[k].append(tuple(sorted(m)))
+    out = ["__device__ __forceinline__ void %s(double* x, double* xd, double alpha, double sw, "
+           "double* red, double* out, int lane) {" % fn]
+    w = out.append
+    tmp = [0]
+    nv = 1 + max(max(m) for k, m in monos if m)
+
+    def fresh(p):
+        tmp[0] += 1
+        return "%s%d" % (p, tmp[0])
+
+    for k in range(c0, c1):
+        j = (k - c0) % 64
+        w("    {")
+        w("        double TA = 0.0, TS = 0.0;")
+        prefix = {}  # sorted prefix tuple -> (value name, tangent name), shared inside a coefficient
+
+        def prod(head):
+            if len(head) == 1:
+                return "x[%d]" % head[0], "xd[%d]" % head[0]
+            if head not in prefix:
+                pv, pt = prod(head[:-1])
+                i, nv, nt = head[-1], fresh("p"), fresh("q")
+                w("        const double %s = %s * x[%d];" % (nv, pv, i))
+                w("        const double %s = __fma_rn(%s, x[%d], %s * xd[%d]);" % (nt, pt, i, pv, i))
+                prefix[head] = (nv, nt)
+            return prefix[head]
+
+        for m in sorted(by.get(k, [])):
+            if len(m) == 0:
+                w("        TA += 1.0;")
+                continue
+            if len(m) == 1:
+                w("        TA += x[%d];" % m[0])
+                w("        TS += xd[%d];" % m[0])
+                continue
+            pv, pt = prod(m[:-1])  # value and tangent of all but the last variable
+            l = m[-1]
+            w("        TA = __fma_rn(%s, x[%d], TA);" % (pv, l))
+            w("        TS = __fma_rn(%s, xd[%d], __fma_rn(%s, x[%d], TS));" % (pv, l, pt, l))
+        w("        red[%d * 65 + lane] = __fma_rn(alpha, TA, sw * TS);" % j)
+        w("    }")
+        # the variables are redefined opaquely between coefficients: products repeated in
+        # other coefficients are recomputed there, not kept live from here
+        w("    ad_opaque<%d>(x, xd);" % nv)
+        if j == 63 or k == c1 - 1:
+            base = k - j
+            w("    ad_basis_flush(red, out + %d, %d, lane);" % (base, j + 1))
+    w("}")
+    return "\n".join(out) + "\n"
+
+
+# Differentiated polynomials: dual-number parts (as the E+G schedule's NPARTS pieces) and
+# coefficient-basis slices.
+AD_POLYS = {"mbpol_2b": 16, "mbpol_3b": 16}  # name -> basis slices
+
+
+def generate_ad(name, out_dir):
+    """gen/ad_<name>_dual.{inc,hip} and gen/ad_<name>_basis.{inc,hip}: two translation units
+    (compiled in parallel), each with its kernel's launcher."""
+    nvars, coef, monos = load_table(name)
+    s = Schedule(nvars, [(k, m) for k, m in monos])
+    assign = s.partition(NPARTS)
+    fn = "ad_%s" % name
+    sl = basis_slices(monos, len(coef), AD_POLYS[name])
+    head = lambda what: ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+                         "// %s" % what, "#pragma once", "#include \"poly_support.hpp\"", "#include \"ad_kernel.hpp\"",
+                         "", "namespace mbxg {", ""]
+    # dual-number polynomial, NPARTS parts
+    body = [emit_dual(s, "%s_dual_p%d" % (fn, k), k, assign) for k in range(NPARTS)]
+    d = ["__device__ __forceinline__ void %s_dual(int part, mbxg_cptr C, const double* x, const double* xd, "
+         "double* g, double* gd, double& E, double& dE) {" % fn]
+    d += ["    if (part == %d) %s_dual_p%d(C, x, xd, g, gd, E, dE);" % (k, fn, k) for k in range(NPARTS)]
+    d += ["}", "", "}  // namespace mbxg", ""]
+    write_if_changed(os.path.join(out_dir, "ad_%s_dual.inc" % name),
+                     "\n".join(head("Dual-number polynomial in %d parts (value, tangent, gradient, gradient "
+                                    "tangent)." % NPARTS)) + "\n".join(body) + "\n".join(d))
+    # coefficient basis, in slices of near-equal cost
+    body = [emit_basis("%s_basis_s%d" % (fn, i), monos, c0, c1) for i, (c0, c1) in enumerate(sl)]
+    d = ["__device__ __forceinline__ void %s_basis(int slice, double* x, double* xd, double alpha, "
+         "double sw, double* red, double* out, int lane) {" % fn]
+    d += ["    if (slice == %d) %s_basis_s%d(x, xd, alpha, sw, red, out, lane);" % (i, fn, i) for i in range(len(sl))]
+    d += ["}", "", "}  // namespace mbxg", ""]
+    write_if_changed(os.path.join(out_dir, "ad_%s_basis.inc" % name),
+                     "\n".join(head("Coefficient basis and its tangent in %d slices (coefficient ranges %s)."
+                                    % (len(sl), ", ".join("%d-%d" % (c0, c1 - 1) for c0, c1 in sl))))
+                     + "\n".join(body) + "\n".join(d))
+    tu = lambda what, inc, launcher: "\n".join([
+        "// GENERATED by mbx_amd/gen/polygen.py -- do not edit.",
+        "// %s of the %s polynomial (%d variables, %d coefficients)." % (what, name, nvars, len(coef)),
+        "#include \"gen/%s\"" % inc, "", "namespace mbxg {", "", launcher, "", "}  // namespace mbxg", ""])
+    write_if_changed(os.path.join(out_dir, "ad_%s_dual.hip" % name), tu(
+        "Dual-number evaluation", "ad_%s_dual.inc" % name,
+        "hipError_t launch_ad_dual_%s(const double* xs, const double* xds, int n, int ld, const double* coef, "
+        "double* gs, double* gds, double* ps, double* pds, hipStream_t st) {\n"
+        "    if (n <= 0) return hipSuccess;\n"
+        "    hipLaunchKernelGGL((k_ad_poly<%d, %d, %s_dual>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, n, "
+        "ld,\n                       (uint64_t)(uintptr_t)coef, gs, gds, ps, pds);\n"
+        "    return hipGetLastError();\n}" % (name, nvars, NPARTS, fn, NPARTS)))
+    write_if_changed(os.path.join(out_dir, "ad_%s_basis.hip" % name), tu(
+        "Coefficient basis", "ad_%s_basis.inc" % name,
+        "hipError_t launch_ad_basis_%s(const double* xs, const double* xds, const double* sw, const double* swd, "
+        "const double* g_energy, int n, int ld, double* partial, hipStream_t st) {\n"
+        "    if (n <= 0) return hipSuccess;\n"
+        "    hipLaunchKernelGGL((k_ad_basis<%d, %d, %s_basis>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, sw, "
+        "swd,\n                       g_energy, n, ld, partial);\n"
+        "    return hipGetLastError();\n}" % (name, nvars, len(coef), fn, len(sl))))
+    return dict(parts=NPARTS, slices=len(sl))
+
+
 def main():
     out_dir = os.path.join(ROOT, "mbx_amd", "csrc", "gen")
     os.makedirs(out_dir, exist_ok=True)
     for name, fn in (("mbpol_2b", "poly_mbpol_2b"), ("mbpol_3b", "poly_mbpol_3b")):
         path, c = generate(name, fn, out_dir)
         print(path, c)
+    for name in AD_POLYS:
+        generate_ad(name, out_dir)
     for name in NRG_PARTS:
         print(name, generate_nrg(name, out_dir))
         generate_holder(name, out_dir)

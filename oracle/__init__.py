@@ -142,6 +142,11 @@ def lib():
         L.orc_mbpol2b.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         L.orc_mbpol3b.restype = ctypes.c_double
         L.orc_mbpol3b.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_mbpol2b_t.restype = ctypes.c_double
+        L.orc_mbpol2b_t.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_mbpol3b_t.restype = ctypes.c_double
+        L.orc_mbpol3b_t.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_mbpol_nmers.argtypes = [ctypes.c_int, P, _dp, _dp, ctypes.c_long, _dp, _dp, _dp]
         L.orc_invert_unit_cell.argtypes = [_dp, _dp]
         L.orc_fix_monomer_coordinates.argtypes = [_dp, _dp, _dp, ctypes.c_int, _ip, _ip]
         L.orc_close_image.argtypes = [_dp, _dp, _dp, ctypes.c_int, _dp]
@@ -179,26 +184,56 @@ def poly_basis(tab, x):
     return b
 
 
-def mbpol2b(w1, w2, a=None):
-    """One dimer: returns (E, g1, g2, virial, evaluated)."""
+def mbpol2b(w1, w2, a=None, theta=None):
+    """One dimer: returns (E, g1, g2, virial, evaluated).  theta: the ten nonlinear parameters
+    (k_HH_intra k_OH_intra k_HH_coul k_OH_coul k_OO_coul k_XH_main k_XO_main k_XX_main
+    in_plane_gamma out_of_plane_gamma), None = the reference's."""
     w1, w2 = _arr(w1), _arr(w2)
     g1, g2, v = np.zeros(9), np.zeros(9), np.zeros(9)
     ev = ctypes.c_int(0)
     a = _arr(a) if a is not None else None
-    e = lib().orc_mbpol2b(ctypes.byref(table("mbpol_2b").c), _d(a), _d(w1), _d(w2), _d(g1), _d(g2), _d(v),
-                          ctypes.byref(ev))
+    th = _arr(theta) if theta is not None else None
+    e = lib().orc_mbpol2b_t(ctypes.byref(table("mbpol_2b").c), _d(a), _d(th), _d(w1), _d(w2), _d(g1), _d(g2),
+                            _d(v), ctypes.byref(ev))
     return e, g1, g2, v, ev.value
 
 
-def mbpol3b(w1, w2, w3, a=None):
-    """One trimer: returns (E, g1, g2, g3, virial, evaluated)."""
+def mbpol3b(w1, w2, w3, a=None, theta=None):
+    """One trimer: returns (E, g1, g2, g3, virial, evaluated).  theta: kHH_intra kOH_intra kHH kOH
+    kOO dHH_intra dOH_intra dHH dOH dOO, None = the reference's."""
     w1, w2, w3 = _arr(w1), _arr(w2), _arr(w3)
     g1, g2, g3, v = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(9)
     ev = ctypes.c_int(0)
     a = _arr(a) if a is not None else None
-    e = lib().orc_mbpol3b(ctypes.byref(table("mbpol_3b").c), _d(a), _d(w1), _d(w2), _d(w3), _d(g1), _d(g2),
-                          _d(g3), _d(v), ctypes.byref(ev))
+    th = _arr(theta) if theta is not None else None
+    e = lib().orc_mbpol3b_t(ctypes.byref(table("mbpol_3b").c), _d(a), _d(th), _d(w1), _d(w2), _d(w3), _d(g1),
+                            _d(g2), _d(g3), _d(v), ctypes.byref(ev))
     return e, g1, g2, g3, v, ev.value
+
+
+MBPOL_THETA = {  # the reference's nonlinear parameters (x2b-v9x.cpp:1330-1345, x3b-v2x.cpp:1256-1271)
+    "mbpol_2b": np.array([-6.480884773303821e-01, 1.674518993682975e+00, 1.148231864355956e+00,
+                          1.205989761123099e+00, 1.395357065790959e+00, 7.347036852042255e-01,
+                          7.998249864422826e-01, 7.960663960630585e-01, -9.721486914088159e-02,
+                          9.859272078406150e-02]),
+    "mbpol_3b": np.array([-2.254303257872797e+00, -2.564771901404151e-01, 4.247920544074333e-01,
+                          8.128985941165371e-01, 3.749457984616480e-02, 1.690594510379166e+00,
+                          -2.999999868517452e+00, 3.499031358429095e+00, 4.854042963514281e+00,
+                          4.816312044947604e-08]),
+}
+
+
+def mbpol_nmers(family, xyz, a=None, theta=None, grad=True):
+    """Independent dimers ([n][18]) or trimers ([n][27]) as given: (E[n], G[n][18|27] or None)."""
+    order = 2 if family == "mbpol_2b" else 3
+    x = _arr(xyz).reshape(-1, 9 * order)
+    n = len(x)
+    e = np.zeros(n)
+    g = np.zeros((n, 9 * order)) if grad else None
+    a = _arr(a) if a is not None else None
+    th = _arr(theta) if theta is not None else None
+    lib().orc_mbpol_nmers(order, ctypes.byref(table(family).c), _d(a), _d(th), n, _d(x), _d(e), _d(g))
+    return e, g
 
 
 def invert_unit_cell(box):

@@ -99,42 +99,9 @@ static double v_coul2(double r0, double k, const double* xc, int o1, int o2, dou
     return val;
 }
 
-/* monomer::setup (x2b-v9x.cpp:117-135): the two M-sites (X1, X2) of a water. */
-static void msite_setup(const double* ohh, double* oh1, double* oh2, double* x1, double* x2) {
-    for (int i = 0; i < 3; ++i) {
-        oh1[i] = ohh[i + 3] - ohh[i];
-        oh2[i] = ohh[i + 6] - ohh[i];
-    }
-    const double v[3] = {oh1[1] * oh2[2] - oh1[2] * oh2[1], oh1[2] * oh2[0] - oh1[0] * oh2[2],
-                         oh1[0] * oh2[1] - oh1[1] * oh2[0]};
-    for (int i = 0; i < 3; ++i) {
-        const double in_plane = ohh[i] + 0.5 * in_plane_gamma * (oh1[i] + oh2[i]);
-        const double out_of_plane = out_of_plane_gamma * v[i];
-        x1[i] = in_plane + out_of_plane;
-        x2[i] = in_plane - out_of_plane;
-    }
-}
-
-/* monomer::grads (x2b-v9x.cpp:139-160): X-site gradients back onto O, H1, H2. */
-static void msite_grads(const double* oh1, const double* oh2, const double* g1, const double* g2, double* grd) {
-    const double gm[3] = {g1[0] - g2[0], g1[1] - g2[1], g1[2] - g2[2]};
-    const double t1[3] = {oh2[1] * gm[2] - oh2[2] * gm[1], oh2[2] * gm[0] - oh2[0] * gm[2],
-                          oh2[0] * gm[1] - oh2[1] * gm[0]};
-    const double t2[3] = {oh1[1] * gm[2] - oh1[2] * gm[1], oh1[2] * gm[0] - oh1[0] * gm[2],
-                          oh1[0] * gm[1] - oh1[1] * gm[0]};
-    for (int i = 0; i < 3; ++i) {
-        const double gsum = g1[i] + g2[i];
-        const double in_plane = 0.5 * in_plane_gamma * gsum;
-        const double gh1 = in_plane + out_of_plane_gamma * t1[i];
-        const double gh2 = in_plane - out_of_plane_gamma * t2[i];
-        grd[i + 0] += gsum - (gh1 + gh2);
-        grd[i + 3] += gh1;
-        grd[i + 6] += gh2;
-    }
-}
-
-/* The same lone-pair construction with explicit gammas: the MB-nrg CH4-H2O holder builds its
- * Z sites with MB-pol's values (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:119-160, 866-872). */
+/* monomer::setup (x2b-v9x.cpp:117-135): the two M-sites (X1, X2) of a water from the in-plane and
+ * out-of-plane gammas; the MB-nrg CH4-H2O holder builds its Z sites the same way with MB-pol's
+ * values (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:119-160, 866-872). */
 static void msite_setup_w(const double* ohh, double ipg, double oopg, double* oh1, double* oh2, double* x1,
                           double* x2) {
     for (int i = 0; i < 3; ++i) {
@@ -151,6 +118,7 @@ static void msite_setup_w(const double* ohh, double ipg, double oopg, double* oh
     }
 }
 
+/* monomer::grads (x2b-v9x.cpp:139-160): X-site gradients back onto O, H1, H2. */
 static void msite_grads_w(const double* oh1, const double* oh2, double ipg, double oopg, const double* g1,
                           const double* g2, double* grd) {
     const double gm[3] = {g1[0] - g2[0], g1[1] - g2[1], g1[2] - g2[2]};
@@ -204,10 +172,15 @@ static const var2_def V2[31] = {
     {2, KXO, Ob, Xa1},   {2, KXO, Ob, Xa2},   {2, KXX, Xa1, Xb1},  {2, KXX, Xa1, Xb2},  {2, KXX, Xa2, Xb1},
     {2, KXX, Xa2, Xb2}};
 
-double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const double* w2, double* g1, double* g2,
-                   double* vir, int* evaluated) {
-    static const double kv[8] = {k_HH_intra, k_OH_intra, k_HH_coul, k_OH_coul,
-                                 k_OO_coul,  k_XH_main,  k_XO_main, k_XX_main};
+/* orc_mbpol2b with the ten nonlinear parameters given (theta, in the order k_HH_intra k_OH_intra
+ * k_HH_coul k_OH_coul k_OO_coul k_XH_main k_XO_main k_XX_main in_plane_gamma out_of_plane_gamma;
+ * NULL = the reference's values). */
+double orc_mbpol2b_t(const orc_poly* p, const double* a, const double* theta, const double* w1, const double* w2,
+                     double* g1, double* g2, double* vir, int* evaluated) {
+    static const double th0[10] = {k_HH_intra, k_OH_intra, k_HH_coul,      k_OH_coul,     k_OO_coul,
+                                   k_XH_main,  k_XO_main,  k_XX_main, in_plane_gamma, out_of_plane_gamma};
+    const double* th = theta ? theta : th0;
+    const double* kv = th;
     /* r_OO and the cutoff test (x2b-v9x.cpp:1589-1600) */
     const double dOO[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
     const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
@@ -219,8 +192,8 @@ double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const d
     memcpy(xc, w1, 9 * sizeof(double));
     memcpy(xc + 9, w2, 9 * sizeof(double));
     double oh1a[3], oh2a[3], oh1b[3], oh2b[3];
-    msite_setup(xc + Oa, oh1a, oh2a, xc + Xa1, xc + Xa2);
-    msite_setup(xc + Ob, oh1b, oh2b, xc + Xb1, xc + Xb2);
+    msite_setup_w(xc + Oa, th[8], th[9], oh1a, oh2a, xc + Xa1, xc + Xa2);
+    msite_setup_w(xc + Ob, th[8], th[9], oh1b, oh2b, xc + Xb1, xc + Xb2);
 
     double v[31], vg[31][3];
     for (int j = 0; j < 31; ++j) {
@@ -249,8 +222,8 @@ double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const d
                 xg[V2[j].o2 + i] -= dd;
             }
         }
-        msite_grads(oh1a, oh2a, xg + Xa1, xg + Xa2, xg + Oa);
-        msite_grads(oh1b, oh2b, xg + Xb1, xg + Xb2, xg + Ob);
+        msite_grads_w(oh1a, oh2a, th[8], th[9], xg + Xa1, xg + Xa2, xg + Oa);
+        msite_grads_w(oh1b, oh2b, th[8], th[9], xg + Xb1, xg + Xb2, xg + Ob);
         double gd1[9], gd2[9];
         for (int j = 0; j < 9; ++j) {
             gd1[j] = sw * xg[j];
@@ -281,6 +254,11 @@ double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const d
         }
     }
     return sw * e2b;
+}
+
+double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const double* w2, double* g1, double* g2,
+                   double* vir, int* evaluated) {
+    return orc_mbpol2b_t(p, a, NULL, w1, w2, g1, g2, vir, evaluated);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -353,10 +331,14 @@ static const var3_def V3[36] = {
     {P_OH, CO, AH2},   {P_OH, CO, BH1},   {P_OH, CO, BH2},   {P_OO, AO, BO},    {P_OO, AO, CO},
     {P_OO, BO, CO}};
 
-double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const double* w2, const double* w3,
-                   double* g1, double* g2, double* g3, double* vir, int* evaluated) {
-    static const double ks[5] = {kHH_intra, kOH_intra, kHH, kOH, kOO};
-    static const double ds[5] = {dHH_intra, dOH_intra, dHH, dOH, dOO};
+/* orc_mbpol3b with the ten nonlinear parameters given (theta = kHH_intra kOH_intra kHH kOH kOO
+ * dHH_intra dOH_intra dHH dOH dOO; NULL = the reference's values). */
+double orc_mbpol3b_t(const orc_poly* p, const double* a, const double* theta, const double* w1, const double* w2,
+                     const double* w3, double* g1, double* g2, double* g3, double* vir, int* evaluated) {
+    static const double th0[10] = {kHH_intra, kOH_intra, kHH, kOH, kOO, dHH_intra, dOH_intra, dHH, dOH, dOO};
+    const double* th = theta ? theta : th0;
+    const double* ks = th;
+    const double* ds = th + 5;
     /* O-O distances and the skip rule (x3b-v2x.cpp:1495-1515) */
     double dab[3], dac[3], dbc[3];
     for (int j = 0; j < 3; ++j) {
@@ -426,6 +408,30 @@ double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const d
         }
     }
     return s * e3;
+}
+
+double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const double* w2, const double* w3,
+                   double* g1, double* g2, double* g3, double* vir, int* evaluated) {
+    return orc_mbpol3b_t(p, a, NULL, w1, w2, w3, g1, g2, g3, vir, evaluated);
+}
+
+/* Independent n-mers (order 2: [n][18] = O H H O H H; order 3: [n][27]), each evaluated as
+ * given (no imaging): e[n] and, when g is not NULL, g[n][18 or 27].  Parallel over n-mers. */
+void orc_mbpol_nmers(int order, const orc_poly* p, const double* a, const double* theta, long n, const double* xyz,
+                     double* e, double* g) {
+    const int w = 9 * order;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < n; ++i) {
+        const double* x = xyz + (size_t)i * w;
+        double gl[27] = {0};
+        if (order == 2)
+            e[i] = orc_mbpol2b_t(p, a, theta, x, x + 9, g ? gl : NULL, g ? gl + 9 : NULL, NULL, NULL);
+        else
+            e[i] = orc_mbpol3b_t(p, a, theta, x, x + 9, x + 18, g ? gl : NULL, g ? gl + 9 : NULL,
+                                 g ? gl + 18 : NULL, NULL, NULL);
+        if (g)
+            for (int j = 0; j < w; ++j) g[(size_t)i * w + j] = gl[j];
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -46,10 +46,22 @@ void orc_poly_basis(const orc_poly* p, const double* x, double* basis);
  * reference x2b_v9x on random dimers, and central finite differences of E. */
 double orc_mbpol2b(const orc_poly* p, const double* a, const double* w1, const double* w2, double* g1, double* g2,
                    double* vir, int* evaluated);
+/* The same with the ten nonlinear parameters given (NULL = the reference's values):
+ * k_HH_intra k_OH_intra k_HH_coul k_OH_coul k_OO_coul k_XH_main k_XO_main k_XX_main
+ * in_plane_gamma out_of_plane_gamma. */
+double orc_mbpol2b_t(const orc_poly* p, const double* a, const double* theta, const double* w1, const double* w2,
+                     double* g1, double* g2, double* vir, int* evaluated);
 
 /* MB-pol 3B holder for ONE trimer (potential/3b/x3b-v2x.cpp:1403-1806).  Same conventions. */
 double orc_mbpol3b(const orc_poly* p, const double* a, const double* w1, const double* w2, const double* w3,
                    double* g1, double* g2, double* g3, double* vir, int* evaluated);
+/* The same with theta = kHH_intra kOH_intra kHH kOH kOO dHH_intra dOH_intra dHH dOH dOO. */
+double orc_mbpol3b_t(const orc_poly* p, const double* a, const double* theta, const double* w1, const double* w2,
+                     const double* w3, double* g1, double* g2, double* g3, double* vir, int* evaluated);
+/* Independent n-mers of one order (2: [n][18], 3: [n][27]) as given (no imaging): energies e[n]
+ * and, when g is not NULL, gradients g[n][18 or 27]. */
+void orc_mbpol_nmers(int order, const orc_poly* p, const double* a, const double* theta, long n, const double* xyz,
+                     double* e, double* g);
 
 /* An MB-nrg holder as data (mbx_amd/tables/<name>.holder.json, extracted from the holder's
  * text): monomers in the holder's argument order, sites = real atoms in that order (+ the
