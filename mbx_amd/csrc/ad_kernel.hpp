@@ -9,7 +9,7 @@
 // k_ad_basis accumulates, for every coefficient k, sum_n [alpha_n M_k(x_n) + sw_n dM_k(x_n)]
 // over the n-mers of a block (M_k = the basis function multiplying coefficient k, dM_k its
 // tangent), coefficients in slices (grid.y = slice).  Each block writes one row of per-block
-// partial sums; k_ad_colsum adds the rows in block order, so the result is deterministic.
+// partial sums; k_ad_colsum adds the rows in a fixed order, so the result is deterministic.
 //
 // Buffers are variable-major: entry (j, t) at j*ld + t.
 #pragma once
@@ -20,6 +20,9 @@
 #include "gen/poly_support.hpp"
 
 namespace mbxg {
+
+// Pieces of the dual-number polynomial (one wavefront each per 64 n-mers).
+constexpr int kAdParts = 16;
 
 // Column sums of the block's LDS tile red[64][65]: lane j adds row j (its coefficient's values
 // from the 64 lanes) in lane order.
@@ -94,8 +97,5 @@ __global__ void __launch_bounds__(64) k_ad_basis(const double* __restrict__ xs, 
     FN(blockIdx.y, x, xd, alpha, s, red, partial + (size_t)blockIdx.x * NCOEF, lane);
 }
 
-// out[k] = sum over rows r < nrows of partial[r * ncol + k], in row order.
-__global__ void __launch_bounds__(256) k_ad_colsum(const double* __restrict__ partial, int nrows, int ncol,
-                                                   double* __restrict__ out);
 
 }  // namespace mbxg

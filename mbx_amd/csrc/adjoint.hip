@@ -69,7 +69,7 @@ constexpr int kNTheta = 10;
 
 // Sites 0..5 = Oa Ha1 Ha2 Ob Hb1 Hb2, 6..9 = Xa1 Xa2 Xb1 Xb2 (x2b-v9x.cpp:1640-1675).
 struct Ad2b {
-    static constexpr int kNV = 31, kNAt = 6, kNS = 10, kPairs = 1, kParts = 8;
+    static constexpr int kNV = 31, kNAt = 6, kNS = 10, kPairs = 1, kParts = kAdParts;
     static constexpr bool kMsites = true;
     static constexpr double kRIn = p2::r2i, kROut = p2::r2f;
     static constexpr AdVar kVars[31] = {
@@ -87,7 +87,7 @@ struct Ad2b {
 // Sites 0..8 = Oa Ha1 Ha2 Ob Hb1 Hb2 Oc Hc1 Hc2 (x3b-v2x.cpp:1548-1584); parameter set s uses
 // k = theta[s], d = theta[5 + s].
 struct Ad3b {
-    static constexpr int kNV = 36, kNAt = 9, kNS = 9, kPairs = 3, kParts = 8;
+    static constexpr int kNV = 36, kNAt = 9, kNS = 9, kPairs = 3, kParts = kAdParts;
     static constexpr bool kMsites = false;
     static constexpr double kRIn = p3::r3i, kROut = p3::r3f;
     static constexpr AdVar kVars[36] = {
@@ -296,9 +296,9 @@ __global__ void __launch_bounds__(64) k_ad_vars(const double* __restrict__ xyz, 
 }
 
 // Stage 4.  Forward (BWD = false): energy_out = sw * P and dx_out = dE/dx from the plain
-// polynomial's value and gradient (one slice).  Backward: dx_out = dphi/dx and per-block
-// partial sums of dphi/dtheta (theta_partial[block * 16 + i]) from the dual polynomial's
-// `parts` slices.
+// polynomial's value and gradient.  Backward: dx_out = dphi/dx and per-block partial sums of
+// dphi/dtheta (theta_partial[block * 16 + i]) from the dual polynomial's outputs.  Polynomial
+// outputs come in `parts` slices, added here.
 template <class F, bool BWD>
 __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz, const double* __restrict__ u,
                                                  const double* __restrict__ theta,
@@ -486,23 +486,41 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
     }
 }
 
-__global__ void __launch_bounds__(256) k_ad_colsum(const double* __restrict__ partial, int nrows, int ncol,
-                                                   double* __restrict__ out) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
+// out[k] = sum over rows r < nrows of partial[r * stride + k] for k < ncol: one wavefront per
+// column, 64 rows at a time, a fixed butterfly order (deterministic).
+__global__ void __launch_bounds__(64) k_ad_colsum(const double* __restrict__ partial, int nrows, int ncol, int stride,
+                                                  double* __restrict__ out) {
+    const int k = blockIdx.x;
     if (k >= ncol) return;
     double s = 0.0;
-    for (int r = 0; r < nrows; ++r) s += partial[(size_t)r * ncol + k];
-    out[k] = s;
+    for (int r = threadIdx.x; r < nrows; r += 64) s += partial[(size_t)r * stride + k];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[k] = s;
 }
 
-// theta: rows of 16 (10 used).
-__global__ void __launch_bounds__(64) k_ad_theta_sum(const double* __restrict__ partial, int nrows,
-                                                     double* __restrict__ out) {
-    const int k = threadIdx.x;
-    if (k >= kNTheta) return;
-    double s = 0.0;
-    for (int r = 0; r < nrows; ++r) s += partial[(size_t)r * 16 + k];
-    out[k] = s;
+// Sum the parts' slices of the dual polynomial's outputs into slice 0: rows 0..nv-1 of gs and
+// gds, then ps and pds (entry (row, t) of part q at (q * rows + row) * ld + t).  One thread per
+// entry, parts added in order.
+__global__ void __launch_bounds__(256) k_ad_sum_parts(double* __restrict__ gs, double* __restrict__ gds,
+                                                      double* __restrict__ ps, double* __restrict__ pds, int nv,
+                                                      int ld, int parts) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t ng = (size_t)nv * ld;
+    if (i >= 2 * ng + 2 * (size_t)ld) return;
+    double* base;
+    size_t e, rows;
+    if (i < ng) {
+        base = gs, e = i, rows = ng;
+    } else if (i < 2 * ng) {
+        base = gds, e = i - ng, rows = ng;
+    } else if (i < 2 * ng + ld) {
+        base = ps, e = i - 2 * ng, rows = ld;
+    } else {
+        base = pds, e = i - 2 * ng - ld, rows = ld;
+    }
+    double s = base[e];
+    for (int q = 1; q < parts; ++q) s += base[q * rows + e];
+    base[e] = s;
 }
 
 __global__ void k_ad_set_int(int* p, int v) { *p = v; }
@@ -536,9 +554,9 @@ bool ad_shape(int family, int& nv, int& nat, int& ncoef) {
     return false;
 }
 
-// Workspace: stage buffers for ld = n rounded up to 64 lanes, 8 polynomial parts.
+// Workspace: stage buffers for ld = n rounded up to 64 lanes, kAdParts polynomial parts.
 AdLayout ad_layout(int nv, int ncoef, int n) {
-    const size_t ld = (size_t)((n + 63) / 64) * 64, nblk = ld / 64, parts = 8;
+    const size_t ld = (size_t)((n + 63) / 64) * 64, nblk = ld / 64, parts = kAdParts;
     AdLayout L;
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -692,18 +710,24 @@ extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const doub
                            swd);
         ADCHK(launch_ad_dual_mbpol_2b(xs, xds, n, ld, coef, gs, gds, ps, pds, st));
         ADCHK(launch_ad_basis_mbpol_2b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
-        hipLaunchKernelGGL((k_ad_chain<Ad2b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld,
-                           Ad2b::kParts, gs, gds, ps, pds, nullptr, d_xyz, thp);
+        const size_t tot = 2 * (size_t)(Ad2b::kNV + 1) * ld;
+        hipLaunchKernelGGL(k_ad_sum_parts, dim3((tot + 255) / 256), dim3(256), 0, st, gs, gds, ps, pds, Ad2b::kNV,
+                           ld, Ad2b::kParts);
+        hipLaunchKernelGGL((k_ad_chain<Ad2b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld, 1,
+                           gs, gds, ps, pds, nullptr, d_xyz, thp);
     } else {
         hipLaunchKernelGGL((k_ad_vars<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, n, ld, xs, xds, sw,
                            swd);
         ADCHK(launch_ad_dual_mbpol_3b(xs, xds, n, ld, coef, gs, gds, ps, pds, st));
         ADCHK(launch_ad_basis_mbpol_3b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
-        hipLaunchKernelGGL((k_ad_chain<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld,
-                           Ad3b::kParts, gs, gds, ps, pds, nullptr, d_xyz, thp);
+        const size_t tot = 2 * (size_t)(Ad3b::kNV + 1) * ld;
+        hipLaunchKernelGGL(k_ad_sum_parts, dim3((tot + 255) / 256), dim3(256), 0, st, gs, gds, ps, pds, Ad3b::kNV,
+                           ld, Ad3b::kParts);
+        hipLaunchKernelGGL((k_ad_chain<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld, 1,
+                           gs, gds, ps, pds, nullptr, d_xyz, thp);
     }
-    hipLaunchKernelGGL(k_ad_colsum, dim3((ncoef + 255) / 256), dim3(256), 0, st, basis, nb, ncoef, d_coef);
-    hipLaunchKernelGGL(k_ad_theta_sum, dim3(1), dim3(64), 0, st, thp, nb, d_theta);
+    hipLaunchKernelGGL(k_ad_colsum, dim3(ncoef), dim3(64), 0, st, basis, nb, ncoef, ncoef, d_coef);
+    hipLaunchKernelGGL(k_ad_colsum, dim3(kNTheta), dim3(64), 0, st, thp, nb, kNTheta, 16, d_theta);
     ADCHK(hipGetLastError());
     return MBXG_OK;
 }

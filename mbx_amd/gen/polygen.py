@@ -630,8 +630,9 @@ def emit_basis(fn, monos, c0, c1):
     return "\n".join(out) + "\n"
 
 
-# Differentiated polynomials: dual-number parts (as the E+G schedule's NPARTS pieces) and
-# coefficient-basis slices.
+# Differentiated polynomials: dual-number parts (the schedule partitioned into AD_PARTS pieces,
+# one wavefront each; kAdParts in ad_kernel.hpp) and coefficient-basis slices.
+AD_PARTS = 16
 AD_POLYS = {"mbpol_2b": 16, "mbpol_3b": 16}  # name -> basis slices
 
 
@@ -640,21 +641,21 @@ def generate_ad(name, out_dir):
     (compiled in parallel), each with its kernel's launcher."""
     nvars, coef, monos = load_table(name)
     s = Schedule(nvars, [(k, m) for k, m in monos])
-    assign = s.partition(NPARTS)
+    assign = s.partition(AD_PARTS)
     fn = "ad_%s" % name
     sl = basis_slices(monos, len(coef), AD_POLYS[name])
     head = lambda what: ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
                          "// %s" % what, "#pragma once", "#include \"poly_support.hpp\"", "#include \"ad_kernel.hpp\"",
                          "", "namespace mbxg {", ""]
     # dual-number polynomial, NPARTS parts
-    body = [emit_dual(s, "%s_dual_p%d" % (fn, k), k, assign) for k in range(NPARTS)]
+    body = [emit_dual(s, "%s_dual_p%d" % (fn, k), k, assign) for k in range(AD_PARTS)]
     d = ["__device__ __forceinline__ void %s_dual(int part, mbxg_cptr C, const double* x, const double* xd, "
          "double* g, double* gd, double& E, double& dE) {" % fn]
-    d += ["    if (part == %d) %s_dual_p%d(C, x, xd, g, gd, E, dE);" % (k, fn, k) for k in range(NPARTS)]
+    d += ["    if (part == %d) %s_dual_p%d(C, x, xd, g, gd, E, dE);" % (k, fn, k) for k in range(AD_PARTS)]
     d += ["}", "", "}  // namespace mbxg", ""]
     write_if_changed(os.path.join(out_dir, "ad_%s_dual.inc" % name),
                      "\n".join(head("Dual-number polynomial in %d parts (value, tangent, gradient, gradient "
-                                    "tangent)." % NPARTS)) + "\n".join(body) + "\n".join(d))
+                                    "tangent)." % AD_PARTS)) + "\n".join(body) + "\n".join(d))
     # coefficient basis, in slices of near-equal cost
     body = [emit_basis("%s_basis_s%d" % (fn, i), monos, c0, c1) for i, (c0, c1) in enumerate(sl)]
     d = ["__device__ __forceinline__ void %s_basis(int slice, double* x, double* xd, double alpha, "
@@ -673,10 +674,11 @@ def generate_ad(name, out_dir):
         "Dual-number evaluation", "ad_%s_dual.inc" % name,
         "hipError_t launch_ad_dual_%s(const double* xs, const double* xds, int n, int ld, const double* coef, "
         "double* gs, double* gds, double* ps, double* pds, hipStream_t st) {\n"
+        "    static_assert(kAdParts == %d, \"part count of the generated code\");\n"
         "    if (n <= 0) return hipSuccess;\n"
         "    hipLaunchKernelGGL((k_ad_poly<%d, %d, %s_dual>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, n, "
         "ld,\n                       (uint64_t)(uintptr_t)coef, gs, gds, ps, pds);\n"
-        "    return hipGetLastError();\n}" % (name, nvars, NPARTS, fn, NPARTS)))
+        "    return hipGetLastError();\n}" % (name, AD_PARTS, nvars, AD_PARTS, fn, AD_PARTS)))
     write_if_changed(os.path.join(out_dir, "ad_%s_basis.hip" % name), tu(
         "Coefficient basis", "ad_%s_basis.inc" % name,
         "hipError_t launch_ad_basis_%s(const double* xs, const double* xds, const double* sw, const double* swd, "
@@ -685,7 +687,7 @@ def generate_ad(name, out_dir):
         "    hipLaunchKernelGGL((k_ad_basis<%d, %d, %s_basis>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, sw, "
         "swd,\n                       g_energy, n, ld, partial);\n"
         "    return hipGetLastError();\n}" % (name, nvars, len(coef), fn, len(sl))))
-    return dict(parts=NPARTS, slices=len(sl))
+    return dict(parts=AD_PARTS, slices=len(sl))
 
 
 def main():

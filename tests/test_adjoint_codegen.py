@@ -120,12 +120,12 @@ def test_dual_parts(name):
     ref = Reference(name)
     monos = _monos(ref)
     s = polygen.Schedule(ref.nv, list(zip(ref.k.tolist(), monos)))
-    assign = s.partition(polygen.NPARTS)
+    assign = s.partition(polygen.AD_PARTS)
     rng = np.random.default_rng(3)
     x, xd = rng.uniform(0.1, 0.9, ref.nv), rng.normal(size=ref.nv)
     a = rng.normal(size=len(ref.coef))
     E, dE, g, gd = 0.0, 0.0, np.zeros(ref.nv), np.zeros(ref.nv)
-    for p in range(polygen.NPARTS):
+    for p in range(polygen.AD_PARTS):
         f = dual_fn(polygen.emit_dual(s, "f", p, assign), "f")
         gp, gdp = [0.0] * ref.nv, [0.0] * ref.nv
         e, de = f(list(a), list(x), list(xd), gp, gdp, 0.0, 0.0)
