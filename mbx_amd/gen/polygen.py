@@ -281,8 +281,9 @@ class Schedule:
         else:
             w("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x%s) {" % (fn, e0))
         name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
+        addr = lambda r, q: q
         for j, q in enumerate(coefs[0]):
-            w("    const double %s = C[%d];" % (name(0, q), q))
+            w("    const double %s = C[%d];" % (name(0, q), addr(0, q)))
         last = None
         for r, reg in enumerate(regions):
             # the fence consumes the previous region's last result, so no load of this or the
@@ -290,7 +291,7 @@ class Schedule:
             w("    MBXG_NODE_FENCE(C, %s);" % (last or ("E0" if part is not None else "x[0]")))
             if r + 1 < len(regions):
                 for q in coefs[r + 1]:
-                    w("    const double %s = C[%d];" % (name(r + 1, q), q))
+                    w("    const double %s = C[%d];" % (name(r + 1, q), addr(r + 1, q)))
             for t, q in reg:
                 w("    " + (t.replace("{c}", name(r, q)) if q is not None else t))
                 if q is not None:
