@@ -93,8 +93,10 @@ struct ListState {
     DBuf<Grid> d_grids;           // one per configuration
     DBuf<int> d_cell_base, d_cell_bound;
     DBuf<double> cx, cy, cz, bx, by, bz;
+    DBuf<PtInfo> pinfo;           // per cloud point: first atom, tag, kind
     DBuf<int> cell, cell_count, cell_start, cell_slot;
     DBuf<int4> counts, offsets, counts_x, offsets_x;
+    DBuf<int4> wpre, wpre_x;      // per (query, wavefront): prefix within the query
     DBuf<int> list2, list3, evals[kFamilies];
     DBuf<double> gscratch;        // match storage when it does not fit in LDS
     // capacities in n-mers: cap = {list2, MB-pol 2B, list3, MB-pol 3B},
@@ -218,6 +220,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.d_cell_base.release();
     L.d_cell_bound.release();
     L.cx.release();
+    L.pinfo.release();
     L.cy.release();
     L.cz.release();
     L.bx.release();
@@ -233,6 +236,8 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.list3.release();
     L.counts_x.release();
     L.offsets_x.release();
+    L.wpre.release();
+    L.wpre_x.release();
     for (DBuf<int>& b : L.evals) b.release();
     L.gscratch.release();
     c->d_pconf.release();
@@ -658,7 +663,8 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     if (!L.layout_ok || L.layout_cut != cut)
         if (int r = prepare_layout(c, cut)) return r;
     if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.bx.reserve(ns) || L.by.reserve(ns) ||
-        L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.counts.reserve(np) ||
+        L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.pinfo.reserve(np) ||
+        L.counts.reserve(np) || L.wpre.reserve((size_t)np * kQueryWaves) || L.wpre_x.reserve((size_t)np * kQueryWaves) ||
         L.offsets.reserve(np + 1) || L.counts_x.reserve(np) || L.offsets_x.reserve(np + 1))
         return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
     const int tb = 256, pb = (np + tb - 1) / tb, sb = (ns + tb - 1) / tb;
@@ -666,14 +672,16 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p,
                            c->d_pt_off.p, L.d_cell_bound.p, cut, L.d_grids.p);
     launch_zero_int(L.cell_count.p, L.ncell_total, st);
-    hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, np, c->box,
-                       L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p);
+    hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, c->d_tag.p,
+                       c->d_kind.p, np, c->box, L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p,
+                       L.cell.p, L.pinfo.p);
     hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p);
     hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_start.p, L.cell_count.p, L.cx.p,
                        L.cy.p, L.cz.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
     HIPCHK(hipGetLastError());
     NmersArgs A{};
+    A.pinfo = L.pinfo.p;
     A.cx = L.cx.p;
     A.cy = L.cy.p;
     A.cz = L.cz.p;
@@ -693,6 +701,8 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     A.counts_x = L.counts_x.p;
     A.offsets = L.offsets.p;
     A.offsets_x = L.offsets_x.p;
+    A.wpre = L.wpre.p;
+    A.wpre_x = L.wpre_x.p;
     A.overflow = c->d_overflow.p;
     A.fam_mask = c->fam_mask;
 

@@ -137,9 +137,10 @@ __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz
 // slots of the points near it.  Cell index = cell_base[config] + local cell.  One thread per
 // point.
 __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
-                        const int* __restrict__ members, int np, Box B, const Grid* __restrict__ grids,
-                        const int* __restrict__ pconf, const int* __restrict__ cell_base, double* __restrict__ cx,
-                        double* __restrict__ cy, double* __restrict__ cz, int* __restrict__ cell) {
+                        const int* __restrict__ members, const int* __restrict__ tag, const int* __restrict__ kind,
+                        int np, Box B, const Grid* __restrict__ grids, const int* __restrict__ pconf,
+                        const int* __restrict__ cell_base, double* __restrict__ cx, double* __restrict__ cy,
+                        double* __restrict__ cz, int* __restrict__ cell, PtInfo* __restrict__ pinfo) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int conf = pconf[p];
@@ -154,8 +155,10 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
         cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
         return base + (cc * G.n[1] + cb) * G.n[0] + ca;
     };
-    const double* x0 = xyz + 3 * (size_t)first_atom[members[p]];
+    const int mp = members[p];
+    const double* x0 = xyz + 3 * (size_t)first_atom[mp];
     double qx = x0[0], qy = x0[1], qz = x0[2];
+    pinfo[p] = PtInfo{qx, qy, qz, tag[mp], kind[mp]};
     if (B.periodic) {
         double u, v, w;
         to_frac(B, qx, qy, qz, u, v, w);
@@ -308,12 +311,11 @@ __device__ __forceinline__ Lds carve(double* d, int cap) {
     return L;
 }
 
-// Bitonic sort of a[0..n) (n a power of two) by one wavefront.
+// Bitonic sort of a[0..n) (n a power of two) by the block.
 __device__ void sort_ints(int* a, int n) {
-    const int lane = threadIdx.x;
     for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < n; i += 64) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 const int ixj = i ^ j;
                 if (ixj > i) {
                     const int ai = a[i], aj = a[ixj];
@@ -331,9 +333,14 @@ __device__ void sort_ints(int* a, int n) {
 
 }  // namespace
 
-// One query: gather, sort, per-match data, dimers, trimers.  L is the block's match storage
-// (LDS, or a global scratch slice when the matches do not fit in LDS).
-__device__ void nmers_query(int q, const double* __restrict__ cx, const double* __restrict__ cy,
+// One query, by the block's kQueryWaves wavefronts: gather, sort, per-match data, dimers,
+// trimers.  L is the block's match storage (LDS, or a global scratch slice when the matches do
+// not fit in LDS); sh_count and sh_w are block-shared.  The dimers and trimers of a query are
+// enumerated in a fixed order and cut into kQueryWaves contiguous ranges, one per wavefront; the
+// count pass stores each wavefront's prefix within the query (wpre, entry q * kQueryWaves + w)
+// beside the query's counts, so the lists keep one deterministic order.
+__device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const double* __restrict__ cx,
+                            const double* __restrict__ cy,
                             const double* __restrict__ cz, const int* __restrict__ cell,
                             const int* __restrict__ cell_start, const int* __restrict__ cell_slot,
                             const double* __restrict__ bx, const double* __restrict__ by,
@@ -342,11 +349,12 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
                             int4* __restrict__ counts_x, const int4* __restrict__ offsets,
                             const int4* __restrict__ offsets_x, int* __restrict__ list2, int* __restrict__ list3,
                             int* const* evals, int fam_mask, int4 cap, int4 capx, int match_cap,
-                            int* __restrict__ overflow, const Lds& L) {
-    const int lane = threadIdx.x;
+                            int* __restrict__ overflow, const Lds& L, int* sh_count, int4* sh_w,
+                            int4* __restrict__ wpre, int4* __restrict__ wpre_x) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int mi = N.members[q];
     if (!N.islocal[mi]) {
-        if (pass == 0 && lane == 0) {
+        if (pass == 0 && tid == 0) {
             counts[q] = make_int4(0, 0, 0, 0);
             counts_x[q] = make_int4(0, 0, 0, 0);
         }
@@ -355,55 +363,114 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
     const double cmsq = c2sq > c3sq ? c2sq : c3sq;
 
     // ---- gather: every slot strictly inside the larger cutoff, from the 27 cells around the
-    // query's (clamped at the grid's faces)
+    // query's (clamped at the grid's faces).  The (up to) three cells along a of each of the 9
+    // rows are contiguous in cell order; the rows' ranges are read first and the candidates of
+    // all rows enumerated as one sequence, two chunks of 64 per step so each step's loads are
+    // in flight together.
     const double qx = cx[q], qy = cy[q], qz = cz[q];
     const int ci = cell[q] - base;
     const int ia = ci % G.n[0], ib = (ci / G.n[0]) % G.n[1], ic = ci / (G.n[0] * G.n[1]);
-    int nm = 0;
+    int rbeg[9], rlen[9];
+    int nrow = 0;
     for (int cc = max(ic - 1, 0); cc <= min(ic + 1, G.n[2] - 1); ++cc) {
         for (int cb = max(ib - 1, 0); cb <= min(ib + 1, G.n[1] - 1); ++cb) {
             const int row = base + (cc * G.n[1] + cb) * G.n[0];
-            // the (up to) three cells along a are contiguous in cell order
-            const int beg = cell_start[row + max(ia - 1, 0)];
-            const int end = cell_start[row + min(ia + 1, G.n[0] - 1) + 1];
-            for (int i0 = beg; i0 < end; i0 += 64) {
-                const int i = i0 + lane;
-                int s = -1;
-                if (i < end && dist2(qx, qy, qz, bx[i], by[i], bz[i]) < cmsq) s = cell_slot[i];
-                const unsigned long long mask = __ballot(s >= 0);
-                const int pos = nm + __popcll(mask & ((1ull << lane) - 1ull));
-                if (s >= 0 && pos < match_cap) L.match[pos] = s;
-                nm += __popcll(mask);
-            }
+            rbeg[nrow] = row + max(ia - 1, 0);
+            rlen[nrow] = row + min(ia + 1, G.n[0] - 1) + 1;
+            ++nrow;
         }
     }
-    if (lane == 0) atomicMax(overflow, nm);
-    if (nm > match_cap) nm = match_cap;
+    int ncand = 0;
+    for (int r = 0; r < 9; ++r) {
+        if (r < nrow) {
+            const int b0 = cell_start[rbeg[r]], e0 = cell_start[rlen[r]];
+            rbeg[r] = b0;
+            rlen[r] = e0 - b0;
+            ncand += e0 - b0;
+        }
+    }
+    // candidate c -> slot-array position (row by row, in order)
+    auto position = [&](int c) {
+        int at = -1;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (r < nrow && at < 0) {
+                if (c < rlen[r])
+                    at = rbeg[r] + c;
+                else
+                    c -= rlen[r];
+            }
+        }
+        return at;
+    };
+    if (tid == 0) *sh_count = 0;
+    __syncthreads();
+    // each wavefront appends its matches at a block-shared cursor (the sort fixes the order)
+    const int stride = 64 * kQueryWaves;
+    for (int c0 = wv * 128; c0 < ncand; c0 += 2 * stride) {
+        const int ca = c0 + lane, cb = c0 + 64 + lane;
+        const int pa = ca < ncand ? position(ca) : -1, pb = cb < ncand ? position(cb) : -1;
+        double ax = 0.0, ay = 0.0, az = 0.0, bxv = 0.0, byv = 0.0, bzv = 0.0;
+        int sa = -1, sb = -1;
+        if (pa >= 0) {
+            ax = bx[pa], ay = by[pa], az = bz[pa];
+            sa = cell_slot[pa];
+        }
+        if (pb >= 0) {
+            bxv = bx[pb], byv = by[pb], bzv = bz[pb];
+            sb = cell_slot[pb];
+        }
+        const int s1 = pa >= 0 && dist2(qx, qy, qz, ax, ay, az) < cmsq ? sa : -1;
+        const int s2 = pb >= 0 && dist2(qx, qy, qz, bxv, byv, bzv) < cmsq ? sb : -1;
+        const unsigned long long m1 = __ballot(s1 >= 0), m2 = __ballot(s2 >= 0);
+        int at0 = 0;
+        if (lane == 0) at0 = atomicAdd(sh_count, __popcll(m1) + __popcll(m2));
+        at0 = __shfl(at0, 0, 64);
+        const int p1 = at0 + __popcll(m1 & ((1ull << lane) - 1ull));
+        if (s1 >= 0 && p1 < match_cap) L.match[p1] = s1;
+        const int p2 = at0 + __popcll(m1) + __popcll(m2 & ((1ull << lane) - 1ull));
+        if (s2 >= 0 && p2 < match_cap) L.match[p2] = s2;
+    }
+    __syncthreads();
+    const int nm = *sh_count;
+    if (tid == 0) atomicMax(overflow, nm);
+    if (nm > match_cap) {
+        // The matches did not fit.  Which of them reached storage depends on the wavefronts'
+        // timing, so the count and emit passes could keep different subsets and disagree on this
+        // query's n-mers; it emits none in either pass.  The caller sees the overflow and
+        // recomputes with more storage.
+        if (pass == 0 && tid == 0) {
+            counts[q] = make_int4(0, 0, 0, 0);
+            counts_x[q] = make_int4(0, 0, 0, 0);
+        }
+        return;
+    }
     // ---- sort the matches by slot index (pad to a power of two; match_cap is one)
     int pw = 1;
     while (pw < nm) pw <<= 1;
-    for (int a = nm + lane; a < pw; a += 64) L.match[a] = 0x7fffffff;
+    for (int a = nm + tid; a < pw; a += blockDim.x) L.match[a] = 0x7fffffff;
     __syncthreads();
     sort_ints(L.match, pw);
 
-    // ---- per-match data
+    // ---- per-match data (one payload load per match: PtInfo of its cloud point)
     const double* oi = N.xyz + 3 * (size_t)N.first_atom[mi];
     const double oix = oi[0], oiy = oi[1], oiz = oi[2];
     const int ti = N.tag[mi], ki = N.kind[mi];
-    for (int a = lane; a < nm; a += 64) {
+    for (int a = tid; a < nm; a += blockDim.x) {
         const int s = L.match[a];
-        const int mj = N.members[s % np];
+        const int p = s % np;
+        const int mj = N.members[p];
+        const PtInfo P = pinfo[p];
         const double x = cx[s], y = cy[s], z = cz[s];
         const double d = dist2(qx, qy, qz, x, y, z);
         L.sx[a] = x;
         L.sy[a] = y;
         L.sz[a] = z;
         L.mon[a] = mj;
-        L.tag[a] = N.tag[mj];
-        L.kind[a] = N.kind[mj];
+        L.tag[a] = P.tag;
+        L.kind[a] = P.kind;
         L.flags[a] = (d < c2sq ? 1 : 0) | (d < c3sq ? 2 : 0);
-        const double* oj = N.xyz + 3 * (size_t)N.first_atom[mj];
-        double jx = oj[0], jy = oj[1], jz = oj[2];
+        double jx = P.x, jy = P.y, jz = P.z;
         if (N.periodic) {
             const Box& B = *N.box;
             double dx, dy, dz;
@@ -421,20 +488,35 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
         L.r[a] = sqrt(rr);
     }
     __syncthreads();
-    // compact the 3B matches (order preserved)
-    int n3 = 0;
-    for (int b = 0; b < nm; b += 64) {
-        const int a = b + lane;
-        const bool in3 = a < nm && (L.flags[a] & 2);
-        const unsigned long long m = __ballot(in3);
-        if (in3) L.m3[n3 + __popcll(m & ((1ull << lane) - 1ull))] = a;
-        n3 += __popcll(m);
+    // compact the 3B matches (order preserved; wavefront 0)
+    if (wv == 0) {
+        int n3w = 0;
+        for (int b = 0; b < nm; b += 64) {
+            const int a = b + lane;
+            const bool in3 = a < nm && (L.flags[a] & 2);
+            const unsigned long long m = __ballot(in3);
+            if (in3) L.m3[n3w + __popcll(m & ((1ull << lane) - 1ull))] = a;
+            n3w += __popcll(m);
+        }
+        if (lane == 0) *sh_count = n3w;
     }
     __syncthreads();
+    const int n3 = *sh_count;
 
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int4 at = pass ? offsets[q] : make_int4(0, 0, 0, 0);
-    const int4 atx = pass ? offsets_x[q] : make_int4(0, 0, 0, 0);
+    int4 at = make_int4(0, 0, 0, 0), atx = make_int4(0, 0, 0, 0);
+    if (pass) {  // this wavefront's base: the query's offset plus the lower wavefronts' counts
+        const int4 o = offsets[q], ox = offsets_x[q];
+        const int4 w = wpre[q * kQueryWaves + wv], wx = wpre_x[q * kQueryWaves + wv];
+        at = make_int4(o.x + w.x, o.y + w.y, o.z + w.z, o.w + w.w);
+        atx = make_int4(ox.x + wx.x, ox.y + wx.y, ox.z + wx.z, ox.w + wx.w);
+    }
+    // this wavefront's range of a sequence of n items: whole chunks of 64, in order
+    auto range = [&](int n, int& b0, int& b1) {
+        const int chunks = (n + 63) / 64, per = (chunks + kQueryWaves - 1) / kQueryWaves;
+        b0 = min(n, wv * per * 64);
+        b1 = min(n, (wv + 1) * per * 64);
+    };
     // evaluate-list bases and capacities per family (FAM_*)
     const int fbase[kFamilies] = {at.y, atx.x, atx.y, at.w, atx.z};
     const int fcap[kFamilies] = {cap.y, capx.x, capx.y, cap.w, capx.z};
@@ -445,11 +527,13 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
     // (x2b-v9x.cpp:1589-1600); the MB-nrg holders skip r > r2f = 9 on the first atoms
     // (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:840, x2b_A1B4_A1B4_deg4_exp0_v1x.cpp:1489).
     if (c2sq > 0.0) {
-        for (int b = 0; b < nm; b += 64) {
+        int b0, b1;
+        range(nm, b0, b1);
+        for (int b = b0; b < b1; b += 64) {
             const int a = b + lane;
             bool listed = false;
             int mj = 0, fam = -1;
-            if (a < nm && (L.flags[a] & 1)) {
+            if (a < b1 && (L.flags[a] & 1)) {
                 mj = L.mon[a];
                 listed = ti < L.tag[a];
                 if (listed) {
@@ -494,14 +578,23 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
     // only when all three first-atom distances exceed 6 (mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp:1364);
     // other mixtures have no 3B term (energy3b.cpp:258-260).
     if (c3sq > 0.0) {
-        const int total = n3 * n3;
-        for (int b = 0; b < total; b += 64) {
+        // pairs u < v of the 3B matches in (u, v) order: pair t lies in row u, which starts at
+        // S(u) = u (2 n3 - u - 1) / 2 (a floating-point first guess, corrected exactly)
+        const int total = n3 * (n3 - 1) / 2;
+        int b0, b1;
+        range(total, b0, b1);
+        for (int b = b0; b < b1; b += 64) {
             const int t = b + lane;
             bool listed = false;
             int mj = 0, mk = 0, fam = -1;
-            if (t < total) {
-                const int u = t / n3, v = t % n3;
-                if (u < v) {
+            if (t < b1) {
+                const double w = 2.0 * n3 - 1.0;
+                int u = (int)((w - sqrt(w * w - 8.0 * t)) * 0.5);
+                u = u < 0 ? 0 : (u > n3 - 2 ? n3 - 2 : u);
+                while (u > 0 && u * (2 * n3 - u - 1) / 2 > t) --u;
+                while (u < n3 - 2 && (u + 1) * (2 * n3 - u - 2) / 2 <= t) ++u;
+                const int v = t - u * (2 * n3 - u - 1) / 2 + u + 1;
+                {
                     const int ja = L.m3[u], ka = L.m3[v];
                     mj = L.mon[ja];
                     mk = L.mon[ka];
@@ -556,9 +649,24 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
             }
         }
     }
-    if (pass == 0 && lane == 0) {
-        counts[q] = make_int4(nl2, ne[FAM_MBPOL_2B], nl3, ne[FAM_MBPOL_3B]);
-        counts_x[q] = make_int4(ne[FAM_CH4H2O_2B], ne[FAM_CH4CH4_2B], ne[FAM_CH4H2O2_3B], 0);
+    if (pass == 0) {  // the query's counts, and each wavefront's prefix within the query
+        if (lane == 0) {
+            sh_w[wv] = make_int4(nl2, ne[FAM_MBPOL_2B], nl3, ne[FAM_MBPOL_3B]);
+            sh_w[kQueryWaves + wv] = make_int4(ne[FAM_CH4H2O_2B], ne[FAM_CH4CH4_2B], ne[FAM_CH4H2O2_3B], 0);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int4 a = make_int4(0, 0, 0, 0), ax = make_int4(0, 0, 0, 0);
+            for (int w = 0; w < kQueryWaves; ++w) {
+                wpre[q * kQueryWaves + w] = a;
+                wpre_x[q * kQueryWaves + w] = ax;
+                const int4 c = sh_w[w], cx = sh_w[kQueryWaves + w];
+                a = make_int4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + c.w);
+                ax = make_int4(ax.x + cx.x, ax.y + cx.y, ax.z + cx.z, ax.w + cx.w);
+            }
+            counts[q] = a;
+            counts_x[q] = ax;
+        }
     }
 }
 
@@ -568,17 +676,20 @@ __device__ void nmers_query(int q, const double* __restrict__ cx, const double* 
 // overflow[0] receives the largest match count of any query (the host compares it with
 // match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
 // the dynamic LDS, otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.
-__global__ void __launch_bounds__(64) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap,
-                                              double* __restrict__ gscratch) {
+__global__ void __launch_bounds__(64 * kQueryWaves) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx,
+                                                            int match_cap, double* __restrict__ gscratch) {
     extern __shared__ double smem[];
+    __shared__ int sh_count;
+    __shared__ int4 sh_w[2 * kQueryWaves];
     double* store = gscratch ? gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles : smem;
     const Lds L = carve(store, match_cap);
     for (int q = blockIdx.x; q < A.np; q += gridDim.x) {
         const int conf = A.pconf[q];
         const Grid G = A.grids[conf];
-        nmers_query(q, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
+        nmers_query(q, A.pinfo, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
                     A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.counts_x, A.offsets, A.offsets_x,
-                    A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, A.overflow, L);
+                    A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, A.overflow, L, &sh_count, sh_w,
+                    A.wpre, A.wpre_x);
         __syncthreads();
     }
 }
@@ -713,7 +824,7 @@ hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap
     if (A.np <= 0) return hipSuccess;
     const int nb = gscratch ? (nblocks < A.np ? nblocks : A.np) : A.np;
     const size_t lds = gscratch ? 0 : nmers_lds_bytes(match_cap);
-    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64), lds, st, A, N, pass, cap, capx, match_cap, gscratch);
+    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64 * kQueryWaves), lds, st, A, N, pass, cap, capx, match_cap, gscratch);
     return hipGetLastError();
 }
 

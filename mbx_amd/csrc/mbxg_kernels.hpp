@@ -18,6 +18,13 @@ struct Grid {
     double inv_w[3];
 };
 
+// Per cloud point p (a monomer, or an image of one): its first atom in the working coordinates
+// (not imaged) and its tag and kind, so the n-mer kernel reads a match's data with one load.
+struct PtInfo {
+    double x, y, z;
+    int tag, kind;
+};
+
 // Everything an n-mer kernel needs to find a monomer's atoms.  Coordinates are the wrapped
 // working copy (3 doubles per real atom, internal monomer order).  The box stays in device
 // memory (read with scalar loads where needed) so it does not pin 48 SGPRs in the kernels.
@@ -44,8 +51,9 @@ constexpr int kMaxCells = kMaxCellsPerDim * kMaxCellsPerDim * kMaxCellsPerDim;
 __global__ void k_fix_monomers(const double* in, double* out, const int* first_atom, const int* nat, int nmon, Box B);
 __global__ void k_grid_gas(const double* xyz, const int* first_atom, const int* members, const int* pt_off,
                            const int* cell_bound, double cut, Grid* grids);
-__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, int np, Box B, const Grid* grids,
-                        const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell);
+__global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, const int* tag, const int* kind,
+                        int np, Box B, const Grid* grids, const int* pconf, const int* cell_base, double* cx,
+                        double* cy, double* cz, int* cell, PtInfo* pinfo);
 __global__ void k_bin_count(const int* cell, int ns, int* cell_count);
 __global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
 __global__ void k_bin_fill(const int* cell, int ns, const int* cell_start, int* cursor, const double* cx,
@@ -65,7 +73,12 @@ enum Family : int { FAM_MBPOL_2B = 0, FAM_CH4H2O_2B = 1, FAM_CH4CH4_2B = 2, FAM_
 constexpr int kFamilies = 5;
 constexpr int family_order(int f) { return f < FAM_MBPOL_3B ? 2 : 3; }
 
+// Wavefronts per query in the n-mer kernel; wpre / wpre_x hold each wavefront's prefix within
+// its query (entry q * kQueryWaves + w), counts / offsets one entry per query.
+constexpr int kQueryWaves = 4;
+
 struct NmersArgs {
+    const PtInfo* pinfo;
     const double* cx;
     const double* cy;
     const double* cz;
@@ -84,6 +97,8 @@ struct NmersArgs {
     int4* counts_x;
     const int4* offsets;
     const int4* offsets_x;
+    int4* wpre;
+    int4* wpre_x;
     int* list2;
     int* list3;
     int* evals[kFamilies];
