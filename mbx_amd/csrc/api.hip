@@ -138,7 +138,6 @@ struct mbxg_ctx {
     DBuf<double> d_coefn[kFamilies];  // MB-nrg coefficient tables (families 1, 2, 4)
     DBuf<double> d_fam_tot;           // per family: energy + 6 virial components
     DBuf<Box> d_box;
-    DBuf<int> d_overflow;
     // configurations (mbxg_set_batch): monomers [mon_off[c], mon_off[c+1]) form configuration c;
     // its cloud points are [pt_off[c], pt_off[c+1])
     int nconf = 1;
@@ -192,7 +191,7 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         return fail(MBXG_EHIP, "mbxg_create: stream/event creation failed: %s", hipGetErrorString(e));
     }
     if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(16) ||
-        c->d_overflow.reserve(1) || c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
+        c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
         c->d_coefn[FAM_CH4H2O_2B].reserve(kNCoef_ch4h2o_2b) || c->d_coefn[FAM_CH4CH4_2B].reserve(kNCoef_ch4ch4_2b) ||
         c->d_coefn[FAM_CH4H2O2_3B].reserve(kNCoef_ch4h2o2_3b)) {
         delete c;
@@ -266,7 +265,6 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_coef2.release();
     c->d_coef3.release();
     c->d_box.release();
-    c->d_overflow.release();
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -703,7 +701,6 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     A.offsets_x = L.offsets_x.p;
     A.wpre = L.wpre.p;
     A.wpre_x = L.wpre_x.p;
-    A.overflow = c->d_overflow.p;
     A.fam_mask = c->fam_mask;
 
     auto scratch = [&]() -> double* {
@@ -715,18 +712,17 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     for (int attempt = 0;; ++attempt) {
         double* gs = scratch();
         if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
-        launch_zero_int(c->d_overflow.p, 1, st);
         HIPCHK(launch_nmers(A, N, 0, none, none, L.match_cap, gs, kScratchBlocks, st));
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p);
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts_x.p, np, L.offsets_x.p);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p, 0);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts_x.p, np, L.offsets_x.p, 1);
         HIPCHK(hipGetLastError());
         if (!sync) break;
         int4 tot, totx;
         int ovf = 0;
         HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        ovf = totx.w;  // the largest match count of any query
         const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
         if (ovf > L.match_cap) {
             if (attempt > 0 || want > 2 * ns + 2)
@@ -875,8 +871,8 @@ int finish(mbxg_ctx* c) {
     int ovf = 0;
     HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&ovf, c->d_overflow.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    ovf = totx.w;  // the largest match count of any query
     if (tot.x > L.cap.x || tot.y > L.cap.y || tot.z > L.cap.z || tot.w > L.cap.w || totx.x > L.capx.x ||
         totx.y > L.capx.y || totx.z > L.capx.z || ovf > L.match_cap) {
         c->need_sync = true;

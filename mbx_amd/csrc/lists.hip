@@ -311,26 +311,6 @@ __device__ __forceinline__ Lds carve(double* d, int cap) {
     return L;
 }
 
-// Bitonic sort of a[0..n) (n a power of two) by the block.
-__device__ void sort_ints(int* a, int n) {
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const int ai = a[i], aj = a[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((ai > aj) == up) {
-                        a[i] = aj;
-                        a[ixj] = ai;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 }  // namespace
 
 // One query, by the block's kQueryWaves wavefronts: gather, sort, per-match data, dimers,
@@ -349,7 +329,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
                             int4* __restrict__ counts_x, const int4* __restrict__ offsets,
                             const int4* __restrict__ offsets_x, int* __restrict__ list2, int* __restrict__ list3,
                             int* const* evals, int fam_mask, int4 cap, int4 capx, int match_cap,
-                            int* __restrict__ overflow, const Lds& L, int* sh_count, int4* sh_w,
+                            const Lds& L, int* sh_count, int4* sh_w,
                             int4* __restrict__ wpre, int4* __restrict__ wpre_x) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int mi = N.members[q];
@@ -405,35 +385,47 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
     };
     if (tid == 0) *sh_count = 0;
     __syncthreads();
-    // each wavefront appends its matches at a block-shared cursor (the sort fixes the order)
-    const int stride = 64 * kQueryWaves;
-    for (int c0 = wv * 128; c0 < ncand; c0 += 2 * stride) {
-        const int ca = c0 + lane, cb = c0 + 64 + lane;
-        const int pa = ca < ncand ? position(ca) : -1, pb = cb < ncand ? position(cb) : -1;
-        double ax = 0.0, ay = 0.0, az = 0.0, bxv = 0.0, byv = 0.0, bzv = 0.0;
-        int sa = -1, sb = -1;
-        if (pa >= 0) {
-            ax = bx[pa], ay = by[pa], az = bz[pa];
-            sa = cell_slot[pa];
+    // each wavefront takes a contiguous share of the candidates, issues the loads of up to four
+    // chunks of 64 before testing any, and appends its matches at a block-shared cursor (the
+    // sort fixes the order)
+    const int per_w = ((ncand + 63) / 64 + kQueryWaves - 1) / kQueryWaves;
+    const int cbeg = wv * per_w * 64;
+    for (int k0 = 0; k0 < per_w; k0 += 4) {
+        double px[4], py[4], pz[4];
+        int ps[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = cbeg + (k0 + k) * 64 + lane;
+            const int pos = (k0 + k < per_w && c < ncand) ? position(c) : -1;
+            ps[k] = -1;
+            px[k] = py[k] = pz[k] = 0.0;
+            if (pos >= 0) {
+                px[k] = bx[pos];
+                py[k] = by[pos];
+                pz[k] = bz[pos];
+                ps[k] = cell_slot[pos];
+            }
         }
-        if (pb >= 0) {
-            bxv = bx[pb], byv = by[pb], bzv = bz[pb];
-            sb = cell_slot[pb];
+        unsigned long long m[4];
+        int tot = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (ps[k] >= 0 && !(dist2(qx, qy, qz, px[k], py[k], pz[k]) < cmsq)) ps[k] = -1;
+            m[k] = __ballot(ps[k] >= 0);
+            tot += __popcll(m[k]);
         }
-        const int s1 = pa >= 0 && dist2(qx, qy, qz, ax, ay, az) < cmsq ? sa : -1;
-        const int s2 = pb >= 0 && dist2(qx, qy, qz, bxv, byv, bzv) < cmsq ? sb : -1;
-        const unsigned long long m1 = __ballot(s1 >= 0), m2 = __ballot(s2 >= 0);
         int at0 = 0;
-        if (lane == 0) at0 = atomicAdd(sh_count, __popcll(m1) + __popcll(m2));
+        if (lane == 0 && tot > 0) at0 = atomicAdd(sh_count, tot);
         at0 = __shfl(at0, 0, 64);
-        const int p1 = at0 + __popcll(m1 & ((1ull << lane) - 1ull));
-        if (s1 >= 0 && p1 < match_cap) L.match[p1] = s1;
-        const int p2 = at0 + __popcll(m1) + __popcll(m2 & ((1ull << lane) - 1ull));
-        if (s2 >= 0 && p2 < match_cap) L.match[p2] = s2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int pk = at0 + __popcll(m[k] & ((1ull << lane) - 1ull));
+            if (ps[k] >= 0 && pk < match_cap) L.match[pk] = ps[k];
+            at0 += __popcll(m[k]);
+        }
     }
     __syncthreads();
     const int nm = *sh_count;
-    if (tid == 0) atomicMax(overflow, nm);
     if (nm > match_cap) {
         // The matches did not fit.  Which of them reached storage depends on the wavefronts'
         // timing, so the count and emit passes could keep different subsets and disagree on this
@@ -441,16 +433,22 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
         // recomputes with more storage.
         if (pass == 0 && tid == 0) {
             counts[q] = make_int4(0, 0, 0, 0);
-            counts_x[q] = make_int4(0, 0, 0, 0);
+            counts_x[q] = make_int4(0, 0, 0, nm);
         }
         return;
     }
-    // ---- sort the matches by slot index (pad to a power of two; match_cap is one)
-    int pw = 1;
-    while (pw < nm) pw <<= 1;
-    for (int a = nm + tid; a < pw; a += blockDim.x) L.match[a] = 0x7fffffff;
+    // ---- sort the matches by slot index (distinct keys): each key's rank is the number of
+    // smaller keys (every thread reads the same LDS word at a time); L.mon is free until the
+    // per-match data are written
+    for (int a = tid; a < nm; a += blockDim.x) {
+        const int key = L.match[a];
+        int rank = 0;
+        for (int b = 0; b < nm; ++b) rank += L.match[b] < key;
+        L.mon[rank] = key;
+    }
     __syncthreads();
-    sort_ints(L.match, pw);
+    for (int a = tid; a < nm; a += blockDim.x) L.match[a] = L.mon[a];
+    __syncthreads();
 
     // ---- per-match data (one payload load per match: PtInfo of its cloud point)
     const double* oi = N.xyz + 3 * (size_t)N.first_atom[mi];
@@ -665,7 +663,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
                 ax = make_int4(ax.x + cx.x, ax.y + cx.y, ax.z + cx.z, ax.w + cx.w);
             }
             counts[q] = a;
-            counts_x[q] = ax;
+            counts_x[q] = make_int4(ax.x, ax.y, ax.z, nm);  // .w: the query's match count
         }
     }
 }
@@ -673,11 +671,11 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 // pass 0: counts[q] = {listed 2B, evaluated MB-pol 2B, listed 3B, evaluated MB-pol 3B},
 // counts_x[q] = {evaluated CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, 0};
 // pass 1: writes at offsets[q] / offsets_x[q], bounded by cap / capx (same orders).
-// overflow[0] receives the largest match count of any query (the host compares it with
+// counts_x[q].w receives query q's match count (the scan maximises it: the host compares it with
 // match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
 // the dynamic LDS, otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.
-__global__ void __launch_bounds__(64 * kQueryWaves) k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx,
-                                                            int match_cap, double* __restrict__ gscratch) {
+__global__ void __launch_bounds__(64 * kQueryWaves)
+    k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap, double* __restrict__ gscratch) {
     extern __shared__ double smem[];
     __shared__ int sh_count;
     __shared__ int4 sh_w[2 * kQueryWaves];
@@ -688,7 +686,7 @@ __global__ void __launch_bounds__(64 * kQueryWaves) k_nmers(NmersArgs A, NmerCtx
         const Grid G = A.grids[conf];
         nmers_query(q, A.pinfo, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
                     A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.counts_x, A.offsets, A.offsets_x,
-                    A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, A.overflow, L, &sh_count, sh_w,
+                    A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, L, &sh_count, sh_w,
                     A.wpre, A.wpre_x);
         __syncthreads();
     }
@@ -696,38 +694,29 @@ __global__ void __launch_bounds__(64 * kQueryWaves) k_nmers(NmersArgs A, NmerCtx
 
 // Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).
 __global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ counts, int n,
-                                                      int4* __restrict__ offsets) {
+                                                      int4* __restrict__ offsets, int max_w) {
     __shared__ int4 sh[1024];
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int beg = t * per, end = min(n, beg + per);
+    // .w is summed, or with max_w maximised (its "offsets" are then running maxima)
+    auto combine = [max_w](int4 a, int4 c) {
+        return make_int4(a.x + c.x, a.y + c.y, a.z + c.z, max_w ? max(a.w, c.w) : a.w + c.w);
+    };
     int4 acc = make_int4(0, 0, 0, 0);
-    for (int i = beg; i < end; ++i) {
-        const int4 c = counts[i];
-        acc.x += c.x;
-        acc.y += c.y;
-        acc.z += c.z;
-        acc.w += c.w;
-    }
+    for (int i = beg; i < end; ++i) acc = combine(acc, counts[i]);
     sh[t] = acc;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
         const int4 v = t >= off ? sh[t - off] : make_int4(0, 0, 0, 0);
         __syncthreads();
-        sh[t].x += v.x;
-        sh[t].y += v.y;
-        sh[t].z += v.z;
-        sh[t].w += v.w;
+        sh[t] = combine(sh[t], v);
         __syncthreads();
     }
-    int4 o = make_int4(sh[t].x - acc.x, sh[t].y - acc.y, sh[t].z - acc.z, sh[t].w - acc.w);
+    int4 o = t > 0 ? sh[t - 1] : make_int4(0, 0, 0, 0);
     for (int i = beg; i < end; ++i) {
         offsets[i] = o;
-        const int4 c = counts[i];
-        o.x += c.x;
-        o.y += c.y;
-        o.z += c.z;
-        o.w += c.w;
+        o = combine(o, counts[i]);
     }
     if (t == 1023) offsets[n] = sh[t];
 }

@@ -58,7 +58,7 @@ __global__ void k_bin_count(const int* cell, int ns, int* cell_count);
 __global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
 __global__ void k_bin_fill(const int* cell, int ns, const int* cell_start, int* cursor, const double* cx,
                            const double* cy, const double* cz, int* cell_slot, double* bx, double* by, double* bz);
-__global__ void k_scan_counts(const int4* counts, int n, int4* offsets);
+__global__ void k_scan_counts(const int4* counts, int n, int4* offsets, int max_w);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_int(int* p, size_t n);
 
@@ -103,7 +103,6 @@ struct NmersArgs {
     int* list3;
     int* evals[kFamilies];
     int fam_mask;   // bit f: family f is evaluated (listed n-mers of a cleared family are not)
-    int* overflow;  // largest match count seen (zeroed before pass 0)
 };
 // match_cap (a power of two) bounds the matches one query may hold.  With gscratch == nullptr
 // they live in LDS (nmers_lds_bytes(match_cap) per block, one block per query); otherwise
