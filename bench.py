@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 METRIC = "2B+3B dimers+trimers/sec (E+F, FP64) at 2048 H2O; batched vs plain speedup"
 FP64_VECTOR_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_poly3b_water2048.txt")
+LIST_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lists_water2048.txt")
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
 
 def host_cores():
@@ -39,10 +41,10 @@ def host_cores():
     return max(1, min(n, 16))
 
 
-def read_traffic():
+def read_traffic(path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in open(path):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -353,6 +355,12 @@ def main():
                      "flops_per_unit": f3, "units_per_launch": int(counts[3]), "launch_ms": t_poly * 1e3},
         "cpu_baseline": None,
     }
+    # list build (fix, cloud, binning, both n-mer passes): PMC bytes per step over its HIP-event time
+    lb = read_traffic(LIST_TRAFFIC_FILE, "list_build_bytes_per_step:")
+    lt = (stage["fix"] + stage["lists_count"] + stage["lists_emit"]) / args.steps * 1e-3
+    out["config"]["list_build"] = {"ms": lt * 1e3, "hbm_bytes": lb, "GB_per_s": lb / lt / 1e9 if lb else None,
+                                   "frac_of_hbm_peak": lb / lt / HBM_PEAK if lb else None,
+                                   "bound": "latency (one 4-wavefront block per query; gathers, LDS sort)"}
     if not args.no_batch:
         out["config"]["batched"] = batched_leg(args.steps, args.warmup)
         out["config"]["pimd_batched"] = pimd_leg(args.steps, args.warmup)
