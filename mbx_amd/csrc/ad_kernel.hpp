@@ -1,10 +1,10 @@
 // Differentiation kernels around the generated polynomial (gen/ad_<name>.inc).
 //
-// k_ad_poly evaluates the polynomial in dual numbers: with variables x and their tangents xd
-// (the derivative of x along a direction of the atom coordinates) it returns P, dP = grad P . xd,
-// grad P and its tangent H xd, one lane per n-mer.  The schedule's parts run as separate
-// wavefronts (grid.y = part); each writes its own slice of the outputs and the consumer sums
-// the slices in part order.
+// k_ad_tan evaluates the tangent of the polynomial's gradient: with variables x and their
+// tangents xd (the derivative of x along a direction of the atom coordinates) it returns H xd,
+// one lane per n-mer (P and grad P come from the plain E+G kernel, dP = grad P . xd).  The
+// schedule's parts run as separate wavefronts (grid.y = part); each writes its own slice and the
+// consumer sums the slices in part order.
 //
 // k_ad_basis accumulates, for every coefficient k, sum_n [alpha_n M_k(x_n) + sw_n dM_k(x_n)]
 // over the n-mers of a block (M_k = the basis function multiplying coefficient k, dM_k its
@@ -21,7 +21,7 @@
 
 namespace mbxg {
 
-// Pieces of the dual-number polynomial (one wavefront each per 64 n-mers).
+// Pieces of the gradient-tangent polynomial (one wavefront each per 64 n-mers).
 constexpr int kAdParts = 16;
 
 // Column sums of the block's LDS tile red[64][65]: lane j adds row j (its coefficient's values
@@ -45,35 +45,28 @@ __device__ __forceinline__ void ad_opaque(double* x, double* xd) {
     for (int i = 0; i < NV; ++i) asm volatile("" : "+v"(x[i]), "+v"(xd[i]));
 }
 
-typedef void (*ad_dual_fn)(int, mbxg_cptr, const double*, const double*, double*, double*, double&, double&);
+typedef void (*ad_tan_fn)(int, mbxg_cptr, const double*, const double*, double*);
 typedef void (*ad_basis_fn)(int, double*, double*, double, double, double*, double*, int);
 
-template <int NV, int PARTS, ad_dual_fn FN>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-    k_ad_poly(const double* __restrict__ xs, const double* __restrict__ xds, int n, int ld, uint64_t coef_addr,
-              double* __restrict__ gs, double* __restrict__ gds, double* __restrict__ ps, double* __restrict__ pds) {
+template <int NV, int PARTS, ad_tan_fn FN>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    k_ad_tan(const double* __restrict__ xs, const double* __restrict__ xds, int n, int ld, uint64_t coef_addr,
+             double* __restrict__ gds) {
     const int t = blockIdx.x * 64 + threadIdx.x;
     const int part = blockIdx.y;
     if (t >= n) return;
     const mbxg_cptr C = (mbxg_cptr)coef_addr;
-    double x[NV], xd[NV], g[NV], gd[NV];
+    double x[NV], xd[NV], gd[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         x[j] = xs[(size_t)j * ld + t];
         xd[j] = xds[(size_t)j * ld + t];
-        g[j] = 0.0;
         gd[j] = 0.0;
     }
-    double E = 0.0, dE = 0.0;
-    FN(part, C, x, xd, g, gd, E, dE);
+    FN(part, C, x, xd, gd);
     const size_t base = (size_t)part * NV * ld;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-        gs[base + (size_t)j * ld + t] = g[j];
-        gds[base + (size_t)j * ld + t] = gd[j];
-    }
-    ps[(size_t)part * ld + t] = E;
-    pds[(size_t)part * ld + t] = dE;
+    for (int j = 0; j < NV; ++j) gds[base + (size_t)j * ld + t] = gd[j];
 }
 
 // alpha_n = g_energy_n * sw_n + dsw_n (the weight of M_k), sw_n (the weight of dM_k).

@@ -17,8 +17,8 @@
 //
 //   stage 1  k_ad_vars     sites (the M-sites of x2b-v9x.cpp:117-135 for 2B), variables and
 //                          switch, with tangents
-//   stage 2  k_ad_poly     polynomial in dual numbers (gen/ad_<name>_dual.inc); the forward
-//                          pass uses the plain E+G polynomial kernel instead
+//   stage 2  P and grad P from the plain E+G polynomial kernel; in the backward pass also the
+//            gradient tangent H vd by k_ad_tan (gen/ad_<name>_tan.inc), dP = grad P . vd
 //   stage 3  k_ad_basis    coefficient gradient sum_n [alpha_n M_k + sw_n dM_k] (gen/ad_<name>_basis.inc)
 //   stage 4  k_ad_chain    reverse pass to coordinates and theta; in the forward pass, E and dE/dx
 //
@@ -43,10 +43,8 @@
 
 namespace mbxg {
 
-hipError_t launch_ad_dual_mbpol_2b(const double*, const double*, int, int, const double*, double*, double*, double*,
-                                   double*, hipStream_t);
-hipError_t launch_ad_dual_mbpol_3b(const double*, const double*, int, int, const double*, double*, double*, double*,
-                                   double*, hipStream_t);
+hipError_t launch_ad_tan_mbpol_2b(const double*, const double*, int, int, const double*, double*, hipStream_t);
+hipError_t launch_ad_tan_mbpol_3b(const double*, const double*, int, int, const double*, double*, hipStream_t);
 hipError_t launch_ad_basis_mbpol_2b(const double*, const double*, const double*, const double*, const double*, int,
                                     int, double*, hipStream_t);
 hipError_t launch_ad_basis_mbpol_3b(const double*, const double*, const double*, const double*, const double*, int,
@@ -297,14 +295,14 @@ __global__ void __launch_bounds__(64) k_ad_vars(const double* __restrict__ xyz, 
 
 // Stage 4.  Forward (BWD = false): energy_out = sw * P and dx_out = dE/dx from the plain
 // polynomial's value and gradient.  Backward: dx_out = dphi/dx and per-block partial sums of
-// dphi/dtheta (theta_partial[block * 16 + i]) from the dual polynomial's outputs.  Polynomial
-// outputs come in `parts` slices, added here.
+// dphi/dtheta (theta_partial[block * 16 + i]) from P, grad P, the gradient tangent gds and the
+// variable tangents xds (dP = grad P . xds).  P and grad P come in `parts` slices, added here.
 template <class F, bool BWD>
 __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz, const double* __restrict__ u,
                                                  const double* __restrict__ theta,
                                                  const double* __restrict__ g_energy, int n, int ld, int parts,
                                                  const double* __restrict__ gs, const double* __restrict__ gds,
-                                                 const double* __restrict__ ps, const double* __restrict__ pds,
+                                                 const double* __restrict__ ps, const double* __restrict__ xds,
                                                  double* __restrict__ energy_out, double* __restrict__ dx_out,
                                                  double* __restrict__ theta_partial) {
     constexpr int NV = F::kNV;
@@ -320,18 +318,14 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
         double s[3 * F::kNS], sd[3 * F::kNS], oh[2][6], uh[2][6];
         ad_sites<F, BWD>(xyz, u, t, th, s, sd, oh, uh);
         double P = 0.0, Pd = 0.0;
-        for (int q = 0; q < parts; ++q) {
-            P += ps[(size_t)q * ld + t];
-            if (BWD) Pd += pds[(size_t)q * ld + t];
-        }
+        for (int q = 0; q < parts; ++q) P += ps[(size_t)q * ld + t];
         double d[3][3], dd[3][3], r[3], rho[3], sv[3], f1[3], f2[3], sdot[3], sw, swd;
         ad_pairs<F, BWD>(s, sd, d, dd, r, rho, sv, f1, f2);
         ad_total_switch<F>(sv, f1, r, rho, sw, swd, sdot);
         const double gE = BWD ? g_energy[t] : 1.0;
-        // adjoints of P, dP (= sw), sw and swd
+        // adjoints of P and dP (= sw); those of sw and swd follow the variable loop, which also
+        // accumulates dP = grad P . vd
         const double alpha = BWD ? gE * sw + swd : sw;
-        const double a_sw = BWD ? gE * P + Pd : P;
-        const double b_sw = P;
         double sb[3 * F::kNS], sdb[12];
 #pragma unroll
         for (int i = 0; i < 3 * F::kNS; ++i) sb[i] = 0.0;
@@ -341,9 +335,10 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
             constexpr int j = decltype(J)::value;
             constexpr AdVar V = F::kVars[j];
             double g = 0.0, gd = 0.0;
-            for (int q = 0; q < parts; ++q) {
-                g += gs[((size_t)q * NV + j) * ld + t];
-                if (BWD) gd += gds[((size_t)q * NV + j) * ld + t];
+            for (int q = 0; q < parts; ++q) g += gs[((size_t)q * NV + j) * ld + t];
+            if (BWD) {
+                gd = gds[(size_t)j * ld + t];
+                Pd += g * xds[(size_t)j * ld + t];
             }
             const double vb = BWD ? alpha * g + sw * gd : sw * g;  // dphi/dv
             const double vdb = sw * g;                              // dphi/dvd
@@ -390,6 +385,8 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
                 if (V.dp >= 0) thb[V.dp] += vb * fd + vdb * rdot * f1d;
             }
         });
+        const double a_sw = BWD ? gE * P + Pd : P;
+        const double b_sw = P;
         // switches: pair q's value enters sw (weight a_q) and swd (weight b_q)
 #pragma unroll
         for (int q = 0; q < F::kPairs; ++q) {
@@ -498,29 +495,14 @@ __global__ void __launch_bounds__(64) k_ad_colsum(const double* __restrict__ par
     if (threadIdx.x == 0) out[k] = s;
 }
 
-// Sum the parts' slices of the dual polynomial's outputs into slice 0: rows 0..nv-1 of gs and
-// gds, then ps and pds (entry (row, t) of part q at (q * rows + row) * ld + t).  One thread per
-// entry, parts added in order.
-__global__ void __launch_bounds__(256) k_ad_sum_parts(double* __restrict__ gs, double* __restrict__ gds,
-                                                      double* __restrict__ ps, double* __restrict__ pds, int nv,
-                                                      int ld, int parts) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t ng = (size_t)nv * ld;
-    if (i >= 2 * ng + 2 * (size_t)ld) return;
-    double* base;
-    size_t e, rows;
-    if (i < ng) {
-        base = gs, e = i, rows = ng;
-    } else if (i < 2 * ng) {
-        base = gds, e = i - ng, rows = ng;
-    } else if (i < 2 * ng + ld) {
-        base = ps, e = i - 2 * ng, rows = ld;
-    } else {
-        base = pds, e = i - 2 * ng - ld, rows = ld;
-    }
-    double s = base[e];
-    for (int q = 1; q < parts; ++q) s += base[q * rows + e];
-    base[e] = s;
+// Sum the parts' slices of the gradient tangent into slice 0 (entry e of part q at
+// q * rows + e, rows = nv * ld).  One thread per entry, parts added in order.
+__global__ void __launch_bounds__(256) k_ad_sum_parts(double* __restrict__ gds, size_t rows, int parts) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows) return;
+    double s = gds[e];
+    for (int q = 1; q < parts; ++q) s += gds[q * rows + e];
+    gds[e] = s;
 }
 
 __global__ void k_ad_set_int(int* p, int v) { *p = v; }
@@ -532,7 +514,7 @@ using namespace mbxg;
 namespace {
 
 struct AdLayout {
-    size_t xs, xds, sw, swd, gs, gds, ps, pds, ndev, basis, theta, total;
+    size_t xs, xds, sw, swd, gs, gds, ps, ndev, basis, theta, total;
 };
 
 constexpr size_t kAlign = 256;
@@ -554,7 +536,8 @@ bool ad_shape(int family, int& nv, int& nat, int& ncoef) {
     return false;
 }
 
-// Workspace: stage buffers for ld = n rounded up to 64 lanes, kAdParts polynomial parts.
+// Workspace: stage buffers for ld = n rounded up to 64 lanes; kAdParts slices of the gradient
+// tangent, one slice of P and grad P (the plain kernel sums its parts itself).
 AdLayout ad_layout(int nv, int ncoef, int n) {
     const size_t ld = (size_t)((n + 63) / 64) * 64, nblk = ld / 64, parts = kAdParts;
     AdLayout L;
@@ -568,10 +551,9 @@ AdLayout ad_layout(int nv, int ncoef, int n) {
     L.xds = take(sizeof(double) * nv * ld);
     L.sw = take(sizeof(double) * ld);
     L.swd = take(sizeof(double) * ld);
-    L.gs = take(sizeof(double) * parts * nv * ld);
+    L.gs = take(sizeof(double) * nv * ld);
     L.gds = take(sizeof(double) * parts * nv * ld);
-    L.ps = take(sizeof(double) * parts * ld);
-    L.pds = take(sizeof(double) * parts * ld);
+    L.ps = take(sizeof(double) * ld);
     L.ndev = take(sizeof(int));
     L.basis = take(sizeof(double) * nblk * ncoef);
     L.theta = take(sizeof(double) * nblk * 16);
@@ -700,31 +682,31 @@ extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const doub
     double* gs = (double*)(w + L.gs);
     double* gds = (double*)(w + L.gds);
     double* ps = (double*)(w + L.ps);
-    double* pds = (double*)(w + L.pds);
+    int* ndev = (int*)(w + L.ndev);
     double* basis = (double*)(w + L.basis);
     double* thp = (double*)(w + L.theta);
     const int ld = (n + 63) / 64 * 64, nb = ld / 64;
     // a null g_grad is the zero direction (an energy-only loss)
+    const size_t rows = (size_t)nv * ld;
+    hipLaunchKernelGGL(k_ad_set_int, dim3(1), dim3(1), 0, st, ndev, n);
     if (family == MBXG_AD_MBPOL_2B) {
         hipLaunchKernelGGL((k_ad_vars<Ad2b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, n, ld, xs, xds, sw,
                            swd);
-        ADCHK(launch_ad_dual_mbpol_2b(xs, xds, n, ld, coef, gs, gds, ps, pds, st));
+        ADCHK(launch_poly_mbpol2b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
+        ADCHK(launch_ad_tan_mbpol_2b(xs, xds, n, ld, coef, gds, st));
         ADCHK(launch_ad_basis_mbpol_2b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
-        const size_t tot = 2 * (size_t)(Ad2b::kNV + 1) * ld;
-        hipLaunchKernelGGL(k_ad_sum_parts, dim3((tot + 255) / 256), dim3(256), 0, st, gs, gds, ps, pds, Ad2b::kNV,
-                           ld, Ad2b::kParts);
+        hipLaunchKernelGGL(k_ad_sum_parts, dim3((rows + 255) / 256), dim3(256), 0, st, gds, rows, Ad2b::kParts);
         hipLaunchKernelGGL((k_ad_chain<Ad2b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld, 1,
-                           gs, gds, ps, pds, nullptr, d_xyz, thp);
+                           gs, gds, ps, xds, nullptr, d_xyz, thp);
     } else {
         hipLaunchKernelGGL((k_ad_vars<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, n, ld, xs, xds, sw,
                            swd);
-        ADCHK(launch_ad_dual_mbpol_3b(xs, xds, n, ld, coef, gs, gds, ps, pds, st));
+        ADCHK(launch_poly_mbpol3b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
+        ADCHK(launch_ad_tan_mbpol_3b(xs, xds, n, ld, coef, gds, st));
         ADCHK(launch_ad_basis_mbpol_3b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
-        const size_t tot = 2 * (size_t)(Ad3b::kNV + 1) * ld;
-        hipLaunchKernelGGL(k_ad_sum_parts, dim3((tot + 255) / 256), dim3(256), 0, st, gs, gds, ps, pds, Ad3b::kNV,
-                           ld, Ad3b::kParts);
+        hipLaunchKernelGGL(k_ad_sum_parts, dim3((rows + 255) / 256), dim3(256), 0, st, gds, rows, Ad3b::kParts);
         hipLaunchKernelGGL((k_ad_chain<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, g_energy, n, ld, 1,
-                           gs, gds, ps, pds, nullptr, d_xyz, thp);
+                           gs, gds, ps, xds, nullptr, d_xyz, thp);
     }
     hipLaunchKernelGGL(k_ad_colsum, dim3(ncoef), dim3(64), 0, st, basis, nb, ncoef, ncoef, d_coef);
     hipLaunchKernelGGL(k_ad_colsum, dim3(kNTheta), dim3(64), 0, st, thp, nb, kNTheta, 16, d_theta);

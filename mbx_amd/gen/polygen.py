@@ -516,14 +516,20 @@ def dual_stmt(t):
     raise ValueError("no tangent rule for: " + t)
 
 
-def emit_dual(s, fn, part, assign, region_coefs=12, region_ops=96):
+def emit_dual(s, fn, part, assign, region_coefs=12, region_ops=96, tangent_only=False):
     """Dual-number version of Schedule.emit for one part: value, tangent, gradient and gradient
-    tangent accumulate into E, dE, g, gd (passed in, not cleared)."""
+    tangent accumulate into E, dE, g, gd (passed in, not cleared).
+
+    With tangent_only, only the gradient tangent gd (the Hessian-vector product H xd) is kept:
+    the statements accumulating E, dE and g are dropped (the plain E+G kernel provides E and g,
+    and dE = g . xd), which frees a third of the accumulator registers; the trie values and their
+    tangents that gd needs stay."""
+    drop = lambda t: tangent_only and _lhs(t) in ("E", "dE") or (tangent_only and _lhs(t).startswith("g["))
     ops = []
     for t, q in s.ops(True, part, assign):
         if t == "double E = E0;":
             continue
-        ops += [(d, q if "{c}" in d else None) for d in dual_stmt(t)]
+        ops += [(d, q if "{c}" in d else None) for d in dual_stmt(t) if not drop(d)]
     regions, coefs, nar = [[]], [[]], 0
     for t, q in ops:
         cur = coefs[-1]
@@ -536,13 +542,16 @@ def emit_dual(s, fn, part, assign, region_coefs=12, region_ops=96):
             cur.append(q)
         regions[-1].append((t, q))
         nar += 1
-    out = ["__device__ __forceinline__ void %s(mbxg_cptr C, const double* x, const double* xd, double* g, "
-           "double* gd, double& E, double& dE) {" % fn]
+    if tangent_only:
+        out = ["__device__ __forceinline__ void %s(mbxg_cptr C, const double* x, const double* xd, double* gd) {" % fn]
+    else:
+        out = ["__device__ __forceinline__ void %s(mbxg_cptr C, const double* x, const double* xd, double* g, "
+               "double* gd, double& E, double& dE) {" % fn]
     w = out.append
     name = lambda r, q: "k%d_%d" % (r, coefs[r].index(q))
     for q in coefs[0]:
         w("    const double %s = C[%d];" % (name(0, q), q))
-    last = "E"
+    last = "x[0]" if tangent_only else "E"
     for r, reg in enumerate(regions):
         w("    MBXG_NODE_FENCE(C, %s);" % last)
         if r + 1 < len(regions):
@@ -631,14 +640,14 @@ def emit_basis(fn, monos, c0, c1):
     return "\n".join(out) + "\n"
 
 
-# Differentiated polynomials: dual-number parts (the schedule partitioned into AD_PARTS pieces,
-# one wavefront each; kAdParts in ad_kernel.hpp) and coefficient-basis slices.
+# Differentiated polynomials: gradient-tangent parts (the schedule partitioned into AD_PARTS
+# pieces, one wavefront each; kAdParts in ad_kernel.hpp) and coefficient-basis slices.
 AD_PARTS = 16
 AD_POLYS = {"mbpol_2b": 16, "mbpol_3b": 16}  # name -> basis slices
 
 
 def generate_ad(name, out_dir):
-    """gen/ad_<name>_dual.{inc,hip} and gen/ad_<name>_basis.{inc,hip}: two translation units
+    """gen/ad_<name>_tan.{inc,hip} and gen/ad_<name>_basis.{inc,hip}: two translation units
     (compiled in parallel), each with its kernel's launcher."""
     nvars, coef, monos = load_table(name)
     s = Schedule(nvars, [(k, m) for k, m in monos])
@@ -648,15 +657,15 @@ def generate_ad(name, out_dir):
     head = lambda what: ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
                          "// %s" % what, "#pragma once", "#include \"poly_support.hpp\"", "#include \"ad_kernel.hpp\"",
                          "", "namespace mbxg {", ""]
-    # dual-number polynomial, NPARTS parts
-    body = [emit_dual(s, "%s_dual_p%d" % (fn, k), k, assign) for k in range(AD_PARTS)]
-    d = ["__device__ __forceinline__ void %s_dual(int part, mbxg_cptr C, const double* x, const double* xd, "
-         "double* g, double* gd, double& E, double& dE) {" % fn]
-    d += ["    if (part == %d) %s_dual_p%d(C, x, xd, g, gd, E, dE);" % (k, fn, k) for k in range(AD_PARTS)]
+    # gradient tangent (Hessian-vector product), AD_PARTS parts
+    body = [emit_dual(s, "%s_tan_p%d" % (fn, k), k, assign, tangent_only=True) for k in range(AD_PARTS)]
+    d = ["__device__ __forceinline__ void %s_tan(int part, mbxg_cptr C, const double* x, const double* xd, "
+         "double* gd) {" % fn]
+    d += ["    if (part == %d) %s_tan_p%d(C, x, xd, gd);" % (k, fn, k) for k in range(AD_PARTS)]
     d += ["}", "", "}  // namespace mbxg", ""]
-    write_if_changed(os.path.join(out_dir, "ad_%s_dual.inc" % name),
-                     "\n".join(head("Dual-number polynomial in %d parts (value, tangent, gradient, gradient "
-                                    "tangent)." % AD_PARTS)) + "\n".join(body) + "\n".join(d))
+    write_if_changed(os.path.join(out_dir, "ad_%s_tan.inc" % name),
+                     "\n".join(head("Gradient tangent of the polynomial (H xd) in %d parts." % AD_PARTS))
+                     + "\n".join(body) + "\n".join(d))
     # coefficient basis, in slices of near-equal cost
     body = [emit_basis("%s_basis_s%d" % (fn, i), monos, c0, c1) for i, (c0, c1) in enumerate(sl)]
     d = ["__device__ __forceinline__ void %s_basis(int slice, double* x, double* xd, double alpha, "
@@ -671,14 +680,14 @@ def generate_ad(name, out_dir):
         "// GENERATED by mbx_amd/gen/polygen.py -- do not edit.",
         "// %s of the %s polynomial (%d variables, %d coefficients)." % (what, name, nvars, len(coef)),
         "#include \"gen/%s\"" % inc, "", "namespace mbxg {", "", launcher, "", "}  // namespace mbxg", ""])
-    write_if_changed(os.path.join(out_dir, "ad_%s_dual.hip" % name), tu(
-        "Dual-number evaluation", "ad_%s_dual.inc" % name,
-        "hipError_t launch_ad_dual_%s(const double* xs, const double* xds, int n, int ld, const double* coef, "
-        "double* gs, double* gds, double* ps, double* pds, hipStream_t st) {\n"
+    write_if_changed(os.path.join(out_dir, "ad_%s_tan.hip" % name), tu(
+        "Gradient tangent", "ad_%s_tan.inc" % name,
+        "hipError_t launch_ad_tan_%s(const double* xs, const double* xds, int n, int ld, const double* coef, "
+        "double* gds, hipStream_t st) {\n"
         "    static_assert(kAdParts == %d, \"part count of the generated code\");\n"
         "    if (n <= 0) return hipSuccess;\n"
-        "    hipLaunchKernelGGL((k_ad_poly<%d, %d, %s_dual>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, n, "
-        "ld,\n                       (uint64_t)(uintptr_t)coef, gs, gds, ps, pds);\n"
+        "    hipLaunchKernelGGL((k_ad_tan<%d, %d, %s_tan>), dim3((n + 63) / 64, %d), dim3(64), 0, st, xs, xds, n, ld,\n"
+        "                       (uint64_t)(uintptr_t)coef, gds);\n"
         "    return hipGetLastError();\n}" % (name, AD_PARTS, nvars, AD_PARTS, fn, AD_PARTS)))
     write_if_changed(os.path.join(out_dir, "ad_%s_basis.hip" % name), tu(
         "Coefficient basis", "ad_%s_basis.inc" % name,

@@ -5,8 +5,9 @@ evaluation of the polynomial from its monomial table (mbx_amd/tables/<name>.tbl)
 differentiation: with z = x + i h xd (h = 1e-30), Re P(z) = P(x) and Im P(z) / h = grad P . xd
 exactly up to rounding, and the same for grad P (whose tangent is the Hessian-vector product
 H xd) and for each coefficient's basis function B_k = sum of its monomials.  The dual-number
-parts must add up to P, grad P . xd, grad P and H xd; the basis slices must give B_k and
-grad B_k . xd for every coefficient of the slice.
+parts must add up to P, grad P . xd, grad P and H xd (the tangent-only parts, which the backward
+pass runs, to H xd alone); the basis slices must give B_k and grad B_k . xd for every coefficient
+of the slice.
 """
 import re
 import sys
@@ -56,6 +57,13 @@ def _translate(lines):
 def dual_fn(src, fname):
     code = "def f(C, x, xd, g, gd, E, dE):\n" + "\n".join("    " + l for l in _translate(_body(src, fname))) + \
         "\n    return E, dE\n"
+    ns = {"fma": _fma}
+    exec(compile(code, fname, "exec"), ns)
+    return ns["f"]
+
+
+def tan_fn(src, fname):
+    code = "def f(C, x, xd, gd):\n" + "\n".join("    " + l for l in _translate(_body(src, fname))) + "\n"
     ns = {"fma": _fma}
     exec(compile(code, fname, "exec"), ns)
     return ns["f"]
@@ -134,6 +142,28 @@ def test_dual_parts(name):
     assert E == pytest.approx(p_ref, rel=1e-12, abs=1e-14 * scale)
     assert dE == pytest.approx(dp_ref, rel=1e-11, abs=1e-13 * scale)
     np.testing.assert_allclose(g, g_ref, rtol=1e-11, atol=1e-13 * scale)
+    np.testing.assert_allclose(gd, gd_ref, rtol=1e-10, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("name", ["mbpol_2b", "mbpol_3b"])
+def test_tangent_only_parts(name):
+    """The kernel the backward pass runs: only H xd, with E, dE and g dropped (the generated
+    function must not reference them at all)."""
+    ref = Reference(name)
+    s = polygen.Schedule(ref.nv, list(zip(ref.k.tolist(), _monos(ref))))
+    assign = s.partition(polygen.AD_PARTS)
+    rng = np.random.default_rng(4)
+    x, xd = rng.uniform(0.1, 0.9, ref.nv), rng.normal(size=ref.nv)
+    a = rng.normal(size=len(ref.coef))
+    gd = np.zeros(ref.nv)
+    for p in range(polygen.AD_PARTS):
+        src = polygen.emit_dual(s, "f", p, assign, tangent_only=True)
+        body = "\n".join(_body(src, "f"))
+        assert not re.search(r"\b(E|dE)\b|\bg\[", body)
+        gdp = [0.0] * ref.nv
+        tan_fn(src, "f")(list(a), list(x), list(xd), gdp)
+        gd += np.array(gdp)
+    _, _, _, gd_ref, scale = ref.dual(x, xd, a)
     np.testing.assert_allclose(gd, gd_ref, rtol=1e-10, atol=1e-12 * scale)
 
 
