@@ -141,8 +141,9 @@ int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
 int mbxg_set_graph(mbxg_ctx* ctx, int on);
 
 /* Schedule parameters.  "split_2b" / "split_3b": wavefronts (1, 2, 4 or 8) that share each
- * block of 64 dimers / trimers in the polynomial stage, each evaluating 1/split of the
- * polynomial (defaults 4 and 2).  mbxg_autotune times every split of both orders on the
+ * block of 64 dimers / trimers in the MB-pol polynomial stage, each evaluating 1/split of the
+ * polynomial (defaults 4 and 2); "split_ch4h2o2_3b": the same for the CH4-H2O-H2O polynomial
+ * (default 4, not autotuned).  mbxg_autotune times every split of both orders on the
  * current inputs (the polynomial stage, `reps` timed repetitions after one warm-up), keeps the
  * fastest and recomputes once with it.  No counterpart in the reference (its kernels have no
  * launch shape). */

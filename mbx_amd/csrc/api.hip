@@ -161,7 +161,7 @@ struct mbxg_ctx {
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
-    int split2 = 4, split3 = 2;
+    int split2 = 4, split3 = 2, split3n = 4;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
     hipEvent_t ev[kNumEvents] = {};
@@ -828,7 +828,7 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                                       o.partials.p, o.ws.p, c->split3, sk, ev ? ev + 8 : nullptr));
             else
                 HIPCHK(launch_nrg_family(f, L.evals[f].p, nd, cap, N, c->d_coefn[f].p, do_grads, c->d_grad.p,
-                                         o.e_nmer.p, vir, o.partials.p, o.ws.p, sk, nullptr));
+                                         o.e_nmer.p, vir, o.partials.p, o.ws.p, c->split3n, sk, nullptr));
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
                                c->d_fam_tot.p + kPartials * f);
             if (batch) {
@@ -988,12 +988,14 @@ bool valid_split(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
 extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
     if (!c || !key) return fail(MBXG_EINVAL, "mbxg_set_tuning: bad arguments");
     const std::string k(key);
-    if ((k == "split_2b" || k == "split_3b") && !valid_split(value))
+    if ((k == "split_2b" || k == "split_3b" || k == "split_ch4h2o2_3b") && !valid_split(value))
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
     if (k == "split_2b")
         c->split2 = value;
     else if (k == "split_3b")
         c->split3 = value;
+    else if (k == "split_ch4h2o2_3b")
+        c->split3n = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1009,6 +1011,8 @@ extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
         *value = c->split2;
     else if (k == "split_3b")
         *value = c->split3;
+    else if (k == "split_ch4h2o2_3b")
+        *value = c->split3n;
     else
         return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
     return MBXG_OK;

@@ -333,8 +333,32 @@ size_t workspace_nrg(int cap) {
 typedef hipError_t (*piece_launch)(const double*, const int*, int, int, const double*, int, double*, double*,
                                    hipStream_t);
 
-template <class H>
-hipError_t launch_nrg(const piece_launch* pieces, const int* list, const int* n_dev, int cap, const NmerCtx& N,
+// The polynomial stage of a pieced family: one kernel per piece, each into its own slice.
+struct Pieced {
+    const piece_launch* pieces;
+    int count;
+    hipError_t operator()(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
+                          double* gs, double* ps, int nv, hipStream_t st) const {
+        for (int q = 0; q < count; ++q) {
+            hipError_t e = pieces[q](xs, n_dev, cap, ld, coef, do_grads, gs + (size_t)q * nv * ld, ps + (size_t)q * ld, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
+// The polynomial stage of a blocked family: one kernel, `split` wavefronts per 64 n-mers.
+struct Blocked {
+    hipError_t (*launch)(const double*, const int*, int, int, const double*, int, double*, double*, int, hipStream_t);
+    int split;
+    hipError_t operator()(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
+                          double* gs, double* ps, int, hipStream_t st) const {
+        return launch(xs, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
+    }
+};
+
+template <class H, class Poly>
+hipError_t launch_nrg(const Poly& poly, const int* list, const int* n_dev, int cap, const NmerCtx& N,
                       const double* coef, int do_grads, double* grad, double* e_out, double* vir_out,
                       double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
     if (cap <= 0) return hipSuccess;
@@ -349,10 +373,8 @@ hipError_t launch_nrg(const piece_launch* pieces, const int* list, const int* n_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    for (int q = 0; q < H::kPieces; ++q) {
-        e = pieces[q](xs, n_dev, cap, ld, coef, do_grads, gs + (size_t)q * H::kNVars * ld, ps + (size_t)q * ld, st);
-        if (e != hipSuccess) return e;
-    }
+    e = poly(xs, n_dev, cap, ld, coef, do_grads, gs, ps, H::kNVars, st);
+    if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
     const int ngb = (cap + kGradBlockNrg - 1) / kGradBlockNrg;
     if (do_grads)
@@ -371,10 +393,6 @@ hipError_t launch_nrg(const piece_launch* pieces, const int* list, const int* n_
 // The polynomial pieces (gen/poly_<name>_p<k>.hip).
 hipError_t launch_poly_ch4h2o_2b_p0(const double*, const int*, int, int, const double*, int, double*, double*,
                                     hipStream_t);
-hipError_t launch_poly_ch4h2o2_3b_p0(const double*, const int*, int, int, const double*, int, double*, double*,
-                                     hipStream_t);
-hipError_t launch_poly_ch4h2o2_3b_p1(const double*, const int*, int, int, const double*, int, double*, double*,
-                                     hipStream_t);
 #define MBXG_CH4CH4_PIECE(k)                                                                                      \
     hipError_t launch_poly_ch4ch4_2b_p##k(const double*, const int*, int, int, const double*, int, double*, double*, \
                                           hipStream_t);
@@ -403,22 +421,21 @@ size_t workspace_nrg_family(int family, int cap) {
 
 hipError_t launch_nrg_family(int family, const int* list, const int* n_dev, int cap, const NmerCtx& N,
                              const double* coef, int do_grads, double* grad, double* e_out, double* vir_out,
-                             double* partials, double* ws, hipStream_t st, hipEvent_t* ev) {
+                             double* partials, double* ws, int split, hipStream_t st, hipEvent_t* ev) {
     static const piece_launch p_ch4h2o[1] = {launch_poly_ch4h2o_2b_p0};
-    static const piece_launch p_ch4h2o2[2] = {launch_poly_ch4h2o2_3b_p0, launch_poly_ch4h2o2_3b_p1};
     static const piece_launch p_ch4ch4[8] = {launch_poly_ch4ch4_2b_p0, launch_poly_ch4ch4_2b_p1,
                                              launch_poly_ch4ch4_2b_p2, launch_poly_ch4ch4_2b_p3,
                                              launch_poly_ch4ch4_2b_p4, launch_poly_ch4ch4_2b_p5,
                                              launch_poly_ch4ch4_2b_p6, launch_poly_ch4ch4_2b_p7};
     switch (family) {
         case FAM_CH4H2O_2B:
-            return launch_nrg<Holder_ch4h2o_2b>(p_ch4h2o, list, n_dev, cap, N, coef, do_grads, grad, e_out, vir_out,
+            return launch_nrg<Holder_ch4h2o_2b>(Pieced{p_ch4h2o, 1}, list, n_dev, cap, N, coef, do_grads, grad, e_out, vir_out,
                                                 partials, ws, st, ev);
         case FAM_CH4CH4_2B:
-            return launch_nrg<Holder_ch4ch4_2b>(p_ch4ch4, list, n_dev, cap, N, coef, do_grads, grad, e_out, vir_out,
+            return launch_nrg<Holder_ch4ch4_2b>(Pieced{p_ch4ch4, 8}, list, n_dev, cap, N, coef, do_grads, grad, e_out, vir_out,
                                                 partials, ws, st, ev);
         case FAM_CH4H2O2_3B:
-            return launch_nrg<Holder_ch4h2o2_3b>(p_ch4h2o2, list, n_dev, cap, N, coef, do_grads, grad, e_out,
+            return launch_nrg<Holder_ch4h2o2_3b>(Blocked{launch_poly_ch4h2o2_3b, split}, list, n_dev, cap, N, coef, do_grads, grad, e_out,
                                                  vir_out, partials, ws, st, ev);
         default:
             return hipErrorInvalidValue;

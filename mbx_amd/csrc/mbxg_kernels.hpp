@@ -132,11 +132,12 @@ hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const Nmer
                           int split, hipStream_t st, hipEvent_t* ev);
 // MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and
 // FAM_CH4H2O2_3B; list entries in emission order (the kernels order each n-mer's monomers
-// for its holder).
+// for its holder).  `split` (1, 2, 4 or 8) is the wavefronts per 64 n-mers of the blocked
+// CH4-H2O-H2O polynomial; the pieced 2B polynomials ignore it.
 size_t workspace_nrg_family(int family, int cap);
 hipError_t launch_nrg_family(int family, const int* list, const int* n_dev, int cap, const NmerCtx& N,
                              const double* coef, int do_grads, double* grad, double* e_out, double* vir_out,
-                             double* partials, double* ws, hipStream_t st, hipEvent_t* ev);
+                             double* partials, double* ws, int split, hipStream_t st, hipEvent_t* ev);
 
 // Polynomial stage alone: xs/gs variable-major with leading dimension ld (= cap rounded up
 // to 64); ps values; n read from n_dev as above; `split` wavefronts (1, 2, 4 or 8) share each
@@ -145,6 +146,8 @@ hipError_t launch_poly_mbpol2b(const double* xs, const int* n_dev, int cap, int 
                                double* gs, double* ps, int split, hipStream_t st);
 hipError_t launch_poly_mbpol3b(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
                                double* gs, double* ps, int split, hipStream_t st);
+hipError_t launch_poly_ch4h2o2_3b(const double* xs, const int* n_dev, int cap, int ld, const double* coef,
+                                  int do_grads, double* gs, double* ps, int split, hipStream_t st);
 
 // Sum the partial records of min(*n_dev, n_cap) n-mers into out[kPartials] in a fixed order
 // (deterministic totals).
@@ -160,7 +163,5 @@ __global__ void k_reduce_configs(const double* e_out, const double* vir_out, int
                                  const int* pt_off, int comp, int cap, double* part);
 __global__ void k_reduce_slices(const double* part, int nconf, int nsl, double* out);
 __global__ void k_combine_families(const double* fam, int nconf, double* out);
-__global__ void k_reduce_configs(const double* e_out, const double* vir_out, int ld, const int4* offsets,
-                                 const int* pt_off, int comp, int cap, double* out);
 
 }  // namespace mbxg

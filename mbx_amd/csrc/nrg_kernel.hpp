@@ -28,7 +28,7 @@ typedef double (*nrg_e_fn)(mbxg_cptr, const double*, double);
 
 // One piece of a polynomial: gs/ps are the piece's own slices ([NV][ld] and [ld]).
 template <int NV, nrg_eg_fn FN>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_poly_piece_eg(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld,
                     uint64_t coef_addr, double* __restrict__ gs, double* __restrict__ ps) {
     const int t = blockIdx.x * 64 + threadIdx.x;

@@ -29,9 +29,10 @@ typedef double (*poly_e_part_fn)(int, int, mbxg_cptr, const double*);
 constexpr int kRedChunk = 12;
 constexpr size_t poly_lds_bytes(int P) { return (size_t)(P - 1) * (kRedChunk + 1) * 64 * sizeof(double); }
 
-// Energy and gradient.  Three waves per SIMD (at most 168 VGPRs).
-template <int NV, int Q, poly_eg_part_fn PART>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3)))
+// Energy and gradient, WPE waves per SIMD: 3 for MB-pol (at most 168 VGPRs), 2 for the
+// 55-variable CH4-H2O-H2O polynomial (x and g alone take 220 VGPRs).
+template <int NV, int Q, poly_eg_part_fn PART, int WPE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     k_poly_eg(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld, uint64_t coef_addr,
               double* __restrict__ gs, double* __restrict__ ps) {
     extern __shared__ double red[];  // [P-1][kRedChunk+1][64]
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(512) k_poly_e(const double* __restrict__ xs, c
 }
 
 // Launch helper: P wavefronts per 64 n-mers (P in {1, 2, 4, 8}, dividing Q).
-template <int NV, int Q, poly_eg_part_fn PEG, poly_e_part_fn PE>
+template <int NV, int Q, poly_eg_part_fn PEG, poly_e_part_fn PE, int WPE = 3>
 hipError_t launch_poly(const double* xs, const int* n_dev, int cap, int ld, const double* coef, int do_grads,
                        double* gs, double* ps, int P, hipStream_t st) {
     if (cap <= 0) return hipSuccess;
@@ -110,7 +111,7 @@ hipError_t launch_poly(const double* xs, const int* n_dev, int cap, int ld, cons
     const int nb = (cap + 63) / 64;
     const uint64_t ca = (uint64_t)(uintptr_t)coef;
     if (do_grads)
-        hipLaunchKernelGGL((k_poly_eg<NV, Q, PEG>), dim3(nb), dim3(64 * P), poly_lds_bytes(P), st, xs, n_dev, cap,
+        hipLaunchKernelGGL((k_poly_eg<NV, Q, PEG, WPE>), dim3(nb), dim3(64 * P), poly_lds_bytes(P), st, xs, n_dev, cap,
                            ld, ca, gs, ps);
     else
         hipLaunchKernelGGL((k_poly_e<NV, Q, PE>), dim3(nb), dim3(64 * P), (size_t)(P - 1) * 64 * sizeof(double), st,

@@ -345,8 +345,11 @@ def coef_source(name, coef):
 
 
 # MB-nrg polynomials: each piece is its own translation unit (compiled in parallel) with its
-# own kernel writing a partial value and gradient; the chain-rule stage sums the pieces.
-NRG_PARTS = {"ch4h2o_2b": 1, "ch4h2o2_3b": 2, "ch4ch4_2b": 8}
+# own kernel writing a partial value and gradient; the chain-rule stage sums the pieces.  The
+# CH4-H2O-H2O 3B polynomial instead takes the blocked form of the MB-pol ones (generate(): one
+# translation unit, NPARTS parts shared by the wavefronts of a block, one output slice).
+NRG_PARTS = {"ch4h2o_2b": 1, "ch4ch4_2b": 8}
+NRG_BLOCKED = ("ch4h2o2_3b",)
 
 
 def generate_nrg(name, out_dir):
@@ -404,7 +407,7 @@ def generate_holder(name, out_dir):
            "    static constexpr double kRIn = %.17g;" % h["switch"]["inner"],
            "    static constexpr double kROut = %.17g;" % h["switch"]["outer"],
            "    static constexpr int kNVars = %d;" % len(h["vars"]),
-           "    static constexpr int kPieces = %d;" % NRG_PARTS[name],
+           "    static constexpr int kPieces = %d;" % NRG_PARTS.get(name, 1),
            "    static constexpr NrgVar kVars[%d] = {" % len(h["vars"])]
     for f, k, d, s1, s2 in h["vars"]:
         out.append("        {%d, %.17g, %.17g, %d, %d}," % (forms[f], k, d, s1, s2))
@@ -710,6 +713,9 @@ def main():
         generate_ad(name, out_dir)
     for name in NRG_PARTS:
         print(name, generate_nrg(name, out_dir))
+        generate_holder(name, out_dir)
+    for name in NRG_BLOCKED:
+        print(generate(name, "poly_" + name, out_dir))
         generate_holder(name, out_dir)
 
 

@@ -115,3 +115,24 @@ def test_mixed_gas_cluster_and_energy_only():
     e.compute(do_grads=False)
     f2, f3, _, _ = e.results(grad=False)
     assert close(f2, r["e2b"], s["e2b"]) and close(f3, r["e3b"], s["e3b"]), (f2, f3)
+
+
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+def test_ch4h2o2_splits(systems, split):
+    """Every wavefront split of the blocked CH4-H2O-H2O polynomial (split_ch4h2o2_3b) gives the
+    oracle's 3B energy and gradients on the CH4-rich box."""
+    xyz, kinds, box, c2, c3, _ = mixed_case(systems, "ch4_342_h2o_228_rect")
+    print("ch4h2o2 split", split, flush=True)
+    e = engine.Engine()
+    e.set_topology(kinds)
+    e.set_box(box)
+    e.set_cutoffs(c2, c3)
+    e.set_families(("ch4h2o2_3b",))
+    e.set_tuning("split_ch4h2o2_3b", split)
+    assert e.tuning("split_ch4h2o2_3b") == split
+    e.set_coords(xyz)
+    e.compute()
+    _, e3, g, _ = e.results()
+    r = oracle.system(xyz, kinds, box, c2, c3, tags=np.arange(len(kinds)), families=("ch4h2o2_3b",))
+    assert e3 != 0.0 and e3 == pytest.approx(r["e3b"], rel=E_REL)
+    assert np.abs(g - r["grad"]).max() <= G_ABS
