@@ -160,6 +160,7 @@ struct mbxg_ctx {
     int g_terms = -1, g_grads = -1;       // what gexec computes
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
+    int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
@@ -340,6 +341,8 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
         acc += nat[i];
     }
     c->natoms = acc;
+    c->n_h2o = (int)std::count(kind, kind + nmon, MBXG_H2O);
+    c->n_ch4 = (int)std::count(kind, kind + nmon, MBXG_CH4);
     // cloud members: every local monomer (AddClusters with use_ghost = false, sys_tools.cpp:508)
     c->members.clear();
     for (int i = 0; i < nmon; ++i)
@@ -588,6 +591,20 @@ int fam_tot(const ListState& L, int f) {
     }
 }
 
+// Whether a family's stages are launched: it is evaluated (mbxg_set_families) and the topology
+// holds the monomers it needs.  A family that is not launched has an empty evaluate list, and
+// its totals keep the zeros they are cleared to before the families run.
+bool family_launched(const mbxg_ctx* c, int f) {
+    if (!((c->fam_mask >> f) & 1)) return false;
+    switch (f) {
+        case FAM_MBPOL_2B: return c->n_h2o >= 2;
+        case FAM_CH4H2O_2B: return c->n_h2o >= 1 && c->n_ch4 >= 1;
+        case FAM_CH4CH4_2B: return c->n_ch4 >= 2;
+        case FAM_MBPOL_3B: return c->n_h2o >= 3;
+        default: return c->n_h2o >= 2 && c->n_ch4 >= 1;
+    }
+}
+
 // The int4 component of a family's count in offsets / offsets_x (k_reduce_configs' comp).
 int fam_component(int f) {
     switch (f) {
@@ -805,7 +822,7 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
         for (int i = 0; i < kFamilies; ++i) {
             const int f = order_of[i];
             const bool two = family_order(f) == 2;
-            if (!(two ? want2 : want3)) continue;
+            if (!(two ? want2 : want3) || !family_launched(c, f)) continue;
             const int cap = fam_cap(L, f);
             if (cap <= 0) continue;
             // with stage timing on, everything runs on the main stream so each stage is timed alone
