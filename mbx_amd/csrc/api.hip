@@ -105,6 +105,15 @@ struct ListState {
     int match_cap = 256;          // power of two
     int4 tot = {0, 0, 0, 0}, totx = {0, 0, 0, 0};
     bool built = false;           // the last compute built lists
+    // one-pass builds: per-(query, wavefront) staging of each family's evaluated n-mers, sized
+    // from the per-wavefront maxima (maxw) of the last synchronised build
+    DBuf<int> stage[kFamilies];
+    DBuf<int4> wmax, wmax_x, maxw;  // per-query largest per-wavefront counts, their maxima (2)
+    int scap[kFamilies] = {0, 0, 0, 0, 0};
+    bool staged = false;          // the last build was one-pass (listed lists not written)
+    bool lists_written = false;   // list2 / list3 hold the last build's listed n-mers
+    NmersArgs args{};             // the last build's arguments (to write the listed lists later)
+    NmerCtx nctx{};
 };
 
 // Per-order evaluation buffers.
@@ -591,6 +600,15 @@ int fam_tot(const ListState& L, int f) {
     }
 }
 
+// Per-family largest per-wavefront counts from the two maxima records (counts / counts_x layout).
+void wave_maxima(const int4* m, int* mw) {
+    mw[FAM_MBPOL_2B] = m[0].y;
+    mw[FAM_CH4H2O_2B] = m[1].x;
+    mw[FAM_CH4CH4_2B] = m[1].y;
+    mw[FAM_MBPOL_3B] = m[0].w;
+    mw[FAM_CH4H2O2_3B] = m[1].z;
+}
+
 // Whether a family's stages are launched: it is evaluated (mbxg_set_families) and the topology
 // holds the monomers it needs.  A family that is not launched has an empty evaluate list, and
 // its totals keep the zeros they are cleared to before the families run.
@@ -668,7 +686,13 @@ int prepare_layout(mbxg_ctx* c, double cut) {
 
 // Dimer and trimer lists at the larger cutoff (one gather for both orders).  With sync the
 // totals and the largest match count are read back and the buffers sized from them.
-int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double cmax, bool sync, hipEvent_t* ev) {
+//
+// Two builds: the count pass, scan and emit pass (listed lists and evaluate lists), used when
+// synchronising or when the listed lists are wanted (need_lists); otherwise one pass that counts
+// and stages the evaluated n-mers per (query, wavefront), the scan, and a compaction into the
+// evaluate lists (the listed lists are written later, if asked for, by mbxg_get_lists).
+int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double cmax, bool sync, bool need_lists,
+                hipEvent_t* ev) {
     ListState& L = c->L;
     hipStream_t st = c->stream;
     const int np = c->np;
@@ -680,7 +704,8 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.bx.reserve(ns) || L.by.reserve(ns) ||
         L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.pinfo.reserve(np) ||
         L.counts.reserve(np) || L.wpre.reserve((size_t)np * kQueryWaves) || L.wpre_x.reserve((size_t)np * kQueryWaves) ||
-        L.offsets.reserve(np + 1) || L.counts_x.reserve(np) || L.offsets_x.reserve(np + 1))
+        L.offsets.reserve(np + 1) || L.counts_x.reserve(np) || L.offsets_x.reserve(np + 1) ||
+        L.wmax.reserve(np) || L.wmax_x.reserve(np) || L.maxw.reserve(2))
         return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
     const int tb = 256, pb = (np + tb - 1) / tb, sb = (ns + tb - 1) / tb;
     if (!c->box.periodic)
@@ -719,6 +744,14 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     A.wpre = L.wpre.p;
     A.wpre_x = L.wpre_x.p;
     A.fam_mask = c->fam_mask;
+    A.wmax = L.wmax.p;
+    A.wmax_x = L.wmax_x.p;
+    const bool one_pass = !sync && !need_lists && L.scap[0] > 0;
+    if (one_pass)
+        for (int f = 0; f < kFamilies; ++f) {
+            A.stage[f] = L.stage[f].p;
+            A.scap[f] = L.scap[f];
+        }
 
     auto scratch = [&]() -> double* {
         if (L.match_cap <= kMaxLdsMatches) return nullptr;
@@ -729,16 +762,20 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     for (int attempt = 0;; ++attempt) {
         double* gs = scratch();
         if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
-        HIPCHK(launch_nmers(A, N, 0, none, none, L.match_cap, gs, kScratchBlocks, st));
+        HIPCHK(launch_nmers(A, N, one_pass ? 2 : 0, none, none, L.match_cap, gs, kScratchBlocks, st));
+        hipLaunchKernelGGL(k_max_counts, dim3(1), dim3(1024), 0, st, L.wmax.p, L.wmax_x.p, np, L.maxw.p);
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p, 0);
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts_x.p, np, L.offsets_x.p, 1);
         HIPCHK(hipGetLastError());
         if (!sync) break;
         int4 tot, totx;
-        int ovf = 0;
+        int ovf = 0, mw[kFamilies];
+        int4 m2[2];
         HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + np, sizeof(int4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(m2, L.maxw.p, sizeof(m2), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        wave_maxima(m2, mw);
         ovf = totx.w;  // the largest match count of any query
         const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
         if (ovf > L.match_cap) {
@@ -755,8 +792,12 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         L.cap = cap;
         L.capx = capx;
         bool bad = L.list2.reserve(2 * (size_t)cap.x) || L.list3.reserve(3 * (size_t)cap.z);
-        for (int f = 0; f < kFamilies; ++f)
+        for (int f = 0; f < kFamilies; ++f) {
             bad = bad || L.evals[f].reserve((size_t)family_order(f) * fam_cap(L, f) + 1);
+            const int sc = mw[f] + mw[f] / 4 + 16;  // staging per (query, wavefront), with headroom
+            bad = bad || L.stage[f].reserve((size_t)np * kQueryWaves * sc * family_order(f) + 1);
+            if (!bad) L.scap[f] = sc;
+        }
         if (bad) return fail(MBXG_ENOMEM, "n-mer lists: device allocation failed (%d + %d listed)", tot.x, tot.z);
         if (L.match_cap > kMaxLdsMatches && !scratch())
             return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
@@ -766,6 +807,23 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     A.list2 = L.list2.p;
     A.list3 = L.list3.p;
     for (int f = 0; f < kFamilies; ++f) A.evals[f] = L.evals[f].p;
+    L.args = A;
+    L.nctx = N;
+    L.staged = one_pass;
+    L.lists_written = !one_pass;
+    if (one_pass) {
+        CompactArgs C;
+        for (int f = 0; f < kFamilies; ++f) {
+            C.stage[f] = L.stage[f].p;
+            C.scap[f] = L.scap[f];
+            C.evals[f] = L.evals[f].p;
+            C.fcap[f] = fam_cap(L, f);
+        }
+        hipLaunchKernelGGL(k_compact_evals, dim3(std::min(np, 16384)), dim3(64 * kQueryWaves), 0, st, C, np,
+                           L.counts.p, L.counts_x.p, L.offsets.p, L.offsets_x.p, L.wpre.p, L.wpre_x.p);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     HIPCHK(launch_nmers(A, N, 1, L.cap, L.capx, L.match_cap,
                         L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr, kScratchBlocks, st));
     return 0;
@@ -800,7 +858,8 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     const double c3sq = want3 ? c->cut3 * c->cut3 : 0.0;
     L.built = (want2 || want3) && c->np >= 2;
     if (L.built) {
-        if (int r = build_lists(c, N, c2sq, c3sq, std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), sync, ev))
+        if (int r = build_lists(c, N, c2sq, c3sq, std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), sync,
+                                (terms & MBXG_LISTS_ONLY) != 0, ev))
             return r;
     } else {
         L.tot = make_int4(0, 0, 0, 0);
@@ -885,13 +944,18 @@ int finish(mbxg_ctx* c) {
     if (!c->have_results || c->checked) return MBXG_OK;
     ListState& L = c->L;
     int4 tot, totx;
-    int ovf = 0;
+    int ovf = 0, mw[kFamilies] = {0, 0, 0, 0, 0};
+    int4 m2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     HIPCHK(hipMemcpyAsync(&tot, L.offsets.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&totx, L.offsets_x.p + c->np, sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    if (L.staged) HIPCHK(hipMemcpyAsync(m2, L.maxw.p, sizeof(m2), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    wave_maxima(m2, mw);
     ovf = totx.w;  // the largest match count of any query
+    bool staging_short = false;  // a one-pass build staged more n-mers per wavefront than it holds
+    for (int f = 0; f < kFamilies; ++f) staging_short = staging_short || mw[f] > L.scap[f];
     if (tot.x > L.cap.x || tot.y > L.cap.y || tot.z > L.cap.z || tot.w > L.cap.w || totx.x > L.capx.x ||
-        totx.y > L.capx.y || totx.z > L.capx.z || ovf > L.match_cap) {
+        totx.y > L.capx.y || totx.z > L.capx.z || ovf > L.match_cap || staging_short) {
         c->need_sync = true;
         if (c->inputs_changed) {
             c->have_results = false;
@@ -959,6 +1023,7 @@ extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     HIPCHK(hipGraphLaunch(c->gexec, st));
     c->have_results = true;
     c->checked = !c->L.built;
+    c->L.lists_written = !c->L.staged;  // the replayed build is the captured one
     return MBXG_OK;
 }
 
@@ -1142,6 +1207,14 @@ extern "C" int mbxg_get_lists(mbxg_ctx* c, int32_t* dimers, int64_t cap2, int32_
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_lists: null context");
     if (int r = finish(c)) return r;
     if (!c->have_results) return MBXG_OK;
+    ListState& L = c->L;
+    if (L.built && !L.lists_written) {
+        // a one-pass build left the listed lists unwritten: the emit pass writes them from the
+        // same cloud, counts and offsets (its evaluate-list writes repeat the compaction's)
+        HIPCHK(launch_nmers(L.args, L.nctx, 1, L.cap, L.capx, L.match_cap,
+                            L.match_cap > kMaxLdsMatches ? L.gscratch.p : nullptr, kScratchBlocks, c->stream));
+        L.lists_written = true;
+    }
     const int4 t = c->L.tot;
     if (dimers && t.x > 0) {
         const int64_t n = t.x < cap2 ? t.x : cap2;

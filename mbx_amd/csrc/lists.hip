@@ -330,13 +330,15 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
                             const int4* __restrict__ offsets_x, int* __restrict__ list2, int* __restrict__ list3,
                             int* const* evals, int fam_mask, int4 cap, int4 capx, int match_cap,
                             const Lds& L, int* sh_count, int4* sh_w,
-                            int4* __restrict__ wpre, int4* __restrict__ wpre_x) {
+                            int4* __restrict__ wpre, int4* __restrict__ wpre_x, int* const* stage,
+                            const int* scap, int4* __restrict__ wmax, int4* __restrict__ wmax_x) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int mi = N.members[q];
     if (!N.islocal[mi]) {
-        if (pass == 0 && tid == 0) {
+        if (pass != 1 && tid == 0) {
             counts[q] = make_int4(0, 0, 0, 0);
             counts_x[q] = make_int4(0, 0, 0, 0);
+            if (wmax) wmax[q] = wmax_x[q] = make_int4(0, 0, 0, 0);
         }
         return;
     }
@@ -431,9 +433,10 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
         // timing, so the count and emit passes could keep different subsets and disagree on this
         // query's n-mers; it emits none in either pass.  The caller sees the overflow and
         // recomputes with more storage.
-        if (pass == 0 && tid == 0) {
+        if (pass != 1 && tid == 0) {
             counts[q] = make_int4(0, 0, 0, 0);
             counts_x[q] = make_int4(0, 0, 0, nm);
+            if (wmax) wmax[q] = wmax_x[q] = make_int4(0, 0, 0, 0);
         }
         return;
     }
@@ -503,7 +506,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 
     const unsigned long long below = (1ull << lane) - 1ull;
     int4 at = make_int4(0, 0, 0, 0), atx = make_int4(0, 0, 0, 0);
-    if (pass) {  // this wavefront's base: the query's offset plus the lower wavefronts' counts
+    if (pass == 1) {  // this wavefront's base: the query's offset plus the lower wavefronts' counts
         const int4 o = offsets[q], ox = offsets_x[q];
         const int4 w = wpre[q * kQueryWaves + wv], wx = wpre_x[q * kQueryWaves + wv];
         at = make_int4(o.x + w.x, o.y + w.y, o.z + w.z, o.w + w.w);
@@ -548,7 +551,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
             }
             if (fam >= 0 && !((fam_mask >> fam) & 1)) fam = -1;
             const unsigned long long ml = __ballot(listed);
-            if (pass && listed) {
+            if (pass == 1 && listed) {
                 const int w = at.x + nl2 + __popcll(ml & below);
                 if (w < cap.x) {
                     list2[2 * w] = mi;
@@ -559,11 +562,18 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 #pragma unroll
             for (int f = FAM_MBPOL_2B; f <= FAM_CH4CH4_2B; ++f) {
                 const unsigned long long me = __ballot(fam == f);
-                if (pass && fam == f) {
+                if (pass == 1 && fam == f) {
                     const int w = fbase[f] + ne[f] + __popcll(me & below);
                     if (w < fcap[f]) {
                         evals[f][2 * w] = mi;
                         evals[f][2 * w + 1] = mj;
+                    }
+                } else if (pass == 2 && fam == f) {
+                    const int w = ne[f] + __popcll(me & below);
+                    if (w < scap[f]) {
+                        int* o = stage[f] + ((size_t)(q * kQueryWaves + wv) * scap[f] + w) * 2;
+                        o[0] = mi;
+                        o[1] = mj;
                     }
                 }
                 ne[f] += __popcll(me);
@@ -623,7 +633,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
             }
             if (fam >= 0 && !((fam_mask >> fam) & 1)) fam = -1;
             const unsigned long long ml = __ballot(listed);
-            if (pass && listed) {
+            if (pass == 1 && listed) {
                 const int w = at.z + nl3 + __popcll(ml & below);
                 if (w < cap.z) {
                     list3[3 * w] = mi;
@@ -635,19 +645,27 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 #pragma unroll
             for (int f = FAM_MBPOL_3B; f <= FAM_CH4H2O2_3B; ++f) {
                 const unsigned long long me = __ballot(fam == f);
-                if (pass && fam == f) {
+                if (pass == 1 && fam == f) {
                     const int w = fbase[f] + ne[f] + __popcll(me & below);
                     if (w < fcap[f]) {
                         evals[f][3 * w] = mi;
                         evals[f][3 * w + 1] = mj;
                         evals[f][3 * w + 2] = mk;
                     }
+                } else if (pass == 2 && fam == f) {
+                    const int w = ne[f] + __popcll(me & below);
+                    if (w < scap[f]) {
+                        int* o = stage[f] + ((size_t)(q * kQueryWaves + wv) * scap[f] + w) * 3;
+                        o[0] = mi;
+                        o[1] = mj;
+                        o[2] = mk;
+                    }
                 }
                 ne[f] += __popcll(me);
             }
         }
     }
-    if (pass == 0) {  // the query's counts, and each wavefront's prefix within the query
+    if (pass != 1) {  // the query's counts, and each wavefront's prefix within the query
         if (lane == 0) {
             sh_w[wv] = make_int4(nl2, ne[FAM_MBPOL_2B], nl3, ne[FAM_MBPOL_3B]);
             sh_w[kQueryWaves + wv] = make_int4(ne[FAM_CH4H2O_2B], ne[FAM_CH4CH4_2B], ne[FAM_CH4H2O2_3B], 0);
@@ -664,6 +682,16 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
             }
             counts[q] = a;
             counts_x[q] = make_int4(ax.x, ax.y, ax.z, nm);  // .w: the query's match count
+            if (wmax) {
+                int4 m = sh_w[0], mx = sh_w[kQueryWaves];
+                for (int w = 1; w < kQueryWaves; ++w) {
+                    const int4 c = sh_w[w], cx = sh_w[kQueryWaves + w];
+                    m = make_int4(max(m.x, c.x), max(m.y, c.y), max(m.z, c.z), max(m.w, c.w));
+                    mx = make_int4(max(mx.x, cx.x), max(mx.y, cx.y), max(mx.z, cx.z), max(mx.w, cx.w));
+                }
+                wmax[q] = m;
+                wmax_x[q] = mx;
+            }
         }
     }
 }
@@ -687,9 +715,67 @@ __global__ void __launch_bounds__(64 * kQueryWaves)
         nmers_query(q, A.pinfo, A.cx, A.cy, A.cz, A.cell, A.cell_start, A.cell_slot, A.bx, A.by, A.bz, A.np, G,
                     A.cell_base[conf], N, A.c2sq, A.c3sq, pass, A.counts, A.counts_x, A.offsets, A.offsets_x,
                     A.list2, A.list3, A.evals, A.fam_mask, cap, capx, match_cap, L, &sh_count, sh_w,
-                    A.wpre, A.wpre_x);
+                    A.wpre, A.wpre_x, A.stage, A.scap, A.wmax, A.wmax_x);
         __syncthreads();
     }
+}
+
+// A family's component of the per-query count records: counts = {listed 2B, MB-pol 2B, listed
+// 3B, MB-pol 3B}, counts_x = {CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, matches}.
+__device__ __forceinline__ int fam_count(int f, const int4& c, const int4& cx) {
+    return f == FAM_MBPOL_2B ? c.y : f == FAM_CH4H2O_2B ? cx.x : f == FAM_CH4CH4_2B ? cx.y : f == FAM_MBPOL_3B ? c.w : cx.z;
+}
+
+__global__ void __launch_bounds__(64 * kQueryWaves)
+    k_compact_evals(CompactArgs C, int np, const int4* __restrict__ counts, const int4* __restrict__ counts_x,
+                    const int4* __restrict__ offsets, const int4* __restrict__ offsets_x,
+                    const int4* __restrict__ wpre, const int4* __restrict__ wpre_x) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = blockIdx.x; q < np; q += gridDim.x) {
+        const int4 c = counts[q], cx = counts_x[q];
+#pragma unroll
+        for (int f = 0; f < kFamilies; ++f) {
+            const int total = fam_count(f, c, cx);
+            if (total <= 0) continue;  // wpre is only written for queries with n-mers
+            const int order = family_order(f);
+            const int k = q * kQueryWaves + wv;
+            const int cur = fam_count(f, wpre[k], wpre_x[k]);
+            const int next = wv + 1 < kQueryWaves ? fam_count(f, wpre[k + 1], wpre_x[k + 1]) : total;
+            const int base = fam_count(f, offsets[q], offsets_x[q]) + cur;
+            const int fit = min(next - cur, C.fcap[f] - base);  // n-mers of this wavefront the list holds
+            const int staged = min(fit, C.scap[f]);
+            const int* src = C.stage[f] + (size_t)k * C.scap[f] * order;
+            int* dst = C.evals[f] + (size_t)base * order;
+            // n-mers past the staging capacity were not kept: their entries name monomer 0, so
+            // the evaluation reads valid coordinates (the caller sees maxw above the capacity and
+            // recomputes)
+            for (int i = lane; i < fit * order; i += 64) dst[i] = i < staged * order ? src[i] : 0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_max_counts(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
+                                                     int n, int4* __restrict__ out) {
+    __shared__ int4 sh[2][1024];
+    const int t = threadIdx.x;
+    int4 m = make_int4(0, 0, 0, 0), mx = make_int4(0, 0, 0, 0);
+    for (int i = t; i < n; i += 1024) {
+        const int4 c = wmax[i], cx = wmax_x[i];
+        m = make_int4(max(m.x, c.x), max(m.y, c.y), max(m.z, c.z), max(m.w, c.w));
+        mx = make_int4(max(mx.x, cx.x), max(mx.y, cx.y), max(mx.z, cx.z), max(mx.w, cx.w));
+    }
+    sh[0][t] = m;
+    sh[1][t] = mx;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 2; ++k) {
+                const int4 a = sh[k][t], b = sh[k][t + s];
+                sh[k][t] = make_int4(max(a.x, b.x), max(a.y, b.y), max(a.z, b.z), max(a.w, b.w));
+            }
+        __syncthreads();
+    }
+    if (t < 2) out[t] = sh[t][0];
 }
 
 // Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).

@@ -103,7 +103,30 @@ struct NmersArgs {
     int* list3;
     int* evals[kFamilies];
     int fam_mask;   // bit f: family f is evaluated (listed n-mers of a cleared family are not)
+    // one-pass build (pass 2): wavefront w of query q stages its family-f n-mers at
+    // stage[f] + (q * kQueryWaves + w) * scap[f] * order, at most scap[f] of them.  Every count
+    // pass stores query q's largest per-wavefront counts in wmax[q] / wmax_x[q] (the layouts of
+    // counts / counts_x; nullable)
+    int* stage[kFamilies];
+    int scap[kFamilies];
+    int4* wmax;
+    int4* wmax_x;
 };
+
+// out[0] / out[1] = componentwise maxima of wmax[0..n) / wmax_x[0..n) (one workgroup).
+__global__ void k_max_counts(const int4* wmax, const int4* wmax_x, int n, int4* out);
+
+// The staged n-mers of a one-pass build, moved to the evaluate lists: block q (striding),
+// wavefront w copies its family-f n-mers to evals[f] at the family's offset of query q plus the
+// wavefront's prefix, bounded by fcap[f] (the list capacity in n-mers) and scap[f].
+struct CompactArgs {
+    const int* stage[kFamilies];
+    int scap[kFamilies];
+    int* evals[kFamilies];
+    int fcap[kFamilies];
+};
+__global__ void k_compact_evals(CompactArgs C, int np, const int4* counts, const int4* counts_x,
+                                const int4* offsets, const int4* offsets_x, const int4* wpre, const int4* wpre_x);
 // match_cap (a power of two) bounds the matches one query may hold.  With gscratch == nullptr
 // they live in LDS (nmers_lds_bytes(match_cap) per block, one block per query); otherwise
 // nblocks blocks stride over the queries, each with nmers_lds_bytes(match_cap) bytes of

@@ -1,0 +1,64 @@
+"""The one-pass list build (count pass that stages each family's evaluated n-mers per query and
+wavefront, scan, compaction into the evaluate lists), which every unsynchronised compute that
+does not ask for the listed lists takes, against the two-pass build of a synchronised compute:
+identical evaluate lists (so bit-identical energies; gradients and virials to rounding, their
+atomic accumulation order varies), listed lists written on demand by mbxg_get_lists, in eager
+and graph mode.  Staging overflow (a compute with
+more n-mers per wavefront than the last synchronised one) is covered by
+test_gpu_parity.py::test_lists_outgrow_buffers."""
+import numpy as np
+import pytest
+
+from conftest import sorted_tuples
+
+import oracle
+from mbx_amd import configs, engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(e):
+    e2, e3, g, v = e.results()
+    d, t = e.lists()
+    return e2, e3, g, v, d, t
+
+
+def _same(a, b):
+    assert a[0] == b[0] and a[1] == b[1]
+    np.testing.assert_allclose(a[2], b[2], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(a[3], b[3], rtol=1e-12, atol=1e-10)
+    assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_one_pass_equals_two_pass(graph):
+    xyz, box, c2, c3 = configs.build(configs.C1)
+    rng = np.random.default_rng(7)
+    moved = xyz + rng.normal(0.0, 0.05, xyz.shape)
+    print("one-pass build, graph", graph, flush=True)
+    e = engine.water_engine(configs.C1["nmon"], box, c2, c3)
+    e.set_graph(graph)
+    e.set_coords(xyz)
+    e.compute()  # synchronised: two passes, sizes the staging
+    first = _state(e)
+    # a fresh context's synchronised compute of the moved coordinates is the two-pass answer
+    f = engine.water_engine(configs.C1["nmon"], box, c2, c3)
+    f.set_coords(moved)
+    f.compute()
+    ref = _state(f)
+    for k in range(3):  # steady state: one pass (graph mode: captured, then replayed)
+        e.set_coords(moved)
+        e.compute()
+        _same(_state(e), ref)
+        # results read without the lists first, then the lists
+        e.set_coords(xyz)
+        e.compute()
+        e2, e3, g, v = e.results()
+        assert e2 == first[0] and e3 == first[1]
+        np.testing.assert_allclose(g, first[2], rtol=0, atol=1e-11)
+        d, t = e.lists()
+        assert np.array_equal(d, first[4]) and np.array_equal(t, first[5])
+    # and the lists are the oracle's
+    fx = oracle.fix_first(xyz, box)
+    assert np.array_equal(sorted_tuples(first[4]), sorted_tuples(oracle.clusters(2, c2, fx, box)))
+    assert np.array_equal(sorted_tuples(first[5]), sorted_tuples(oracle.clusters(3, c3, fx, box)))
