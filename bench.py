@@ -9,11 +9,13 @@ into the engine's working buffer is part of each step); results stay on the devi
 Throughput is listed dimers + trimers per second (the n-mers the reference's Get2B / Get3B walk).
 
 The JSON line also carries
-  roofline:     the 3B polynomial kernel (the dominant kernel) against MI355X FP64 vector peak,
-                with the reference's algorithmic FLOPs per evaluated trimer (81,549, counted from
-                the reference's generated E+G code: mbx_amd/tables/flops.json), timed with HIP
-                events on the engine stream during the timed steps; traffic from the committed
-                rocprofv3 PMC summary when present;
+  roofline:     the polynomial launch (the dominant kernel: the MB-pol 3B and 2B polynomials in
+                one launch, k_poly_eg_pair; the 3B one alone when the pairing is off) against
+                MI355X FP64 vector peak, with the reference's algorithmic FLOPs per evaluated
+                trimer (81,549) and dimer (27,208), counted from the reference's generated E+G
+                code (mbx_amd/tables/flops.json), timed with HIP events on the engine stream
+                during the timed steps; traffic from the committed rocprofv3 PMC summary when
+                present;
   cpu_baseline: the reference's own compiled 2B/3B leaves (oracle/_ref, OpenMP driver with the
                 reference's batch-of-8 dispatch) on this host's cores, on the same workload.
 """
@@ -326,10 +328,13 @@ def main():
     e2, e3, grad, vir = eng.results()
 
     flops = json.load(open(os.path.join(ROOT, "mbx_amd", "tables", "flops.json")))
-    f3 = flops["mbpol_3b_eg"]["flops"]
+    f3, f2 = flops["mbpol_3b_eg"]["flops"], flops["mbpol_2b_eg"]["flops"]
+    paired = eng.tuning("fuse_poly") == 1 and counts[1] > 0 and counts[3] > 0
     t_poly = float(np.mean(poly_ms)) * 1e-3
-    achieved = counts[3] * f3 / t_poly
+    achieved = (counts[3] * f3 + (counts[1] * f2 if paired else 0)) / t_poly
     traffic = read_traffic()
+    kernel = ("k_poly_eg_pair<36, 31> (MB-pol 3B and 2B polynomials E+G, one launch)" if paired
+              else "k_poly_eg<36> (MB-pol 3B polynomial E+G)")
 
     out = {
         "metric": METRIC,
@@ -349,10 +354,12 @@ def main():
                    "listed_trimers": int(counts[2]), "evaluated_trimers": int(counts[3]),
                    "stage_ms": {k: v / args.steps for k, v in stage.items()}, "e2b": e2, "e3b": e3,
                    "schedule": schedule},
-        "roofline": {"bound": "fp64-valu", "kernel": "k_poly_eg<36> (MB-pol 3B polynomial E+G)",
+        "roofline": {"bound": "fp64-valu", "kernel": kernel,
                      "achieved": achieved / 1e12, "peak": FP64_VECTOR_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK, "traffic": traffic,
-                     "flops_per_unit": f3, "units_per_launch": int(counts[3]), "launch_ms": t_poly * 1e3},
+                     "flops_per_unit": {"trimer": f3, "dimer": f2 if paired else 0},
+                     "units_per_launch": {"trimers": int(counts[3]), "dimers": int(counts[1]) if paired else 0},
+                     "launch_ms": t_poly * 1e3},
         "cpu_baseline": None,
     }
     # list build (fix, cloud, binning, both n-mer passes): PMC bytes per step over its HIP-event time

@@ -172,6 +172,8 @@ struct mbxg_ctx {
     int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
+    int fuse_poly = 1;       // MB-pol 2B and 3B polynomials in one launch when both are evaluated
+    bool fused_last = false;  // the last compute did so (its 2B polynomial time is in the 3B one)
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
     hipEvent_t ev[kNumEvents] = {};
@@ -878,24 +880,76 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
         HIPCHK(hipStreamWaitEvent(c->stream2, c->fork, 0));
         const bool batch = c->nconf > 1;
         const int order_of[kFamilies] = {FAM_MBPOL_3B, FAM_CH4H2O2_3B, FAM_MBPOL_2B, FAM_CH4H2O_2B, FAM_CH4CH4_2B};
-        for (int i = 0; i < kFamilies; ++i) {
-            const int f = order_of[i];
-            const bool two = family_order(f) == 2;
-            if (!(two ? want2 : want3) || !family_launched(c, f)) continue;
+        auto active = [&](int f) {
+            return (family_order(f) == 2 ? want2 : want3) && family_launched(c, f) && fam_cap(L, f) > 0;
+        };
+        // the MB-pol 2B and 3B polynomials share one launch (k_poly_eg_pair) when both run with
+        // gradients; everything of both families then runs on the main stream
+        const bool fuse = c->fuse_poly && do_grads && active(FAM_MBPOL_2B) && active(FAM_MBPOL_3B);
+        c->fused_last = fuse;
+        auto buffers = [&](int f) -> bool {  // false: allocation failed
             const int cap = fam_cap(L, f);
-            if (cap <= 0) continue;
-            // with stage timing on, everything runs on the main stream so each stage is timed alone
-            hipStream_t sk = (two && !ev) ? c->stream2 : st;
             EvalState& o = c->fam[f];
             const int ld = (cap + 63) / 64 * 64;
             const size_t wsz = f == FAM_MBPOL_2B   ? workspace_mbpol2b(cap)
                                : f == FAM_MBPOL_3B ? workspace_mbpol3b(cap)
                                                    : workspace_nrg_family(f, cap);
-            if (o.e_nmer.reserve(cap) || o.partials.reserve((size_t)kPartials * partials_records(cap)) ||
-                o.ws.reserve(wsz) || (batch && o.vir_nmer.reserve((size_t)6 * ld)))
+            return !(o.e_nmer.reserve(cap) || o.partials.reserve((size_t)kPartials * partials_records(cap)) ||
+                     o.ws.reserve(wsz) || (batch && o.vir_nmer.reserve((size_t)6 * ld)));
+        };
+        auto reduce = [&](int f, hipStream_t sk) {
+            const int cap = fam_cap(L, f);
+            const int ld = (cap + 63) / 64 * 64;
+            EvalState& o = c->fam[f];
+            const int* nd = fam_ndev(L, c->np, f);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
+                               c->d_fam_tot.p + kPartials * f);
+            if (batch) {
+                double* part = c->d_conf_part.p + (size_t)f * kConfSlices * kPartials * c->nconf;
+                hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf, kConfSlices), dim3(256), 0, sk, o.e_nmer.p,
+                                   o.vir_nmer.p, ld, f == FAM_MBPOL_2B || f == FAM_MBPOL_3B ? L.offsets.p : L.offsets_x.p,
+                                   c->d_pt_off.p, fam_component(f), cap, part);
+                hipLaunchKernelGGL(k_reduce_slices, dim3((c->nconf * kPartials + 255) / 256), dim3(256), 0, sk, part,
+                                   c->nconf, kConfSlices, c->d_conf_fam.p + (size_t)f * kPartials * c->nconf);
+            }
+        };
+        for (int i = 0; i < kFamilies; ++i) {
+            const int f = order_of[i];
+            const bool two = family_order(f) == 2;
+            if (!active(f) || (fuse && f == FAM_MBPOL_2B)) continue;
+            const int cap = fam_cap(L, f);
+            // with stage timing on, everything runs on the main stream so each stage is timed alone
+            hipStream_t sk = (two && !ev) ? c->stream2 : st;
+            EvalState& o = c->fam[f];
+            if (!buffers(f) || (fuse && f == FAM_MBPOL_3B && !buffers(FAM_MBPOL_2B)))
                 return fail(MBXG_ENOMEM, "evaluation buffers: device allocation failed");
             double* vir = batch ? o.vir_nmer.p : nullptr;
             const int* nd = fam_ndev(L, c->np, f);
+            if (fuse && f == FAM_MBPOL_3B) {
+                const int f2 = FAM_MBPOL_2B, cap2 = fam_cap(L, f2);
+                EvalState& o2 = c->fam[f2];
+                const int* nd2 = fam_ndev(L, c->np, f2);
+                if (ev) HIPCHK(hipEventRecord(ev[4], st));
+                HIPCHK(launch_mbpol2b_vars(L.evals[f2].p, nd2, cap2, N, o2.ws.p, st));
+                if (ev) HIPCHK(hipEventRecord(ev[5], st));
+                if (ev) HIPCHK(hipEventRecord(ev[8], st));
+                HIPCHK(launch_mbpol3b_vars(L.evals[f].p, nd, cap, N, o.ws.p, st));
+                if (ev) HIPCHK(hipEventRecord(ev[9], st));
+                HIPCHK(launch_poly_mbpol_pair(o.ws.p, nd, cap, c->d_coef3.p, o2.ws.p, nd2, cap2, c->d_coef2.p,
+                                              c->split3, st));
+                if (ev) HIPCHK(hipEventRecord(ev[10], st));
+                HIPCHK(launch_mbpol3b_chain(L.evals[f].p, nd, cap, N, 1, c->d_grad.p, o.e_nmer.p, vir, o.partials.p,
+                                            o.ws.p, st));
+                if (ev) HIPCHK(hipEventRecord(ev[11], st));
+                if (ev) HIPCHK(hipEventRecord(ev[6], st));
+                HIPCHK(launch_mbpol2b_chain(L.evals[f2].p, nd2, cap2, N, 1, c->d_grad.p, o2.e_nmer.p,
+                                            batch ? o2.vir_nmer.p : nullptr, o2.partials.p, o2.ws.p, st));
+                if (ev) HIPCHK(hipEventRecord(ev[7], st));
+                reduce(f, st);
+                reduce(f2, st);
+                HIPCHK(hipGetLastError());
+                continue;
+            }
             if (f == FAM_MBPOL_2B)
                 HIPCHK(launch_mbpol2b(L.evals[f].p, nd, cap, N, c->d_coef2.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
                                       o.partials.p, o.ws.p, c->split2, sk, ev ? ev + 4 : nullptr));
@@ -905,16 +959,7 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
             else
                 HIPCHK(launch_nrg_family(f, L.evals[f].p, nd, cap, N, c->d_coefn[f].p, do_grads, c->d_grad.p,
                                          o.e_nmer.p, vir, o.partials.p, o.ws.p, c->split3n, sk, nullptr));
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
-                               c->d_fam_tot.p + kPartials * f);
-            if (batch) {
-                double* part = c->d_conf_part.p + (size_t)f * kConfSlices * kPartials * c->nconf;
-                hipLaunchKernelGGL(k_reduce_configs, dim3(c->nconf, kConfSlices), dim3(256), 0, sk, o.e_nmer.p, vir,
-                                   ld, f == FAM_MBPOL_2B || f == FAM_MBPOL_3B ? L.offsets.p : L.offsets_x.p,
-                                   c->d_pt_off.p, fam_component(f), cap, part);
-                hipLaunchKernelGGL(k_reduce_slices, dim3((c->nconf * kPartials + 255) / 256), dim3(256), 0, sk, part,
-                                   c->nconf, kConfSlices, c->d_conf_fam.p + (size_t)f * kPartials * c->nconf);
-            }
+            reduce(f, sk);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(c->join, c->stream2));
@@ -1058,6 +1103,7 @@ extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
         HIPCHK(hipEventElapsedTime(&t, c->ev[pairs[k][0]], c->ev[pairs[k][1]]));
         ms[k] = t;
     }
+    if (c->fused_last && n > 4) ms[4] = 0.0;  // the 2B polynomial ran inside the 3B launch
     return MBXG_OK;
 }
 
@@ -1072,12 +1118,15 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
     const std::string k(key);
     if ((k == "split_2b" || k == "split_3b" || k == "split_ch4h2o2_3b") && !valid_split(value))
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
+    if (k == "fuse_poly" && value != 0 && value != 1) return fail(MBXG_EINVAL, "mbxg_set_tuning: fuse_poly is 0 or 1");
     if (k == "split_2b")
         c->split2 = value;
     else if (k == "split_3b")
         c->split3 = value;
     else if (k == "split_ch4h2o2_3b")
         c->split3n = value;
+    else if (k == "fuse_poly")
+        c->fuse_poly = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1095,6 +1144,8 @@ extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
         *value = c->split3;
     else if (k == "split_ch4h2o2_3b")
         *value = c->split3n;
+    else if (k == "fuse_poly")
+        *value = c->fuse_poly;
     else
         return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
     return MBXG_OK;

@@ -257,24 +257,27 @@ size_t workspace_mbpol2b(int cap) {
     return (2 * 31 + 2) * ld;
 }
 
-hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
-                          int split, hipStream_t st, hipEvent_t* ev) {
+// The stages of launch_mbpol2b on its own, for callers that run the polynomial stage
+// elsewhere (launch_poly_mbpol_pair): variables and switch into ws, and the chain rule from ws.
+hipError_t launch_mbpol2b_vars(const int* list, const int* n_dev, int cap, const NmerCtx& N, double* ws,
+                               hipStream_t st) {
     if (cap <= 0) return hipSuccess;
     const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
     double* xs = ws;
-    double* gs = xs + (size_t)31 * ld;
+    double* sw = xs + (size_t)(2 * 31 + 1) * ld;
+    hipLaunchKernelGGL(k_vars_mbpol2b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_mbpol2b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
+                                double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                                hipStream_t st) {
+    if (cap <= 0) return hipSuccess;
+    const int ld = (cap + 63) / 64 * 64;
+    double* gs = ws + (size_t)31 * ld;
     double* ps = gs + (size_t)31 * ld;
     double* sw = ps + ld;
-    if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_vars_mbpol2b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol2b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
-    if (e != hipSuccess) return e;
-    if (ev) (void)hipEventRecord(ev[2], st);
     const int ngb = (cap + kGradBlock2 - 1) / kGradBlock2;
     if (do_grads)
         hipLaunchKernelGGL(k_grad_mbpol2b<true>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps, sw,
@@ -282,7 +285,24 @@ hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const Nmer
     else
         hipLaunchKernelGGL(k_grad_mbpol2b<false>, dim3(ngb), dim3(kGradBlock2), 0, st, list, n_dev, cap, N, gs, ps,
                            sw, ld, grad, e_out, vir_out, partials);
-    e = hipGetLastError();
+    return hipGetLastError();
+}
+
+hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev) {
+    if (cap <= 0) return hipSuccess;
+    const int ld = (cap + 63) / 64 * 64;
+    double* gs = ws + (size_t)31 * ld;
+    double* ps = gs + (size_t)31 * ld;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipError_t e = launch_mbpol2b_vars(list, n_dev, cap, N, ws, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], st);
+    e = launch_poly_mbpol2b(ws, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[2], st);
+    e = launch_mbpol2b_chain(list, n_dev, cap, N, do_grads, grad, e_out, vir_out, partials, ws, st);
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;
 }

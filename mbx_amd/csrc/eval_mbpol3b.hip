@@ -218,24 +218,28 @@ size_t workspace_mbpol3b(int cap) {
     return (2 * 36 + 2) * ld;
 }
 
-hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
-                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
-                          int split, hipStream_t st, hipEvent_t* ev) {
+// The stages of launch_mbpol3b on its own, for callers that run the polynomial stage
+// elsewhere (launch_poly_mbpol_pair): variables and switch into ws, and the chain rule from ws.
+hipError_t launch_mbpol3b_vars(const int* list, const int* n_dev, int cap, const NmerCtx& N, double* ws,
+                               hipStream_t st) {
     if (cap <= 0) return hipSuccess;
     const int nb = (cap + 63) / 64;
     const int ld = nb * 64;
     double* xs = ws;
+    double* sw = xs + (size_t)(2 * 36 + 1) * ld;
+    hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_mbpol3b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
+                                double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                                hipStream_t st) {
+    if (cap <= 0) return hipSuccess;
+    const int ld = (cap + 63) / 64 * 64;
+    double* xs = ws;
     double* gs = xs + (size_t)36 * ld;
     double* ps = gs + (size_t)36 * ld;
     double* sw = ps + ld;
-    if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (ev) (void)hipEventRecord(ev[1], st);
-    e = launch_poly_mbpol3b(xs, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
-    if (e != hipSuccess) return e;
-    if (ev) (void)hipEventRecord(ev[2], st);
     const int ngb = (cap + kGradBlock3 - 1) / kGradBlock3;
     if (do_grads)
         hipLaunchKernelGGL(k_grad_mbpol3b<true>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs, ps,
@@ -243,7 +247,24 @@ hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const Nmer
     else
         hipLaunchKernelGGL(k_grad_mbpol3b<false>, dim3(ngb), dim3(kGradBlock3), 0, st, list, n_dev, cap, N, xs, gs,
                            ps, sw, ld, grad, e_out, vir_out, partials);
-    e = hipGetLastError();
+    return hipGetLastError();
+}
+
+hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
+                          int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                          int split, hipStream_t st, hipEvent_t* ev) {
+    if (cap <= 0) return hipSuccess;
+    const int ld = (cap + 63) / 64 * 64;
+    double* gs = ws + (size_t)36 * ld;
+    double* ps = gs + (size_t)36 * ld;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipError_t e = launch_mbpol3b_vars(list, n_dev, cap, N, ws, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], st);
+    e = launch_poly_mbpol3b(ws, n_dev, cap, ld, coef, do_grads, gs, ps, split, st);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[2], st);
+    e = launch_mbpol3b_chain(list, n_dev, cap, N, do_grads, grad, e_out, vir_out, partials, ws, st);
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;
 }

@@ -153,6 +153,22 @@ hipError_t launch_mbpol2b(const int* list, const int* n_dev, int cap, const Nmer
 hipError_t launch_mbpol3b(const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
                           int do_grads, double* grad, double* e_out, double* vir_out, double* partials, double* ws,
                           int split, hipStream_t st, hipEvent_t* ev);
+// The variable and chain-rule stages alone (same workspace layout), and both MB-pol polynomial
+// stages with gradients in one launch (poly_mbpol_pair.hip): the 3B blocks first, then the 2B
+// ones, `split` wavefronts per 64 n-mers for both, so the short 2B blocks fill the machine while
+// the last 3B blocks drain.
+hipError_t launch_mbpol2b_vars(const int* list, const int* n_dev, int cap, const NmerCtx& N, double* ws,
+                               hipStream_t st);
+hipError_t launch_mbpol3b_vars(const int* list, const int* n_dev, int cap, const NmerCtx& N, double* ws,
+                               hipStream_t st);
+hipError_t launch_mbpol2b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
+                                double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                                hipStream_t st);
+hipError_t launch_mbpol3b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
+                                double* grad, double* e_out, double* vir_out, double* partials, double* ws,
+                                hipStream_t st);
+hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
+                                  const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);
 // MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and
 // FAM_CH4H2O2_3B; list entries in emission order (the kernels order each n-mer's monomers
 // for its holder).  `split` (1, 2, 4 or 8) is the wavefronts per 64 n-mers of the blocked

@@ -29,18 +29,16 @@ typedef double (*poly_e_part_fn)(int, int, mbxg_cptr, const double*);
 constexpr int kRedChunk = 12;
 constexpr size_t poly_lds_bytes(int P) { return (size_t)(P - 1) * (kRedChunk + 1) * 64 * sizeof(double); }
 
-// Energy and gradient, WPE waves per SIMD: 3 for MB-pol (at most 168 VGPRs), 2 for the
-// 55-variable CH4-H2O-H2O polynomial (x and g alone take 220 VGPRs).
-template <int NV, int Q, poly_eg_part_fn PART, int WPE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    k_poly_eg(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld, uint64_t coef_addr,
-              double* __restrict__ gs, double* __restrict__ ps) {
-    extern __shared__ double red[];  // [P-1][kRedChunk+1][64]
+// One block's work: 64 n-mers of group `grp`, P = blockDim.x / 64 wavefronts.
+template <int NV, int Q, poly_eg_part_fn PART>
+__device__ __forceinline__ void poly_eg_group(int grp, const double* __restrict__ xs, const int* __restrict__ n_dev,
+                                              int cap, int ld, uint64_t coef_addr, double* __restrict__ gs,
+                                              double* __restrict__ ps, double* red) {
     const int P = blockDim.x >> 6;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = blockIdx.x * 64 + lane;
+    const int t = grp * 64 + lane;
     const int n = min(*n_dev, cap);
-    if (blockIdx.x * 64 >= n) return;  // the whole block is past the end (uniform per block)
+    if (grp * 64 >= n) return;  // the whole block is past the end (uniform per block)
     const bool active = t < n;
     const mbxg_cptr C = (mbxg_cptr)coef_addr;
     double x[NV], g[NV];
@@ -76,6 +74,38 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE, W
         for (int j = 0; j < NV; ++j) gs[(size_t)j * ld + t] = g[j];
         ps[t] = E;
     }
+}
+
+// Energy and gradient, WPE waves per SIMD: 3 for MB-pol (at most 168 VGPRs), 2 for the
+// 55-variable CH4-H2O-H2O polynomial (x and g alone take 220 VGPRs).
+template <int NV, int Q, poly_eg_part_fn PART, int WPE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+    k_poly_eg(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld, uint64_t coef_addr,
+              double* __restrict__ gs, double* __restrict__ ps) {
+    extern __shared__ double red[];  // [P-1][kRedChunk+1][64]
+    poly_eg_group<NV, Q, PART>(blockIdx.x, xs, n_dev, cap, ld, coef_addr, gs, ps, red);
+}
+
+// Two polynomials in one launch: blocks [0, nb_a) evaluate polynomial A (groups of 64 n-mers in
+// order), the rest polynomial B.  Blocks are dispatched in index order, so B's blocks take the
+// slots that A's last blocks leave idle.  Both use blockDim.x / 64 wavefronts per group.
+struct PolyJob {
+    const double* xs;
+    const int* n_dev;
+    int cap, ld;
+    uint64_t coef_addr;
+    double* gs;
+    double* ps;
+};
+
+template <int NVA, int QA, poly_eg_part_fn PA, int NVB, int QB, poly_eg_part_fn PB>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3)))
+    k_poly_eg_pair(PolyJob A, PolyJob B, int nb_a) {
+    extern __shared__ double red[];
+    if ((int)blockIdx.x < nb_a)
+        poly_eg_group<NVA, QA, PA>(blockIdx.x, A.xs, A.n_dev, A.cap, A.ld, A.coef_addr, A.gs, A.ps, red);
+    else
+        poly_eg_group<NVB, QB, PB>(blockIdx.x - nb_a, B.xs, B.n_dev, B.cap, B.ld, B.coef_addr, B.gs, B.ps, red);
 }
 
 // Energy only.

@@ -62,3 +62,32 @@ def test_one_pass_equals_two_pass(graph):
     fx = oracle.fix_first(xyz, box)
     assert np.array_equal(sorted_tuples(first[4]), sorted_tuples(oracle.clusters(2, c2, fx, box)))
     assert np.array_equal(sorted_tuples(first[5]), sorted_tuples(oracle.clusters(3, c3, fx, box)))
+
+
+def test_paired_polynomials_match_separate_launches():
+    """The MB-pol 3B and 2B polynomials in one launch (fuse_poly = 1, the default) against
+    separate launches: the same per-n-mer schedules, so energies and gradients agree to rounding
+    (the 2B parts are summed over split_3b wavefronts instead of split_2b: gradients of
+    cancelling terms differ by a few 1e-10); both meet the oracle's bar."""
+    xyz, box, c2, c3 = configs.build(configs.C1)
+    out = {}
+    for fuse in (1, 0):
+        print("fuse_poly", fuse, flush=True)
+        e = engine.water_engine(configs.C1["nmon"], box, c2, c3)
+        e.set_tuning("fuse_poly", fuse)
+        assert e.tuning("fuse_poly") == fuse
+        e.set_coords(xyz)
+        e.compute()
+        e.set_timing(True)
+        e.compute()
+        tm = e.timings()
+        assert (tm["poly_2b"] == 0.0) == (fuse == 1) and tm["poly_3b"] > 0.0
+        out[fuse] = e.results()
+    (a2, a3, ag, av), (b2, b3, bg, bv) = out[1], out[0]
+    assert a2 == pytest.approx(b2, rel=1e-12) and a3 == pytest.approx(b3, rel=1e-12)
+    np.testing.assert_allclose(ag, bg, rtol=0, atol=5e-9)
+    np.testing.assert_allclose(av, bv, rtol=1e-10, atol=1e-8)
+    r = oracle.water_system(xyz, box, c2, c3)
+    for e2, e3, g in ((a2, a3, ag), (b2, b3, bg)):
+        assert e2 == pytest.approx(r["e2b"], rel=1e-10) and e3 == pytest.approx(r["e3b"], rel=1e-10)
+        assert np.abs(g - r["grad"]).max() <= 1e-8
