@@ -70,6 +70,14 @@ enum {
 };
 int mbxg_set_families(mbxg_ctx* ctx, int mask);
 
+/* Local/ghost decomposition (the LAMMPS path): with on = 1 the neighbour search holds every
+ * monomer, ghosts (islocal = 0 in mbxg_set_topology) included, and n-mers are still found only
+ * from local monomers, so local-ghost n-mers are evaluated once and ghost-only ones never; with
+ * on = 0 (default) the ghosts are left out of the search.  Replaces the use_ghost argument of
+ * System::TwoBodyEnergy / ThreeBodyEnergy / GetPairList (bblock/system.h:1088,1100,246), which
+ * reaches systools::AddClusters (bblock/sys_tools.cpp:472-537). */
+int mbxg_set_use_ghost(mbxg_ctx* ctx, int on);
+
 /* Batches: split the monomers (in topology order) into nconf independent configurations,
  * configuration c holding monomers [mon_offsets[c], mon_offsets[c+1]) (mon_offsets[0] = 0,
  * mon_offsets[nconf] = nmon).  Lists never cross configurations; one compute evaluates all of

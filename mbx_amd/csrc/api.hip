@@ -169,6 +169,7 @@ struct mbxg_ctx {
     int g_terms = -1, g_grads = -1;       // what gexec computes
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
+    int use_ghost = 0;                    // the cloud holds ghost monomers too (mbxg_set_use_ghost)
     int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
@@ -292,6 +293,19 @@ namespace {
 
 // Split the monomers into configurations (CSR offsets over monomers; nullptr = one).  Cloud
 // points keep monomer order, so each configuration's points are a contiguous range.
+// Cloud members (AddClusters, sys_tools.cpp:502-515): every monomer with use_ghost, the local
+// ones otherwise; queries are the local members either way (sys_tools.cpp:537).
+int set_members(mbxg_ctx* c) {
+    c->members.clear();
+    for (int i = 0; i < c->nmon; ++i)
+        if (c->use_ghost || c->islocal[i]) c->members.push_back(i);
+    c->np = (int)c->members.size();
+    if (c->d_members.reserve(c->nmon > 0 ? c->nmon : 1)) return fail(MBXG_ENOMEM, "members: device allocation failed");
+    if (c->np > 0)
+        HIPCHK(hipMemcpy(c->d_members.p, c->members.data(), sizeof(int) * c->np, hipMemcpyHostToDevice));
+    return MBXG_OK;
+}
+
 int set_configs(mbxg_ctx* c, int nconf, const int* mon_off) {
     std::vector<int> off;
     if (!mon_off || nconf <= 1) {
@@ -354,11 +368,6 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     c->natoms = acc;
     c->n_h2o = (int)std::count(kind, kind + nmon, MBXG_H2O);
     c->n_ch4 = (int)std::count(kind, kind + nmon, MBXG_CH4);
-    // cloud members: every local monomer (AddClusters with use_ghost = false, sys_tools.cpp:508)
-    c->members.clear();
-    for (int i = 0; i < nmon; ++i)
-        if (c->islocal[i]) c->members.push_back(i);
-    c->np = (int)c->members.size();
     const size_t nm = nmon > 0 ? nmon : 1;
     if (c->d_kind.reserve(nm) || c->d_nat.reserve(nm) || c->d_first.reserve(nm) || c->d_tag.reserve(nm) ||
         c->d_members.reserve(nm) || c->d_islocal.reserve(nm) || c->d_in.reserve(3 * (size_t)acc + 3) ||
@@ -372,8 +381,7 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
         HIPCHK(hipMemcpy(c->d_tag.p, c->tag.data(), sizeof(int) * nmon, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_islocal.p, c->islocal.data(), nmon, hipMemcpyHostToDevice));
     }
-    if (c->np > 0)
-        HIPCHK(hipMemcpy(c->d_members.p, c->members.data(), sizeof(int) * c->np, hipMemcpyHostToDevice));
+    if (int r = set_members(c)) return r;
     if (int r = set_configs(c, 1, nullptr)) return r;
     c->have_coords = false;
     c->have_results = false;
@@ -387,6 +395,23 @@ extern "C" int mbxg_set_families(mbxg_ctx* c, int mask) {
     c->fam_mask = mask;
     c->need_sync = true;
     c->inputs_changed = true;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_set_use_ghost(mbxg_ctx* c, int on) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_set_use_ghost: null context");
+    on = on ? 1 : 0;
+    if (on == c->use_ghost) return MBXG_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->use_ghost = on;
+    if (int r = set_members(c)) return r;
+    // configuration ranges over the new cloud (needs a new list layout and a synchronised compute)
+    if (int r = set_configs(c, c->nconf, c->nconf > 1 ? c->mon_off.data() : nullptr)) return r;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_terms = c->g_grads = -1;
+    c->eager_terms = c->eager_grads = -1;
     return MBXG_OK;
 }
 

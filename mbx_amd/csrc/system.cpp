@@ -214,8 +214,8 @@ void System::check_initialized(const char* func, int line) {
 }
 
 double System::run_terms(int terms, bool do_grads, bool use_ghost) {
-    if (use_ghost)
-        MBX_THROW("use_ghost (local/ghost monomer decomposition) is not supported by the GPU engine");
+    // use_ghost: local-ghost n-mers from local query monomers (sys_tools.cpp:502-537)
+    check(mbxg_set_use_ghost(ctx_, use_ghost ? 1 : 0));
     std::fill(grad_.begin(), grad_.end(), 0.0);
     std::fill(virial_.begin(), virial_.end(), 0.0);
     // ignore_2b_poly / ignore_3b_poly (system.cpp:2606-2619, 2940-2953): those n-mers are listed
@@ -314,7 +314,7 @@ std::vector<size_t> System::GetPairList(size_t nmax, double cutoff, size_t istar
     if (nmax != 2 && nmax != 3)
         MBX_THROW("nmax value of " + std::to_string(nmax) + " is not acceptable. Possible values are 2 or 3.");
     check_initialized(__func__, __LINE__);
-    if (use_ghost) MBX_THROW("use_ghost is not supported by the GPU engine");
+    check(mbxg_set_use_ghost(ctx_, use_ghost ? 1 : 0));
     check(mbxg_set_cutoffs(ctx_, nmax == 2 ? cutoff : 0.0, nmax == 3 ? cutoff : 0.0));
     check(mbxg_set_coords(ctx_, xyz_.data()));
     check(mbxg_compute(ctx_, (nmax == 2 ? MBXG_2B : MBXG_3B) | MBXG_LISTS_ONLY, 0));

@@ -49,6 +49,7 @@ PROTOTYPES = {
     "mbxg_synchronize": (ctypes.c_int, [_vp]),
     "mbxg_set_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "mbxg_set_families": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mbxg_set_use_ghost": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
     "mbxg_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
@@ -131,6 +132,11 @@ class Engine:
     def set_families(self, families=FAMILIES):
         """Evaluate only these n-mer families (names as FAMILIES); the others stay listed."""
         _check(lib().mbxg_set_families(self._c, sum(1 << i for i, f in enumerate(FAMILIES) if f in families)))
+
+    def set_use_ghost(self, on=True):
+        """Neighbour search over local and ghost monomers, n-mers found from local ones only
+        (see mbxg_set_use_ghost)."""
+        _check(lib().mbxg_set_use_ghost(self._c, int(bool(on))))
 
     def set_batch(self, mon_offsets):
         """Split the monomers into configurations: configuration c = monomers

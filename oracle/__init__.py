@@ -442,21 +442,21 @@ def ref():
     return _REF
 
 
-def ref_add_clusters(nmax, cutoff, first_xyz, box=None):
+def ref_add_clusters(nmax, cutoff, first_xyz, box=None, islocal=None, use_ghost=False):
     first_xyz = _arr(first_xyz).reshape(-1)
     nmon = len(first_xyz) // 3
     fi = np.arange(nmon, dtype=np.uintp)
-    loc = np.ones(nmon, np.uintp)
+    loc = np.ones(nmon, np.uintp) if islocal is None else np.ascontiguousarray(islocal, np.uintp)
     tag = np.arange(nmon, dtype=np.int32)
     b = _arr(box) if box is not None else None
     bi = invert_unit_cell(b) if b is not None else None
     sz = lambda a: a.ctypes.data_as(_szp)
     L = ref()
     n = L.ref_add_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), nmon, sz(fi),
-                           sz(loc), _i(tag), 0, None, 0)
+                           sz(loc), _i(tag), int(use_ghost), None, 0)
     out = np.zeros((max(n, 1), nmax), np.uintp)
     L.ref_add_clusters(nmax, cutoff, nmon, int(b is not None), _d(b), _d(bi), _d(first_xyz), nmon, sz(fi), sz(loc),
-                       _i(tag), 0, sz(out), n)
+                       _i(tag), int(use_ghost), sz(out), n)
     return out[:n].astype(np.int64)
 
 

@@ -260,6 +260,38 @@ def test_lists_vs_compiled_reference(systems, case):
         assert len(got) == len(want) and np.array_equal(got, want)
 
 
+def _ghost_case():
+    """A gas-phase water cluster, monomers on one side of a plane marked ghost."""
+    from mbx_amd import configs
+
+    xyz, _ = configs.water_box(120, 16.0, (5, 5, 5), seed=31)
+    first = xyz.reshape(-1, 9)[:, :3]
+    islocal = (first[:, 0] < np.median(first[:, 0])).astype(np.int32)
+    return first, islocal
+
+
+def test_ghost_lists_are_the_local_queries_lists():
+    """With use_ghost the search holds every monomer and only local ones query, so the lists are
+    the all-local lists' n-mers found from a local monomer (the first of each tuple); without
+    use_ghost the ghosts are not searched at all (sys_tools.cpp:502-537)."""
+    first, islocal = _ghost_case()
+    for nmax, c in ((2, 7.0), (3, 5.0)):
+        full = oracle.clusters(nmax, c, first, None)
+        got = oracle.clusters(nmax, c, first, None, islocal=islocal, use_ghost=True)
+        want = full[islocal[full[:, 0]] == 1]
+        assert len(got) > 0 and np.array_equal(sorted_tuples(got), sorted_tuples(want))
+        assert np.all(islocal[got[:, 0]] == 1) and np.any(islocal[got] == 0)
+
+
+@ref_only
+def test_ghost_lists_vs_compiled_reference():
+    first, islocal = _ghost_case()
+    for nmax, c in ((2, 7.0), (3, 5.0)):
+        got = oracle.clusters(nmax, c, first, None, islocal=islocal, use_ghost=True)
+        want = oracle.ref_add_clusters(nmax, c, first, None, islocal=islocal, use_ghost=True)
+        assert len(got) == len(want) and np.array_equal(sorted_tuples(got), sorted_tuples(want))
+
+
 # ---------------------------------------------------------------------------------------------
 # system totals
 
