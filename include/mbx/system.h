@@ -47,6 +47,7 @@ class System {
 
     // system.h:1057, 1088, 1100
     double Energy(bool do_grads);
+    double OneBodyEnergy(bool do_grads);
     double TwoBodyEnergy(bool do_grads, bool use_ghost = 0);
     double ThreeBodyEnergy(bool do_grads, bool use_ghost = 0);
 

@@ -78,6 +78,14 @@ int mbxg_set_families(mbxg_ctx* ctx, int mask);
  * reaches systools::AddClusters (bblock/sys_tools.cpp:472-537). */
 int mbxg_set_use_ghost(mbxg_ctx* ctx, int on);
 
+/* MB-pol 1B: the Partridge-Schwenke energy of every local water monomer at the current
+ * coordinates (after FixMonomerCoordinates), ps::pot_nasa (potential/1b/ps.cpp:346) as reached
+ * through System::OneBodyEnergy / Get1B (bblock/system.cpp:2140-2158, 2269-2401).  e1b receives
+ * the total (kcal/mol, summed in monomer order); with do_grads, grad (3 doubles per atom,
+ * nullable) is overwritten with the 1B gradient and virial (9, nullable) with the 1B virial.
+ * Synchronous.  Only water monomers have a 1B term here (other kinds: MBXG_EINVAL). */
+int mbxg_eval_1b(mbxg_ctx* ctx, int do_grads, double* e1b, double* grad, double* virial);
+
 /* Batches: split the monomers (in topology order) into nconf independent configurations,
  * configuration c holding monomers [mon_offsets[c], mon_offsets[c+1]) (mon_offsets[0] = 0,
  * mon_offsets[nconf] = nmon).  Lists never cross configurations; one compute evaluates all of

@@ -170,6 +170,9 @@ struct mbxg_ctx {
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
     int use_ghost = 0;                    // the cloud holds ghost monomers too (mbxg_set_use_ghost)
+    std::vector<int> water_local;         // local water monomers (MB-pol 1B, mbxg_eval_1b)
+    DBuf<int> d_w1b;
+    DBuf<double> d_e1b, d_v1b, d_g1b;
     int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
@@ -383,6 +386,13 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     }
     if (int r = set_members(c)) return r;
     if (int r = set_configs(c, 1, nullptr)) return r;
+    c->water_local.clear();
+    for (int i = 0; i < nmon; ++i)
+        if (c->kind[i] == MBXG_H2O && c->islocal[i]) c->water_local.push_back(i);
+    if (c->d_w1b.reserve(nm)) return fail(MBXG_ENOMEM, "mbxg_set_topology: device allocation failed");
+    if (!c->water_local.empty())
+        HIPCHK(hipMemcpy(c->d_w1b.p, c->water_local.data(), sizeof(int) * c->water_local.size(),
+                         hipMemcpyHostToDevice));
     c->have_coords = false;
     c->have_results = false;
     c->need_sync = true;
@@ -414,6 +424,7 @@ extern "C" int mbxg_set_use_ghost(mbxg_ctx* c, int on) {
     c->eager_terms = c->eager_grads = -1;
     return MBXG_OK;
 }
+
 
 extern "C" int mbxg_set_batch(mbxg_ctx* c, int nconf, const int* mon_offsets) {
     if (!c || nconf < 1 || (nconf > 1 && !mon_offsets)) return fail(MBXG_EINVAL, "mbxg_set_batch: bad arguments");
@@ -1351,4 +1362,53 @@ extern "C" int mbxg_stream(mbxg_ctx* c, void** stream) {
 extern "C" int mbxg_synchronize(mbxg_ctx* c) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_synchronize: null context");
     return finish(c);
+}
+
+extern "C" int mbxg_eval_1b(mbxg_ctx* c, int do_grads, double* e1b, double* grad, double* virial) {
+    if (!c || !e1b) return fail(MBXG_EINVAL, "mbxg_eval_1b: bad arguments");
+    if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_eval_1b: coordinates not set");
+    for (int i = 0; i < c->nmon; ++i)
+        if (c->kind[i] != MBXG_H2O)
+            return fail(MBXG_EINVAL, "mbxg_eval_1b: monomer %d: only h2o has a GPU 1B term", i);
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int n = (int)c->water_local.size();
+    const size_t na3 = 3 * (size_t)c->natoms;
+    if (c->d_e1b.reserve(n > 0 ? n : 1) || c->d_v1b.reserve(6 * (size_t)(n > 0 ? n : 1)) ||
+        c->d_g1b.reserve(na3 > 0 ? na3 : 1))
+        return fail(MBXG_ENOMEM, "mbxg_eval_1b: device allocation failed");
+    // FixMonomerCoordinates of the current input, as a compute does (the energy is intramolecular;
+    // the virial takes the wrapped coordinates, as ps::pot_nasa takes System::xyz_)
+    if (c->nmon > 0)
+        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p, c->d_xyz.p,
+                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
+    if (do_grads && na3 > 0) launch_zero(c->d_g1b.p, na3, st);
+    HIPCHK(launch_1b_ps(c->d_xyz.p, c->d_first.p, c->d_w1b.p, n, c->d_e1b.p, do_grads ? c->d_g1b.p : nullptr,
+                        do_grads ? c->d_v1b.p : nullptr, st));
+    std::vector<double> e(n), v(do_grads ? 6 * (size_t)n : 0);
+    if (n > 0) HIPCHK(hipMemcpyAsync(e.data(), c->d_e1b.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (do_grads && n > 0)
+        HIPCHK(hipMemcpyAsync(v.data(), c->d_v1b.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
+    if (do_grads && grad && na3 > 0)
+        HIPCHK(hipMemcpyAsync(grad, c->d_g1b.p, sizeof(double) * na3, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // totals in monomer order (deterministic)
+    double tot = 0.0;
+    for (int t = 0; t < n; ++t) tot += e[t];
+    *e1b = tot;
+    if (virial) {
+        const int slot[6] = {0, 1, 2, 4, 5, 8};
+        for (int q = 0; q < 9; ++q) virial[q] = 0.0;
+        if (do_grads)
+            for (int q = 0; q < 6; ++q) {
+                double s = 0.0;
+                for (int t = 0; t < n; ++t) s += v[(size_t)q * n + t];
+                virial[slot[q]] = s;
+            }
+        virial[3] = virial[1];
+        virial[6] = virial[2];
+        virial[7] = virial[5];
+    }
+    if (!do_grads && grad && na3 > 0) std::fill(grad, grad + na3, 0.0);
+    return MBXG_OK;
 }

@@ -167,6 +167,10 @@ hipError_t launch_mbpol2b_chain(const int* list, const int* n_dev, int cap, cons
 hipError_t launch_mbpol3b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
                                 double* grad, double* e_out, double* vir_out, double* partials, double* ws,
                                 hipStream_t st);
+// MB-pol 1B (eval_1b.hip): energies e_out[t] of waters mons[0..n), and with g_out the 9 gradient
+// components of each (3 doubles per atom, written at its atoms) and vir_out[q * n + t].
+hipError_t launch_1b_ps(const double* xyz, const int* first_atom, const int* mons, int n, double* e_out,
+                        double* g_out, double* vir_out, hipStream_t st);
 hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
                                   const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);
 // MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and

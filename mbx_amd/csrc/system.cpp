@@ -249,6 +249,19 @@ double System::Energy(bool do_grads) {
     return run_terms(MBXG_2B | MBXG_3B, do_grads, false);
 }
 
+// System::OneBodyEnergy (bblock/system.cpp:2140-2158): the MB-pol 1B energy of the local waters
+// (Get1B, :2269-2401), gradients into the internal-order grad_ and the virial.
+double System::OneBodyEnergy(bool do_grads) {
+    check_initialized(__func__, __LINE__);
+    std::fill(grad_.begin(), grad_.end(), 0.0);
+    std::fill(virial_.begin(), virial_.end(), 0.0);
+    check(mbxg_set_coords(ctx_, xyz_.data()));
+    double e1 = 0.0;
+    check(mbxg_eval_1b(ctx_, do_grads ? 1 : 0, &e1, do_grads ? grad_.data() : nullptr, virial_.data()));
+    check(mbxg_get_coords(ctx_, xyz_.data()));  // SetPBC wraps the working coordinates
+    return e1;
+}
+
 double System::TwoBodyEnergy(bool do_grads, bool use_ghost) {
     check_initialized(__func__, __LINE__);
     return run_terms(MBXG_2B, do_grads, use_ghost);

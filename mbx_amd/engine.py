@@ -50,6 +50,7 @@ PROTOTYPES = {
     "mbxg_set_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "mbxg_set_families": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_set_use_ghost": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mbxg_eval_1b": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
     "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
     "mbxg_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
@@ -137,6 +138,16 @@ class Engine:
         """Neighbour search over local and ghost monomers, n-mers found from local ones only
         (see mbxg_set_use_ghost)."""
         _check(lib().mbxg_set_use_ghost(self._c, int(bool(on))))
+
+    def eval_1b(self, do_grads=True):
+        """MB-pol 1B energy of the local waters at the current coordinates: (E1B, gradient per
+        atom or None, virial[9]) (see mbxg_eval_1b)."""
+        e = ctypes.c_double()
+        g = np.zeros(3 * self.natoms) if do_grads else None
+        v = np.zeros(9)
+        _check(lib().mbxg_eval_1b(self._c, int(bool(do_grads)), ctypes.byref(e), _d(g) if do_grads else None,
+                                  _d(v)))
+        return e.value, g, v
 
     def set_batch(self, mon_offsets):
         """Split the monomers into configurations: configuration c = monomers

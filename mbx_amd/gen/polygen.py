@@ -703,6 +703,46 @@ def generate_ad(name, out_dir):
     return dict(parts=AD_PARTS, slices=len(sl))
 
 
+def generate_ps(out_dir):
+    """gen/ps_1b.inc: the MB-pol 1B (Partridge-Schwenke) data of mbx_amd/tables/ps_1b.json as
+    constants, the derived ones evaluated in the reference's operation order (c5z: ps.cpp:445-447;
+    deoh, phh1: :450-451; cm-1 -> kcal/mol: constants.h:47-61), and the four coefficient sums of
+    ps.cpp:505-526 as straight-line code in the reference's order (no contraction)."""
+    import json
+    import math
+
+    d = json.load(open(os.path.join(TABLES, "ps_1b.json")))
+    c5z = [d["f5z"] * d["c5zA"][j] + d["fbasis"] * d["cbasis"][j] + d["fcore"] * d["ccore"][j] + d["frest"] * d["crest"][j]
+           for j in range(245)]
+    consts = {
+        "kReoh": d["reoh"], "kB1": d["b1"], "kRoh": d["roh"], "kAlphaoh": d["alphaoh"],
+        "kDeoh": d["f5z"] * d["deohA"], "kPhh1": d["f5z"] * d["phh1A"] * math.exp(d["phh2"]), "kPhh2": d["phh2"],
+        "kCosthe": d["costhe"], "kCorrection": d["correction"],
+        "kCm1": (d["Eh_J"] * d["Na"] / d["kcal_J"]) / (1.0e-2 * d["Eh_J"] / (d["c0"] * d["h_Js"])),
+        "kC5z0": c5z[0],
+    }
+    out = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/ps_1b.json -- do not edit.",
+           "#pragma once", "", "namespace ps1b {", ""]
+    for k, v in consts.items():
+        out.append("constexpr double %s = %s;" % (k, repr(float(v))))
+    out += ["", "// s0..s3 of ps.cpp:505-526 from the power tables f0, f1, f2 (fmat[0..2][0..15])",
+            "__device__ __forceinline__ void sums(const double* f0, const double* f1, const double* f2, double& s0,",
+            "                                     double& s1, double& s2, double& s3) {"]
+    for j in range(1, 245):
+        I, J, K = d["idx1"][j], d["idx2"][j], d["idx3"][j]
+        c = repr(float(c5z[j]))
+        out.append("    s0 += %s * (f0[%d] * f1[%d] + f0[%d] * f1[%d]) * f2[%d];" % (c, I, J, J, I, K))
+        out.append("    s1 += %s * (%r * f0[%d] * f1[%d] + %r * f0[%d] * f1[%d]) * f2[%d];" % (
+            c, float(I - 1), I - 1, J, float(J - 1), J - 1, I, K))
+        out.append("    s2 += %s * (%r * f0[%d] * f1[%d] + %r * f0[%d] * f1[%d]) * f2[%d];" % (
+            c, float(J - 1), I, J - 1, float(I - 1), J, I - 1, K))
+        out.append("    s3 += %s * (f0[%d] * f1[%d] + f0[%d] * f1[%d]) * %r * f2[%d];" % (c, I, J, J, I, float(K - 1), K - 1))
+    out += ["}", "", "}  // namespace ps1b", ""]
+    path = os.path.join(out_dir, "ps_1b.inc")
+    write_if_changed(path, "\n".join(out))
+    return path
+
+
 def main():
     out_dir = os.path.join(ROOT, "mbx_amd", "csrc", "gen")
     os.makedirs(out_dir, exist_ok=True)
@@ -717,6 +757,7 @@ def main():
     for name in NRG_BLOCKED:
         print(generate(name, "poly_" + name, out_dir))
         generate_holder(name, out_dir)
+    print(generate_ps(out_dir))
 
 
 if __name__ == "__main__":

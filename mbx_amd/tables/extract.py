@@ -15,7 +15,9 @@ What is extracted, per polynomial (MB-pol 2B/3B and the MB-nrg CH4 families):
     routines belongs to exactly one coefficient with weight 1; the script
     checks this and aborts otherwise.
 and per MB-nrg holder (``<name>.holder.json``): its variables as data (form, k, d, sites),
-its lone-pair sites and switching radii.
+its lone-pair sites and switching radii; and the data of the MB-pol 1B water potential
+(``ps_1b.json``: the Partridge-Schwenke constants and coefficient tables of
+src/potential/1b/ps.cpp).
 
 Usage:  python mbx_amd/tables/extract.py [/root/reference]
 """
@@ -257,9 +259,41 @@ def write_table(path, name, nvars, coef, poly):
     return len(terms)
 
 
+def parse_ps(text, consts_text):
+    """The Partridge-Schwenke water monomer potential's data (src/potential/1b/ps.cpp:46-89 and
+    the tables c5zA, cbasis, ccore, crest, idx1-3 at :61-295; the cm-1 -> kcal/mol constants of
+    src/tools/constants.h:43-61): scalars and arrays, as given in the text."""
+    src = re.sub(r"//[^\n]*", "", text)
+    out = {}
+    for name in ("reoh", "thetae", "b1", "roh", "alphaoh", "deohA", "phh1A", "phh2", "f5z", "fbasis", "fcore",
+                 "frest"):
+        m = re.search(r"const double %s\s*=\s*([^;]+);" % name, src)
+        out[name] = float(m.group(1))
+    for name, typ in (("idx1", int), ("idx2", int), ("idx3", int), ("c5zA", float), ("cbasis", float),
+                      ("ccore", float), ("crest", float)):
+        m = re.search(r"const (?:size_t|double) %s\[245\]\s*=\s*\{([^}]*)\}" % name, src)
+        vals = [typ(v.strip()) for v in m.group(1).split(",") if v.strip()]
+        assert len(vals) == 245, (name, len(vals))
+        out[name] = vals
+    m = re.search(r"const double costhe\s*=\s*([^;]+);", src)
+    out["costhe"] = float(m.group(1))
+    m = re.search(r"e1\[nv\]\s*\+=\s*([0-9.eE+-]+);", src)
+    out["correction"] = float(m.group(1))
+    csrc = re.sub(r"//[^\n]*", "", consts_text)
+    for name in ("Eh_J", "Na", "kcal_J", "c0", "h_Js"):
+        m = re.search(r"const double %s\s*=\s*([^;]+);" % name, csrc)
+        out[name] = float(m.group(1))
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     import json
+    ps = parse_ps(open(os.path.join(ref, "src/potential/1b/ps.cpp")).read(),
+                  open(os.path.join(ref, "src/tools/constants.h")).read())
+    with open(os.path.join(HERE, "ps_1b.json"), "w") as f:
+        json.dump(ps, f, indent=0)
+    print("ps_1b: %d coefficients" % len(ps["c5zA"]))
     for name, holder_path, mons, nats in HOLDERS:
         h = parse_holder(open(os.path.join(ref, holder_path)).read(), mons, nats)
         h["name"] = name

@@ -470,3 +470,74 @@ def ref_water_2b3b(xyz, box, cutoff2b, cutoff3b, batch=8, nthreads=1):
     ref().ref_water_2b3b(nmon, _d(xyz), _d(b), cutoff2b, cutoff3b, batch, nthreads, ctypes.byref(e2),
                          ctypes.byref(e3), _d(grad), _d(vir), counts.ctypes.data_as(_szp))
     return dict(e2b=e2.value, e3b=e3.value, grad=grad, virial=vir, n2=int(counts[0]), n3=int(counts[1]), xyz=xyz)
+
+
+# ---------------------------------------------------------------------------------------------
+# MB-pol 1B: the Partridge-Schwenke water monomer (potential/1b/ps.cpp:346-633)
+
+class _Ps(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("reoh", "b1", "roh", "alphaoh", "deoh", "phh1", "phh2", "costhe",
+                                               "correction", "cm1")] + \
+               [("idx1", ctypes.c_int * 245), ("idx2", ctypes.c_int * 245), ("idx3", ctypes.c_int * 245),
+                ("c5z", ctypes.c_double * 245)]
+
+
+_PS = None
+
+
+def ps_table():
+    """The 1B data of mbx_amd/tables/ps_1b.json in the reference's derived form: c5z
+    (ps.cpp:445-447), deoh and phh1 (:450-451) and the cm-1 -> kcal/mol factor (constants.h:47-61),
+    each evaluated in the reference's operation order."""
+    global _PS
+    if _PS is None:
+        import json
+        import math
+        d = json.load(open(os.path.join(ROOT, "mbx_amd", "tables", "ps_1b.json")))
+        p = _Ps()
+        for n in ("reoh", "b1", "roh", "alphaoh", "phh2", "costhe", "correction"):
+            setattr(p, n, d[n])
+        p.deoh = d["f5z"] * d["deohA"]
+        p.phh1 = d["f5z"] * d["phh1A"] * math.exp(d["phh2"])
+        eh_kcalmol = d["Eh_J"] * d["Na"] / d["kcal_J"]
+        eh_cm1 = 1.0e-2 * d["Eh_J"] / (d["c0"] * d["h_Js"])
+        p.cm1 = eh_kcalmol / eh_cm1
+        for j in range(245):
+            p.idx1[j], p.idx2[j], p.idx3[j] = d["idx1"][j], d["idx2"][j], d["idx3"][j]
+            p.c5z[j] = d["f5z"] * d["c5zA"][j] + d["fbasis"] * d["cbasis"][j] + d["fcore"] * d["ccore"][j] + \
+                d["frest"] * d["crest"][j]
+        L = lib()
+        L.orc_ps_1b.restype = ctypes.c_double
+        L.orc_ps_1b.argtypes = [ctypes.POINTER(_Ps), _dp, _dp, _dp]
+        _PS = p
+    return _PS
+
+
+def ps_1b(w, grad=True):
+    """MB-pol 1B energies of waters w [n][9] (O H1 H2): (E[n], G[n][9] or None, virial[6])."""
+    p = ps_table()
+    w = _arr(w).reshape(-1, 9)
+    e = np.zeros(len(w))
+    g = np.zeros_like(w) if grad else None
+    vir = np.zeros(6)
+    for i in range(len(w)):
+        wi = np.ascontiguousarray(w[i])
+        gi = np.zeros(9) if grad else None
+        e[i] = lib().orc_ps_1b(ctypes.byref(p), _d(wi), _d(gi) if grad else None, _d(vir) if grad else None)
+        if grad:
+            g[i] = gi
+    return e, g, vir
+
+
+def ref_ps_1b(w, virial=None):
+    """The compiled reference's ps::pot_nasa on waters w [n][9]: (E[n], G[n][9], virial[9])."""
+    L = ref()
+    L.ref_ps_1b.restype = None
+    L.ref_ps_1b.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp]
+    w = np.ascontiguousarray(_arr(w).reshape(-1))
+    nw = len(w) // 9
+    g = np.zeros(9 * nw)
+    e = np.zeros(nw)
+    v = np.zeros(9) if virial is None else np.array(virial, np.float64)
+    L.ref_ps_1b(_d(w), _d(g), nw, _d(e), _d(v))
+    return e, g.reshape(nw, 9), v

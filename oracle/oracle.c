@@ -1021,3 +1021,82 @@ int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* 
     free(first);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------------------------
+ * MB-pol 1B, ps::pot_nasa (potential/1b/ps.cpp:346-633), one water at a time.  The operation
+ * order of the reference is kept (no contraction): bond vectors and lengths (:393-433), cos of
+ * the HOH angle (:437-440), the Morse and H-H terms (:456-466), the polynomial in x1, x2, x3
+ * through the power tables fmat (:475-491) and the four sums (:505-526), energy (:533-539) and
+ * the chain rule to H1, H2 and O (:542-582), virial (:584-603). */
+double orc_ps_1b(const orc_ps* p, const double* w, double* g, double* vir6) {
+    double roh1[3], roh2[3], rhh[3];
+    for (int i = 0; i < 3; ++i) {
+        roh1[i] = w[3 + i] - w[i];
+        roh2[i] = w[6 + i] - w[i];
+        rhh[i] = w[3 + i] - w[6 + i];
+    }
+    double d1 = roh1[0] * roh1[0] + roh1[1] * roh1[1];
+    d1 = d1 + roh1[2] * roh1[2];
+    double d2 = roh2[0] * roh2[0] + roh2[1] * roh2[1];
+    d2 = d2 + roh2[2] * roh2[2];
+    double d3 = rhh[0] * rhh[0] + rhh[1] * rhh[1];
+    d3 = d3 + rhh[2] * rhh[2];
+    d1 = sqrt(d1);
+    d2 = sqrt(d2);
+    d3 = sqrt(d3);
+    const double costh = (roh1[0] * roh2[0] + roh1[1] * roh2[1] + roh1[2] * roh2[2]) / (d1 * d2);
+    const double exp1 = exp(-p->alphaoh * (d1 - p->roh));
+    const double exp2 = exp(-p->alphaoh * (d2 - p->roh));
+    const double Va = p->deoh * (exp1 * (exp1 - 2.0) + exp2 * (exp2 - 2.0));
+    const double Vb = p->phh1 * exp(-p->phh2 * d3);
+    const double dVa1 = 2.0 * p->alphaoh * p->deoh * exp1 * (1.0 - exp1) / d1;
+    const double dVa2 = 2.0 * p->alphaoh * p->deoh * exp2 * (1.0 - exp2) / d2;
+    const double dVb = -p->phh2 * Vb / d3;
+    const double x1 = (d1 - p->reoh) / p->reoh;
+    const double x2 = (d2 - p->reoh) / p->reoh;
+    const double x3 = costh - p->costhe;
+    double f[3][16];
+    for (int i = 0; i < 3; ++i) {
+        f[i][0] = 0.0;
+        f[i][1] = 1.0;
+    }
+    for (int j = 2; j < 16; ++j) {
+        f[0][j] = f[0][j - 1] * x1;
+        f[1][j] = f[1][j - 1] * x2;
+        f[2][j] = f[2][j - 1] * x3;
+    }
+    const double efac = exp(-p->b1 * (pow(d1 - p->reoh, 2) + pow(d2 - p->reoh, 2)));
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int j = 1; j < 245; ++j) {
+        const int I = p->idx1[j], J = p->idx2[j], K = p->idx3[j];
+        const double c = p->c5z[j];
+        s0 += c * (f[0][I] * f[1][J] + f[0][J] * f[1][I]) * f[2][K];
+        s1 += c * ((double)(I - 1) * f[0][I - 1] * f[1][J] + (double)(J - 1) * f[0][J - 1] * f[1][I]) * f[2][K];
+        s2 += c * ((double)(J - 1) * f[0][I] * f[1][J - 1] + (double)(I - 1) * f[0][J] * f[1][I - 1]) * f[2][K];
+        s3 += c * (f[0][I] * f[1][J] + f[0][J] * f[1][I]) * (double)(K - 1) * f[2][K - 1];
+    }
+    const double Vc = 2 * p->c5z[0] + efac * s0;
+    double e = Va + Vb + Vc;
+    e += p->correction;
+    e *= p->cm1;
+    if (g) {
+        const double dVcdr1 = (-2 * p->b1 * efac * (d1 - p->reoh) * s0 + efac * s1 / p->reoh) / d1;
+        const double dVcdr2 = (-2 * p->b1 * efac * (d2 - p->reoh) * s0 + efac * s2 / p->reoh) / d2;
+        const double dVcdcth = efac * s3;
+        for (int i = 0; i < 3; ++i) {
+            const double h1 = (dVa1 * roh1[i] + dVb * rhh[i] + dVcdr1 * roh1[i] +
+                               dVcdcth * (roh2[i] / (d1 * d2) - costh * roh1[i] / (d1 * d1))) * p->cm1;
+            const double h2 = (dVa2 * roh2[i] - dVb * rhh[i] + dVcdr2 * roh2[i] +
+                               dVcdcth * (roh1[i] / (d1 * d2) - costh * roh2[i] / (d2 * d2))) * p->cm1;
+            g[3 + i] = h1;
+            g[6 + i] = h2;
+            g[i] = -(h1 + h2);
+        }
+        if (vir6) {
+            const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2};
+            for (int q = 0; q < 6; ++q)
+                vir6[q] += -w[va[q]] * g[vb[q]] - w[3 + va[q]] * g[3 + vb[q]] - w[6 + va[q]] * g[6 + vb[q]];
+        }
+    }
+    return e;
+}

@@ -144,6 +144,19 @@ int orc_system(const orc_poly* p2, const orc_poly* p3, const orc_holder* const* 
                const double* box, double cutoff2b, double cutoff3b, double* e2b, double* e3b, double* grad,
                double* virial, long* counts);
 
+/* MB-pol 1B: the Partridge-Schwenke water monomer energy, ps::pot_nasa
+ * (potential/1b/ps.cpp:346-633) for one water w[9] (O H1 H2).  The data (ps.cpp:46-295,
+ * constants.h:43-61) come in orc_ps: c5z = f5z c5zA + fbasis cbasis + fcore ccore + frest crest
+ * (ps.cpp:445-447), deoh = f5z deohA, phh1 = f5z phh1A exp(phh2) (:450-451), cm1 the cm-1 ->
+ * kcal/mol factor.  Returns E (kcal/mol); g[9] (nullable) receives dE/dw, vir6 (nullable) is
+ * incremented by the reference's virial terms -sum_atoms w_a g_b (xx xy xz yy yz zz). */
+typedef struct {
+    double reoh, b1, roh, alphaoh, deoh, phh1, phh2, costhe, correction, cm1;
+    int idx1[245], idx2[245], idx3[245];
+    double c5z[245];
+} orc_ps;
+double orc_ps_1b(const orc_ps* p, const double* w, double* g, double* vir6);
+
 #ifdef __cplusplus
 }
 #endif

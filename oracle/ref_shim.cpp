@@ -20,6 +20,7 @@
 // System::Get2B / System::Get3B (bblock/system.cpp:2425-2725, 2769-3073) over
 // those leaves for a pure-water system; it is the reference-faithful CPU
 // baseline.
+#include "potential/1b/ps.h"
 #include <omp.h>
 
 #include <algorithm>
@@ -267,6 +268,16 @@ double ref_x3b_ch4h2o2(const double* c, const double* w1, const double* w2, size
     const double e = pot.eval(c, w1, w2, gc, g1, g2, n, &v);
     std::copy(v.begin(), v.end(), vir);
     return e;
+}
+
+// ps::pot_nasa (potential/1b/ps.cpp:346): energies of nw waters, gradient dr[9 nw] (nullable),
+// virial[9] incremented (nullable).
+void ref_ps_1b(const double* rr, double* dr, size_t nw, double* energies, double* virial) {
+    std::vector<double> v(9, 0.0);
+    if (virial) v.assign(virial, virial + 9);
+    std::vector<double> e = ps::pot_nasa(rr, dr, nw, virial ? &v : nullptr);
+    std::copy(e.begin(), e.end(), energies);
+    if (virial) std::copy(v.begin(), v.end(), virial);
 }
 
 }  // extern "C"

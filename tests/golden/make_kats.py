@@ -8,7 +8,8 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     poly_3b    src/unittests/unittest-poly-3b.cpp         TEST_CASE("poly-3b-v2x::class")
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
-    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases (see CASES).
+    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, and the MB-pol
+    1B (ps::pot_nasa) case (see CASES).
 
 Usage:  python tests/golden/make_kats.py [/root/reference]
 """
@@ -34,6 +35,8 @@ CASES = {
     "energy_ch4ch4_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-ch4'),
     "energy_ch4h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-h2o'),
     "energy_ch4h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:ch4-h2o-h2o'),
+    # MB-pol 1B (Partridge-Schwenke water monomer)
+    "ps_1b": ("src/unittests/unittest-ps.cpp", 'ps::energy'),
 }
 
 

@@ -315,3 +315,58 @@ def test_system_vs_compiled_reference():
     assert r["e3b"] == pytest.approx(q["e3b"], rel=1e-11)
     assert np.abs(r["grad"] - q["grad"]).max() < 1e-9
     np.testing.assert_allclose(r["virial"], q["virial"], rtol=1e-10, atol=1e-8)
+
+
+# ---------------------------------------------------------------------------------------------
+# MB-pol 1B (Partridge-Schwenke water monomer, potential/1b/ps.cpp)
+
+def test_ps_1b_kat():
+    """unittest-ps.cpp TEST_CASE("ps::energy"): energies, gradients and the incremented virial
+    of two waters (the reference's own tolerance, 1e-6)."""
+    k = json.load(open(os.path.join(GOLDEN, "kats.json")))["ps_1b"]
+    e, g, vir = oracle.ps_1b(np.array(k["xyz"]).reshape(2, 9))
+    np.testing.assert_allclose(e, k["energies_expected"], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(g.reshape(-1), k["grad_expected"], rtol=1e-6, atol=1e-6)
+    v = np.array(k["virial"], np.float64)
+    slot = [0, 1, 2, 4, 5, 8]
+    for q in range(6):
+        v[slot[q]] += vir[q]
+    v[3], v[6], v[7] = v[1], v[2], v[5]
+    np.testing.assert_allclose(v, k["virial_expected"], rtol=1e-6, atol=1e-6)
+
+
+def test_ps_1b_finite_differences():
+    from mbx_amd import configs
+
+    rng = np.random.default_rng(3)
+    w = configs.water_geometries(rng, 6).reshape(6, 9)
+    e, g, _ = oracle.ps_1b(w)
+    h = 1e-5
+    for i in range(6):
+        for j in range(9):
+            def en(d):
+                x = w[i:i + 1].copy()
+                x[0, j] += d
+                return oracle.ps_1b(x, grad=False)[0][0]
+            fd = (-en(2 * h) + 8 * en(h) - 8 * en(-h) + en(-2 * h)) / (12 * h)
+            assert fd == pytest.approx(g[i, j], rel=1e-6, abs=1e-7)
+
+
+@ref_only
+def test_ps_1b_vs_compiled_reference():
+    from mbx_amd import configs
+
+    rng = np.random.default_rng(4)
+    w = configs.water_geometries(rng, 64).reshape(64, 9)
+    e, g, vir = oracle.ps_1b(w)
+    er, gr, vr = oracle.ref_ps_1b(w)
+    np.testing.assert_allclose(e, er, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(g, gr, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(vir, vr[[0, 1, 2, 4, 5, 8]], rtol=1e-12, atol=1e-10)
+
+
+def test_ps_1b_published_256_water(systems):
+    """timing/256_h2o_cubic_box/output: "1B = 3.0996035371e+02" (11 significant digits)."""
+    xyz = systems["h2o_256_cubic/xyz"].reshape(-1, 9)
+    e, _, _ = oracle.ps_1b(xyz, grad=False)
+    assert e.sum() == pytest.approx(3.0996035371e+02, rel=5e-11)
