@@ -368,6 +368,15 @@ def main():
     out["config"]["list_build"] = {"ms": lt * 1e3, "hbm_bytes": lb, "GB_per_s": lb / lt / 1e9 if lb else None,
                                    "frac_of_hbm_peak": lb / lt / HBM_PEAK if lb else None,
                                    "bound": "latency (one 4-wavefront block per query; gathers, LDS sort)"}
+    # MB-pol 1B of the same 2048 waters (mbxg_eval_1b: synchronous, host copies of the per-water
+    # energies, virial terms and the gradient included)
+    eng.eval_1b()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e1b, _, _ = eng.eval_1b()
+    t1b = (time.perf_counter() - t0) / args.steps
+    out["config"]["one_body"] = {"e1b": e1b, "ms_per_call": t1b * 1e3, "waters_per_s": cfg["nmon"] / t1b,
+                                 "call": "mbxg_eval_1b with gradients, results copied to the host"}
     if not args.no_batch:
         out["config"]["batched"] = batched_leg(args.steps, args.warmup)
         out["config"]["pimd_batched"] = pimd_leg(args.steps, args.warmup)
