@@ -8,8 +8,8 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     poly_3b    src/unittests/unittest-poly-3b.cpp         TEST_CASE("poly-3b-v2x::class")
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
-    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, and the MB-pol
-    1B (ps::pot_nasa) case (see CASES).
+    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, the MB-pol
+    1B (ps::pot_nasa) case (see CASES), and the ch4 and h2o rows of unittest-energy1b.cpp.
 
 Usage:  python tests/golden/make_kats.py [/root/reference]
 """
@@ -66,6 +66,37 @@ def values(body):
     return out
 
 
+def energy1b(text, mons=("ch4", "h2o")):
+    """unittest-energy1b.cpp keeps one row per monomer kind in nested vectors (xyz, grad, virial,
+    grad_expected, virial_expected), indexed by monid, plus nmon and energies_expected: the rows
+    of the named kinds."""
+    src = re.sub(r"//[^\n]*", "", text)
+
+    def vec(name, nested):
+        m = re.search(r"std::vector<(?:std::vector<double> ?|double|size_t|std::string)> ?%s\s*=\s*\{" % name, src)
+        i = m.end() - 1
+        depth, j = 0, i
+        while True:
+            depth += {"{": 1, "}": -1}.get(src[j], 0)
+            if depth == 0:
+                break
+            j += 1
+        body = src[i + 1:j]
+        if not nested:
+            return [v.strip().strip('"') for v in body.split(",") if v.strip()]
+        return [[float(v) for v in row.split(",") if v.strip()] for row in re.findall(r"\{([^{}]*)\}", body)]
+
+    monid = vec("monid", False)
+    nmon = [int(v) for v in vec("nmon", False)]
+    energies = [float(v) for v in vec("energies_expected", False)]
+    rows = {n: vec(n, True) for n in ("xyz", "grad", "virial", "grad_expected", "virial_expected")}
+    out = {}
+    for mon in mons:
+        k = monid.index(mon)
+        out[mon] = {"nmon": nmon[k], "energy_expected": energies[k], **{n: r[k] for n, r in rows.items()}}
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     res = {}
@@ -73,6 +104,10 @@ def main():
         res[key] = values(block(open(os.path.join(ref, path)).read(), name))
         res[key]["_source"] = ('%s SECTION("%s")' % (path, name[8:]) if name.startswith("SECTION:")
                                else '%s TEST_CASE("%s")' % (path, name))
+    path = "src/unittests/unittest-energy1b.cpp"
+    res["energy1b"] = energy1b(open(os.path.join(ref, path)).read())
+    res["energy1b"]["_source"] = path + ' TEST_CASE("energy1b::get_1b_energy_grad"), rows ch4 and h2o'
+    
     with open(os.path.join(HERE, "kats.json"), "w") as f:
         json.dump(res, f)
     for k, v in res.items():

@@ -370,3 +370,24 @@ def test_ps_1b_published_256_water(systems):
     xyz = systems["h2o_256_cubic/xyz"].reshape(-1, 9)
     e, _, _ = oracle.ps_1b(xyz, grad=False)
     assert e.sum() == pytest.approx(3.0996035371e+02, rel=5e-11)
+
+
+def _energy1b_kat(mon):
+    k = json.load(open(os.path.join(GOLDEN, "kats.json")))["energy1b"][mon]
+    return k
+
+
+def test_energy1b_kat():
+    """unittest-energy1b.cpp TEST_CASE("energy1b::get_1b_energy_grad"), row h2o (6 waters): the
+    dispatcher's total energy, gradients (overwritten) and incremented virial, tolerance 1e-6."""
+    k = _energy1b_kat("h2o")
+    e, g, vir = oracle.ps_1b(np.array(k["xyz"]).reshape(k["nmon"], 9))
+    assert e.sum() == pytest.approx(k["energy_expected"], abs=1e-6)
+    np.testing.assert_allclose(g.reshape(-1), k["grad_expected"], rtol=1e-6, atol=1e-6)
+    v = np.array(k["virial"], np.float64)
+    slot = [0, 1, 2, 4, 5, 8]
+    for q in range(6):
+        v[slot[q]] += vir[q]
+    v[3], v[6], v[7] = v[1], v[2], v[5]
+    np.testing.assert_allclose(v, k["virial_expected"], rtol=1e-6, atol=1e-6)
+
