@@ -195,10 +195,13 @@ size_t mbxg_ad_workspace_bytes(int family, int n);
 int mbxg_ad_forward(int family, int n, const double* xyz, const double* coef, const double* theta, double* energy,
                     double* grad, void* workspace, size_t workspace_bytes, void* stream);
 /* With phi = sum_n g_energy[n] E_n + g_grad[n] . dE_n/dx_n: d_xyz = dphi/dx ([n][3*atoms]),
- * d_coef = dphi/da, d_theta = dphi/dtheta.  g_grad may be NULL (phi = sum g_energy E). */
+ * d_coef = dphi/da, d_theta = dphi/dtheta.  g_grad may be NULL (phi = sum g_energy E).
+ * from_forward = 1: `workspace` is the one mbxg_ad_forward used on these same inputs (and
+ * nothing has written it since), so the polynomial's value and gradient are taken from it
+ * instead of being evaluated again; 0: any workspace. */
 int mbxg_ad_backward(int family, int n, const double* xyz, const double* coef, const double* theta,
                      const double* g_energy, const double* g_grad, double* d_xyz, double* d_coef, double* d_theta,
-                     void* workspace, size_t workspace_bytes, void* stream);
+                     void* workspace, size_t workspace_bytes, int from_forward, void* stream);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,7 @@ class NmerEnergy(torch.autograd.Function):
                                                    _ptr(grad), _ptr(ws), nbytes, stream))
         ctx.save_for_backward(xyz, coef, theta)
         ctx.fam, ctx.n = fam, n
+        ctx.ws, ctx.nbytes = ws, nbytes  # holds the polynomial's value and gradient for the backward
         return energy, grad
 
     @staticmethod
@@ -105,11 +106,11 @@ class NmerEnergy(torch.autograd.Function):
         g_energy = g_energy.contiguous()
         g_grad = g_grad.contiguous() if g_grad is not None else None
         d_xyz, d_coef, d_theta = torch.empty_like(xyz), torch.empty_like(coef), torch.empty_like(theta)
-        ws, nbytes = _workspace(fam, n, xyz.device)
+        ws, nbytes = ctx.ws, ctx.nbytes  # read-only for the polynomial part: repeated backwards may share it
         stream = ctypes.c_void_p(torch.cuda.current_stream(xyz.device).cuda_stream)
         engine._check(engine.lib().mbxg_ad_backward(fam, n, _ptr(xyz), _ptr(coef), _ptr(theta), _ptr(g_energy),
                                                     _ptr(g_grad), _ptr(d_xyz), _ptr(d_coef), _ptr(d_theta),
-                                                    _ptr(ws), nbytes, stream))
+                                                    _ptr(ws), nbytes, 1, stream))
         return d_xyz, d_coef, d_theta, None
 
 

@@ -661,7 +661,8 @@ extern "C" int mbxg_ad_forward(int family, int n, const double* xyz, const doubl
 
 extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const double* coef, const double* theta,
                                 const double* g_energy, const double* g_grad, double* d_xyz, double* d_coef,
-                                double* d_theta, void* workspace, size_t workspace_bytes, void* stream) {
+                                double* d_theta, void* workspace, size_t workspace_bytes, int from_forward,
+                                void* stream) {
     int nv, nat, ncoef;
     AdLayout L;
     if (int e = ad_check("mbxg_ad_backward", family, n, workspace, workspace_bytes, nv, nat, ncoef, L)) return e;
@@ -692,7 +693,7 @@ extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const doub
     if (family == MBXG_AD_MBPOL_2B) {
         hipLaunchKernelGGL((k_ad_vars<Ad2b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, n, ld, xs, xds, sw,
                            swd);
-        ADCHK(launch_poly_mbpol2b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
+        if (!from_forward) ADCHK(launch_poly_mbpol2b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
         ADCHK(launch_ad_tan_mbpol_2b(xs, xds, n, ld, coef, gds, st));
         ADCHK(launch_ad_basis_mbpol_2b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
         hipLaunchKernelGGL(k_ad_sum_parts, dim3((rows + 255) / 256), dim3(256), 0, st, gds, rows, Ad2b::kParts);
@@ -701,7 +702,7 @@ extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const doub
     } else {
         hipLaunchKernelGGL((k_ad_vars<Ad3b, true>), dim3(nb), dim3(64), 0, st, xyz, g_grad, theta, n, ld, xs, xds, sw,
                            swd);
-        ADCHK(launch_poly_mbpol3b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
+        if (!from_forward) ADCHK(launch_poly_mbpol3b(xs, ndev, n, ld, coef, 1, gs, ps, 8, st));
         ADCHK(launch_ad_tan_mbpol_3b(xs, xds, n, ld, coef, gds, st));
         ADCHK(launch_ad_basis_mbpol_3b(xs, xds, sw, swd, g_energy, n, ld, basis, st));
         hipLaunchKernelGGL(k_ad_sum_parts, dim3((rows + 255) / 256), dim3(256), 0, st, gds, rows, Ad3b::kParts);

@@ -61,7 +61,7 @@ PROTOTYPES = {
     "mbxg_ad_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
     "mbxg_ad_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "mbxg_ad_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                        ctypes.c_size_t, _vp]),
+                                        ctypes.c_size_t, ctypes.c_int, _vp]),
 }
 
 FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")
