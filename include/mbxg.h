@@ -144,8 +144,10 @@ int mbxg_device_outputs(mbxg_ctx* ctx, double** d_grad, double** d_totals);
 int mbxg_stream(mbxg_ctx* ctx, void** stream);
 
 /* Stage timing with HIP events on the context stream.  After a compute with timing on,
- * mbxg_get_timings writes up to 10 durations in ms: FixMonomerCoordinates, list count pass (grid,
- * cloud, binning, counts, scan), list emit pass, 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total. */
+ * mbxg_get_timings writes up to 11 durations in ms: FixMonomerCoordinates, list count pass (grid,
+ * cloud, binning, counts, scan), list emit pass, 2B variables, 2B polynomial, 2B chain rule, 3B variables, 3B polynomial, 3B chain rule, total,
+ * CH4-H2O-H2O family (variables, polynomial and chain rule together).  A stage that did not run
+ * reads about 0 (two back-to-back events). */
 int mbxg_set_timing(mbxg_ctx* ctx, int on);
 int mbxg_get_timings(mbxg_ctx* ctx, double* ms, int n);
 
@@ -159,9 +161,10 @@ int mbxg_set_graph(mbxg_ctx* ctx, int on);
 /* Schedule parameters.  "split_2b" / "split_3b": wavefronts (1, 2, 4 or 8) that share each
  * block of 64 dimers / trimers in the MB-pol polynomial stage, each evaluating 1/split of the
  * polynomial (defaults 4 and 2); "split_ch4h2o2_3b": the same for the CH4-H2O-H2O polynomial
- * (default 4, not autotuned).  mbxg_autotune times every split of both orders on the
- * current inputs (the polynomial stage, `reps` timed repetitions after one warm-up), keeps the
- * fastest and recomputes once with it.  No counterpart in the reference (its kernels have no
+ * (default 4).  mbxg_autotune times every split of both MB-pol orders on the current inputs
+ * (the polynomial stage, `reps` timed repetitions after one warm-up) and, when CH4-H2O-H2O trimers
+ * are evaluated, every split of that family (its whole evaluation), keeps the fastest of each and
+ * recomputes once with them.  No counterpart in the reference (its kernels have no
  * launch shape). */
 int mbxg_set_tuning(mbxg_ctx* ctx, const char* key, int value);
 int mbxg_get_tuning(mbxg_ctx* ctx, const char* key, int* value);

@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(64) k_ad_vars(const double* __restrict__ xyz, 
         }
         const double r = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
         double f1, f2, fk, f1k, fd, f1d;
-        const double f = ad_value<false>(V, th[V.kp], V.dp >= 0 ? th[V.dp] : V.d, r, f1, f2, fk, f1k, fd, f1d);
+        const double f = ad_value<false>(V, th[V.kp], V.dp >= 0 ? th[V.dp >= 0 ? V.dp : 0] : V.d, r, f1, f2, fk, f1k, fd, f1d);
         xs[(size_t)j * ld + t] = f;
         if (TAN) {
             const double rho = dv[0] * ddv[0] + dv[1] * ddv[1] + dv[2] * ddv[2];
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
             }
             const double rr = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
             const double rinv = 1.0 / rr;
-            const double k = th[V.kp], dpar = V.dp >= 0 ? th[V.dp] : V.d;
+            const double k = th[V.kp], dpar = V.dp >= 0 ? th[V.dp >= 0 ? V.dp : 0] : V.d;
             double vf1, vf2, fk, f1k, fd, f1d;
             ad_value<BWD>(V, k, dpar, rr, vf1, vf2, fk, f1k, fd, f1d);
             if (!BWD) {
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(64) k_ad_chain(const double* __restrict__ xyz,
                     }
                 }
                 thb[V.kp] += vb * fk + vdb * rdot * f1k;
-                if (V.dp >= 0) thb[V.dp] += vb * fd + vdb * rdot * f1d;
+                if (V.dp >= 0) thb[V.dp >= 0 ? V.dp : 0] += vb * fd + vdb * rdot * f1d;
             }
         });
         const double a_sw = BWD ? gE * P + Pd : P;

@@ -39,7 +39,7 @@ namespace {
 
 // Timing events: 0 start, 1 after fix, 2 after the list count pass, 3 after the emit pass,
 // 4..7 2B stages, 8..11 3B stages, 12 end.
-constexpr int kNumEvents = 13;
+constexpr int kNumEvents = 15;
 
 thread_local std::string g_err;
 
@@ -992,9 +992,13 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
             else if (f == FAM_MBPOL_3B)
                 HIPCHK(launch_mbpol3b(L.evals[f].p, nd, cap, N, c->d_coef3.p, do_grads, c->d_grad.p, o.e_nmer.p, vir,
                                       o.partials.p, o.ws.p, c->split3, sk, ev ? ev + 8 : nullptr));
-            else
+            else {
+                const bool timed = ev && f == FAM_CH4H2O2_3B;
+                if (timed) HIPCHK(hipEventRecord(ev[13], st));
                 HIPCHK(launch_nrg_family(f, L.evals[f].p, nd, cap, N, c->d_coefn[f].p, do_grads, c->d_grad.p,
                                          o.e_nmer.p, vir, o.partials.p, o.ws.p, c->split3n, sk, nullptr));
+                if (timed) HIPCHK(hipEventRecord(ev[14], st));
+            }
             reduce(f, sk);
             HIPCHK(hipGetLastError());
         }
@@ -1132,9 +1136,10 @@ extern "C" int mbxg_get_timings(mbxg_ctx* c, double* ms, int n) {
     if (!c || !ms) return fail(MBXG_EINVAL, "mbxg_get_timings: bad arguments");
     if (!c->timing || !c->have_results) return fail(MBXG_ESTATE, "mbxg_get_timings: no timed compute");
     if (int r = finish(c)) return r;
-    // fix, list count pass, list emit pass, vars 2B, poly 2B, chain 2B, vars 3B, poly 3B, chain 3B, total
-    const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {8, 9}, {9, 10}, {10, 11}, {0, 12}};
-    for (int k = 0; k < n && k < 10; ++k) {
+    // fix, list count pass, list emit pass, vars 2B, poly 2B, chain 2B, vars 3B, poly 3B, chain 3B, total,
+    // CH4-H2O-H2O family (variables, polynomial, chain rule)
+    const int pairs[11][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {8, 9}, {9, 10}, {10, 11}, {0, 12}, {13, 14}};
+    for (int k = 0; k < n && k < 11; ++k) {
         float t = 0.0f;
         HIPCHK(hipEventElapsedTime(&t, c->ev[pairs[k][0]], c->ev[pairs[k][1]]));
         ms[k] = t;
@@ -1219,6 +1224,25 @@ extern "C" int mbxg_autotune(mbxg_ctx* c, int terms, int do_grads, int reps) {
     }
     c->split2 = pick[0];
     c->split3 = pick[1];
+    if ((terms & MBXG_3B) && c->cut3 > 0.0 && family_launched(c, FAM_CH4H2O2_3B)) {
+        double best3n = 1e300;
+        int pick3n = c->split3n;
+        for (int i = 0; i < 4; ++i) {
+            c->split3n = cand[i];
+            double t = 1e300;
+            for (int r = 0; r < reps + 1; ++r) {
+                if (int e = mbxg_compute(c, terms, do_grads)) return e;
+                double ms[11];
+                if (int e = mbxg_get_timings(c, ms, 11)) return e;
+                if (r > 0) t = std::min(t, ms[10]);
+            }
+            if (t < best3n) {
+                best3n = t;
+                pick3n = cand[i];
+            }
+        }
+        c->split3n = pick3n;
+    }
     if (int r = mbxg_set_timing(c, was_timing ? 1 : 0)) return r;
     return mbxg_compute(c, terms, do_grads);
 }

@@ -66,7 +66,7 @@ PROTOTYPES = {
 
 FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")
 TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
-                 "chain_3b", "total")
+                 "chain_3b", "total", "ch4h2o2_3b")
 
 
 class MbxgError(RuntimeError):
@@ -177,7 +177,7 @@ class Engine:
     def autotune(self, terms=TERM_2B | TERM_3B, do_grads=True, reps=3):
         """Pick the fastest polynomial schedule for the current inputs (see mbxg_autotune)."""
         _check(lib().mbxg_autotune(self._c, int(terms), int(bool(do_grads)), int(reps)))
-        return {"split_2b": self.tuning("split_2b"), "split_3b": self.tuning("split_3b")}
+        return {k: self.tuning(k) for k in ("split_2b", "split_3b", "split_ch4h2o2_3b")}
 
     def set_box(self, box):
         if box is None or len(box) == 0:

@@ -136,3 +136,27 @@ def test_ch4h2o2_splits(systems, split):
     r = oracle.system(xyz, kinds, box, c2, c3, tags=np.arange(len(kinds)), families=("ch4h2o2_3b",))
     assert e3 != 0.0 and e3 == pytest.approx(r["e3b"], rel=E_REL)
     assert np.abs(g - r["grad"]).max() <= G_ABS
+
+
+def test_autotune_ch4h2o2(systems):
+    """mbxg_autotune also picks the CH4-H2O-H2O split when that family is evaluated (its whole
+    evaluation is timed as a stage of its own), and the picked schedule keeps the oracle's values."""
+    xyz, kinds, box, c2, c3, _ = mixed_case(systems, "ch4_342_h2o_228_rect")
+    print("autotune ch4h2o2", flush=True)
+    e = engine.Engine()
+    e.set_topology(kinds)
+    e.set_box(box)
+    e.set_cutoffs(c2, c3)
+    e.set_coords(xyz)
+    e.compute()
+    e.set_timing(True)
+    e.compute()
+    assert e.timings()["ch4h2o2_3b"] > 0.0
+    e.set_timing(False)
+    picked = e.autotune(reps=1)
+    assert all(picked[k] in (1, 2, 4, 8) for k in ("split_2b", "split_3b", "split_ch4h2o2_3b"))
+    e2, e3, g, _ = e.results()
+    r = oracle.system(xyz, kinds, box, c2, c3, tags=np.arange(len(kinds)))
+    assert e2 == pytest.approx(r["e2b"], rel=E_REL)
+    assert e3 == pytest.approx(r["e3b"], rel=E_REL)
+    assert np.abs(g - r["grad"]).max() <= G_ABS
