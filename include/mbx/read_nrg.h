@@ -1,0 +1,56 @@
+// tools::ReadNrg -- the reference's NRG reader (src/io_tools/read_nrg.h:72-110,
+// src/io_tools/read_nrg.cpp:43-298) over this library's bblock::System.
+//
+// File format (keywords case-insensitive, anything after a keyword on its line ignored):
+//
+//   SYSTEM
+//   MOLECULE
+//   MONOMER <id>
+//   <atom name> <x> <y> <z>        (one line per atom, Angstrom)
+//   ENDMON
+//   MONOMER <id> ... ENDMON         (more monomers of the same molecule)
+//   ENDMOL
+//   MOLECULE ... ENDMOL             (more molecules)
+//   ENDSYS
+//   SYSTEM ... ENDSYS               (more systems; an empty line or the end of the file ends the list)
+//
+// Monomer ids are lower-cased (the reference's std::transform(::tolower)); atom names are kept as
+// written.  Coordinates are read with std::stod, as the reference does.
+#pragma once
+
+#include <istream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mbx/system.h"
+
+namespace tools {
+
+struct NrgMonomer {
+    std::string id;
+    std::vector<std::string> atoms;
+    std::vector<double> xyz;  // 3 per atom
+};
+
+struct NrgSystem {
+    std::vector<NrgMonomer> monomers;             // file order
+    std::vector<std::vector<size_t>> molecules;   // monomer indices (into `monomers`) per molecule
+};
+
+// The reader of read_nrg.cpp:43-298 without the System: appends every system of `is` to
+// `systems`.  A malformed file throws bblock::CUException naming the line (the reference's
+// messages: "Expected SYSTEM keyword in line N", "Expected MOLECULE keyword ...", "Expected
+// MONOMER keyword ..., along with the monomer id", "Expected coordinates or ENDMON ...",
+// "Expected numerical coordinates ...").  Unlike the reference, a line that is neither MOLECULE
+// nor ENDSYS after an ENDMOL is an error instead of being skipped.
+void ParseNrg(std::istream& is, std::vector<NrgSystem>& systems);
+
+// read_nrg.h:72 (ReadNrg(const char*, std::vector<bblock::System>&)).  bblock::System here owns
+// a device context and cannot be copied, so the systems are appended as unique_ptrs.  Each one
+// gets its monomers (AddMonomer) and molecules (AddMolecule) in file order and is initialized
+// (Initialize), as read_nrg.cpp:280, :293 and :135 do.  A file that cannot be opened throws
+// CUException("Could not open NRG file <name> for reading.").
+void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>& systems);
+
+}  // namespace tools

@@ -38,6 +38,8 @@ class System {
     // system.h:564, system.cpp:848-870
     void AddMonomer(std::vector<double> xyz, std::vector<std::string> atoms, std::string id, size_t islocal = 1,
                     int tag = 0);
+    // system.h:578, system.cpp:872: a molecule = the indices (input order) of its monomers
+    void AddMolecule(std::vector<size_t> molec);
     // system.h:713, system.cpp:1000
     void Initialize();
     // system.h:770-782, system.cpp:1198-1608.  Keys read under "MBX": box, twobody_cutoff,
@@ -74,6 +76,9 @@ class System {
     std::vector<size_t> GetPairList(size_t nmax, double cutoff, size_t istart, size_t iend, bool use_ghost = false);
 
     size_t GetNumMon();
+    // system.cpp:136 and :387 (the count is molecules_.size(), as system.cpp:1032 sets it at Initialize)
+    size_t GetNumMol();
+    std::vector<size_t> GetMolecule(size_t n);
     size_t GetNumRealSites();
     std::string GetMonId(size_t n);
 
@@ -91,6 +96,7 @@ class System {
     std::vector<double> in_xyz_;
     std::vector<std::string> in_atoms_;
     std::vector<std::string> in_mon_;
+    std::vector<std::vector<size_t>> molecules_;
     std::vector<size_t> in_local_;
     std::vector<int> in_tag_;  // per atom
     std::vector<size_t> in_nat_, in_first_;

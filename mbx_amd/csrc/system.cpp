@@ -95,6 +95,8 @@ void System::AddMonomer(std::vector<double> xyz, std::vector<std::string> atoms,
     tag_counter_ += (int)atoms.size();
 }
 
+void System::AddMolecule(std::vector<size_t> molec) { molecules_.push_back(std::move(molec)); }
+
 void System::Initialize() {
     if (initialized_) MBX_THROW("The system has already been initialized. Reinitialization is not possible");
     const size_t nmon = in_mon_.size();
@@ -352,6 +354,11 @@ std::vector<size_t> System::GetPairList(size_t nmax, double cutoff, size_t istar
 }
 
 size_t System::GetNumMon() { return in_mon_.size(); }
+size_t System::GetNumMol() { return molecules_.size(); }
+std::vector<size_t> System::GetMolecule(size_t n) {
+    if (n >= molecules_.size()) MBX_THROW("Molecule " + std::to_string(n) + " does not exist");
+    return molecules_[n];
+}
 size_t System::GetNumRealSites() { return numat_; }
 
 std::string System::GetMonId(size_t n) {

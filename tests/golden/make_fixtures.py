@@ -8,12 +8,15 @@ is imported or executed.
 
     timing/<case>/{input.nrg, mbx.json, output}
     src/unittests/unittests_inputs/{2048_water.nrg, 2048_water.json}
+    src/unittests/unittests_inputs/input_h2o_co2_ch4_dp1_unittest_system.nrg (copied as it is:
+        the reference's multi-system NRG test input, for the NRG reader's tests)
 
 Usage:  python tests/golden/make_fixtures.py [/root/reference]
 """
 import json
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -65,8 +68,17 @@ def read_output(path):
     return out
 
 
+NRG_FILES = {"input_h2o_co2_ch4_dp1.nrg": "src/unittests/unittests_inputs/input_h2o_co2_ch4_dp1_unittest_system.nrg"}
+
+
+def copy_nrg(ref):
+    for name, src in NRG_FILES.items():
+        shutil.copyfile(os.path.join(ref, src), os.path.join(HERE, name))
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    copy_nrg(ref)
     data = {}
     for key, d in CASES.items():
         mons, nat, names, xyz = read_nrg(os.path.join(ref, d, "input.nrg"))

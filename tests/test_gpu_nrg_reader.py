@@ -1,0 +1,55 @@
+"""GPU: an NRG file through the reference's single-point flow (tools::ReadNrg -> SetUpFromJson ->
+Energy(true), OneBodyEnergy / TwoBodyEnergy / ThreeBodyEnergy; src/tests/single_point.cpp) with
+the prebuilt mbx_amd/single_point.  The reference's inputs (systems.npz) are written out as NRG
+and mbx.json files; the printed energies must equal the reference's published per-term energies
+(11 significant digits) and the gradients the oracle's (1e-8 kcal/mol/A)."""
+import numpy as np
+import pytest
+
+import nrg_files
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _files(systems, tmp_path, case):
+    mons = [str(m) for m in systems[case + "/mon"]]
+    nat = systems[case + "/nat"]
+    names = [str(a) for a in systems[case + "/atoms"]]
+    xyz = systems[case + "/xyz"]
+    c2, c3 = (float(v) for v in systems[case + "/cutoffs"])
+    nrg, js = str(tmp_path / "input.nrg"), str(tmp_path / "mbx.json")
+    nrg_files.write_nrg(nrg, [(mons, nat, names, xyz)])
+    nrg_files.write_json(js, systems[case + "/box"], c2, c3)
+    return nrg, js, mons, xyz, c2, c3
+
+
+def test_single_point_water_published(systems, tmp_path):
+    nrg, js, _, _, _, _ = _files(systems, tmp_path, "h2o_256_cubic")
+    print("single_point h2o_256_cubic", flush=True)
+    r = nrg_files.run(nrg, js)
+    assert r.returncode == 0, r.stderr
+    e, _ = nrg_files.energies(r.stdout)
+    pub = systems["h2o_256_cubic/published"]
+    assert e["1B"] == pytest.approx(pub[0], rel=5e-11)
+    assert e["2B"] == pytest.approx(pub[1], rel=5e-11)
+    assert e["3B"] == pytest.approx(pub[2], rel=5e-11)
+    # one combined compute vs two separate ones: the same terms up to the atomics' rounding order
+    assert e["total"] == pytest.approx(e["2B"] + e["3B"], rel=1e-12)
+
+
+def test_single_point_clathrate_grads(systems, tmp_path):
+    nrg, js, mons, xyz, c2, c3 = _files(systems, tmp_path, "clathrate")
+    print("single_point clathrate", flush=True)
+    r = nrg_files.run(nrg, js, "--grads")
+    assert r.returncode == 0, r.stderr
+    e, g = nrg_files.energies(r.stdout)
+    assert "1B" not in e and "1B term not computed" in r.stderr  # CH4 has no 1B term here
+    pub = systems["clathrate/published"]
+    assert e["2B"] == pytest.approx(pub[1], rel=5e-11)
+    assert e["3B"] == pytest.approx(pub[2], rel=5e-11)
+    kinds = np.array([0 if m == "h2o" else 1 for m in mons])
+    box = systems["clathrate/box"]
+    ref = oracle.system(xyz, kinds, box if box.size else None, c2, c3)
+    assert g.shape == (len(xyz) // 3, 3)
+    assert np.abs(g.reshape(-1) - ref["grad"]).max() <= 1e-8
