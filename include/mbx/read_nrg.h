@@ -19,6 +19,7 @@
 #pragma once
 
 #include <istream>
+#include <ostream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -52,5 +53,15 @@ void ParseNrg(std::istream& is, std::vector<NrgSystem>& systems);
 // (Initialize), as read_nrg.cpp:280, :293 and :135 do.  A file that cannot be opened throws
 // CUException("Could not open NRG file <name> for reading.").
 void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>& systems);
+
+// write_nrg.h:65-75 / write_nrg.cpp: every system as SYSTEM i / MOLECULE i.j / MONOMER <id> /
+// atom lines (name left-aligned in 5 columns, coordinates in 20 columns, scientific, `digits`
+// significant digits after the point: 8 in the reference) / ENDMON / ENDMOL / ENDSYS.  Molecules
+// come from AddMolecule (a system without molecules writes none, as in the reference);
+// monomers, names and coordinates (GetRealXyz) are taken in input order.  The systems must be
+// initialized.  WriteNrg throws std::runtime_error("could not open NRG file for writting") --
+// the reference's text -- when the file cannot be opened.
+void SaveNrg(std::ostream& os, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits = 8);
+void WriteNrg(const char* filename, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits = 8);
 
 }  // namespace tools

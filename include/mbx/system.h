@@ -81,6 +81,11 @@ class System {
     std::vector<size_t> GetMolecule(size_t n);
     size_t GetNumRealSites();
     std::string GetMonId(size_t n);
+    // system.h:151, 207, 353 -- here in input order (real atoms only): atoms of monomer n, its first
+    // atom's index, every atom's name
+    size_t GetMonNumAt(size_t n);
+    size_t GetFirstInd(size_t n);
+    std::vector<std::string> GetRealAtomNames();
 
     // The n-mer counts of the last energy call: listed / evaluated dimers, listed / evaluated trimers.
     std::vector<long> GetNmerCounts();

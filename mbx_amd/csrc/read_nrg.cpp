@@ -1,10 +1,12 @@
-// NRG reader (include/mbx/read_nrg.h): a line-oriented parser over the grammar of
-// src/io_tools/read_nrg.cpp:43-298, then bblock::System construction.
+// NRG reader and writer (include/mbx/read_nrg.h): a line-oriented parser over the grammar of
+// src/io_tools/read_nrg.cpp:43-298, then bblock::System construction; the writer of
+// src/io_tools/write_nrg.cpp.
 #include "mbx/read_nrg.h"
 
 #include <algorithm>
 #include <cctype>
 #include <fstream>
+#include <iomanip>
 #include <sstream>
 #include <stdexcept>
 
@@ -127,6 +129,37 @@ void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>&
         sys->Initialize();
         systems.push_back(std::move(sys));
     }
+}
+
+void SaveNrg(std::ostream& os, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits) {
+    for (size_t i = 0; i < systems.size(); ++i) {
+        bblock::System& sys = *systems[i];
+        const std::vector<double> xyz = sys.GetRealXyz();
+        const std::vector<std::string> names = sys.GetRealAtomNames();
+        os << "SYSTEM " << i << std::endl;
+        for (size_t j = 0; j < sys.GetNumMol(); ++j) {
+            os << "MOLECULE " << i << "." << j << std::endl;
+            for (size_t m : sys.GetMolecule(j)) {
+                os << "MONOMER " << sys.GetMonId(m) << std::endl;
+                const size_t first = sys.GetFirstInd(m), nat = sys.GetMonNumAt(m);
+                for (size_t a = first; a < first + nat; ++a) {
+                    os << std::scientific << std::setprecision(digits) << std::setw(5) << std::left << names[a];
+                    for (int k = 0; k < 3; ++k) os << std::setw(20) << std::right << xyz[3 * a + k];
+                    os << std::endl;
+                }
+                os << "ENDMON " << std::endl;
+            }
+            os << "ENDMOL " << std::endl;
+        }
+        os << "ENDSYS " << std::endl;
+    }
+}
+
+void WriteNrg(const char* filename, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits) {
+    if (!filename) throw std::runtime_error("could not open NRG file for writting");
+    std::ofstream ofs(filename);
+    if (!ofs) throw std::runtime_error("could not open NRG file for writting");
+    SaveNrg(ofs, systems, digits);
 }
 
 }  // namespace tools

@@ -3,7 +3,9 @@
 // SetUpFromJson with the optional json file, then Energy(true) and the per-term block of its
 // PRINT_VIRIAL / PRINT_GRADS output) for the terms this library computes.
 //
-//   single_point <file.nrg> [mbx.json] [--grads]
+//   single_point <file.nrg> [mbx.json] [--grads] [--write <out.nrg>]
+//                                       --write: every system read, written back with WriteNrg
+//                                       (the reference writes input_out.nrg unconditionally)
 //   single_point --dump <file.nrg>     every system, molecule, monomer and atom the file holds
 //                                       (the parser alone: no device is used)
 #include <cstdio>
@@ -57,10 +59,13 @@ int main(int argc, char** argv) {
         if (argc == 3 && std::strcmp(argv[1], "--dump") == 0) return dump(argv[2]);
         const char* nrg = nullptr;
         char* json = nullptr;
+        const char* out = nullptr;
         bool grads = false;
         for (int i = 1; i < argc; ++i) {
             if (std::strcmp(argv[i], "--grads") == 0)
                 grads = true;
+            else if (std::strcmp(argv[i], "--write") == 0 && i + 1 < argc)
+                out = argv[++i];
             else if (!nrg)
                 nrg = argv[i];
             else if (!json)
@@ -69,12 +74,13 @@ int main(int argc, char** argv) {
                 nrg = nullptr, i = argc;
         }
         if (!nrg) {
-            std::fprintf(stderr, "usage: %s <file.nrg> [mbx.json] [--grads]\n       %s --dump <file.nrg>\n", argv[0],
+            std::fprintf(stderr, "usage: %s <file.nrg> [mbx.json] [--grads] [--write <out.nrg>]\n       %s --dump <file.nrg>\n", argv[0],
                          argv[0]);
             return 1;
         }
         std::vector<std::unique_ptr<bblock::System>> systems;
         tools::ReadNrg(nrg, systems);
+        if (out) tools::WriteNrg(out, systems);
         bblock::System& sys = *systems[0];
         sys.SetUpFromJson(json);
         const double e = sys.Energy(true);

@@ -366,6 +366,18 @@ std::string System::GetMonId(size_t n) {
     return in_mon_[n];
 }
 
+size_t System::GetMonNumAt(size_t n) {
+    if (n >= in_nat_.size()) MBX_THROW("monomer index out of range");
+    return in_nat_[n];
+}
+
+size_t System::GetFirstInd(size_t n) {
+    if (n >= in_first_.size()) MBX_THROW("monomer index out of range");
+    return in_first_[n];
+}
+
+std::vector<std::string> System::GetRealAtomNames() { return in_atoms_; }
+
 std::vector<long> System::GetNmerCounts() { return counts_; }
 
 }  // namespace bblock
