@@ -20,7 +20,6 @@
 
 #include <istream>
 #include <ostream>
-#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,21 +46,22 @@ struct NrgSystem {
 // nor ENDSYS after an ENDMOL is an error instead of being skipped.
 void ParseNrg(std::istream& is, std::vector<NrgSystem>& systems);
 
-// read_nrg.h:72 (ReadNrg(const char*, std::vector<bblock::System>&)).  bblock::System here owns
-// a device context and cannot be copied, so the systems are appended as unique_ptrs.  Each one
-// gets its monomers (AddMonomer) and molecules (AddMolecule) in file order and is initialized
-// (Initialize), as read_nrg.cpp:280, :293 and :135 do.  A file that cannot be opened throws
+// read_nrg.h:72.  Every system of the file is appended to `systems`: its monomers (AddMonomer)
+// and molecules (AddMolecule) in file order, initialized (Initialize), as read_nrg.cpp:280,
+// :293 and :135 do.  A System holds no device memory until its first computation, so reading a
+// file of many configurations allocates nothing on the GPU.  A file that cannot be opened throws
 // CUException("Could not open NRG file <name> for reading.").
-void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>& systems);
+void ReadNrg(const char* filename, std::vector<bblock::System>& systems);
 
 // write_nrg.h:65-75 / write_nrg.cpp: every system as SYSTEM i / MOLECULE i.j / MONOMER <id> /
 // atom lines (name left-aligned in 5 columns, coordinates in 20 columns, scientific, `digits`
 // significant digits after the point: 8 in the reference) / ENDMON / ENDMOL / ENDSYS.  Molecules
 // come from AddMolecule (a system without molecules writes none, as in the reference);
-// monomers, names and coordinates (GetRealXyz) are taken in input order.  The systems must be
-// initialized.  WriteNrg throws std::runtime_error("could not open NRG file for writting") --
-// the reference's text -- when the file cannot be opened.
-void SaveNrg(std::ostream& os, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits = 8);
-void WriteNrg(const char* filename, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits = 8);
+// monomers, names and coordinates (GetRealXyz: wrapped under PBC) are taken in input order.  The
+// systems must be initialized.  Taken by value, as the reference does.  WriteNrg throws
+// std::runtime_error("could not open NRG file for writting") -- the reference's text -- when the
+// file cannot be opened.
+void SaveNrg(std::ostream& os, std::vector<bblock::System> systems, int digits = 8);
+void WriteNrg(const char* filename, std::vector<bblock::System> systems, int digits = 8);
 
 }  // namespace tools

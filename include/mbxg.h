@@ -32,7 +32,7 @@ typedef struct mbxg_ctx mbxg_ctx;
 /* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", ...). */
 enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2 };
 /* Terms for mbxg_compute; MBXG_LISTS_ONLY builds the requested lists without evaluating. */
-enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4 };
+enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4, MBXG_1B = 8 };
 /* Status codes. */
 enum { MBXG_OK = 0, MBXG_EINVAL = -1, MBXG_EHIP = -2, MBXG_ESTATE = -3, MBXG_ENOMEM = -4 };
 
@@ -78,13 +78,27 @@ int mbxg_set_families(mbxg_ctx* ctx, int mask);
  * reaches systools::AddClusters (bblock/sys_tools.cpp:472-537). */
 int mbxg_set_use_ghost(mbxg_ctx* ctx, int on);
 
-/* MB-pol 1B: the Partridge-Schwenke energy of every local water monomer at the current
- * coordinates (after FixMonomerCoordinates), ps::pot_nasa (potential/1b/ps.cpp:346) as reached
- * through System::OneBodyEnergy / Get1B (bblock/system.cpp:2140-2158, 2269-2401).  e1b receives
- * the total (kcal/mol, summed in monomer order); with do_grads, grad (3 doubles per atom,
- * nullable) is overwritten with the 1B gradient and virial (9, nullable) with the 1B virial.
- * Synchronous.  Only water monomers have a 1B term here (other kinds: MBXG_EINVAL). */
+/* The 1B term alone (mbxg_compute(ctx, MBXG_1B, do_grads), then its results): the energy of every
+ * local monomer at the current coordinates (after FixMonomerCoordinates), as reached through
+ * System::OneBodyEnergy / Get1B (bblock/system.cpp:2140-2158, 2269-2401) and e1b::get_1b_energy
+ * (potential/1b/energy1b.cpp:118-147): water by the MB-pol Partridge-Schwenke potential
+ * ps::pot_nasa (potential/1b/ps.cpp:346), CH4 by the MB-nrg x1b_A1B4_v1x holder
+ * (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371).  e1b receives the total (kcal/mol); with do_grads,
+ * grad (3 doubles per atom, nullable) is overwritten with the 1B gradient and virial (9, nullable)
+ * with the 1B virial.  Synchronous.  Monomers other than h2o and ch4: MBXG_EINVAL. */
 int mbxg_eval_1b(mbxg_ctx* ctx, int do_grads, double* e1b, double* grad, double* virial);
+
+/* Energies of the last compute, per term (each pointer nullable; a term not computed reads 0).
+ * The 1B total is a fixed-order sum over the local monomers. */
+int mbxg_get_energies(mbxg_ctx* ctx, double* e1b, double* e2b, double* e3b);
+
+/* FixMonomerCoordinates (bblock/sys_tools.cpp:257-320) of host coordinates (3*natoms doubles,
+ * internal order) under the current box, into xyz_out (may equal xyz_in): each monomer's first
+ * atom wrapped into the cell (fractional x - floor(x + 0.5)), the others to their images nearest
+ * to it; a copy in the gas phase.  The same kernel and arithmetic as the wrap of every compute,
+ * on its own buffers (the last compute's results are untouched).  Synchronous.  Used by
+ * System::SetPBC / GetRealXyz, which wrap the stored coordinates as the reference does. */
+int mbxg_fix_coordinates(mbxg_ctx* ctx, const double* xyz_in, double* xyz_out);
 
 /* Batches: split the monomers (in topology order) into nconf independent configurations,
  * configuration c holding monomers [mon_offsets[c], mon_offsets[c+1]) (mon_offsets[0] = 0,
@@ -95,8 +109,9 @@ int mbxg_eval_1b(mbxg_ctx* ctx, int do_grads, double* e1b, double* grad, double*
  * configurations or path-integral beads at once. */
 int mbxg_set_batch(mbxg_ctx* ctx, int nconf, const int* mon_offsets);
 
-/* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163).
- * a == NULL restores the fitted values. */
+/* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163,
+ * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207).  a == NULL restores
+ * the fitted values. */
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
 
 /* Coordinates of all atoms (3*natoms doubles, internal order) from host or device memory.
@@ -105,7 +120,9 @@ int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int
 int mbxg_set_coords(mbxg_ctx* ctx, const double* xyz);
 int mbxg_set_coords_device(mbxg_ctx* ctx, const double* d_xyz);
 
-/* Run lists + evaluation for `terms` (MBXG_2B | MBXG_3B) on the context stream.  Results stay
+/* Run lists + evaluation for `terms` (MBXG_1B | MBXG_2B | MBXG_3B) on the context stream.  The 1B
+ * term (see mbxg_eval_1b) runs first and its gradient and virial join the others'; it is not
+ * evaluated over batches (mbxg_set_batch with more than one configuration: MBXG_EINVAL).  Results stay
  * on the device until mbxg_get_results / the device accessors.  The first compute after a new
  * topology, box or cutoffs synchronises once to size the list buffers (with headroom); later
  * computes enqueue everything without waiting.  A compute whose lists outgrow the buffers is
@@ -113,9 +130,10 @@ int mbxg_set_coords_device(mbxg_ctx* ctx, const double* d_xyz);
  * changed in between, that call fails with MBXG_ESTATE and the caller computes again. */
 int mbxg_compute(mbxg_ctx* ctx, int terms, int do_grads);
 
-/* Wait for the context stream and copy results: energies, gradient (3*natoms, internal
- * order; may be NULL), virial (9; may be NULL; the reference's ordering with [3]=[1],
- * [6]=[2], [7]=[5]). */
+/* Wait for the context stream and copy results: 2B and 3B energies, gradient of every computed
+ * term (3*natoms, internal order; may be NULL), virial of every computed term (9; may be NULL;
+ * the reference's ordering with [3]=[1], [6]=[2], [7]=[5]).  mbxg_get_energies gives the 1B
+ * energy. */
 int mbxg_get_results(mbxg_ctx* ctx, double* e2b, double* e3b, double* grad, double* virial);
 
 /* Per-configuration results of the last compute: e2b[nconf], e3b[nconf], virial[9*nconf]
@@ -136,8 +154,8 @@ int mbxg_get_nmer_energies(mbxg_ctx* ctx, int order, int32_t* nmers, double* ene
 int mbxg_get_coords(mbxg_ctx* ctx, double* xyz);
 
 /* Device pointers into the context (valid until the next compute that grows buffers):
- * gradient (3*natoms) and totals (14 doubles: e2b and its 6 virial components xx xy xz yy yz zz,
- * then e3b and its 6). */
+ * gradient (3*natoms) and totals (21 doubles: e2b and its 6 virial components xx xy xz yy yz zz,
+ * then e3b and its 6, then e1b and its 6). */
 int mbxg_device_outputs(mbxg_ctx* ctx, double** d_grad, double** d_totals);
 
 /* The context's HIP stream (hipStream_t), for callers that enqueue around the engine. */

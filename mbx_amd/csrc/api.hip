@@ -170,9 +170,12 @@ struct mbxg_ctx {
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
     int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
     int use_ghost = 0;                    // the cloud holds ghost monomers too (mbxg_set_use_ghost)
-    std::vector<int> water_local;         // local water monomers (MB-pol 1B, mbxg_eval_1b)
+    // 1B term (MBXG_1B): the local waters, then the local CH4s; per monomer energy and virial rows
+    std::vector<int> water_local, ch4_local;
     DBuf<int> d_w1b;
-    DBuf<double> d_e1b, d_v1b, d_g1b;
+    DBuf<double> d_e1b, d_v1b, d_coef1b;
+    bool one_body_last = false;  // the last compute evaluated the 1B term
+    DBuf<double> d_wrap_in, d_wrap_out;  // mbxg_fix_coordinates
     int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
@@ -206,7 +209,11 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         delete c;
         return fail(MBXG_EHIP, "mbxg_create: stream/event creation failed: %s", hipGetErrorString(e));
     }
-    if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(16) ||
+    int n1b = 0;
+    const double* fit1b = ch4_1b_fit(&n1b);
+    // totals: 2B record, 3B record, 1B record (kPartials each)
+    if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) ||
+        c->d_totals.reserve(3 * kPartials) || c->d_coef1b.reserve(n1b) ||
         c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
         c->d_coefn[FAM_CH4H2O_2B].reserve(kNCoef_ch4h2o_2b) || c->d_coefn[FAM_CH4CH4_2B].reserve(kNCoef_ch4ch4_2b) ||
         c->d_coefn[FAM_CH4H2O2_3B].reserve(kNCoef_ch4h2o2_3b)) {
@@ -214,6 +221,8 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         return fail(MBXG_ENOMEM, "mbxg_create: device allocation failed");
     }
     HIPCHK(hipMemcpy(c->d_coef2.p, kFit_mbpol_2b, sizeof(double) * kNCoef_mbpol_2b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_coef1b.p, fit1b, sizeof(double) * n1b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->d_totals.p, 0, sizeof(double) * 3 * kPartials));
     HIPCHK(hipMemcpy(c->d_coef3.p, kFit_mbpol_3b, sizeof(double) * kNCoef_mbpol_3b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_coefn[FAM_CH4H2O_2B].p, kFit_ch4h2o_2b, sizeof(double) * kNCoef_ch4h2o_2b,
                      hipMemcpyHostToDevice));
@@ -278,6 +287,12 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_xyz.release();
     c->d_grad.release();
     c->d_totals.release();
+    c->d_w1b.release();
+    c->d_e1b.release();
+    c->d_v1b.release();
+    c->d_coef1b.release();
+    c->d_wrap_in.release();
+    c->d_wrap_out.release();
     c->d_coef2.release();
     c->d_coef3.release();
     c->d_box.release();
@@ -387,12 +402,17 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     if (int r = set_members(c)) return r;
     if (int r = set_configs(c, 1, nullptr)) return r;
     c->water_local.clear();
-    for (int i = 0; i < nmon; ++i)
+    c->ch4_local.clear();
+    for (int i = 0; i < nmon; ++i) {
         if (c->kind[i] == MBXG_H2O && c->islocal[i]) c->water_local.push_back(i);
-    if (c->d_w1b.reserve(nm)) return fail(MBXG_ENOMEM, "mbxg_set_topology: device allocation failed");
-    if (!c->water_local.empty())
-        HIPCHK(hipMemcpy(c->d_w1b.p, c->water_local.data(), sizeof(int) * c->water_local.size(),
-                         hipMemcpyHostToDevice));
+        if (c->kind[i] == MBXG_CH4 && c->islocal[i]) c->ch4_local.push_back(i);
+    }
+    std::vector<int> m1b(c->water_local);
+    m1b.insert(m1b.end(), c->ch4_local.begin(), c->ch4_local.end());
+    if (c->d_w1b.reserve(nm) || c->d_e1b.reserve(nm) || c->d_v1b.reserve(6 * nm))
+        return fail(MBXG_ENOMEM, "mbxg_set_topology: device allocation failed");
+    if (!m1b.empty())
+        HIPCHK(hipMemcpy(c->d_w1b.p, m1b.data(), sizeof(int) * m1b.size(), hipMemcpyHostToDevice));
     c->have_coords = false;
     c->have_results = false;
     c->need_sync = true;
@@ -533,6 +553,9 @@ extern "C" int mbxg_set_coefficients(mbxg_ctx* c, const char* model, const doubl
         dst = c->d_coefn[FAM_CH4H2O2_3B].p;
         fit = kFit_ch4h2o2_3b;
         need = kNCoef_ch4h2o2_3b;
+    } else if (m == "ch4_1b") {
+        dst = c->d_coef1b.p;
+        fit = ch4_1b_fit(&need);
     } else {
         return fail(MBXG_EINVAL, "mbxg_set_coefficients: unknown model '%s'", model);
     }
@@ -906,6 +929,20 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     if (c->natoms > 0) launch_zero(c->d_grad.p, 3 * (size_t)c->natoms, st);
     launch_zero(c->d_fam_tot.p, (size_t)kFamilies * kPartials, st);
+    // 1B (System::Get1B, system.cpp:2269-2401): every local monomer's gradient is written at its own
+    // atoms of the zeroed gradient before any 2B/3B kernel adds to it (they start after the fork)
+    c->one_body_last = (terms & MBXG_1B) && !(terms & MBXG_LISTS_ONLY);
+    if (c->one_body_last) {
+        const int nw = (int)c->water_local.size(), nc = (int)c->ch4_local.size(), n1 = nw + nc;
+        double* g = do_grads ? c->d_grad.p : nullptr;
+        double* v = do_grads ? c->d_v1b.p : nullptr;
+        HIPCHK(launch_1b_ps(c->d_xyz.p, c->d_first.p, c->d_w1b.p, nw, c->d_e1b.p, g, v, n1, st));
+        HIPCHK(launch_1b_ch4(c->d_xyz.p, c->d_first.p, c->d_w1b.p + nw, nc, c->d_coef1b.p, c->d_e1b.p + nw, g,
+                             v ? v + nw : nullptr, n1, st));
+        hipLaunchKernelGGL(k_reduce_1b, dim3(1), dim3(256), 0, st, c->d_e1b.p, v, n1, n1, c->d_totals.p + 2 * kPartials);
+    } else {
+        launch_zero(c->d_totals.p + 2 * kPartials, kPartials, st);
+    }
     if (c->nconf > 1) launch_zero(c->d_conf_fam.p, (size_t)kFamilies * kPartials * c->nconf, st);
     HIPCHK(hipGetLastError());
     c->evaluated = L.built && !(terms & MBXG_LISTS_ONLY);
@@ -1014,7 +1051,7 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev) HIPCHK(hipEventRecord(ev[12], st));
     c->have_results = true;
     c->checked = sync || !L.built;
-    if (sync) {
+    if (sync && L.built) {  // the list buffers are sized (a 1B-only compute sizes nothing)
         c->need_sync = false;
         drop_graph(c);  // buffers may have moved
     }
@@ -1062,6 +1099,10 @@ int finish(mbxg_ctx* c) {
 extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_compute: null context");
     if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_compute: coordinates not set");
+    if (terms & ~(MBXG_1B | MBXG_2B | MBXG_3B | MBXG_LISTS_ONLY))
+        return fail(MBXG_EINVAL, "mbxg_compute: unknown terms 0x%x", terms);
+    if ((terms & MBXG_1B) && c->nconf > 1)
+        return fail(MBXG_EINVAL, "mbxg_compute: the 1B term is not evaluated over batches");
     if (!(terms & MBXG_LISTS_ONLY))
         for (int i = 0; i < c->nmon; ++i)
             if (c->kind[i] != MBXG_H2O && c->kind[i] != MBXG_CH4)
@@ -1251,7 +1292,7 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_results: null context");
     if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_results: nothing computed");
     if (int r = finish(c)) return r;
-    double tot[16];
+    double tot[3 * kPartials];
     HIPCHK(hipMemcpyAsync(tot, c->d_totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     if (grad && c->natoms > 0)
         HIPCHK(hipMemcpyAsync(grad, c->d_grad.p, sizeof(double) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream));
@@ -1262,7 +1303,8 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
         // xx xy xz yy yz zz -> 3x3 with the reference's mirroring
         const int slot[6] = {0, 1, 2, 4, 5, 8};
         for (int q = 0; q < 9; ++q) virial[q] = 0.0;
-        for (int q = 0; q < 6; ++q) virial[slot[q]] = tot[1 + q] + tot[kPartials + 1 + q];
+        for (int q = 0; q < 6; ++q)
+            virial[slot[q]] = tot[2 * kPartials + 1 + q] + tot[1 + q] + tot[kPartials + 1 + q];
         virial[3] = virial[1];
         virial[6] = virial[2];
         virial[7] = virial[5];
@@ -1390,49 +1432,39 @@ extern "C" int mbxg_synchronize(mbxg_ctx* c) {
 
 extern "C" int mbxg_eval_1b(mbxg_ctx* c, int do_grads, double* e1b, double* grad, double* virial) {
     if (!c || !e1b) return fail(MBXG_EINVAL, "mbxg_eval_1b: bad arguments");
-    if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_eval_1b: coordinates not set");
-    for (int i = 0; i < c->nmon; ++i)
-        if (c->kind[i] != MBXG_H2O)
-            return fail(MBXG_EINVAL, "mbxg_eval_1b: monomer %d: only h2o has a GPU 1B term", i);
+    if (int r = mbxg_compute(c, MBXG_1B, do_grads)) return r;
+    if (int r = mbxg_get_energies(c, e1b, nullptr, nullptr)) return r;
+    if (int r = mbxg_get_results(c, nullptr, nullptr, do_grads ? grad : nullptr, virial)) return r;
+    if (!do_grads && grad && c->natoms > 0) std::fill(grad, grad + 3 * (size_t)c->natoms, 0.0);
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_energies(mbxg_ctx* c, double* e1b, double* e2b, double* e3b) {
+    if (!c) return fail(MBXG_EINVAL, "mbxg_get_energies: null context");
+    if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_energies: nothing computed");
+    if (int r = finish(c)) return r;
+    double tot[3 * kPartials];
+    HIPCHK(hipMemcpyAsync(tot, c->d_totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (e1b) *e1b = tot[2 * kPartials];
+    if (e2b) *e2b = tot[0];
+    if (e3b) *e3b = tot[kPartials];
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_fix_coordinates(mbxg_ctx* c, const double* xyz_in, double* xyz_out) {
+    if (!c || (c->natoms > 0 && (!xyz_in || !xyz_out))) return fail(MBXG_EINVAL, "mbxg_fix_coordinates: bad arguments");
+    if (c->natoms == 0) return MBXG_OK;
     HIPCHK(hipSetDevice(c->device));
+    const size_t n3 = 3 * (size_t)c->natoms;
+    if (c->d_wrap_in.reserve(n3) || c->d_wrap_out.reserve(n3))
+        return fail(MBXG_ENOMEM, "mbxg_fix_coordinates: device allocation failed");
     hipStream_t st = c->stream;
-    const int n = (int)c->water_local.size();
-    const size_t na3 = 3 * (size_t)c->natoms;
-    if (c->d_e1b.reserve(n > 0 ? n : 1) || c->d_v1b.reserve(6 * (size_t)(n > 0 ? n : 1)) ||
-        c->d_g1b.reserve(na3 > 0 ? na3 : 1))
-        return fail(MBXG_ENOMEM, "mbxg_eval_1b: device allocation failed");
-    // FixMonomerCoordinates of the current input, as a compute does (the energy is intramolecular;
-    // the virial takes the wrapped coordinates, as ps::pot_nasa takes System::xyz_)
-    if (c->nmon > 0)
-        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p, c->d_xyz.p,
-                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
-    if (do_grads && na3 > 0) launch_zero(c->d_g1b.p, na3, st);
-    HIPCHK(launch_1b_ps(c->d_xyz.p, c->d_first.p, c->d_w1b.p, n, c->d_e1b.p, do_grads ? c->d_g1b.p : nullptr,
-                        do_grads ? c->d_v1b.p : nullptr, st));
-    std::vector<double> e(n), v(do_grads ? 6 * (size_t)n : 0);
-    if (n > 0) HIPCHK(hipMemcpyAsync(e.data(), c->d_e1b.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (do_grads && n > 0)
-        HIPCHK(hipMemcpyAsync(v.data(), c->d_v1b.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
-    if (do_grads && grad && na3 > 0)
-        HIPCHK(hipMemcpyAsync(grad, c->d_g1b.p, sizeof(double) * na3, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->d_wrap_in.p, xyz_in, sizeof(double) * n3, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_wrap_in.p, c->d_wrap_out.p,
+                       c->d_first.p, c->d_nat.p, c->nmon, c->box);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(xyz_out, c->d_wrap_out.p, sizeof(double) * n3, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    // totals in monomer order (deterministic)
-    double tot = 0.0;
-    for (int t = 0; t < n; ++t) tot += e[t];
-    *e1b = tot;
-    if (virial) {
-        const int slot[6] = {0, 1, 2, 4, 5, 8};
-        for (int q = 0; q < 9; ++q) virial[q] = 0.0;
-        if (do_grads)
-            for (int q = 0; q < 6; ++q) {
-                double s = 0.0;
-                for (int t = 0; t < n; ++t) s += v[(size_t)q * n + t];
-                virial[slot[q]] = s;
-            }
-        virial[3] = virial[1];
-        virial[6] = virial[2];
-        virial[7] = virial[5];
-    }
-    if (!do_grads && grad && na3 > 0) std::fill(grad, grad + na3, 0.0);
     return MBXG_OK;
 }

@@ -1,11 +1,22 @@
-// MB-pol 1B: the Partridge-Schwenke water monomer energy and gradient, ps::pot_nasa
-// (potential/1b/ps.cpp:346-633), one lane per water.  The reference's operation order is kept
-// (the build disables contraction): bond vectors and lengths (:393-433), cos of the HOH angle
-// (:437-440), Morse and H-H terms (:456-466), the power tables fmat (:475-491), the four
-// coefficient sums (gen/ps_1b.inc, :505-526), energy (:533-539), chain rule to H1, H2, O
-// (:542-582) and the virial terms -sum_atoms w_a g_b (:584-603).
+// 1B terms, one lane per monomer.
+//
+// MB-pol water: the Partridge-Schwenke monomer energy and gradient, ps::pot_nasa
+// (potential/1b/ps.cpp:346-633).  The reference's operation order is kept (the build disables
+// contraction): bond vectors and lengths (:393-433), cos of the HOH angle (:437-440), Morse and
+// H-H terms (:456-466), the power tables fmat (:475-491), the four coefficient sums
+// (gen/ps_1b.inc, :505-526), energy (:533-539), chain rule to H1, H2, O (:542-582) and the
+// virial terms -sum_atoms w_a g_b (:584-603).
+//
+// MB-nrg CH4: x1b_A1B4_v1x::eval (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371-470): ten
+// variables exp(k (d - r)) over the C-H and H-H pairs (holder data gen/holder_ch4_1b.inc), the
+// degree-5 polynomial poly_1b_A1B4_deg5_exp0_v1x (gen/poly_ch4_1b.inc, value and gradient in one
+// reverse-mode pass), the chain rule per variable in the reference's order (variable::grads,
+// :56-63) and the virial -sum_atoms x_a g_b (:442-462).
 #include <hip/hip_runtime.h>
 
+#include "gen/coef_ch4_1b.inc"
+#include "gen/holder_ch4_1b.inc"
+#include "gen/poly_ch4_1b.inc"
 #include "gen/ps_1b.inc"
 #include "mbxg_kernels.hpp"
 
@@ -13,11 +24,11 @@ namespace mbxg {
 
 namespace {
 
-// e_out[t], and with grads g_out (3 doubles per atom, the water's 9 written) and vir_out[q * n + t]
+// e_out[t], and with grads g_out (3 doubles per atom, the water's 9 written) and vir_out[q * ld + t]
 // (xx xy xz yy yz zz) for water mons[t] (first atom first_atom[mons[t]], atoms O H1 H2).
 __global__ void __launch_bounds__(64) k_1b_ps(const double* __restrict__ xyz, const int* __restrict__ first_atom,
                                               const int* __restrict__ mons, int n, double* __restrict__ e_out,
-                                              double* __restrict__ g_out, double* __restrict__ vir_out) {
+                                              double* __restrict__ g_out, double* __restrict__ vir_out, int ld) {
     using namespace ps1b;
     const int t = blockIdx.x * 64 + threadIdx.x;
     if (t >= n) return;
@@ -93,19 +104,113 @@ __global__ void __launch_bounds__(64) k_1b_ps(const double* __restrict__ xyz, co
         const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
         for (int q = 0; q < 6; ++q)
-            vir_out[(size_t)q * n + t] =
+            vir_out[(size_t)q * ld + t] =
                 -w[va[q]] * g[vb[q]] - w[3 + va[q]] * g[3 + vb[q]] - w[6 + va[q]] * g[6 + vb[q]];
+    }
+}
+
+// The CH4 holder: the same outputs for monomers mons[0..n) of H's kind.
+template <class H>
+__global__ void __launch_bounds__(64) k_1b_nrg(const double* __restrict__ xyz, const int* __restrict__ first_atom,
+                                               const int* __restrict__ mons, int n, const double* __restrict__ coef,
+                                               double* __restrict__ e_out, double* __restrict__ g_out,
+                                               double* __restrict__ vir_out, int ld) {
+    constexpr int NA = H::kNReal, NV = H::kNVars;
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= n) return;
+    const size_t base = 3 * (size_t)first_atom[mons[t]];
+    double w[3 * NA];
+#pragma unroll
+    for (int i = 0; i < 3 * NA; ++i) w[i] = xyz[base + i];
+    // variable::v_exp0 (x1b_A1B4_deg5_exp0_v1x.cpp:67-84): v = exp(k (d - r)), and the vector
+    // (p1 - p2) * (-k v / r) that the chain rule scales
+    double v[NV], dv[NV][3];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const NrgVar V = H::kVars[j];
+        const double* p1 = w + 3 * V.s1;
+        const double* p2 = w + 3 * V.s2;
+        double d[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double e1 = exp(V.k * (V.d - r));
+        const double gg = -V.k * e1 / r;
+        v[j] = e1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dv[j][i] = d[i] * gg;
+    }
+    double gp[NV];
+    const double E = poly_ch4_1b(coef, v, gp);
+    e_out[t] = E;
+    if (!g_out) return;
+    double g[3 * NA];
+#pragma unroll
+    for (int i = 0; i < 3 * NA; ++i) g[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {  // variable::grads (:56-63), variables in order
+        const NrgVar V = H::kVars[j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double dd = gp[j] * dv[j][i];
+            g[3 * V.s1 + i] += dd;
+            g[3 * V.s2 + i] -= dd;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3 * NA; ++i) g_out[base + i] = g[i];
+    if (vir_out) {
+        const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double s = -w[va[q]] * g[vb[q]];
+#pragma unroll
+            for (int a = 1; a < NA; ++a) s = s - w[3 * a + va[q]] * g[3 * a + vb[q]];
+            vir_out[(size_t)q * ld + t] = s;
+        }
     }
 }
 
 }  // namespace
 
 hipError_t launch_1b_ps(const double* xyz, const int* first_atom, const int* mons, int n, double* e_out,
-                        double* g_out, double* vir_out, hipStream_t st) {
+                        double* g_out, double* vir_out, int ld, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_1b_ps, dim3((n + 63) / 64), dim3(64), 0, st, xyz, first_atom, mons, n, e_out, g_out,
-                       vir_out);
+                       vir_out, ld);
     return hipGetLastError();
+}
+
+hipError_t launch_1b_ch4(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
+                         double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_1b_nrg<Holder_ch4_1b>, dim3((n + 63) / 64), dim3(64), 0, st, xyz, first_atom, mons, n,
+                       coef, e_out, g_out, vir_out, ld);
+    return hipGetLastError();
+}
+
+const double* ch4_1b_fit(int* n) {
+    *n = kNCoef_ch4_1b;
+    return kFit_ch4_1b;
+}
+
+// One workgroup: out[0] = sum of e[0..n), out[1 + q] = sum of vir[q * ld + 0..n) (zeros when vir is
+// null); each thread sums a fixed stride, then a fixed tree (deterministic totals).
+__global__ void __launch_bounds__(256) k_reduce_1b(const double* __restrict__ e, const double* __restrict__ vir, int n,
+                                                   int ld, double* __restrict__ out) {
+    __shared__ double s[kPartials][256];
+    const int t = threadIdx.x;
+    for (int q = 0; q < kPartials; ++q) {
+        double a = 0.0;
+        if (q == 0 || vir)
+            for (int i = t; i < n; i += 256) a += q == 0 ? e[i] : vir[(size_t)(q - 1) * ld + i];
+        s[q][t] = a;
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int q = 0; q < kPartials; ++q) s[q][t] += s[q][t + h];
+        __syncthreads();
+    }
+    if (t < kPartials) out[t] = s[t][0];
 }
 
 }  // namespace mbxg

@@ -167,10 +167,16 @@ hipError_t launch_mbpol2b_chain(const int* list, const int* n_dev, int cap, cons
 hipError_t launch_mbpol3b_chain(const int* list, const int* n_dev, int cap, const NmerCtx& N, int do_grads,
                                 double* grad, double* e_out, double* vir_out, double* partials, double* ws,
                                 hipStream_t st);
-// MB-pol 1B (eval_1b.hip): energies e_out[t] of waters mons[0..n), and with g_out the 9 gradient
-// components of each (3 doubles per atom, written at its atoms) and vir_out[q * n + t].
+// 1B terms (eval_1b.hip): energies e_out[t] of monomers mons[0..n), and with g_out the gradient
+// components of each (3 doubles per atom, written at its atoms) and vir_out[q * ld + t] (q = xx xy
+// xz yy yz zz).  Water: MB-pol (Partridge-Schwenke); CH4: MB-nrg A1B4 with coefficients coef
+// (ch4_1b_fit: the fitted ones).  k_reduce_1b sums n energies and virial rows into out[kPartials].
 hipError_t launch_1b_ps(const double* xyz, const int* first_atom, const int* mons, int n, double* e_out,
-                        double* g_out, double* vir_out, hipStream_t st);
+                        double* g_out, double* vir_out, int ld, hipStream_t st);
+hipError_t launch_1b_ch4(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
+                         double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st);
+const double* ch4_1b_fit(int* n);
+__global__ void k_reduce_1b(const double* e, const double* vir, int n, int ld, double* out);
 hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
                                   const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);
 // MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and

@@ -64,7 +64,10 @@ void read_monomers(Reader& r, NrgSystem& sys) {
         m.id = lower(id);
         while (true) {
             if (!r.next(line)) fail(__func__, r, "Expected coordinates or ENDMON", "end of file");
-            if (keyword(line) == "endmon") break;
+            // the first line after MONOMER is always an atom line (read_nrg.cpp:229-246): a
+            // monomer without atoms is an error
+            if (keyword(line) == "endmon" && !m.atoms.empty()) break;
+            if (keyword(line) == "endmon") fail(__func__, r, "Expected coordinates or ENDMON", line);
             std::istringstream as(line);
             std::string name, x, y, z;
             if (!(as >> name >> x)) fail(__func__, r, "Expected coordinates or ENDMON", line);
@@ -114,7 +117,7 @@ void ParseNrg(std::istream& is, std::vector<NrgSystem>& systems) {
     }
 }
 
-void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>& systems) {
+void ReadNrg(const char* filename, std::vector<bblock::System>& systems) {
     if (!filename) throw CUException(__func__, __FILE__, __LINE__, "null NRG file name");
     std::ifstream ifs(filename);
     if (!ifs)
@@ -123,17 +126,17 @@ void ReadNrg(const char* filename, std::vector<std::unique_ptr<bblock::System>>&
     std::vector<NrgSystem> parsed;
     ParseNrg(ifs, parsed);
     for (const NrgSystem& p : parsed) {
-        auto sys = std::make_unique<bblock::System>();
-        for (const NrgMonomer& m : p.monomers) sys->AddMonomer(m.xyz, m.atoms, m.id);
-        for (const auto& mol : p.molecules) sys->AddMolecule(mol);
-        sys->Initialize();
+        bblock::System sys;
+        for (const NrgMonomer& m : p.monomers) sys.AddMonomer(m.xyz, m.atoms, m.id);
+        for (const auto& mol : p.molecules) sys.AddMolecule(mol);
+        sys.Initialize();
         systems.push_back(std::move(sys));
     }
 }
 
-void SaveNrg(std::ostream& os, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits) {
+void SaveNrg(std::ostream& os, std::vector<bblock::System> systems, int digits) {
     for (size_t i = 0; i < systems.size(); ++i) {
-        bblock::System& sys = *systems[i];
+        bblock::System& sys = systems[i];
         const std::vector<double> xyz = sys.GetRealXyz();
         const std::vector<std::string> names = sys.GetRealAtomNames();
         os << "SYSTEM " << i << std::endl;
@@ -155,11 +158,11 @@ void SaveNrg(std::ostream& os, const std::vector<std::unique_ptr<bblock::System>
     }
 }
 
-void WriteNrg(const char* filename, const std::vector<std::unique_ptr<bblock::System>>& systems, int digits) {
+void WriteNrg(const char* filename, std::vector<bblock::System> systems, int digits) {
     if (!filename) throw std::runtime_error("could not open NRG file for writting");
     std::ofstream ofs(filename);
     if (!ofs) throw std::runtime_error("could not open NRG file for writting");
-    SaveNrg(ofs, systems, digits);
+    SaveNrg(ofs, std::move(systems), digits);
 }
 
 }  // namespace tools

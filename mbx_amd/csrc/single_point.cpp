@@ -1,4 +1,4 @@
-// single_point: the 1B (water-only systems), 2B and 3B energies, virials and, on request, gradients of the
+// single_point: the 1B, 2B and 3B energies, virials and, on request, gradients of the
 // first system of an NRG file -- the reference's src/tests/single_point.cpp (ReadNrg, then
 // SetUpFromJson with the optional json file, then Energy(true) and the per-term block of its
 // PRINT_VIRIAL / PRINT_GRADS output) for the terms this library computes.
@@ -13,7 +13,6 @@
 #include <exception>
 #include <fstream>
 #include <iostream>
-#include <memory>
 #include <string>
 #include <vector>
 
@@ -78,29 +77,24 @@ int main(int argc, char** argv) {
                          argv[0]);
             return 1;
         }
-        std::vector<std::unique_ptr<bblock::System>> systems;
+        std::vector<bblock::System> systems;
         tools::ReadNrg(nrg, systems);
         if (out) tools::WriteNrg(out, systems);
-        bblock::System& sys = *systems[0];
+        bblock::System& sys = systems[0];
         sys.SetUpFromJson(json);
         const double e = sys.Energy(true);
         std::printf("Energy= %.15e\n", e);
         const std::vector<double> g = sys.GetRealGrads();
-        try {  // the 1B term exists for water only: other monomers make OneBodyEnergy throw
-            const double e1 = sys.OneBodyEnergy(true);
-            std::printf(" 1B Energy [kcal/mol] %.15e\n", e1);
-            print_virial("1B", sys.GetVirial());
-        } catch (const std::exception& ex) {
-            std::fflush(stdout);
-            std::cerr << " ** Warning ** : 1B term not computed: " << ex.what() << std::endl;
-        }
+        const double e1 = sys.OneBodyEnergy(true);
+        std::printf(" 1B Energy [kcal/mol] %.15e\n", e1);
+        print_virial("1B", sys.GetVirial());
         const double e2 = sys.TwoBodyEnergy(true);
         std::printf(" 2B Energy [kcal/mol] %.15e\n", e2);
         print_virial("2B", sys.GetVirial());
         const double e3 = sys.ThreeBodyEnergy(true);
         std::printf(" 3B Energy [kcal/mol] %.15e\n", e3);
         print_virial("3B", sys.GetVirial());
-        if (grads)  // the gradients of Energy(true): 2B + 3B, input atom order
+        if (grads)  // the gradients of Energy(true): 1B + 2B + 3B, input atom order
             for (size_t i = 0; i < g.size() / 3; ++i)
                 std::printf("%.15e %.15e %.15e\n", g[3 * i], g[3 * i + 1], g[3 * i + 2]);
     } catch (const std::exception& ex) {

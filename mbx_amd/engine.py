@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmbxg.so")
 
 H2O, CH4, OTHER = 0, 1, 2
-TERM_2B, TERM_3B, LISTS_ONLY = 1, 2, 4
+TERM_2B, TERM_3B, LISTS_ONLY, TERM_1B = 1, 2, 4, 8
 KIND_OF = {"h2o": H2O, "ch4": CH4}
 NAT_OF = {H2O: 3, CH4: 5}
 
@@ -51,6 +51,8 @@ PROTOTYPES = {
     "mbxg_set_families": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_set_use_ghost": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_eval_1b": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+    "mbxg_get_energies": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "mbxg_fix_coordinates": (ctypes.c_int, [_vp, _dp, _dp]),
     "mbxg_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
     "mbxg_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
@@ -139,9 +141,22 @@ class Engine:
         (see mbxg_set_use_ghost)."""
         _check(lib().mbxg_set_use_ghost(self._c, int(bool(on))))
 
+    def energies(self):
+        """(E1B, E2B, E3B) of the last compute (see mbxg_get_energies)."""
+        e = [ctypes.c_double() for _ in range(3)]
+        _check(lib().mbxg_get_energies(self._c, *[ctypes.byref(v) for v in e]))
+        return tuple(v.value for v in e)
+
+    def fix_coordinates(self, xyz):
+        """FixMonomerCoordinates of host coordinates under the current box (mbxg_fix_coordinates)."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        out = np.zeros_like(x)
+        _check(lib().mbxg_fix_coordinates(self._c, _d(x), _d(out)))
+        return out
+
     def eval_1b(self, do_grads=True):
-        """MB-pol 1B energy of the local waters at the current coordinates: (E1B, gradient per
-        atom or None, virial[9]) (see mbxg_eval_1b)."""
+        """1B energy of the local monomers (MB-pol water, MB-nrg CH4) at the current coordinates:
+        (E1B, gradient per atom or None, virial[9]) (see mbxg_eval_1b)."""
         e = ctypes.c_double()
         g = np.zeros(3 * self.natoms) if do_grads else None
         v = np.zeros(9)

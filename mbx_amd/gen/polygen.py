@@ -743,6 +743,74 @@ def generate_ps(out_dir):
     return path
 
 
+# ------------------------------------------------------------------------------------------
+# 1B polynomials (any degree): reverse mode over the full monomial trie.  Node n (a sorted
+# prefix of variable indices, reached from its parent over the edge variable v) has the product
+# P(n) = P(parent) * x_v; E = sum_n c(n) P(n), with c(n) the coefficient of the monomial equal
+# to n (zero if none).  A depth-first walk returns the adjoint R(n) = c(n) + sum_children
+# x_v' R(child) and adds P(parent) R(n) to g_v on the way back, so the value and gradient come
+# out of one pass with the path's products and adjoints as the only live temporaries.
+ONE_BODY = ("ch4_1b",)
+
+
+def emit_trie(fn, nvars, monos):
+    coef_of = {m: k for k, m in monos}
+    children = collections.defaultdict(list)
+    for _, m in monos:
+        for d in range(1, len(m) + 1):
+            parent, v = m[:d - 1], m[d - 1]
+            if v not in children[parent]:
+                children[parent].append(v)
+    for kids in children.values():
+        kids.sort()
+    out = ["__device__ __forceinline__ double %s(const double* __restrict__ C, const double* x, double* g) {" % fn,
+           "    for (int i = 0; i < %d; ++i) g[i] = 0.0;" % nvars, "    double E = 0.0;"]
+    w = out.append
+    counter = [0]
+
+    def walk(node, pname):
+        """Statements for the subtree of `node`; returns the name holding R(node)."""
+        i = counter[0]
+        counter[0] += 1
+        r = "R%d" % i
+        k = coef_of.get(node)
+        w("    double %s = %s;" % (r, "C[%d]" % k if k is not None else "0.0"))
+        for v in children.get(node, []):
+            child = node + (v,)
+            if children.get(child):
+                pc = "P%d" % counter[0]
+                w("    const double %s = %s;" % (pc, ("%s * x[%d]" % (pname, v)) if pname else "x[%d]" % v))
+            else:
+                pc = None
+            rc = walk(child, pc)
+            w("    g[%d] %s;" % (v, ("= __fma_rn(%s, %s, g[%d])" % (pname, rc, v)) if pname else "+= %s" % rc))
+            w("    %s = __fma_rn(x[%d], %s, %s);" % (r, v, rc, r))
+        return r
+
+    for v in children[()]:
+        pc = "P%d" % counter[0] if children.get((v,)) else None
+        if pc:
+            w("    const double %s = x[%d];" % (pc, v))
+        rc = walk((v,), pc)
+        w("    g[%d] += %s;" % (v, rc))
+        w("    E = __fma_rn(x[%d], %s, E);" % (v, rc))
+    w("    return E;")
+    w("}")
+    return "\n".join(out) + "\n"
+
+
+def generate_1b(name, out_dir):
+    nvars, coef, monos = load_table(name)
+    hdr = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+           "// %d monomials, %d coefficients: value and gradient by reverse mode over the monomial trie."
+           % (len(monos), len(coef)), "#pragma once", ""]
+    src = "\n".join(hdr) + emit_trie("poly_%s" % name, nvars, monos)
+    write_if_changed(os.path.join(out_dir, "poly_%s.inc" % name), src)
+    write_if_changed(os.path.join(out_dir, "coef_%s.inc" % name), coef_source(name, coef))
+    generate_holder(name, out_dir)
+    return len(monos)
+
+
 def main():
     out_dir = os.path.join(ROOT, "mbx_amd", "csrc", "gen")
     os.makedirs(out_dir, exist_ok=True)
@@ -758,6 +826,8 @@ def main():
         print(generate(name, "poly_" + name, out_dir))
         generate_holder(name, out_dir)
     print(generate_ps(out_dir))
+    for name in ONE_BODY:
+        print(name, generate_1b(name, out_dir))
 
 
 if __name__ == "__main__":

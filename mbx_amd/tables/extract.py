@@ -14,7 +14,7 @@ What is extracted, per polynomial (MB-pol 2B/3B and the MB-nrg CH4 families):
     src/potential/3b/poly-3b-v2x-nogrd.cpp:39).  Every monomial of those
     routines belongs to exactly one coefficient with weight 1; the script
     checks this and aborts otherwise.
-and per MB-nrg holder (``<name>.holder.json``): its variables as data (form, k, d, sites),
+and per MB-nrg holder (``<name>.holder.json``, 2B/3B and the CH4 1B): its variables as data (form, k, d, sites),
 its lone-pair sites and switching radii; and the data of the MB-pol 1B water potential
 (``ps_1b.json``: the Partridge-Schwenke constants and coefficient tables of
 src/potential/1b/ps.cpp).
@@ -40,6 +40,14 @@ POLYS = [
      "src/potential/2b/x2b_A1B4_A1B4_deg4_exp0_v1x.cpp", 45, 1345),
     ("ch4h2o2_3b", "src/potential/3b/poly_3b_A1B4_C1D2_C1D2_deg3_nograd_v1.cpp",
      "src/potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.cpp", 55, 1254),
+    # MB-nrg CH4 1B (degree 5)
+    ("ch4_1b", "src/potential/1b/poly_1b_A1B4_deg5_exp0_v1.cpp",
+     "src/potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp", 10, 207),
+]
+
+# 1B holders: the monomer, its atom count and the holder text (x1b_A1B4_deg5_exp0_v1x.cpp:309-343)
+HOLDERS_1B = [
+    ("ch4_1b", "src/potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp", "ch4", 5),
 ]
 
 # Holders of the MB-nrg families: argument order of the monomers, and how the variables are
@@ -88,6 +96,26 @@ def parse_holder(text, mons, nats):
     sw = {"inner": par.get("m_r2i", par.get("m_ri")), "outer": par.get("m_r2f", par.get("m_ro"))}
     return {"monomers": mons, "atoms": nats, "nreal": nreal, "nsites": nreal + (2 if vsite else 0),
             "vsite": vsite, "switch": sw, "vars": [vars_[i] for i in range(nv)]}
+
+
+def parse_holder_1b(text, mon, nat):
+    """Variables of a generated MB-nrg 1B holder: v[i] = vr[i].v_exp0(d, k, P, Q) over the
+    monomer's atoms (sites named A_1, B_1, ... at xcrd + 3 * site), with the parameters m_k_*,
+    m_d_* of the holder's constructor.  Form exp0: exp(k (d - r))."""
+    src = re.sub(r"//[^\n]*", "", text)
+    ev = src[src.index("::eval("):]
+    ev = ev[:ev.index("::eval(", 10)]
+    sites = {name: int(off) // 3 for name, off in re.findall(r"const double\*\s*([A-Z]_\d+)\s*=\s*xcrd\s*\+\s*(\d+);", ev)}
+    ctor = src[:src.index("::eval(")]
+    par = {k: float(v) for k, v in re.findall(r"\b(m_\w+)\s*=\s*([-+0-9.eE]+)\s*;", ctor)}
+    vars_ = {}
+    for idx, args in re.findall(r"v\[(\d+)\]\s*=\s*vr\[\d+\]\.v_exp0\(([^)]*)\);", ev):
+        a = [t.strip() for t in args.split(",")]
+        vars_[int(idx)] = ["exp0", par[a[1]], par[a[0]], sites[a[2]], sites[a[3]]]
+    nv = max(vars_) + 1
+    assert sorted(vars_) == list(range(nv)) and len(sites) == nat
+    return {"monomers": [mon], "atoms": [nat], "nreal": nat, "nsites": nat, "vsite": None,
+            "switch": {"inner": 0.0, "outer": 0.0}, "vars": [vars_[i] for i in range(nv)]}
 
 
 class Poly:
@@ -162,6 +190,7 @@ def _routine_body(src):
 
 def expand(text):
     body = re.sub(r"//[^\n]*", "", _routine_body(text))
+    body = re.sub(r"^\s*#[^\n]*", "", body, flags=re.M)  # #ifdef DEBUG ... #endif lines
     # the 2B routine loops over dimers: x[nv + nd * 30] -> x[30]
     body = re.sub(r"x\[\s*nv\s*\+\s*nd\s*\*\s*(\d+)\s*\]", r"x[\1]", body)
     body = re.sub(r"for\s*\([^)]*\)\s*\{", ";", body)
@@ -170,8 +199,8 @@ def expand(text):
     env = {"a": _Coef(), "x": _Var()}
     result = None
     for s in stmts:
-        if s.startswith("const double"):
-            name, expr = s[len("const double"):].split("=", 1)
+        if s.startswith("const double") or s.startswith("double "):
+            name, expr = s[len("const double") if s.startswith("const") else len("double"):].split("=", 1)
             env[name.strip()] = eval(compile(ast.parse(expr.strip(), mode="eval"), "<poly>", "eval"), env)
         elif s.startswith("energy[nv]"):
             result = eval(compile(ast.parse(s.split("=", 1)[1].strip(), mode="eval"), "<poly>", "eval"), env)
@@ -296,6 +325,12 @@ def main():
     print("ps_1b: %d coefficients" % len(ps["c5zA"]))
     for name, holder_path, mons, nats in HOLDERS:
         h = parse_holder(open(os.path.join(ref, holder_path)).read(), mons, nats)
+        h["name"] = name
+        with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
+            json.dump(h, f, indent=1)
+        print("%s holder: %d variables, %d sites" % (name, len(h["vars"]), h["nsites"]))
+    for name, holder_path, mon, nat in HOLDERS_1B:
+        h = parse_holder_1b(open(os.path.join(ref, holder_path)).read(), mon, nat)
         h["name"] = name
         with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
             json.dump(h, f, indent=1)

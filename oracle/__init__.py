@@ -34,7 +34,7 @@ def _i(a):
 
 class _OrcPoly(ctypes.Structure):
     _fields_ = [("nvars", ctypes.c_int), ("ncoef", ctypes.c_int), ("nmono", ctypes.c_int),
-                ("coef", _dp), ("mono_k", _ip), ("mono_deg", _ip), ("mono_idx", _ip)]
+                ("coef", _dp), ("mono_k", _ip), ("mono_deg", _ip), ("mono_idx", _ip), ("stride", ctypes.c_int)]
 
 
 class Table:
@@ -55,16 +55,17 @@ class Table:
         self.nvars, self.ncoef, self.nmono = hdr["nvars"], hdr["ncoef"], hdr["nmono"]
         self.coef = np.array([float(next(it)) for _ in range(self.ncoef)])
         assert next(it).startswith("monomials")
+        rows = [[int(t) for t in next(it).split()] for _ in range(self.nmono)]
+        self.stride = max(4, max(v[1] for v in rows))
         k = np.zeros(self.nmono, np.int32)
         deg = np.zeros(self.nmono, np.int32)
-        idx = -np.ones((self.nmono, 4), np.int32)
-        for m in range(self.nmono):
-            v = [int(t) for t in next(it).split()]
+        idx = -np.ones((self.nmono, self.stride), np.int32)
+        for m, v in enumerate(rows):
             k[m], deg[m] = v[0], v[1]
             idx[m, :v[1]] = v[2:]
         self.mono_k, self.mono_deg, self.mono_idx = k, deg, idx
         self.c = _OrcPoly(self.nvars, self.ncoef, self.nmono, _d(self.coef), _i(self.mono_k),
-                          _i(self.mono_deg), _i(self.mono_idx))
+                          _i(self.mono_deg), _i(self.mono_idx), self.stride)
 
 
 class _OrcHolder(ctypes.Structure):
@@ -378,7 +379,8 @@ def abs_table(name):
         a = Table.__new__(Table)
         a.__dict__.update(t.__dict__)
         a.coef = np.abs(t.coef)
-        a.c = _OrcPoly(t.nvars, t.ncoef, t.nmono, _d(a.coef), _i(t.mono_k), _i(t.mono_deg), _i(t.mono_idx))
+        a.c = _OrcPoly(t.nvars, t.ncoef, t.nmono, _d(a.coef), _i(t.mono_k), _i(t.mono_deg), _i(t.mono_idx),
+                       t.stride)
         _TABLES[key] = a
     return _TABLES[key]
 
@@ -460,6 +462,36 @@ def ref_add_clusters(nmax, cutoff, first_xyz, box=None, islocal=None, use_ghost=
     return out[:n].astype(np.int64)
 
 
+def ref_invert_unit_cell(box):
+    """The compiled InvertUnitCell (tools/math_tools.cpp:37-64)."""
+    bi = np.zeros(9)
+    ref().ref_invert_unit_cell(_d(_arr(box)), _d(bi))
+    return bi
+
+
+def ref_fix_monomer_coordinates(xyz, box, nat):
+    """The compiled systools::FixMonomerCoordinates (sys_tools.cpp:257-320); monomers of nat[i]
+    atoms, back to back."""
+    xyz = _arr(xyz).copy()
+    box = _arr(box)
+    nat = np.ascontiguousarray(nat, np.uintp)
+    fi = np.ascontiguousarray(np.concatenate([[0], np.cumsum(nat)[:-1]]), np.uintp)
+    sz = lambda a: a.ctypes.data_as(_szp)
+    ref().ref_fix_monomer_coordinates(_d(xyz), len(xyz) // 3, _d(box), _d(ref_invert_unit_cell(box)), len(nat),
+                                      sz(nat), sz(fi))
+    return xyz
+
+
+def ref_close_image(box, xyz1, xyz2):
+    """The compiled systools::GetCloseDimerImage (sys_tools.cpp:322-365) for one dimer: xyz2 moved
+    next to xyz1's first atom."""
+    box = _arr(box)
+    x1 = _arr(xyz1).copy()
+    x2 = _arr(xyz2).copy()
+    ref().ref_close_dimer_image(_d(box), _d(ref_invert_unit_cell(box)), len(x1) // 3, len(x2) // 3, 1, _d(x1), _d(x2))
+    return x2
+
+
 def ref_water_2b3b(xyz, box, cutoff2b, cutoff3b, batch=8, nthreads=1):
     xyz = _arr(xyz).reshape(-1).copy()
     nmon = len(xyz) // 9
@@ -527,6 +559,68 @@ def ps_1b(w, grad=True):
         if grad:
             g[i] = gi
     return e, g, vir
+
+
+def nrg_1b(name, w, a=None, grad=True):
+    """MB-nrg 1B energies (orc_1b_nrg; name "ch4_1b") of monomers w [n][3*atoms]:
+    (E[n], G[n][3*atoms] or None, virial[6])."""
+    L = lib()
+    L.orc_1b_nrg.restype = ctypes.c_double
+    L.orc_1b_nrg.argtypes = [ctypes.POINTER(_OrcHolder), ctypes.POINTER(_OrcPoly), _dp, _dp, _dp, _dp]
+    h, t = holder(name), table(name)
+    na = 3 * h.desc["nreal"]
+    w = _arr(w).reshape(-1, na)
+    e = np.zeros(len(w))
+    g = np.zeros_like(w) if grad else None
+    vir = np.zeros(6)
+    aa = _arr(a) if a is not None else None
+    for i in range(len(w)):
+        wi = np.ascontiguousarray(w[i])
+        gi = np.zeros(na) if grad else None
+        e[i] = L.orc_1b_nrg(ctypes.byref(h.c), ctypes.byref(t.c), _d(aa), _d(wi), _d(gi) if grad else None,
+                            _d(vir) if grad else None)
+        if grad:
+            g[i] = gi
+    return e, g, vir
+
+
+def one_body(xyz, kinds, box=None):
+    """The 1B term of a mixed H2O (kind 0, 3 atoms) / CH4 (kind 1, 5 atoms) system, monomers in the
+    given order: e1b::get_1b_energy per monomer (potential/1b/energy1b.cpp:118-147) on the
+    coordinates System::SetPBC wraps first (FixMonomerCoordinates) when box is given.
+    Returns (E, grad[3 * atoms], virial[9]); E sums in monomer order."""
+    kinds = np.asarray(kinds)
+    nat = np.where(kinds == 0, 3, 5)
+    x = _arr(xyz).reshape(-1)
+    if box is not None and len(box):
+        x = fix_monomer_coordinates(x, box, nat)
+    first = np.concatenate([[0], np.cumsum(nat)[:-1]])
+    e = 0.0
+    g = np.zeros_like(x)
+    vir = np.zeros(6)
+    for m, (k, f) in enumerate(zip(kinds, first)):
+        w = x[3 * f:3 * f + 3 * nat[m]]
+        em, gm, vm = ps_1b(w) if k == 0 else nrg_1b("ch4_1b", w)
+        e += em[0]
+        g[3 * f:3 * f + 3 * nat[m]] = gm[0]
+        vir += vm
+    xx, xy, xz, yy, yz, zz = vir
+    return e, g, np.array([xx, xy, xz, xy, yy, yz, xz, yz, zz])
+
+
+def ref_ch4_1b(w):
+    """The compiled reference's x1b_A1B4_v1x("ch4")::eval on CH4 monomers w [n][15]:
+    (E[n], G[n][15], virial[9])."""
+    L = ref()
+    L.ref_ch4_1b.restype = None
+    L.ref_ch4_1b.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp]
+    w = np.ascontiguousarray(_arr(w).reshape(-1))
+    n = len(w) // 15
+    g = np.zeros(15 * n)
+    e = np.zeros(n)
+    v = np.zeros(9)
+    L.ref_ch4_1b(_d(w), _d(g), n, _d(e), _d(v))
+    return e, g.reshape(n, 15), v
 
 
 def ref_ps_1b(w, virial=None):

@@ -26,9 +26,9 @@ double orc_poly_eval(const orc_poly* p, const double* a, const double* x, double
         for (int j = 0; j < p->nvars; ++j) g[j] = 0.0;
     for (int m = 0; m < p->nmono; ++m) {
         const int d = p->mono_deg[m];
-        const int* id = p->mono_idx + 4 * m;
+        const int* id = p->mono_idx + p->stride * m;
         const double c = a[p->mono_k[m]];
-        double pre[5], suf[5];
+        double pre[9], suf[9];
         pre[0] = 1.0;
         for (int q = 0; q < d; ++q) pre[q + 1] = pre[q] * x[id[q]];
         e += c * pre[d];
@@ -44,7 +44,7 @@ double orc_poly_eval(const orc_poly* p, const double* a, const double* x, double
 void orc_poly_basis(const orc_poly* p, const double* x, double* basis) {
     for (int k = 0; k < p->ncoef; ++k) basis[k] = 0.0;
     for (int m = 0; m < p->nmono; ++m) {
-        const int* id = p->mono_idx + 4 * m;
+        const int* id = p->mono_idx + p->stride * m;
         double v = 1.0;
         for (int q = 0; q < p->mono_deg[m]; ++q) v *= x[id[q]];
         basis[p->mono_k[m]] += v;
@@ -791,6 +791,35 @@ static double nrg_var(int form, double k, double d, const double* p1, const doub
     gvec[1] *= gg;
     gvec[2] *= gg;
     return val;
+}
+
+/* x1b_A1B4_v1x::eval with gradients (x1b_A1B4_deg5_exp0_v1x.cpp:371-470): the variables in
+ * order, the polynomial, variable::grads per variable (:56-63), the virial rows (:442-462). */
+double orc_1b_nrg(const orc_holder* h, const orc_poly* p, const double* a, const double* w, double* g, double* vir6) {
+    double v[64], gv[64][3], gp[64];
+    for (int j = 0; j < h->nvars; ++j)
+        v[j] = nrg_var(h->form[j], h->k[j], h->d[j], w + 3 * h->s1[j], w + 3 * h->s2[j], gv[j]);
+    const double e = orc_poly_eval(p, a, v, g || vir6 ? gp : NULL);
+    if (!g && !vir6) return e;
+    double gw[3 * 16];
+    for (int i = 0; i < 3 * h->nreal; ++i) gw[i] = 0.0;
+    for (int j = 0; j < h->nvars; ++j)
+        for (int i = 0; i < 3; ++i) {
+            const double dd = gp[j] * gv[j][i];
+            gw[3 * h->s1[j] + i] += dd;
+            gw[3 * h->s2[j] + i] -= dd;
+        }
+    if (g)
+        for (int i = 0; i < 3 * h->nreal; ++i) g[i] = gw[i];
+    if (vir6) {
+        const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2};
+        for (int q = 0; q < 6; ++q) {
+            double t = -w[va[q]] * gw[vb[q]];
+            for (int at = 1; at < h->nreal; ++at) t = t - w[3 * at + va[q]] * gw[3 * at + vb[q]];
+            vir6[q] += t;
+        }
+    }
+    return e;
 }
 
 /* f_switch of the MB-nrg holders (x2b_A1B2Z2_C1D4_deg3_exp0_v1x.cpp:804-817,

@@ -19,7 +19,8 @@ extern "C" {
 #endif
 
 /* A polynomial E(x) = sum_k a_k sum_{m in S_k} prod_{i in m} x_i, given as a monomial table
- * (mbx_amd/tables/<name>.tbl).  mono_idx holds 4 variable indices per monomial (-1 padded).
+ * (mbx_amd/tables/<name>.tbl).  mono_idx holds `stride` variable indices per monomial (-1
+ * padded; stride >= the largest degree, at most 8).
  * Restates poly_2b_v6x / poly_3b_v2x (potential/2b/poly-2b-v6x.cpp:12936,
  * potential/3b/poly-3b-v2x.cpp:26982) by direct monomial summation.
  * Pinned by: the reference's 21-digit KATs (unittests/unittest-poly-2b.cpp:2537-3012,
@@ -31,7 +32,8 @@ typedef struct {
     const double* coef;  /* [ncoef] */
     const int* mono_k;   /* [nmono] coefficient index */
     const int* mono_deg; /* [nmono] */
-    const int* mono_idx; /* [4*nmono] */
+    const int* mono_idx; /* [stride*nmono] */
+    int stride;
 } orc_poly;
 
 /* Returns E; if g != NULL writes dE/dx (nvars values, overwritten). */
@@ -94,6 +96,14 @@ typedef struct {
  * differences, and the published per-term energies of the reference's CH4/H2O systems. */
 double orc_nrg_holder(const orc_holder* h, const orc_poly* p, const double* a, const double* const* mons,
                       double* const* grads, double* vir, int* evaluated);
+
+/* MB-nrg 1B holder for ONE monomer: x1b_A1B4_v1x::eval (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371-470;
+ * h: the holder's variables over the monomer's atoms, mbx_amd/tables/ch4_1b.holder.json).
+ * E = P(v); g (nullable) receives dE/dw (overwritten); vir6 (nullable) is incremented by
+ * -sum_atoms w_a g_b (xx xy xz yy yz zz).
+ * Pinned by: unittest-energy1b.cpp's ch4 row (energy, gradient, virial), the compiled reference
+ * holder on perturbed monomers, central finite differences. */
+double orc_1b_nrg(const orc_holder* h, const orc_poly* p, const double* a, const double* w, double* g, double* vir6);
 
 /* InvertUnitCell (tools/math_tools.cpp:37-64).  Returns 0, or -1 if singular. */
 int orc_invert_unit_cell(const double* box, double* box_inv);

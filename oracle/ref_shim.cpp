@@ -35,6 +35,7 @@
 #include "tools/math_tools.h"
 #include "potential/2b/x2b_A1B2Z2_C1D4_deg3_exp0_v1x.h"
 #include "potential/3b/mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1.h"
+#include "potential/1b/x1b_A1B4_deg5_exp0_v1x.h"
 
 extern "C" {
 
@@ -278,6 +279,15 @@ void ref_ps_1b(const double* rr, double* dr, size_t nw, double* energies, double
     std::vector<double> e = ps::pot_nasa(rr, dr, nw, virial ? &v : nullptr);
     std::copy(e.begin(), e.end(), energies);
     if (virial) std::copy(v.begin(), v.end(), virial);
+}
+
+// x1b_A1B4_v1x("ch4")::eval with gradients over n CH4 monomers (15 doubles each).
+void ref_ch4_1b(const double* xyz, double* grad, size_t n, double* energies, double* virial) {
+    x1b_A1B4_deg5_exp0::x1b_A1B4_v1x pot("ch4");
+    std::vector<double> v(virial, virial + 9);
+    std::vector<double> e = pot.eval(xyz, grad, n, &v);
+    std::copy(e.begin(), e.end(), energies);
+    std::copy(v.begin(), v.end(), virial);
 }
 
 }  // extern "C"

@@ -5,6 +5,9 @@ TEST_CASE blocks: every ``std::vector<double> NAME = {...};`` and ``double NAME 
 Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
 
     poly_2b    src/unittests/unittest-poly-2b.cpp         TEST_CASE(":poly_2b_v6x:struct")
+    pair_list  src/unittests/unittest-system.cpp          TEST_CASE("Pair List") (expected n-mer sets)
+    system_h2o_2_ch4_1  src/unittests/setup_h2o_2_ch4_1.h  (System-level 1B/2B/3B energies, gradients,
+                                                            virials of two waters and a methane)
     poly_3b    src/unittests/unittest-poly-3b.cpp         TEST_CASE("poly-3b-v2x::class")
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
@@ -37,6 +40,9 @@ CASES = {
     "energy_ch4h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:ch4-h2o-h2o'),
     # MB-pol 1B (Partridge-Schwenke water monomer)
     "ps_1b": ("src/unittests/unittest-ps.cpp", 'ps::energy'),
+    # MB-nrg CH4 1B: polynomial and holder
+    "poly_ch4_1b": ("src/unittests/unittest-poly-1b.cpp", 'poly_1b_A1B4_deg5_exp0_v1x::struct'),
+    "holder_ch4_1b": ("src/unittests/unittest-poly-holder-1b.cpp", 'x1b_A1B4_deg5_exp0::struct'),
 }
 
 
@@ -97,6 +103,36 @@ def energy1b(text, mons=("ch4", "h2o")):
     return out
 
 
+def pair_list(text):
+    """unittest-system.cpp TEST_CASE("Pair List"): the expected dimer / trimer sets of
+    GetPairList at 10 and 5 Angstrom (input monomer indices, in the order written)."""
+    b = re.sub(r"//[^\n]*", "", block(re.sub(r"//[^\n]*", "", text), "Pair List"))
+    out = {}
+    for name in ("expected_d", "expected_t", "expected_d2", "expected_t2"):
+        m = re.search(r"std::vector<size_t>\s+%s\s*=\s*\{([^}]*)\}" % name, b)
+        out[name] = [int(v) for v in m.group(1).replace("\n", " ").split(",") if v.strip()]
+    out["cutoffs"] = [float(v) for v in re.findall(r"GetPairList\(2,\s*([0-9.]+),", b)]
+    out["input"] = re.search(r'ReadNrg\("([^"]+)"', b).group(1)
+    return out
+
+
+def test_data(text):
+    """The TestEnergyData of a setup_*.h macro (unittests/setup_h2o_2_ch4_1.h): numbers, vectors
+    and strings of every testData.<field> assignment."""
+    src = re.sub(r"\\\n", "\n", text)
+    out = {}
+    for m in re.finditer(r"testData\.(\w+)\s*=\s*(\{[^}]*\}|R\"\((.*?)\)\"|[-+0-9.eE]+)\s*;", src, re.S):
+        name, val = m.group(1), m.group(2)
+        if m.group(3) is not None:
+            out[name] = m.group(3)
+        elif val.startswith("{"):
+            items = [v.strip() for v in val[1:-1].replace("\n", " ").split(",") if v.strip()]
+            out[name] = [v.strip('"') if v.startswith('"') else float(v) for v in items]
+        else:
+            out[name] = float(val)
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     res = {}
@@ -107,6 +143,12 @@ def main():
     path = "src/unittests/unittest-energy1b.cpp"
     res["energy1b"] = energy1b(open(os.path.join(ref, path)).read())
     res["energy1b"]["_source"] = path + ' TEST_CASE("energy1b::get_1b_energy_grad"), rows ch4 and h2o'
+    path = "src/unittests/unittest-system.cpp"
+    res["pair_list"] = pair_list(open(os.path.join(ref, path)).read())
+    res["pair_list"]["_source"] = path + ' TEST_CASE("Pair List")'
+    path = "src/unittests/setup_h2o_2_ch4_1.h"
+    res["system_h2o_2_ch4_1"] = test_data(open(os.path.join(ref, path)).read())
+    res["system_h2o_2_ch4_1"]["_source"] = path + " (TestEnergyData of unittest-system.cpp:215-240)"
     
     with open(os.path.join(HERE, "kats.json"), "w") as f:
         json.dump(res, f)

@@ -81,3 +81,25 @@ def energies(stdout):
         elif len(w) == 3:
             grads.append([float(v) for v in w])
     return e, np.array(grads)
+
+
+def read_first_system(path):
+    """Monomers of the first SYSTEM block of an NRG file, in file order: (id, atom names, xyz)
+    each (a plain tokenisation of MONOMER / atom / ENDMON lines)."""
+    out, cur = [], None
+    for line in open(path):
+        w = line.split()
+        if not w:
+            continue
+        k = w[0].lower()
+        if k == "endsys":
+            break
+        if k == "monomer":
+            cur = (w[1].lower(), [], [])
+        elif k == "endmon":
+            out.append(cur)
+            cur = None
+        elif cur is not None:
+            cur[1].append(w[0])
+            cur[2].extend(float(v) for v in w[1:4])
+    return out

@@ -58,3 +58,82 @@ def test_1b_ghosts_and_kat():
     er, gr, _ = oracle.ps_1b(w)
     assert e1 == pytest.approx(er.sum(), rel=1e-12)
     assert np.abs(g - gr.reshape(-1)).max() <= 1e-10
+
+
+# ---------------------------------------------------------------------------------------------
+# MB-nrg CH4 1B and the 1B term inside a combined compute
+
+
+def _ch4_conditioning(xyz, kinds, box):
+    """Sum over the CH4 monomers of sum_k |a_k| M_k: the FP64 floor of the CH4 1B energy
+    (about 2e7 per monomer for ~3 kcal/mol; see tests/test_oracle_nrg.py)."""
+    nat = np.where(np.asarray(kinds) == 0, 3, 5)
+    x = oracle.fix_monomer_coordinates(xyz, box, nat) if box is not None else np.asarray(xyz, np.float64)
+    first = np.concatenate([[0], np.cumsum(nat)[:-1]])
+    w = [x[3 * f:3 * f + 15] for f, k in zip(first, kinds) if k == 1]
+    if not w:
+        return 0.0
+    a = np.abs(oracle.table("ch4_1b").coef)
+    return float(oracle.nrg_1b("ch4_1b", np.array(w), a=a, grad=False)[0].sum())
+
+
+MIXED = ["ch4_57_h2o_513_rect", "ch4_342_h2o_228_rect", "clathrate"]
+
+
+@pytest.mark.parametrize("name", MIXED)
+def test_1b_mixed_published_and_oracle(systems, name):
+    """The CH4 / H2O reference systems: E1B against the published 1B energy (timing/<case>/output,
+    11 significant digits) and the oracle; the 1B gradient and virial against the oracle."""
+    kinds = np.array([0 if m == "h2o" else 1 for m in systems[name + "/mon"]])
+    xyz = systems[name + "/xyz"]
+    box = systems[name + "/box"]
+    box = box if box.size else None
+    pub = systems[name + "/published"]
+    print("1b", name, flush=True)
+    e = engine.Engine()
+    e.set_topology(kinds)
+    e.set_box(box)
+    e.set_cutoffs(9.0, 0.0)
+    e.set_coords(xyz)
+    e1, g, v = e.eval_1b()
+    er, gr, vr = oracle.one_body(xyz, kinds, box)
+    floor = 2e-16 * _ch4_conditioning(xyz, kinds, box)
+    assert abs(e1 - er) <= max(1e-12 * abs(er), floor)
+    assert abs(e1 - pub[0]) <= max(5e-11 * abs(pub[0]), 5 * floor)
+    assert np.abs(g - gr).max() <= 1e-8
+    np.testing.assert_allclose(v, vr, rtol=1e-9, atol=1e-8)
+
+
+def test_1b_inside_compute(systems):
+    """MBXG_1B | MBXG_2B | MBXG_3B in one compute (what System::Energy runs): each energy equals
+    its own compute, the gradient and virial equal their sums; graph replays give the same."""
+    name = "ch4_57_h2o_513_rect"
+    kinds = np.array([0 if m == "h2o" else 1 for m in systems[name + "/mon"]])
+    xyz = systems[name + "/xyz"]
+    box = systems[name + "/box"]
+    c2, c3 = (float(v) for v in systems[name + "/cutoffs"])
+    print("1b inside compute", flush=True)
+    e = engine.Engine()
+    e.set_topology(kinds)
+    e.set_box(box)
+    e.set_cutoffs(c2, c3)
+    e.set_coords(xyz)
+    e.compute(engine.TERM_1B)
+    e1_only = e.energies()[0]
+    _, _, g1, v1 = e.results()
+    e.compute(engine.TERM_2B | engine.TERM_3B)
+    _, e2_only, e3_only = e.energies()
+    _, _, g23, v23 = e.results()
+    e.set_graph(True)
+    for rep in range(4):  # eager (sizes), eager, then graph replays
+        e.compute(engine.TERM_1B | engine.TERM_2B | engine.TERM_3B)
+        e1, e2, e3 = e.energies()
+        _, _, g, v = e.results()
+        assert e1 == pytest.approx(e1_only, rel=1e-13), rep
+        assert e2 == pytest.approx(e2_only, rel=1e-12) and e3 == pytest.approx(e3_only, rel=1e-12), rep
+        assert np.abs(g - (g1 + g23)).max() <= 1e-10, rep
+        np.testing.assert_allclose(v, v1 + v23, rtol=1e-11, atol=1e-9)
+    e.set_graph(False)
+    with pytest.raises(engine.MbxgError):  # the 1B term is not evaluated over batches
+        e.set_batch([0, 10, len(kinds)])
+        e.compute(engine.TERM_1B)

@@ -34,8 +34,8 @@ def test_single_point_water_published(systems, tmp_path):
     assert e["1B"] == pytest.approx(pub[0], rel=5e-11)
     assert e["2B"] == pytest.approx(pub[1], rel=5e-11)
     assert e["3B"] == pytest.approx(pub[2], rel=5e-11)
-    # one combined compute vs two separate ones: the same terms up to the atomics' rounding order
-    assert e["total"] == pytest.approx(e["2B"] + e["3B"], rel=1e-12)
+    # Energy(true): one combined compute of the three terms vs three separate ones
+    assert e["total"] == pytest.approx(e["1B"] + e["2B"] + e["3B"], rel=1e-12)
 
 
 def test_single_point_clathrate_grads(systems, tmp_path):
@@ -44,15 +44,18 @@ def test_single_point_clathrate_grads(systems, tmp_path):
     r = nrg_files.run(nrg, js, "--grads")
     assert r.returncode == 0, r.stderr
     e, g = nrg_files.energies(r.stdout)
-    assert "1B" not in e and "1B term not computed" in r.stderr  # CH4 has no 1B term here
     pub = systems["clathrate/published"]
+    # the CH4 1B polynomial cancels terms of ~2e7 per monomer: its FP64 floor is ~1e-8 here
+    assert e["1B"] == pytest.approx(pub[0], abs=2e-8)
     assert e["2B"] == pytest.approx(pub[1], rel=5e-11)
     assert e["3B"] == pytest.approx(pub[2], rel=5e-11)
     kinds = np.array([0 if m == "h2o" else 1 for m in mons])
     box = systems["clathrate/box"]
-    ref = oracle.system(xyz, kinds, box if box.size else None, c2, c3)
+    box = box if box.size else None
+    ref = oracle.system(xyz, kinds, box, c2, c3)
+    _, g1, _ = oracle.one_body(xyz, kinds, box)
     assert g.shape == (len(xyz) // 3, 3)
-    assert np.abs(g.reshape(-1) - ref["grad"]).max() <= 1e-8
+    assert np.abs(g.reshape(-1) - (ref["grad"] + g1)).max() <= 1e-8
 
 
 def test_write_nrg_roundtrip(tmp_path):

@@ -1,0 +1,179 @@
+"""GPU: the C++ drop-in boundary as the reference's own callers use it (tests/cpp/system_api, built
+against include/mbx/ and libmbxg.so by __graft_entry__.build()).
+
+  * System-level known answers of unittest-system.cpp:67-123 over setup_h2o_2_ch4_1.h (two waters
+    and a methane, gas phase): 1B, 2B, 3B energies with and without gradients, GetGrads (every
+    site, virtual ones included) and GetRealGrads, virials;
+  * GetPairList's expected sets of unittest-system.cpp:644-718 (10 and 5 Angstrom) and its
+    nmax = 4 error;
+  * copies, assignment and moves of a System (tools::ReadNrg fills a std::vector<System>): the
+    same energies and gradients, each copy on a device context of its own;
+  * SetRealXyz with a monomer split across the cell, then GetRealXyz: FixMonomerCoordinates'
+    wrap bit for bit (the compiled reference's, through the oracle); SetXyz(GetXyz()) round trip;
+  * SetUpFromJson without a "box" key keeps the box set before (system.cpp:1200-1206);
+  * the wrap entry point mbxg_fix_coordinates against the compiled reference's
+    FixMonomerCoordinates on the reference's test boxes.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, sorted_tuples, water_case
+
+import nrg_files
+import oracle
+from mbx_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+KATS = json.load(open(os.path.join(GOLDEN, "kats.json")))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "system_api")
+
+
+def _run(*args):
+    assert os.path.exists(DRIVER), "tests/cpp/system_api is not built (run __graft_entry__.build())"
+    print("system_api", *args[:1], flush=True)
+    r = subprocess.run([DRIVER, *[str(a) for a in args]], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def _kat_system_files(tmp_path):
+    k = KATS["system_h2o_2_ch4_1"]
+    mons = k["monomer_names"]
+    nat = [int(v) for v in k["natoms"]]
+    nrg = str(tmp_path / "h2o_2_ch4_1.nrg")
+    nrg_files.write_nrg(nrg, [(mons, nat, k["real_atom_names"], k["real_coords"])])
+    js = str(tmp_path / "mbx.json")
+    with open(js, "w") as f:
+        f.write(k["json_mbx"])
+    return k, nrg, js
+
+
+def test_system_energy_kat(tmp_path):
+    """unittest-system.cpp TestEnergy (:67-123) for the 1B, 2B and 3B sections, values of
+    setup_h2o_2_ch4_1.h (printed with 11 significant digits)."""
+    k, nrg, js = _kat_system_files(tmp_path)
+    o = _run("terms", nrg, js)
+    assert o["nsites"] == k["tot_sites"] and o["natoms"] == k["tot_atoms"]
+    assert o["atom_names"] == k["atom_names"] and o["real_atom_names"] == k["real_atom_names"]
+    for term in ("1b", "2b", "3b"):
+        e = k["energy_" + term]
+        assert o[term] == pytest.approx(e, rel=1e-10, abs=1e-10), term
+        assert o[term + "_nograd"] == pytest.approx(o[term], rel=1e-12), term
+        np.testing.assert_allclose(o[term + "_grads"], k["grad_" + term], rtol=1e-9, atol=1e-9, err_msg=term)
+        np.testing.assert_allclose(o[term + "_real_grads"], k["real_grad_" + term], rtol=1e-9, atol=1e-9,
+                                   err_msg=term)
+        np.testing.assert_allclose(o[term + "_virial"], k["virial_" + term], rtol=1e-9, atol=1e-9, err_msg=term)
+    # Energy(): the sum of the three terms this library computes (the reference's total also holds
+    # dispersion and electrostatics)
+    assert o["total"] == pytest.approx(o["1b"] + o["2b"] + o["3b"], rel=1e-12)
+    np.testing.assert_allclose(o["total_real_grads"],
+                               np.sum([o[t + "_real_grads"] for t in ("1b", "2b", "3b")], axis=0), rtol=1e-11,
+                               atol=1e-11)
+    np.testing.assert_allclose(o["total_virial"], np.sum([o[t + "_virial"] for t in ("1b", "2b", "3b")], axis=0),
+                               rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_pair_list_kat(which):
+    """unittest-system.cpp:644-718, compared sorted within and across n-mers as its sortEdge does."""
+    k = KATS["pair_list"]
+    nrg = os.path.join(GOLDEN, "input_h2o_co2_ch4_dp1.nrg")
+    cutoff = k["cutoffs"][which]
+    for nmax, key in ((2, "expected_d"), (3, "expected_t")):
+        o = _run("pairlist", nrg, cutoff, nmax)
+        got = np.array(o["nmers"]).reshape(-1, nmax)
+        want = np.array(k[key + ("2" if which else "")]).reshape(-1, nmax)
+        assert len(got) == len(want)
+        assert np.array_equal(sorted_tuples(got), sorted_tuples(want))
+        assert o["nmax4_threw"] == 1
+
+
+def _water_files(systems, tmp_path, case="h2o_256_cubic", with_box=True):
+    xyz, box, c2, c3, pub = water_case(systems, case)
+    n = len(xyz) // 9
+    nrg = str(tmp_path / "w.nrg")
+    nrg_files.write_nrg(nrg, [(["h2o"] * n, [3] * n, ["O", "H", "H"] * n, xyz)])
+    js = str(tmp_path / "mbx.json")
+    nrg_files.write_json(js, box if with_box else None, c2, c3)
+    return xyz, box, c2, c3, pub, nrg, js
+
+
+def test_copies_and_moves(systems, tmp_path):
+    xyz, box, c2, c3, pub, nrg, js = _water_files(systems, tmp_path)
+    o = _run("copies", nrg, js)
+    total = pub[0] + pub[1] + pub[2]
+    assert o["orig"] == pytest.approx(total, rel=5e-11)
+    for k in ("orig_again", "assigned", "grown_last", "moved"):
+        assert o[k] == pytest.approx(o["orig"], rel=1e-12), k
+    np.testing.assert_allclose(o["assigned_grads"], o["orig_grads"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(o["moved_grads"], o["orig_grads"], rtol=0, atol=1e-10)
+
+
+def test_split_monomer_wrap_and_setxyz(systems, tmp_path):
+    """A water split across the cell by SetRealXyz comes back from GetRealXyz wrapped as
+    FixMonomerCoordinates wraps it (the oracle's restatement, pinned bit for bit to the compiled
+    reference); GetXyz / SetXyz round-trip; the energy is that of the unsplit box."""
+    xyz, box, c2, c3, pub, nrg, js = _water_files(systems, tmp_path)
+    L = box[0]
+    o = _run("xyz", nrg, js, 17, L, 0.0, 0.0)
+    n = len(xyz) // 9
+    want = oracle.fix_monomer_coordinates(np.array(o["set"]), box, [3] * n)
+    assert np.array_equal(np.array(o["get_real"]), want)
+    assert np.array_equal(np.array(o["after_setxyz"]), want)
+    allx = np.array(o["get_all"]).reshape(n, 4, 3)
+    assert np.array_equal(allx[:, :3].reshape(-1), want)
+    assert o["energy"] == pytest.approx(pub[0] + pub[1] + pub[2], rel=5e-11)
+    assert o["bad_size_threw"] == 1
+
+
+def test_setupfromjson_keeps_box(systems, tmp_path):
+    """ADVICE: a second SetUpFromJson without "box" keeps the periodic box."""
+    xyz, box, c2, c3, pub, nrg, js = _water_files(systems, tmp_path)
+    nobox = str(tmp_path / "nobox.json")
+    nrg_files.write_json(nobox, None, c2, c3)
+    o = _run("jsonbox", nrg, js, nobox)
+    np.testing.assert_array_equal(o["box"], box)
+    assert o["energy"] == pytest.approx(pub[0] + pub[1] + pub[2], rel=5e-11)
+    r = oracle.water_system(xyz, box, c2, c3)
+    e1, g1, _ = oracle.one_body(xyz, [0] * (len(xyz) // 9), box)
+    assert np.abs(np.array(o["grads"]) - (r["grad"] + g1)).max() <= 1e-8
+
+
+BOXES = {
+    "cubic": [10.0, 0.0, 0.0, 0.0, 10.0, 0.0, 0.0, 0.0, 10.0],
+    "hexagonal": [10.0, 0.0, 0.0, 5.0, 8.660254037844386, 0.0, 0.0, 0.0, 20.0],
+    "triclinic": [10.0, 0.0, 0.0, 10.0, 15.0, 0.0, 5.0, 5.0, 20.0],
+    "clathrate": [24.84242, 0.0, 0.0, -18.63183, 32.27124, 0.0, 0.0, 0.0, 40.26258],
+}
+
+
+@pytest.mark.parametrize("name", sorted(BOXES))
+def test_fix_coordinates_entry_bit_exact(name):
+    """mbxg_fix_coordinates = FixMonomerCoordinates, bit for bit: against the oracle, which the
+    CPU suite pins bit for bit to the compiled reference on the same boxes."""
+    box = np.array(BOXES[name])
+    b = box.reshape(3, 3)
+    rng = np.random.default_rng(21)
+    kinds = rng.integers(0, 2, 300)
+    nat = np.where(kinds == 0, 3, 5)
+    xyz = []
+    for k, m in zip(kinds, nat):
+        first = rng.uniform(-2.5, 2.5, 3) @ b
+        xyz.append(first)
+        for _ in range(m - 1):
+            xyz.append(first + rng.normal(0, 1.0, 3) + rng.integers(-2, 3, 3) @ b)
+    xyz = np.concatenate(xyz)
+    e = engine.Engine()
+    e.set_topology(kinds)
+    e.set_box(box)
+    print("fix_coordinates", name, flush=True)
+    got = e.fix_coordinates(xyz)
+    want = oracle.fix_monomer_coordinates(xyz, box, nat)
+    assert np.array_equal(got, want)
+    if oracle.ref_available():
+        assert np.array_equal(got, oracle.ref_fix_monomer_coordinates(xyz, box, nat))

@@ -203,3 +203,84 @@ def test_published_mixed_systems(systems, name):
     r = oracle.system(xyz, kinds, box, c2, c3, families=PUBLISHED[name])
     assert r["e2b"] == pytest.approx(pub[1], rel=5e-11)
     assert r["e3b"] == pytest.approx(pub[2], rel=5e-11)
+
+
+# ---------------------------------------------------------------------------------------------
+# MB-nrg CH4 1B (orc_1b_nrg over mbx_amd/tables/ch4_1b.*)
+
+
+def _virial9(v6):
+    xx, xy, xz, yy, yz, zz = v6
+    return np.array([xx, xy, xz, xy, yy, yz, xz, yz, zz])
+
+
+def test_ch4_1b_poly_kat():
+    """unittest-poly-1b.cpp:76-100 (21-digit a, x, e, g of poly_1b_A1B4_deg5_exp0_v1x)."""
+    k = KATS["poly_ch4_1b"]
+    t = oracle.table("ch4_1b")
+    assert np.array_equal(np.array(k["a"]), t.coef)
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    assert e == pytest.approx(k["e_expected"], abs=1e-10)
+    np.testing.assert_allclose(g, k["g_expected"], atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["holder_ch4_1b", "energy1b"])
+def test_ch4_1b_holder_kats(case):
+    """unittest-poly-holder-1b.cpp:85-116 (two CH4 monomers: energies, gradients, virial) and the
+    ch4 row of unittest-energy1b.cpp (the same through e1b::get_1b_energy)."""
+    k = KATS[case] if case == "holder_ch4_1b" else KATS["energy1b"]["ch4"]
+    e, g, v = oracle.nrg_1b("ch4_1b", k["xyz"])
+    if case == "holder_ch4_1b":
+        np.testing.assert_allclose(e, k["energies_expected"], rtol=0, atol=1e-9)
+    else:
+        assert e.sum() == pytest.approx(k["energy_expected"], abs=1e-9)
+    np.testing.assert_allclose(g.reshape(-1), k["grad_expected"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(_virial9(v), k["virial_expected"], rtol=0, atol=1e-8)
+
+
+def _ch4_monomers(n, seed):
+    """Ideal tetrahedra (C-H 1.09) with random orientation, bond lengths jittered by U(+-0.05)
+    and angles by random H displacements of 0.05."""
+    rng = np.random.default_rng(seed)
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], np.float64) / np.sqrt(3.0)
+    out = []
+    for _ in range(n):
+        q, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+        c = rng.uniform(-5, 5, 3)
+        hs = (tet @ q.T) * (1.09 + rng.uniform(-0.05, 0.05, (4, 1))) + rng.normal(0, 0.05, (4, 3))
+        out.append(np.concatenate([c, (c + hs).reshape(-1)]))
+    return np.array(out)
+
+
+def ch4_1b_conditioning(w):
+    """Per monomer, sum_k |a_k| M_k(v): the scale of the cancelling terms behind the CH4 1B
+    energy (~2e7 for an energy of ~3 kcal/mol at equilibrium geometries), which bounds the
+    agreement of any two FP64 evaluation orders (the reference's Horner form and a monomial sum
+    or trie differ by ~1e-17 of it)."""
+    return oracle.nrg_1b("ch4_1b", w, a=np.abs(oracle.table("ch4_1b").coef), grad=False)[0]
+
+
+@ref_only
+def test_ch4_1b_vs_compiled_reference():
+    w = _ch4_monomers(64, 3)
+    e, g, v = oracle.nrg_1b("ch4_1b", w)
+    er, gr, vr = oracle.ref_ch4_1b(w)
+    cond = ch4_1b_conditioning(w)
+    assert np.all(np.abs(e - er) <= 2e-16 * cond)
+    np.testing.assert_allclose(g, gr, rtol=0, atol=2e-15 * cond.max())
+    np.testing.assert_allclose(_virial9(v), vr, rtol=0, atol=2e-14 * cond.max())
+
+
+def test_ch4_1b_finite_differences():
+    """Five-point central differences with h = 1e-3: the energy's rounding noise (~1e-17 of the
+    ~2e7 conditioning, about 4e-10) allows no smaller step."""
+    w = _ch4_monomers(4, 9)
+    h = 1e-3
+    E = lambda x: oracle.nrg_1b("ch4_1b", x, grad=False)[0][0]
+    for m in w:
+        _, g, _ = oracle.nrg_1b("ch4_1b", m)
+        for i in range(15):
+            d = np.zeros(15)
+            d[i] = h
+            fd = (8 * (E(m + d) - E(m - d)) - (E(m + 2 * d) - E(m - 2 * d))) / (12 * h)
+            assert fd == pytest.approx(g[0][i], abs=2e-6 * max(1.0, abs(g[0][i])))

@@ -54,17 +54,20 @@ def test_external_call_pbc_water(systems, tmp_path):
     L.get_energy_pbc_g_(c.ctypes.data_as(dp), ctypes.byref(ctypes.c_int(3 * n)), b.ctypes.data_as(dp),
                         ctypes.byref(e), g.ctypes.data_as(dp))
     r = oracle.water_system(xyz, box, c2, c3)
-    assert e.value == pytest.approx(r["e2b"] + r["e3b"], rel=1e-10)
-    assert e.value == pytest.approx(pub[1] + pub[2], rel=5e-11)
-    assert np.abs(g - r["grad"]).max() < 1e-8
+    e1, g1, v1 = oracle.one_body(xyz, [0] * n, box)
+    assert e.value == pytest.approx(e1 + r["e2b"] + r["e3b"], rel=1e-10)
+    assert e.value == pytest.approx(pub[0] + pub[1] + pub[2], rel=5e-11)
+    assert np.abs(g - (r["grad"] + g1)).max() < 1e-8
     v = np.zeros(9)
     L.get_virial_(v.ctypes.data_as(dp))
-    np.testing.assert_allclose(v, r["virial"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(v, r["virial"] + v1, rtol=1e-8, atol=1e-8)
     terms = [ctypes.c_double() for _ in range(7)]
     L.get_energy_decomp_pbc_(c.ctypes.data_as(dp), ctypes.byref(ctypes.c_int(3 * n)), b.ctypes.data_as(dp),
                              *[ctypes.byref(t) for t in terms])
+    assert terms[0].value == pytest.approx(pub[0], rel=5e-11)  # OneBodyEnergy(false)
     assert terms[1].value == pytest.approx(pub[1], rel=5e-11)
     assert terms[2].value == pytest.approx(pub[2], rel=5e-11)
+    assert all(t.value == 0.0 for t in terms[3:])  # 4B, dispersion, Buckingham, electrostatics
     L.finalize_system_()
 
 
@@ -87,6 +90,7 @@ def test_external_call_orders_mixed_input(systems, tmp_path):
     L.get_energy_pbc_g_(c.ctypes.data_as(dp), ctypes.byref(ctypes.c_int(3 * n)), b.ctypes.data_as(dp),
                         ctypes.byref(e), g.ctypes.data_as(dp))
     r = oracle.water_system(xs, box, c2, c3)
-    assert e.value == pytest.approx(r["e2b"] + r["e3b"], rel=1e-10)
-    assert np.abs(g - r["grad"]).max() < 1e-8
+    e1, g1, _ = oracle.one_body(xs, [0] * n, box)
+    assert e.value == pytest.approx(e1 + r["e2b"] + r["e3b"], rel=1e-10)
+    assert np.abs(g - (r["grad"] + g1)).max() < 1e-8
     L.finalize_system_()
