@@ -16,8 +16,14 @@ The JSON line also carries
                 code (mbx_amd/tables/flops.json), timed with HIP events on the engine stream
                 during the timed steps; traffic from the committed rocprofv3 PMC summary when
                 present;
-  cpu_baseline: the reference's own compiled 2B/3B leaves (oracle/_ref, OpenMP driver with the
-                reference's batch-of-8 dispatch) on this host's cores, on the same workload.
+  cpu_baseline: the reference's own compiled 2B/3B leaves (oracle/_ref, OpenMP driver) on this
+                host's cores, on the same workload, in two dispatch modes: the reference's own
+                batch of 8 n-mers per polynomial call and a best-effort batch of 1024; the faster
+                one is the baseline, both are reported with the CPU model.
+Also in "config": the same step with the gradient copied to the host every step
+(with_grad_d2h; PCIe-inclusive, so never the headline value) and the drop-in path a C / Fortran /
+Python caller takes (drop_in: get_energy_g_ = SetRealXyz -> Energy(true) -> GetRealGrads of the
+C++ System, 1B + 2B + 3B, host arrays in and out).
 """
 import argparse
 import json
@@ -36,11 +42,37 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
 
 def host_cores():
+    """Threads for the CPU baseline: the cores this process may run on, capped by OMP_NUM_THREADS
+    where the environment sets it (the GPU box gives one GPU's process a 16-thread share and sets
+    OMP_NUM_THREADS=16; without either, 16)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(n, cap))
+
+
+def host_cpu():
+    """(model name, physical cores, logical CPUs) of this host from /proc/cpuinfo."""
+    model, cores, logical = "unknown", set(), 0
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                logical += 1
+            elif k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return model, len(cores) or logical, logical
 
 
 def read_traffic(path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
@@ -54,28 +86,80 @@ def read_traffic(path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
 
 
 def cpu_baseline(xyz, box, c2, c3, seconds):
-    """Reference leaves on the host (kind 'reference'); the oracle port when they are absent."""
+    """Reference leaves on the host (kind 'reference'), in the reference's batch-of-8 dispatch and
+    a best-effort batch of 1024, each timed for about `seconds`; the faster mode is the baseline.
+    The oracle port when the reference leaves are absent."""
     import oracle
 
     cores = host_cores()
-    reps, t_total = 0, 0.0
-    if oracle.ref_available():
-        kind = "reference"
-        run = lambda: oracle.ref_water_2b3b(xyz, box, c2, c3, batch=8, nthreads=cores)
-    else:
-        kind = "port"
-        os.environ["OMP_NUM_THREADS"] = str(cores)
-        run = lambda: oracle.water_system(xyz, box, c2, c3)
-    r = run()  # warm-up
-    while t_total < seconds or reps < 2:
-        t0 = time.perf_counter()
-        r = run()
-        t_total += time.perf_counter() - t0
-        reps += 1
-    n = r["n2"] + r["n3"]
-    return dict(value=n * reps / t_total, unit="dimers+trimers/s", cores=cores, kind=kind,
-                sample="%d full 2B+3B E+F calls on the 2048-water box (%.1f s of CPU time, %d threads, "
-                       "batch-8 dispatch)" % (reps, t_total, cores)), r
+    model, phys, logical = host_cpu()
+    modes = {}
+    r = None
+    for batch in (8, 1024):
+        reps, t_total = 0, 0.0
+        if oracle.ref_available():
+            kind = "reference"
+            run = lambda: oracle.ref_water_2b3b(xyz, box, c2, c3, batch=batch, nthreads=cores)
+        else:
+            kind = "port"
+            os.environ["OMP_NUM_THREADS"] = str(cores)
+            run = lambda: oracle.water_system(xyz, box, c2, c3)
+        r = run()  # warm-up
+        while t_total < seconds or reps < 2:
+            t0 = time.perf_counter()
+            r = run()
+            t_total += time.perf_counter() - t0
+            reps += 1
+        n = r["n2"] + r["n3"]
+        modes["batch%d" % batch] = {"value": n * reps / t_total, "calls": reps, "cpu_s": t_total,
+                                    "ms_per_call": t_total / reps * 1e3}
+        if kind == "port":
+            break
+    best = max(modes, key=lambda m: modes[m]["value"])
+    return dict(value=modes[best]["value"], unit="dimers+trimers/s", cores=cores, kind=kind,
+                cpu_model=model, host_physical_cores=phys, host_logical_cpus=logical, mode=best, modes=modes,
+                sample="%d full 2B+3B E+F calls on the 2048-water box (%.1f s of CPU time, %d threads; "
+                       "the faster of the reference's batch-of-8 dispatch and a batch of 1024: %s)"
+                       % (modes[best]["calls"], modes[best]["cpu_s"], cores, best)), r
+
+
+def drop_in_leg(xyz, box, c2, c3, steps, tmpdir):
+    """The reference-compatible entry points of a C / Fortran / Python caller: initialize_system_py_
+    once, then get_energy_g_ per step (SetRealXyz -> Energy(true) -> GetRealGrads: 1B + 2B + 3B,
+    host arrays in and out, one synchronous call)."""
+    import ctypes
+
+    import numpy as np
+
+    from mbx_amd import engine
+
+    L = engine.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    cpp = ctypes.POINTER(ctypes.c_char_p)
+    L.initialize_system_py_.argtypes = [dp, ip, cpp, cpp, ip, ctypes.c_char_p]
+    L.get_energy_g_.argtypes = [dp, ip, dp, dp]
+    n = len(xyz) // 9
+    js = os.path.join(tmpdir, "bench_mbx.json")
+    with open(js, "w") as f:
+        json.dump({"MBX": {"box": [float(v) for v in box], "twobody_cutoff": c2, "threebody_cutoff": c3}}, f)
+    c = np.ascontiguousarray(xyz)
+    L.initialize_system_py_(c.ctypes.data_as(dp), (ctypes.c_int * n)(*([3] * n)),
+                            (ctypes.c_char_p * (3 * n))(*([b"O", b"H", b"H"] * n)),
+                            (ctypes.c_char_p * n)(*([b"h2o"] * n)), ctypes.byref(ctypes.c_int(n)), js.encode())
+    e = ctypes.c_double()
+    g = np.zeros(len(xyz))
+    nat = ctypes.c_int(3 * n)
+    for _ in range(3):  # sizing, eager, then graph replays (the System runs its engine in graph mode)
+        L.get_energy_g_(c.ctypes.data_as(dp), ctypes.byref(nat), ctypes.byref(e), g.ctypes.data_as(dp))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.get_energy_g_(c.ctypes.data_as(dp), ctypes.byref(nat), ctypes.byref(e), g.ctypes.data_as(dp))
+    dt = (time.perf_counter() - t0) / steps
+    L.finalize_system_()
+    return {"ms_per_call": dt * 1e3, "energy_1b_2b_3b": e.value,
+            "call": "get_energy_g_: SetRealXyz -> Energy(true) (1B + 2B + 3B) -> GetRealGrads, host arrays, "
+                    "synchronous"}, dt, g
 
 
 def batched_vs_plain(parts, kinds, box, c2, c3, steps, warmup):
@@ -321,6 +405,18 @@ def main():
         for k, v in tm.items():
             stage[k] = stage.get(k, 0.0) + v
     eng.set_timing(False)
+    # the same step with the gradient (and energies) copied to the host every step: synchronous,
+    # PCIe-inclusive (graph mode again)
+    eng.set_graph(True)
+    for _ in range(2):
+        step()
+        eng.results()
+    t0 = time.perf_counter()
+    for _ in range(0 if args.solo else args.steps):
+        step()
+        eng.results()
+    t_d2h = (time.perf_counter() - t0) / max(1, args.steps)
+    eng.set_graph(False)
 
     counts = eng.counts()
     n_listed = int(counts[0] + counts[2])
@@ -363,6 +459,10 @@ def main():
         "cpu_baseline": None,
     }
     # list build (fix, cloud, binning, both n-mer passes): PMC bytes per step over its HIP-event time
+    out["config"]["with_grad_d2h"] = {
+        "ms_per_step": t_d2h * 1e3, "nmers_per_s": n_listed / t_d2h,
+        "step": "the headline step plus mbxg_get_results (energies, virial and the 6144 x 3 gradient copied to "
+                "host memory, host waits each step)"}
     lb = read_traffic(LIST_TRAFFIC_FILE, "list_build_bytes_per_step:")
     lt = (stage["fix"] + stage["lists_count"] + stage["lists_emit"]) / args.steps * 1e-3
     out["config"]["list_build"] = {"ms": lt * 1e3, "hbm_bytes": lb, "GB_per_s": lb / lt / 1e9 if lb else None,
@@ -377,6 +477,17 @@ def main():
     t1b = (time.perf_counter() - t0) / args.steps
     out["config"]["one_body"] = {"e1b": e1b, "ms_per_call": t1b * 1e3, "waters_per_s": cfg["nmon"] / t1b,
                                  "call": "mbxg_eval_1b with gradients, results copied to the host"}
+    if not args.solo:
+        import tempfile
+
+        _, g1b, _ = eng.eval_1b()
+        with tempfile.TemporaryDirectory() as tmp:
+            di, t_di, g_di = drop_in_leg(xyz, box, c2, c3, args.steps, tmp)
+        di["nmers_per_s"] = n_listed / t_di
+        # the drop-in gradient (input order = engine order for waters) against the engine's
+        # 2B + 3B gradient of the headline step plus its 1B gradient
+        di["grad_maxabs_vs_engine"] = float(np.abs(g_di - (grad + g1b)).max())
+        out["config"]["drop_in"] = di
     if not args.no_batch:
         out["config"]["batched"] = batched_leg(args.steps, args.warmup)
         out["config"]["pimd_batched"] = pimd_leg(args.steps, args.warmup)

@@ -110,8 +110,9 @@ def test_copies_and_moves(systems, tmp_path):
     assert o["orig"] == pytest.approx(total, rel=5e-11)
     for k in ("orig_again", "assigned", "grown_last", "moved"):
         assert o[k] == pytest.approx(o["orig"], rel=1e-12), k
-    np.testing.assert_allclose(o["assigned_grads"], o["orig_grads"], rtol=0, atol=1e-10)
-    np.testing.assert_allclose(o["moved_grads"], o["orig_grads"], rtol=0, atol=1e-10)
+    # the gradient scatter's FP64 atomics add in run-dependent order: equal to ~1e-11 relative
+    np.testing.assert_allclose(o["assigned_grads"], o["orig_grads"], rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(o["moved_grads"], o["orig_grads"], rtol=1e-10, atol=1e-9)
 
 
 def test_split_monomer_wrap_and_setxyz(systems, tmp_path):
@@ -124,9 +125,11 @@ def test_split_monomer_wrap_and_setxyz(systems, tmp_path):
     n = len(xyz) // 9
     want = oracle.fix_monomer_coordinates(np.array(o["set"]), box, [3] * n)
     assert np.array_equal(np.array(o["get_real"]), want)
-    assert np.array_equal(np.array(o["after_setxyz"]), want)
     allx = np.array(o["get_all"]).reshape(n, 4, 3)
     assert np.array_equal(allx[:, :3].reshape(-1), want)
+    # SetXyz(GetXyz()) then GetRealXyz: the wrapped coordinates wrapped once more, as the
+    # reference's GetRealXyz -> SetPBC(box_) does
+    assert np.array_equal(np.array(o["after_setxyz"]), oracle.fix_monomer_coordinates(want, box, [3] * n))
     assert o["energy"] == pytest.approx(pub[0] + pub[1] + pub[2], rel=5e-11)
     assert o["bad_size_threw"] == 1
 
