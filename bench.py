@@ -411,11 +411,11 @@ def main():
     for _ in range(2):
         step()
         eng.results()
-    t0 = time.perf_counter()
+    t_d0 = time.perf_counter()
     for _ in range(0 if args.solo else args.steps):
         step()
         eng.results()
-    t_d2h = (time.perf_counter() - t0) / max(1, args.steps)
+    t_d2h = (time.perf_counter() - t_d0) / max(1, args.steps)
     eng.set_graph(False)
 
     counts = eng.counts()

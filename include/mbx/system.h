@@ -127,6 +127,7 @@ class System {
     mbxg_ctx* device();  // the context, created (topology, box) on first use
     void apply_box();
     void wrap_xyz();     // FixMonomerCoordinates of xyz_ under PBC (SetPBC's side effect)
+    std::vector<double> real_xyz_input_order() const;  // xyz_ in input atom order, as stored
 
     mbxg_ctx* ctx_ = nullptr;
     bool initialized_ = false;

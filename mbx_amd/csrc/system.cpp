@@ -451,6 +451,11 @@ std::vector<double> System::GetVirial() { return virial_; }
 
 std::vector<double> System::GetRealXyz() {
     wrap_xyz();  // SetPBC(box_) (system.cpp:400)
+    return real_xyz_input_order();
+}
+
+// The stored real-atom coordinates in input order, as they are (ResetOrderReal3N).
+std::vector<double> System::real_xyz_input_order() const {
     std::vector<double> out(3 * numat_);
     for (size_t i = 0; i < order_.size(); ++i) {
         const size_t k = order_[i];
@@ -462,8 +467,9 @@ std::vector<double> System::GetRealXyz() {
 
 // ResetOrder3N of every site (system.cpp:397): real atoms as GetRealXyz, then each water's M-site
 // as SetPBC's SetVSites recomputes it; other virtual sites read 0 (no term here places them).
+// GetXyz does not wrap (system.cpp:397 reads xyz_ as the last SetPBC left it).
 std::vector<double> System::GetXyz() {
-    const std::vector<double> real = GetRealXyz();
+    const std::vector<double> real = real_xyz_input_order();
     std::vector<double> out(3 * numsites_, 0.0);
     for (size_t k = 0; k < in_mon_.size(); ++k) {
         const double* r = real.data() + 3 * in_first_[k];
