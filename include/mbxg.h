@@ -215,6 +215,17 @@ size_t mbxg_ad_workspace_bytes(int family, int n);
 /* energy[n] = E_n and grad[n][3*atoms] = dE_n/dx_n. */
 int mbxg_ad_forward(int family, int n, const double* xyz, const double* coef, const double* theta, double* energy,
                     double* grad, void* workspace, size_t workspace_bytes, void* stream);
+/* The forward without allocation: mbxg_ad_forward with the stage buffers in the library's own
+ * device scratch (about 4.6 MiB per device, held until the process ends) instead of a caller's
+ * workspace, for at most 8192 n-mers per call (more is an MBXG_EINVAL error).  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree.
+ * The scratch is looked up on the device current at the first call there; pass a stream of the
+ * current device.  Every call on one device shares that device's scratch: calls that overlap on one
+ * device (other streams, other threads) race and must be serialised by the caller.  Its contents
+ * do not feed mbxg_ad_backward (call that with from_forward = 0). */
+int mbxg_ad_forward_noalloc(int family, int n, const double* xyz, const double* coef, const double* theta,
+                            double* energy, double* grad, void* stream);
 /* With phi = sum_n g_energy[n] E_n + g_grad[n] . dE_n/dx_n: d_xyz = dphi/dx ([n][3*atoms]),
  * d_coef = dphi/da, d_theta = dphi/dtheta.  g_grad may be NULL (phi = sum g_energy E).
  * from_forward = 1: `workspace` is the one mbxg_ad_forward used on these same inputs (and

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "ad_kernel.hpp"
 #include "gen/coef_mbpol_2b.inc"
@@ -626,20 +627,12 @@ extern "C" size_t mbxg_ad_workspace_bytes(int family, int n) {
     return ad_layout(nv, ncoef, n).total;
 }
 
-extern "C" int mbxg_ad_forward(int family, int n, const double* xyz, const double* coef, const double* theta,
-                               double* energy, double* grad, void* workspace, size_t workspace_bytes, void* stream) {
-    int nv, nat, ncoef;
-    AdLayout L;
-    if (int e = ad_check("mbxg_ad_forward", family, n, workspace, workspace_bytes, nv, nat, ncoef, L)) return e;
-    if (n == 0) return MBXG_OK;
-    if (!xyz || !coef || !theta || !energy || !grad) return ad_err(MBXG_EINVAL, "mbxg_ad_forward: null pointer");
-    hipStream_t st = (hipStream_t)stream;
-    char* w = (char*)workspace;
-    double* xs = (double*)(w + L.xs);
-    double* sw = (double*)(w + L.sw);
-    double* gs = (double*)(w + L.gs);
-    double* ps = (double*)(w + L.ps);
-    int* ndev = (int*)(w + L.ndev);
+namespace {
+
+// The forward stages over stage buffers xs (nv x ld), sw (ld), gs (nv x ld), ps (ld) and the
+// device n-mer count ndev, wherever they live (a caller's workspace or the library's scratch).
+int ad_forward_run(int family, int n, const double* xyz, const double* coef, const double* theta, double* energy,
+                   double* grad, double* xs, double* sw, double* gs, double* ps, int* ndev, hipStream_t st) {
     const int ld = (n + 63) / 64 * 64, nb = ld / 64;
     hipLaunchKernelGGL(k_ad_set_int, dim3(1), dim3(1), 0, st, ndev, n);
     if (family == MBXG_AD_MBPOL_2B) {
@@ -657,6 +650,94 @@ extern "C" int mbxg_ad_forward(int family, int n, const double* xyz, const doubl
     }
     ADCHK(hipGetLastError());
     return MBXG_OK;
+}
+
+// ---- Forward without allocation -------------------------------------------------------------
+// mbxg_ad_forward_noalloc keeps its stage buffers in this file-scope device array instead of a
+// caller's workspace: forward-only buffers for at most kNoAllocMaxNmers n-mers of either family
+// (sized for the 36 variables of the 3B one).  Each device holds its own copy for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress returns on that
+// device (the array's name in host code is not a device address).
+constexpr int kNoAllocMaxNmers = 8192;
+constexpr size_t fwd_bytes(int nv, int n) {
+    // xs, sw, gs, ps and the count, each rounded up to kAlign bytes
+    return ((sizeof(double) * nv * n + kAlign - 1) / kAlign * kAlign) * 2 +
+           ((sizeof(double) * n + kAlign - 1) / kAlign * kAlign) * 2 + kAlign;
+}
+constexpr size_t kScratchBytes = fwd_bytes(36, kNoAllocMaxNmers);  // 4,849,920 bytes
+static_assert(fwd_bytes(31, kNoAllocMaxNmers) <= kScratchBytes, "the 2B forward fits the 3B scratch");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};  // scratch's address on each device, set on first use there
+
+// The scratch address on the current device: looked up once per device (with that device
+// current), then reused.
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    ADCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: device %d outside 0..%d", dev, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        ADCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ad_err(MBXG_EHIP, "mbxg_ad_forward_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return MBXG_OK;
+}
+
+}  // namespace
+
+extern "C" int mbxg_ad_forward(int family, int n, const double* xyz, const double* coef, const double* theta,
+                               double* energy, double* grad, void* workspace, size_t workspace_bytes, void* stream) {
+    int nv, nat, ncoef;
+    AdLayout L;
+    if (int e = ad_check("mbxg_ad_forward", family, n, workspace, workspace_bytes, nv, nat, ncoef, L)) return e;
+    if (n == 0) return MBXG_OK;
+    if (!xyz || !coef || !theta || !energy || !grad) return ad_err(MBXG_EINVAL, "mbxg_ad_forward: null pointer");
+    char* w = (char*)workspace;
+    return ad_forward_run(family, n, xyz, coef, theta, energy, grad, (double*)(w + L.xs), (double*)(w + L.sw),
+                          (double*)(w + L.gs), (double*)(w + L.ps), (int*)(w + L.ndev), (hipStream_t)stream);
+}
+
+extern "C" int mbxg_ad_forward_noalloc(int family, int n, const double* xyz, const double* coef,
+                                       const double* theta, double* energy, double* grad, void* stream) {
+    int nv, nat, ncoef;
+    if (!ad_shape(family, nv, nat, ncoef)) return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: unknown family %d", family);
+    if (n < 0) return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: negative n-mer count %d", n);
+    if (n > kNoAllocMaxNmers)
+        return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: %d n-mers exceed the scratch limit of %d", n,
+                      kNoAllocMaxNmers);
+    if (n == 0) return MBXG_OK;
+    if (!xyz || !coef || !theta || !energy || !grad) return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: null pointer");
+    void* base = nullptr;
+    if (int e = scratch_on_current_device(&base)) return e;
+    // the forward layout for this n inside the scratch; every stage writes its buffer before
+    // reading it (the count by k_ad_set_int), so nothing needs clearing between calls
+    const size_t ld = (size_t)(n + 63) / 64 * 64;
+    char* w = (char*)base;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char* at = w + o;
+        o = align_up(o + bytes);
+        return at;
+    };
+    double* xs = (double*)take(sizeof(double) * nv * ld);
+    double* gs = (double*)take(sizeof(double) * nv * ld);
+    double* sw = (double*)take(sizeof(double) * ld);
+    double* ps = (double*)take(sizeof(double) * ld);
+    int* ndev = (int*)take(sizeof(int));
+    if (o > kScratchBytes) return ad_err(MBXG_EINVAL, "mbxg_ad_forward_noalloc: layout exceeds the scratch");
+    return ad_forward_run(family, n, xyz, coef, theta, energy, grad, xs, sw, gs, ps, ndev, (hipStream_t)stream);
 }
 
 extern "C" int mbxg_ad_backward(int family, int n, const double* xyz, const double* coef, const double* theta,

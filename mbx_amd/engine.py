@@ -62,6 +62,7 @@ PROTOTYPES = {
     "mbxg_ad_defaults": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
     "mbxg_ad_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
     "mbxg_ad_forward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "mbxg_ad_forward_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mbxg_ad_backward": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         ctypes.c_size_t, ctypes.c_int, _vp]),
 }
