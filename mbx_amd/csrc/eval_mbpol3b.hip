@@ -93,6 +93,17 @@ __device__ __forceinline__ double switches(const double* w, double* dab, double*
     return sab * sac + sab * sbc + sac * sbc;
 }
 
+// An empty asm over 27 doubles and an offset: the values are "rewritten" in place, so whatever
+// computes them stays before the fence and whatever reads them (or addresses through off) after.
+__device__ __forceinline__ void group_fence(size_t& off, double* v) {
+    asm volatile(""
+                 : "+v"(off), "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]),
+                   "+v"(v[7]), "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]));
+    asm volatile(""
+                 : "+v"(off), "+v"(v[14]), "+v"(v[15]), "+v"(v[16]), "+v"(v[17]), "+v"(v[18]), "+v"(v[19]),
+                   "+v"(v[20]), "+v"(v[21]), "+v"(v[22]), "+v"(v[23]), "+v"(v[24]), "+v"(v[25]), "+v"(v[26]));
+}
+
 }  // namespace
 
 // Stage 1: variables and switch product.
@@ -119,9 +130,11 @@ __global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ lis
 // Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.  Blocks of 128
 // trimers (two wavefronts); one partial record per wavefront.
 constexpr int kGradBlock3 = 128;
+// variables per group of the chain rule's software pipeline (a divisor of 36)
+constexpr int kChainGroup3 = 4;
 
 template <bool GRAD>
-__global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restrict__ list,
+__global__ void __launch_bounds__(kGradBlock3) __attribute__((amdgpu_waves_per_eu(3))) k_grad_mbpol3b(const int* __restrict__ list,
                                                               const int* __restrict__ n_dev, int cap, NmerCtx N,
                                                               const double* __restrict__ xs,
                                                               const double* __restrict__ gs,
@@ -137,33 +150,68 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
     __syncthreads();
     double e_sum = 0.0;
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int m1 = -1, m2 = -1, m3 = -1;
+    double gd[27];
+#pragma unroll
+    for (int j = 0; j < 27; ++j) gd[j] = 0.0;
     if (t < n) {
         const double s = sw[t];
         const double P = ps[t];
         if (GRAD) {
-            const int m1 = list[3 * t], m2 = list[3 * t + 1], m3 = list[3 * t + 2];
+            m1 = list[3 * t];
+            m2 = list[3 * t + 1];
+            m3 = list[3 * t + 2];
             double w[27];
             load_trimer(N, m1, m2, m3, w);
-            double gd[27];
+            // g_var (x3b-v2x.cpp:63-76, 1638-1673), kChainGroup3 variables at a time.  At each group
+            // boundary the coordinates, the gradient accumulators and the stage offset pass
+            // through empty asm statements: group g's arithmetic starts after the boundary, the
+            // accumulations of group g - 1 end before it, and the loads of group g + 1 (issued
+            // there, in flight while group g computes) follow it.  Left to itself the compiler
+            // hoists every load and square root and sinks every accumulation: ~500 registers.
+            constexpr int G = kChainGroup3;
+            size_t toff = (size_t)t;
+            double cg[G], ng[G];
 #pragma unroll
-            for (int j = 0; j < 27; ++j) gd[j] = 0.0;
-            // g_var (x3b-v2x.cpp:63-76, 1638-1673)
-#pragma unroll
-            for (int j = 0; j < 36; ++j) {
-                const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
-                const double d0 = w[3 * a1] - w[3 * a2];
-                const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
-                const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
-                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                const double gj = gs[(size_t)j * ld + t] * s;
-                const double gg = -kset(st) * gj * xs[(size_t)j * ld + t] / d;
-                gd[3 * a1] += gg * d0;
-                gd[3 * a1 + 1] += gg * d1;
-                gd[3 * a1 + 2] += gg * d2;
-                gd[3 * a2] -= gg * d0;
-                gd[3 * a2 + 1] -= gg * d1;
-                gd[3 * a2 + 2] -= gg * d2;
+            for (int k = 0; k < G; ++k) {
+                cg[k] = gs[(size_t)k * ld + toff];
             }
+#pragma unroll
+            for (int g = 0; g < 36 / G; ++g) {
+                group_fence(toff, w);
+                group_fence(toff, gd);
+                if (g < 36 / G - 1) {
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        ng[k] = gs[(size_t)(G * g + G + k) * ld + toff];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const int j = G * g + k;
+                    const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
+                    const double d0 = w[3 * a1] - w[3 * a2];
+                    const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
+                    const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
+                    const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    const double gj = cg[k] * s;
+                    // the variable itself, recomputed as k_vars_mbpol3b computed it (same bits; one
+                    // exp is cheaper than reading it back)
+                    const double xv = exp(-kset(st) * (d - dset(st)));
+                    const double gg = -kset(st) * gj * xv / d;
+                    gd[3 * a1] += gg * d0;
+                    gd[3 * a1 + 1] += gg * d1;
+                    gd[3 * a1 + 2] += gg * d2;
+                    gd[3 * a2] -= gg * d0;
+                    gd[3 * a2 + 1] -= gg * d1;
+                    gd[3 * a2 + 2] -= gg * d2;
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    cg[k] = ng[k];
+                }
+            }
+            group_fence(toff, gd);
             // switch derivatives (x3b-v2x.cpp:1676-1686)
             double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
             switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
@@ -185,9 +233,6 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
                 for (int at = 0; at < 9; ++at) acc -= w[3 * at + va[q]] * gd[3 * at + vb[q]];
                 vir[q] = acc;
             }
-            table_add(T, m1, gd, 9, grad, 3 * (size_t)N.first_atom[m1]);
-            table_add(T, m2, gd + 9, 9, grad, 3 * (size_t)N.first_atom[m2]);
-            table_add(T, m3, gd + 18, 9, grad, 3 * (size_t)N.first_atom[m3]);
         }
         const double e = s * P;
         e_out[t] = e;
@@ -197,6 +242,27 @@ __global__ void __launch_bounds__(kGradBlock3) k_grad_mbpol3b(const int* __restr
             for (int q = 0; q < 6; ++q) vir_out[(size_t)q * ld + t] = vir[q];
     }
     if (GRAD) {
+        // The evaluate list is grouped by query monomer, so usually every active lane of the
+        // wavefront has the same monomer 1: its nine gradient values are then summed across the
+        // wavefront (inactive lanes hold zeros) and enter the table once instead of 64 times.
+        const bool act = t < n;
+        const unsigned long long am = __ballot(act);
+        if (am) {
+            const int lane = threadIdx.x & 63, lead = __ffsll((long long)am) - 1;
+            const int mq = __shfl(m1, lead, 64);
+            if (__ballot(act && m1 != mq) == 0) {
+                double r[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) r[c] = wave_sum(gd[c]);
+                if (lane == lead) table_add(T, mq, r, 9, grad, 3 * (size_t)N.first_atom[mq]);
+            } else if (act) {
+                table_add(T, m1, gd, 9, grad, 3 * (size_t)N.first_atom[m1]);
+            }
+            if (act) {
+                table_add(T, m2, gd + 9, 9, grad, 3 * (size_t)N.first_atom[m2]);
+                table_add(T, m3, gd + 18, 9, grad, 3 * (size_t)N.first_atom[m3]);
+            }
+        }
         __syncthreads();
         table_flush(T, N.first_atom, N.nat, grad);
     }
