@@ -98,6 +98,165 @@ def parse_holder(text, mons, nats):
             "vsite": vsite, "switch": sw, "vars": [vars_[i] for i in range(nv)]}
 
 
+# Further MB-nrg families, evaluated on the GPU as data-described holders (eval_xfam.hip):
+# (holder name, holder text, monomer names in the holder's argument order, atoms per monomer,
+#  polynomial structure name).  A holder text may hold several parameter sets (one per ion, or
+#  per CO2 model); the set is the constructor's block for these monomer names
+# (e.g. mbnrg_2b_A1_B1C2X2_deg6_v1.cpp:12 "f-", :929 "cl-", :1846 "br-", :2763 "i-").
+XHOLDERS = [
+    ("f_h2o_2b", "src/potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.cpp", ["f-", "h2o"], [1, 3], "halide_h2o_2b"),
+    ("cl_h2o_2b", "src/potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.cpp", ["cl-", "h2o"], [1, 3], "halide_h2o_2b"),
+    ("br_h2o_2b", "src/potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.cpp", ["br-", "h2o"], [1, 3], "halide_h2o_2b"),
+    ("i_h2o_2b", "src/potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.cpp", ["i-", "h2o"], [1, 3], "halide_h2o_2b"),
+    ("f_h2o2_3b", "src/potential/3b/mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp", ["f-", "h2o", "h2o"], [1, 3, 3],
+     "halide_h2o2_3b"),
+    ("cl_h2o2_3b", "src/potential/3b/mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp", ["cl-", "h2o", "h2o"], [1, 3, 3],
+     "halide_h2o2_3b"),
+    ("br_h2o2_3b", "src/potential/3b/mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp", ["br-", "h2o", "h2o"], [1, 3, 3],
+     "halide_h2o2_3b"),
+    ("i_h2o2_3b", "src/potential/3b/mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp", ["i-", "h2o", "h2o"], [1, 3, 3],
+     "halide_h2o2_3b"),
+    ("co2co2_2b", "src/potential/2b/x2b_A1B2_A1B2_v1x.cpp", ["co2", "co2"], [3, 3], "co2co2_2b"),
+    ("co2x3_3b", "src/potential/3b/mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.cpp", ["co2", "co2", "co2"], [3, 3, 3],
+     "co2x3_3b"),
+    ("h2ox4_4b", "src/potential/4b/mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp", ["h2o"] * 4, [3] * 4, "h2ox4_4b"),
+]
+
+# Their polynomial structures: (name, energy-only routine, variables, coefficients)
+XPOLYS = [
+    ("halide_h2o_2b", "src/potential/2b/poly_2b_A1_B1C2X2_deg6_nograd_v1.cpp", 8, 906),
+    ("halide_h2o2_3b", "src/potential/3b/poly_3b_A1_B1C2X2_B1C2X2_deg4_nograd_v1.cpp", 41, 3173),
+    ("co2co2_2b", "src/potential/2b/poly_2b_A1B2_A1B2_v1.cpp", 15, 2269),
+    ("co2x3_3b", "src/potential/3b/poly_3b_A1B2_A1B2_A1B2_deg4_nograd_v1.cpp", 27, 1067),
+    ("h2ox4_4b", "src/potential/4b/poly_4b_A1B2_A1B2_A1B2_A1B2_deg4_nograd_v1.cpp", 66, 398),
+]
+
+# pairs of an n-mer's monomers in the holders' switch naming (sw12, sw13, sw14, sw23, sw24, sw34)
+PAIRS = {2: [(0, 1)], 3: [(0, 1), (0, 2), (1, 2)], 4: [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]}
+
+
+class _Multi:
+    """Multilinear polynomial in the pair switches: {frozenset(pair indices): coefficient}."""
+
+    def __init__(self, d):
+        self.d = d
+
+    def __add__(self, o):
+        o = o if isinstance(o, _Multi) else _Multi({frozenset(): float(o)})
+        r = dict(self.d)
+        for k, v in o.d.items():
+            r[k] = r.get(k, 0.0) + v
+        return _Multi(r)
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return _Multi({k: -v for k, v in self.d.items()})
+
+    def __sub__(self, o):
+        return self + (-o if isinstance(o, _Multi) else -float(o))
+
+    def __rsub__(self, o):
+        return (-self) + o
+
+    def __mul__(self, o):
+        if not isinstance(o, _Multi):
+            return _Multi({k: v * float(o) for k, v in self.d.items()})
+        r = {}
+        for ka, va in self.d.items():
+            for kb, vb in o.d.items():
+                if ka & kb:
+                    raise ValueError("a switch appears twice in one term")
+                r[ka | kb] = r.get(ka | kb, 0.0) + va * vb
+        return _Multi(r)
+
+    __rmul__ = __mul__
+
+
+def _block(src, start):
+    """Text of the brace block whose opening brace is the first one at or after `start`."""
+    i = src.index("{", start) + 1
+    depth, j = 1, i
+    while depth:
+        depth += src[j] == "{"
+        depth -= src[j] == "}"
+        j += 1
+    return src[i:j - 1]
+
+
+def parameter_set(src, mons):
+    """The constructor block of the parameter set for these monomer names: its coefficients and
+    its m_* parameters."""
+    cond = r"\s*(?:and|&&)\s*".join(r'mon%d\s*==\s*"%s"' % (i + 1, re.escape(m)) for i, m in enumerate(mons))
+    m = re.search(r"if\s*\(\s*" + cond + r"\s*\)", src)
+    if not m:
+        raise RuntimeError("no parameter set for %s" % "-".join(mons))
+    blk = _block(src, m.end())
+    c = re.search(r"coefficients\s*=\s*std::vector<double>\s*\{(.*?)\};", blk, re.S)
+    coef = [float(v) for v in c.group(1).replace(",", " ").split()]
+    par = {k: float(v) for k, v in re.findall(r"\b(m_\w+)\s*=\s*([-+0-9.eE]+)\s*;", blk)}
+    return coef, par
+
+
+def parse_xholder(text, mons, nats):
+    """A generated MB-nrg holder as data: sites (real atoms of the monomers in argument order,
+    then the lone-pair sites of every water it builds, two each), variables (form, k, d, site
+    pair), the switching radii, the pair-switch combination and the skip rule, plus the
+    parameter set of these monomers (coefficients)."""
+    src = re.sub(r"//[^\n]*", "", text)
+    coef, par = parameter_set(src, mons)
+    ev = src[src.index("::eval("):]
+    ev = ev[:ev.index("::eval(", 10)]
+    sites = {}
+    for name, off in re.findall(
+            r"const double\*\s*(?:coords_)?([A-Z]_\d+_[a-z])\s*=\s*(?:xcrd|xyz\.data\(\))\s*\+\s*(\d+);", ev):
+        sites[name] = int(off) // 3
+    nreal = sum(nats)
+    assert sorted(sites.values()) == list(range(nreal)), sites
+    w = {k: float(v) for k, v in re.findall(r"double\s+(w12|w13|wcross)\s*=\s*([-+0-9.eE]+);", ev)}
+    vsites = []
+    for o, g1, g2, x1, x2 in re.findall(
+            r"\w+\.setup\((?:coords_)?([A-Z]_\d+_[a-z]),\s*(\w+),\s*(\w+),\s*(?:coords_)?([A-Z]_\d+_[a-z]),"
+            r"\s*(?:coords_)?([A-Z]_\d+_[a-z])\)", ev):
+        sites[x1] = nreal + 2 * len(vsites)
+        sites[x2] = nreal + 2 * len(vsites) + 1
+        vsites.append({"oxygen_site": sites[o], "in_plane": w[g1], "out_of_plane": w[g2]})
+    vars_ = {}
+    for idx, form, args in re.findall(r"(?:v|xs)\[(\d+)\]\s*=\s*(?:vr|vs)\[\d+\]\.(v_exp0|v_coul0|v_exp|v_coul)\(([^)]*)\);",
+                                      ev):
+        a = [t.strip() for t in args.split(",")]
+        if len(a) == 4:  # (r0, k, p1, p2): exp(k (r0 - r)) [/ r]; x2b_A1B2_A1B2_v1x.cpp:62's own v_exp too
+            f = {"v_exp0": "exp0", "v_coul0": "coul0", "v_exp": "exp0", "v_coul": "coul0"}[form]
+            d, k, s1, s2 = par[a[0]], par[a[1]], a[2], a[3]
+        else:  # (k, p1, p2): exp(-k r) [/ r]
+            f = {"v_exp": "exp", "v_coul": "coul"}[form]
+            d, k, s1, s2 = 0.0, par[a[0]], a[1], a[2]
+        vars_[int(idx)] = [f, k, d, sites[s1.replace("coords_", "")], sites[s2.replace("coords_", "")]]
+    nv = max(vars_) + 1
+    assert sorted(vars_) == list(range(nv))
+    k = len(mons)
+    pairs = PAIRS[k]
+    names = {"sw%d%d" % (a + 1, b + 1): i for i, (a, b) in enumerate(pairs)}
+    expr = [" ".join(e.split()) for e in re.findall(r"(?<![\w.])sw\s*=\s*([^;]+);", ev) if e.strip() != "0.0"][0]
+    if expr.startswith("f_switch("):
+        assert k == 2
+        terms = [[1.0, [0]]]
+    else:
+        env = {n: _Multi({frozenset([i]): 1.0}) for n, i in names.items()}
+        val = eval(compile(ast.parse(expr, mode="eval"), "<sw>", "eval"), env)
+        terms = [[v, sorted(t)] for t, v in sorted(val.d.items(), key=lambda kv: sorted(kv[0])) if v != 0.0]
+    # skip rule: the n-mer is skipped when every first-atom separation exceeds the outer radius
+    outer = par.get("m_ro", par.get("m_r2f"))
+    inner = par.get("m_ri", par.get("m_r2i"))
+    skip = re.search(r"if\s*\(\s*(true\s*&&[^)]*|r12\s*>\s*m_r2f)\)", ev).group(1)
+    want = {"d%d%dr" % (a + 1, b + 1) for a, b in pairs}
+    got = set(re.findall(r"(d\d\dr)\s*>\s*m_ro", skip)) or ({"d12r"} if "r12" in skip else set())
+    assert got == want, (skip, want)
+    return {"monomers": mons, "atoms": nats, "nreal": nreal, "nsites": nreal + 2 * len(vsites), "vsites": vsites,
+            "switch": {"inner": inner, "outer": outer, "pairs": pairs, "terms": terms},
+            "skip": "all_pairs_beyond_outer", "vars": [vars_[i] for i in range(nv)], "coef": coef}
+
+
 def parse_holder_1b(text, mon, nat):
     """Variables of a generated MB-nrg 1B holder: v[i] = vr[i].v_exp0(d, k, P, Q) over the
     monomer's atoms (sites named A_1, B_1, ... at xcrd + 3 * site), with the parameters m_k_*,
@@ -335,6 +494,22 @@ def main():
         with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
             json.dump(h, f, indent=1)
         print("%s holder: %d variables, %d sites" % (name, len(h["vars"]), h["nsites"]))
+    for name, poly_path, nvars, ncoef in XPOLYS:
+        text = open(os.path.join(ref, poly_path)).read()
+        poly = expand_tarray(text) if "fmg1(" in text else expand(text)
+        first = [h for h in XHOLDERS if h[4] == name][0]
+        coef = parse_xholder(open(os.path.join(ref, first[1])).read(), first[2], first[3])["coef"]
+        if len(coef) != ncoef:
+            raise RuntimeError("%s: %d coefficients, expected %d" % (name, len(coef), ncoef))
+        n = write_table(os.path.join(HERE, name + ".tbl"), name, nvars, coef, poly)
+        print("%s: %d monomials, %d coefficients" % (name, n, len(coef)))
+    for name, holder_path, mons, nats, poly in XHOLDERS:
+        h = parse_xholder(open(os.path.join(ref, holder_path)).read(), mons, nats)
+        h["name"], h["poly"] = name, poly
+        with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
+            json.dump(h, f, indent=0)
+        print("%s holder: %d variables, %d sites, %d switch terms, %d coefficients" % (
+            name, len(h["vars"]), h["nsites"], len(h["switch"]["terms"]), len(h["coef"])))
     for name, poly_path, holder_path, nvars, ncoef in POLYS:
         text = open(os.path.join(ref, poly_path)).read()
         poly = expand_tarray(text) if "fmg1(" in text else expand(text)

@@ -548,7 +548,7 @@ long orc_clusters(int nmax, double cutoff, int nmon, int use_pbc, const double* 
         }
     }
     long found = 0;
-    if (np < 2 || (nmax > 2 && np < 3)) goto done;
+    if (np < 2 || (nmax > 2 && np < 3) || (nmax > 3 && np < 4)) goto done;
     {
         /* XyzToCloud (kdtree_utils.h:196-279): slot 0 = re-wrapped point, slots 1..7 images */
         const long size = use_pbc ? 8L * np : np;
@@ -655,6 +655,36 @@ long orc_clusters(int nmax, double cutoff, int nmon, int use_pbc, const double* 
                         }
                     }
                 }
+            } else if (nmax == 4) {
+                /* tetramers, the center-3-neighbour rule (sys_tools.cpp:594-634): no self-image check */
+                for (long j = 0; j < nm; ++j)
+                    for (long k = j + 1; k < nm; ++k)
+                        for (long l = k + 1; l < nm; ++l) {
+                            const double* pj = pts + 3 * mt[j].idx;
+                            const double* pk = pts + 3 * mt[k].idx;
+                            const double* pl = pts + 3 * mt[l].idx;
+                            double a0 = pj[0] - pk[0], a1 = pj[1] - pk[1], a2 = pj[2] - pk[2];
+                            const double d2jk = a0 * a0 + a1 * a1 + a2 * a2;
+                            a0 = pj[0] - pl[0], a1 = pj[1] - pl[1], a2 = pj[2] - pl[2];
+                            const double d2jl = a0 * a0 + a1 * a1 + a2 * a2;
+                            a0 = pk[0] - pl[0], a1 = pk[1] - pl[1], a2 = pk[2] - pl[2];
+                            const double d2kl = a0 * a0 + a1 * a1 + a2 * a2;
+                            const int idj = (int)(mt[j].idx % np);
+                            const int idk = (int)(mt[k].idx % np);
+                            const int idl = (int)(mt[l].idx % np);
+                            const int ti = tag_index[i];
+                            if (!((ti >= tag_index[idj] && d2jk < c2 && d2jl < c2) ||
+                                  (ti >= tag_index[idk] && d2jk < c2 && d2kl < c2) ||
+                                  (ti >= tag_index[idl] && d2kl < c2 && d2jl < c2))) {
+                                if (found < cap) {
+                                    out[4 * found] = mon_index[i];
+                                    out[4 * found + 1] = mon_index[idj];
+                                    out[4 * found + 2] = mon_index[idk];
+                                    out[4 * found + 3] = mon_index[idl];
+                                }
+                                ++found;
+                            }
+                        }
             }
         }
         free(mt);
@@ -1128,4 +1158,115 @@ double orc_ps_1b(const orc_ps* p, const double* w, double* g, double* vir6) {
         }
     }
     return e;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Further MB-nrg holders as data (see oracle.h, orc_xholder).                                 */
+
+/* Variable forms 0-2 as nrg_var, and form 3: variable::v_coul, exp(-k r) / r
+ * (tools/variable.cpp:109-129). */
+static double xvar(int form, double k, double d, const double* p1, const double* p2, double* gvec) {
+    if (form != 3) return nrg_var(form, k, d, p1, p2, gvec);
+    gvec[0] = p1[0] - p2[0];
+    gvec[1] = p1[1] - p2[1];
+    gvec[2] = p1[2] - p2[2];
+    const double rsq = gvec[0] * gvec[0] + gvec[1] * gvec[1] + gvec[2] * gvec[2];
+    const double r = sqrt(rsq);
+    const double exp1 = exp(k * (-r));
+    const double rinv = 1.0 / r;
+    const double val = exp1 * rinv;
+    const double gg = -(k + rinv) * val * rinv;
+    gvec[0] *= gg;
+    gvec[1] *= gg;
+    gvec[2] *= gg;
+    return val;
+}
+
+/* The generated holders' eval with gradients, e.g. mbnrg_2b_A1_B1C2X2_deg6_v1.cpp:3789-3960,
+ * mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp:12955-13230, x2b_A1B2_A1B2_v1x.cpp:18485-18700,
+ * mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.cpp:1223-1460, mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp:657-1100:
+ * first-atom separations of every monomer pair and the skip rule; the water lone-pair sites
+ * (tools/water_monomer_lp.cpp:3-21); variables; P and its gradient; the gradient scaled by the
+ * switch combination sw; variable gradients to the sites; lone-pair gradients back to O, H, H
+ * (water_monomer_lp.cpp:25-48); the switch derivative along every pair; E = sw P; the virial
+ * -sum_atoms x (x) g over the real atoms. */
+double orc_xholder_eval(const orc_xholder* h, const orc_poly* p, const double* a, const double* const* mons,
+                        double* const* grads, double* vir, int* evaluated) {
+    double xs[3 * 24], xg[3 * 24];
+    int off[5] = {0, 0, 0, 0, 0};
+    for (int m = 0; m < h->nmon; ++m) off[m + 1] = off[m] + h->nat[m];
+    for (int m = 0; m < h->nmon; ++m) memcpy(xs + 3 * off[m], mons[m], sizeof(double) * 3 * h->nat[m]);
+    *evaluated = 0;
+    double dd[6][3], r[6];
+    int beyond = 1;
+    for (int q = 0; q < h->npair; ++q) {
+        const double* u = xs + 3 * off[h->pa[q]];
+        const double* v = xs + 3 * off[h->pb[q]];
+        dd[q][0] = u[0] - v[0];
+        dd[q][1] = u[1] - v[1];
+        dd[q][2] = u[2] - v[2];
+        r[q] = sqrt(dd[q][0] * dd[q][0] + dd[q][1] * dd[q][1] + dd[q][2] * dd[q][2]);
+        beyond = beyond && r[q] > h->r_out;
+    }
+    if (beyond) return 0.0;
+    *evaluated = 1;
+    double oh1[2][3], oh2[2][3];
+    for (int w = 0; w < h->nvsite; ++w)
+        msite_setup_w(xs + 3 * h->vo[w], h->in_plane, h->out_of_plane, oh1[w], oh2[w], xs + 3 * (h->nreal + 2 * w),
+                      xs + 3 * (h->nreal + 2 * w + 1));
+    double v[96], gvv[96][3], g[96];
+    for (int j = 0; j < h->nvars; ++j)
+        v[j] = xvar(h->form[j], h->k[j], h->d[j], xs + 3 * h->s1[j], xs + 3 * h->s2[j], gvv[j]);
+    const double P = orc_poly_eval(p, a, v, g);
+    double s[6], gs[6];
+    for (int q = 0; q < h->npair; ++q) s[q] = nrg_switch(h->r_in, h->r_out, r[q], &gs[q]);
+    double sw = 0.0;
+    for (int t = 0; t < h->nterm; ++t) {
+        double prod = h->tcoef[t];
+        for (int q = 0; q < h->npair; ++q)
+            if (h->tmask[t] >> q & 1) prod *= s[q];
+        sw += prod;
+    }
+    for (int j = 0; j < h->nvars; ++j) g[j] *= sw;
+    memset(xg, 0, sizeof(xg));
+    for (int j = 0; j < h->nvars; ++j)
+        for (int c = 0; c < 3; ++c) {
+            const double t = g[j] * gvv[j][c];
+            xg[3 * h->s1[j] + c] += t;
+            xg[3 * h->s2[j] + c] -= t;
+        }
+    for (int w = 0; w < h->nvsite; ++w)
+        msite_grads_w(oh1[w], oh2[w], h->in_plane, h->out_of_plane, xg + 3 * (h->nreal + 2 * w),
+                      xg + 3 * (h->nreal + 2 * w + 1), xg + 3 * h->vo[w]);
+    for (int q = 0; q < h->npair; ++q) {
+        /* d sw / d s_q: the terms holding pair q, with s_q left out */
+        double dsw = 0.0;
+        for (int t = 0; t < h->nterm; ++t) {
+            if (!(h->tmask[t] >> q & 1)) continue;
+            double prod = h->tcoef[t];
+            for (int u = 0; u < h->npair; ++u)
+                if (u != q && (h->tmask[t] >> u & 1)) prod *= s[u];
+            dsw += prod;
+        }
+        const double gq = gs[q] * dsw * P / r[q];
+        for (int c = 0; c < 3; ++c) {
+            xg[3 * off[h->pa[q]] + c] += gq * dd[q][c];
+            xg[3 * off[h->pb[q]] + c] -= gq * dd[q][c];
+        }
+    }
+    for (int m = 0; m < h->nmon; ++m)
+        if (grads && grads[m])
+            for (int i = 0; i < 3 * h->nat[m]; ++i) grads[m][i] += xg[3 * off[m] + i];
+    if (vir) {
+        const int va[6] = {0, 0, 0, 1, 1, 2}, vb[6] = {0, 1, 2, 1, 2, 2}, slot[6] = {0, 1, 2, 4, 5, 8};
+        for (int q = 0; q < 6; ++q) {
+            double acc = 0.0;
+            for (int at = 0; at < h->nreal; ++at) acc += -xs[3 * at + va[q]] * xg[3 * at + vb[q]];
+            vir[slot[q]] += acc;
+        }
+        vir[3] = vir[1];
+        vir[6] = vir[2];
+        vir[7] = vir[5];
+    }
+    return P * sw;
 }

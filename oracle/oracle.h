@@ -97,6 +97,40 @@ typedef struct {
 double orc_nrg_holder(const orc_holder* h, const orc_poly* p, const double* a, const double* const* mons,
                       double* const* grads, double* vir, int* evaluated);
 
+/* A further MB-nrg holder as data (mbx_amd/tables/<name>.holder.json, extracted from the holder's
+ * text: the halide-water 2B and 3B, CO2 2B and 3B and water 4B holders): up to four monomers
+ * in the holder's argument order; sites = real atoms, then two lone-pair sites per water with
+ * them (oxygen site vo[w], the holders' w12 / wcross); variables of forms 0-2 as orc_holder
+ * and 3: exp(-k r) / r; the pair switches of every monomer pair (pa[q], pb[q]) combined as
+ * sum_t tcoef[t] prod_{q in tmask[t]} s_q; skipped when every pair separation exceeds r_out. */
+typedef struct {
+    int nmon;
+    int nat[4];
+    int nreal, nsites;
+    int nvsite;
+    int vo[2];
+    double in_plane, out_of_plane;
+    double r_in, r_out;
+    int nvars;
+    const int* form;
+    const double* k;
+    const double* d;
+    const int* s1;
+    const int* s2;
+    int npair;
+    int pa[6], pb[6];
+    int nterm;
+    const double* tcoef;
+    const int* tmask;
+} orc_xholder;
+
+/* One n-mer through such a holder; conventions of orc_nrg_holder.  Pinned by the reference's
+ * unittest-energy2b.cpp:1322-1557, 1558-1612 (halide-water, CO2-CO2), unittest-energy3b.cpp:1010-1482
+ * (halide-water-water, CO2 x3) and unittest-energy4b.cpp (water x4) known answers, the compiled
+ * reference holders on perturbed n-mers, and central finite differences. */
+double orc_xholder_eval(const orc_xholder* h, const orc_poly* p, const double* a, const double* const* mons,
+                        double* const* grads, double* vir, int* evaluated);
+
 /* MB-nrg 1B holder for ONE monomer: x1b_A1B4_v1x::eval (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371-470;
  * h: the holder's variables over the monomer's atoms, mbx_amd/tables/ch4_1b.holder.json).
  * E = P(v); g (nullable) receives dE/dw (overwritten); vir6 (nullable) is incremented by
@@ -120,7 +154,8 @@ void orc_close_image(const double* box, const double* box_inv, const double* xyz
 /* AddClusters (bblock/sys_tools.cpp:472-641) with XyzToCloud (kdtree/kdtree_utils.h:196-279),
  * GetFractionalCoordinates (:165-194) and nanoflann's strict radius test
  * (kdtree/nanoflann.hpp:166-171), by brute force over the point cloud.
- * first_xyz: 3 doubles per monomer (its first atom).  Writes nmax indices per n-mer to out
+ * first_xyz: 3 doubles per monomer (its first atom).  nmax 2, 3 or 4 (tetramers: the
+ * center-3-neighbour rule of :594-634).  Writes nmax indices per n-mer to out
  * (up to cap n-mers) and returns the number found.  Queries all local monomers in order.
  * Pinned by: unittest-systools.cpp:852-1129 counts, unittest-system.cpp:644-718 explicit
  * sets, the compiled reference AddClusters, and an O(N^2)/O(N^3) minimum-image brute force. */

@@ -43,12 +43,30 @@ CASES = {
     # MB-nrg CH4 1B: polynomial and holder
     "poly_ch4_1b": ("src/unittests/unittest-poly-1b.cpp", 'poly_1b_A1B4_deg5_exp0_v1x::struct'),
     "holder_ch4_1b": ("src/unittests/unittest-poly-holder-1b.cpp", 'x1b_A1B4_deg5_exp0::struct'),
+    # halide-water 2B / 3B, CO2 2B / 3B and water 4B energy routines ("#n": the n-th section of
+    # that name; the first co2-co2 section uses the co2_archive model, the first co2-co2-co2 one is
+    # commented out)
+    "energy_f_h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:f--h2o'),
+    "energy_cl_h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:cl-h2o'),
+    "energy_br_h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:br-h2o'),
+    "energy_i_h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:i-h2o'),
+    "energy_co2co2_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:co2-co2#2'),
+    "energy_f_h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:f-h2o-h2o'),
+    "energy_cl_h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:cl-h2o-h2o'),
+    "energy_br_h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:br-h2o-h2o'),
+    "energy_i_h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:i-h2o-h2o'),
+    "energy_co2x3_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:co2-co2-co2#2'),
+    "energy_h2ox4_4b": ("src/unittests/unittest-energy4b.cpp", 'SECTION:h2o-h2o-h2o-h2o'),
+    "poly_co2co2_2b": ("src/unittests/unittest-poly-2b.cpp", ':poly_2b_A1B2_A1B2_v1x:struct'),
 }
 
 
 def block(text, name):
     if name.startswith("SECTION:"):
-        i = text.index('SECTION("%s")' % name[len("SECTION:"):])
+        sec, _, nth = name[len("SECTION:"):].partition("#")
+        i = -1
+        for _ in range(int(nth or 1)):
+            i = text.index('SECTION("%s")' % sec, i + 1)
     else:
         i = text.index('TEST_CASE("%s")' % name)
     j = text.index("{", i) + 1
@@ -69,6 +87,8 @@ def values(body):
         out[m.group(1)] = float(m.group(2))
     for m in re.finditer(r"size_t\s+(\w+)\s*=\s*(\d+)\s*;", body):
         out[m.group(1)] = int(m.group(2))
+    for m in re.finditer(r"std::string\s+(mon\d)\s*=\s*\"([^\"]*)\"\s*;", body):
+        out[m.group(1)] = m.group(2)
     return out
 
 
@@ -138,7 +158,7 @@ def main():
     res = {}
     for key, (path, name) in CASES.items():
         res[key] = values(block(open(os.path.join(ref, path)).read(), name))
-        res[key]["_source"] = ('%s SECTION("%s")' % (path, name[8:]) if name.startswith("SECTION:")
+        res[key]["_source"] = ('%s SECTION("%s")' % (path, name[8:].split("#")[0]) if name.startswith("SECTION:")
                                else '%s TEST_CASE("%s")' % (path, name))
     path = "src/unittests/unittest-energy1b.cpp"
     res["energy1b"] = energy1b(open(os.path.join(ref, path)).read())
