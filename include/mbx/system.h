@@ -69,6 +69,8 @@ class System {
     double OneBodyEnergy(bool do_grads);
     double TwoBodyEnergy(bool do_grads, bool use_ghost = 0);
     double ThreeBodyEnergy(bool do_grads, bool use_ghost = 0);
+    // system.h:1112: the 4B energy (0 unless Set4bCutoff / "fourbody_cutoff" > 0)
+    double FourBodyEnergy(bool do_grads, bool use_ghost = 0);
 
     // system.h:278, 285, 450: gradients of the last energy call in input order, of every site
     // (virtual sites included, which have none from these terms) or of the real atoms; virial
@@ -98,6 +100,12 @@ class System {
     size_t GetMaxEval3b();
     void Set2bIgnorePoly(std::vector<std::vector<std::string>> ignore_2b);
     void Set3bIgnorePoly(std::vector<std::vector<std::string>> ignore_3b);
+    // system.h:408, 444, 699, 705, 842: the 4B term's cutoff and ignore list
+    void Set4bCutoff(double cutoff4b);
+    double Get4bCutoff();
+    void Set4bIgnorePoly(std::vector<std::vector<std::string>> ignore_4b);
+    void Add4bIgnorePoly(std::string mon1, std::string mon2, std::string mon3, std::string mon4);
+    std::vector<std::vector<std::string>> Get4bIgnorePoly();
 
     // system.h:246, system.cpp:313: n-mers (input monomer indices) with a member in [istart, iend)
     std::vector<size_t> GetPairList(size_t nmax, double cutoff, size_t istart, size_t iend, bool use_ghost = false);
@@ -151,7 +159,8 @@ class System {
     std::vector<double> box_;
     double cutoff2b_ = 100.0, cutoff3b_ = 5.0;
     size_t max2b_ = 1024, max3b_ = 1024;
-    std::vector<std::vector<std::string>> ignore2b_, ignore3b_;
+    double cutoff4b_ = 0.0;  // system.cpp:89
+    std::vector<std::vector<std::string>> ignore2b_, ignore3b_, ignore4b_;
     std::vector<long> counts_ = std::vector<long>(4, 0);
     size_t numat_ = 0, numsites_ = 0;
 };

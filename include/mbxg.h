@@ -29,10 +29,10 @@ extern "C" {
 
 typedef struct mbxg_ctx mbxg_ctx;
 
-/* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", ...). */
-enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2 };
+/* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", "co2", "f-", "cl-", "br-", "i-"). */
+enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2, MBXG_CO2 = 3, MBXG_F = 4, MBXG_CL = 5, MBXG_BR = 6, MBXG_I = 7 };
 /* Terms for mbxg_compute; MBXG_LISTS_ONLY builds the requested lists without evaluating. */
-enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4, MBXG_1B = 8 };
+enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4, MBXG_1B = 8, MBXG_4B = 16 };
 /* Status codes. */
 enum { MBXG_OK = 0, MBXG_EINVAL = -1, MBXG_EHIP = -2, MBXG_ESTATE = -3, MBXG_ENOMEM = -4 };
 
@@ -44,8 +44,9 @@ int mbxg_create(mbxg_ctx** ctx, int device);
 int mbxg_destroy(mbxg_ctx* ctx);
 
 /* Topology, in the engine's (internal) monomer order; monomer i's atoms follow monomer i-1's.
- * kind[i]: MBXG_H2O (3 atoms O H H), MBXG_CH4 (5 atoms C H H H H) or MBXG_OTHER (nat[i] atoms,
- * contributes nothing, like an unknown pair in energy2b.cpp:372-374).  tag[i] orders monomers
+ * kind[i]: MBXG_H2O (3 atoms O H H), MBXG_CH4 (5 atoms C H H H H), MBXG_CO2 (3 atoms C O O), the
+ * halide ions MBXG_F / MBXG_CL / MBXG_BR / MBXG_I (1 atom) or MBXG_OTHER (nat[i] atoms, listed but
+ * without a holder here: only MBXG_LISTS_ONLY computes accept it).  tag[i] orders monomers
  * for n-mer ownership (the reference uses the first atom's tag, sys_tools.cpp:558-592).
  * islocal may be NULL (all local).  Replaces System::AddMonomer/Initialize bookkeeping
  * (system.cpp:848-870, 1794-1969). */
@@ -66,7 +67,15 @@ enum {
     MBXG_FAM_CH4H2O_2B = 2,     /* ch4-h2o, MB-nrg A1B2Z2_C1D4 */
     MBXG_FAM_CH4CH4_2B = 4,     /* ch4-ch4, MB-nrg A1B4_A1B4 */
     MBXG_FAM_MBPOL_3B = 8,      /* h2o x3, MB-pol */
-    MBXG_FAM_CH4H2O2_3B = 16    /* ch4-h2o-h2o, MB-nrg A1B4_C1D2_C1D2 */
+    MBXG_FAM_CH4H2O2_3B = 16,   /* ch4-h2o-h2o, MB-nrg A1B4_C1D2_C1D2 */
+    /* halide-water 2B, MB-nrg A1_B1C2X2 deg6 (energy2b.cpp:261-272) */
+    MBXG_FAM_F_H2O_2B = 1 << 5, MBXG_FAM_CL_H2O_2B = 1 << 6, MBXG_FAM_BR_H2O_2B = 1 << 7, MBXG_FAM_I_H2O_2B = 1 << 8,
+    MBXG_FAM_CO2CO2_2B = 1 << 9, /* co2-co2, x2b_A1B2_A1B2_v1x (energy2b.cpp:293-295) */
+    /* halide-water-water 3B, MB-nrg A1_B1C2X2_B1C2X2 deg4 (energy3b.cpp:229-240) */
+    MBXG_FAM_F_H2O2_3B = 1 << 10, MBXG_FAM_CL_H2O2_3B = 1 << 11, MBXG_FAM_BR_H2O2_3B = 1 << 12,
+    MBXG_FAM_I_H2O2_3B = 1 << 13,
+    MBXG_FAM_CO2X3_3B = 1 << 14, /* co2 x3, MB-nrg A1B2_A1B2_A1B2 deg4 (energy3b.cpp:247-249) */
+    MBXG_FAM_H2OX4_4B = 1 << 15  /* h2o x4, MB-nrg A1B2_A1B2_A1B2_A1B2 deg4 (energy4b.cpp:218-221) */
 };
 int mbxg_set_families(mbxg_ctx* ctx, int mask);
 
@@ -92,6 +101,22 @@ int mbxg_eval_1b(mbxg_ctx* ctx, int do_grads, double* e1b, double* grad, double*
  * The 1B total is a fixed-order sum over the local monomers. */
 int mbxg_get_energies(mbxg_ctx* ctx, double* e1b, double* e2b, double* e3b);
 
+/* The 4B term (System::FourBodyEnergy / Get4B, bblock/system.cpp:2747-2767, 3075-3198): the
+ * cutoff of the tetramer search (System::Set4bCutoff, system.h:842; 0, the default, disables the
+ * term).  Tetramers come from AddClusters with nmax = 4 (bblock/sys_tools.cpp:594-634, the
+ * center-3-neighbour rule) and are evaluated by e4b::get_4b_energy (potential/4b/energy4b.cpp:120;
+ * water x4 only).  A compute with MBXG_4B, or over a topology holding co2 or halide monomers,
+ * runs synchronously (eager, no graph replay). */
+int mbxg_set_cutoff4b(mbxg_ctx* ctx, double cutoff4b);
+/* e[0..3] = the 1B, 2B, 3B and 4B energies of the last compute (2B / 3B include every family). */
+int mbxg_get_term_energies(mbxg_ctx* ctx, double* e);
+/* counts[0..10]: evaluated n-mers of the further families, in the order of their MBXG_FAM_* bits
+ * from MBXG_FAM_F_H2O_2B; counts[11]: listed tetramers. */
+int mbxg_get_xcounts(mbxg_ctx* ctx, int64_t* counts);
+/* The last compute's listed tetramers (4 internal monomer indices each, query monomer first), at
+ * most cap of them; *n receives the number listed. */
+int mbxg_get_tetramers(mbxg_ctx* ctx, int32_t* tetramers, int64_t cap, int64_t* n);
+
 /* FixMonomerCoordinates (bblock/sys_tools.cpp:257-320) of host coordinates (3*natoms doubles,
  * internal order) under the current box, into xyz_out (may equal xyz_in): each monomer's first
  * atom wrapped into the cell (fractional x - floor(x + 0.5)), the others to their images nearest
@@ -110,8 +135,9 @@ int mbxg_fix_coordinates(mbxg_ctx* ctx, const double* xyz_in, double* xyz_out);
 int mbxg_set_batch(mbxg_ctx* ctx, int nconf, const int* mon_offsets);
 
 /* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163,
- * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207).  a == NULL restores
- * the fitted values. */
+ * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207, "f_h2o_2b" / "cl_h2o_2b" /
+ * "br_h2o_2b" / "i_h2o_2b": 906, "co2co2_2b": 2269, "f_h2o2_3b" / "cl_h2o2_3b" / "br_h2o2_3b" /
+ * "i_h2o2_3b": 3173, "co2x3_3b": 1067, "h2ox4_4b": 398).  a == NULL restores the fitted values. */
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
 
 /* Coordinates of all atoms (3*natoms doubles, internal order) from host or device memory.

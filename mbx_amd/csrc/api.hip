@@ -32,6 +32,7 @@
 #include "gen/coef_mbpol_2b.inc"
 #include "gen/coef_mbpol_3b.inc"
 #include "mbxg_kernels.hpp"
+#include "xfam.hpp"
 
 using namespace mbxg;
 
@@ -168,7 +169,7 @@ struct mbxg_ctx {
     hipGraphExec_t gexec = nullptr;
     int g_terms = -1, g_grads = -1;       // what gexec computes
     int eager_terms = -1, eager_grads = -1;  // last eager compute since the buffers were sized
-    int fam_mask = (1 << kFamilies) - 1;  // evaluated families (mbxg_set_families)
+    int fam_mask = (1 << (kFamilies + kXFam)) - 1;  // evaluated families (mbxg_set_families)
     int use_ghost = 0;                    // the cloud holds ghost monomers too (mbxg_set_use_ghost)
     // 1B term (MBXG_1B): the local waters, then the local CH4s; per monomer energy and virial rows
     std::vector<int> water_local, ch4_local;
@@ -184,6 +185,18 @@ struct mbxg_ctx {
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
     hipEvent_t ev[kNumEvents] = {};
+    // further MB-nrg families and the 4B term (xfam.hip); they run synchronously
+    double cut4 = 0.0;
+    bool has_xkind = false;                  // the topology holds co2 or halide monomers
+    DBuf<int> x_code, x_blk, x_off, x_tot;   // classification scratch; x_tot: [order 2, 3, 4][kXFam]
+    DBuf<int> x_list[kXFam];
+    int x_count[kXFam] = {};
+    EvalState xf[kXFam];
+    DBuf<double> d_xcoef[kXFam];
+    DBuf<double> d_xfam_tot;                 // per further family: energy + 6 virial components
+    DBuf<int> t_counts, t_off, t_list, t_ovf;  // tetramers
+    int64_t n4_listed = 0;
+    bool x_last = false;                     // the last compute ran the further families / 4B
 };
 
 int mbxg::report_error(int code, const char* msg) {
@@ -211,9 +224,15 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
     }
     int n1b = 0;
     const double* fit1b = ch4_1b_fit(&n1b);
-    // totals: 2B record, 3B record, 1B record (kPartials each)
-    if (c->d_coef2.reserve(kNCoef_mbpol_2b) || c->d_coef3.reserve(kNCoef_mbpol_3b) ||
-        c->d_totals.reserve(3 * kPartials) || c->d_coef1b.reserve(n1b) ||
+    // totals: 2B record, 3B record, 1B record, 4B record (kPartials each)
+    bool xbad = false;
+    for (int f = 0; f < kXFam; ++f) {
+        int nx = 0;
+        xfam_default_coef(f, &nx);
+        xbad = xbad || c->d_xcoef[f].reserve(nx);
+    }
+    if (xbad || c->d_xfam_tot.reserve(kXFam * kPartials) || c->d_coef2.reserve(kNCoef_mbpol_2b) ||
+        c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(4 * kPartials) || c->d_coef1b.reserve(n1b) ||
         c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
         c->d_coefn[FAM_CH4H2O_2B].reserve(kNCoef_ch4h2o_2b) || c->d_coefn[FAM_CH4CH4_2B].reserve(kNCoef_ch4ch4_2b) ||
         c->d_coefn[FAM_CH4H2O2_3B].reserve(kNCoef_ch4h2o2_3b)) {
@@ -222,7 +241,12 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
     }
     HIPCHK(hipMemcpy(c->d_coef2.p, kFit_mbpol_2b, sizeof(double) * kNCoef_mbpol_2b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_coef1b.p, fit1b, sizeof(double) * n1b, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(c->d_totals.p, 0, sizeof(double) * 3 * kPartials));
+    HIPCHK(hipMemset(c->d_totals.p, 0, sizeof(double) * 4 * kPartials));
+    for (int f = 0; f < kXFam; ++f) {
+        int nx = 0;
+        const double* fit = xfam_default_coef(f, &nx);
+        HIPCHK(hipMemcpy(c->d_xcoef[f].p, fit, sizeof(double) * nx, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(c->d_coef3.p, kFit_mbpol_3b, sizeof(double) * kNCoef_mbpol_3b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_coefn[FAM_CH4H2O_2B].p, kFit_ch4h2o_2b, sizeof(double) * kNCoef_ch4h2o_2b,
                      hipMemcpyHostToDevice));
@@ -271,6 +295,22 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_conf_part.release();
     c->d_fam_tot.release();
     for (DBuf<double>& b : c->d_coefn) b.release();
+    for (DBuf<double>& b : c->d_xcoef) b.release();
+    for (DBuf<int>& b : c->x_list) b.release();
+    for (EvalState* o = c->xf; o != c->xf + kXFam; ++o) {
+        o->e_nmer.release();
+        o->partials.release();
+        o->ws.release();
+    }
+    c->d_xfam_tot.release();
+    c->x_code.release();
+    c->x_blk.release();
+    c->x_off.release();
+    c->x_tot.release();
+    c->t_counts.release();
+    c->t_off.release();
+    c->t_list.release();
+    c->t_ovf.release();
     for (EvalState* o = c->fam; o != c->fam + kFamilies; ++o) {
         o->e_nmer.release();
         o->vir_nmer.release();
@@ -379,6 +419,10 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     for (int i = 0; i < nmon; ++i) {
         if (kind[i] == MBXG_H2O && nat[i] != 3) return fail(MBXG_EINVAL, "monomer %d: h2o needs 3 atoms", i);
         if (kind[i] == MBXG_CH4 && nat[i] != 5) return fail(MBXG_EINVAL, "monomer %d: ch4 needs 5 atoms", i);
+        if (kind[i] == MBXG_CO2 && nat[i] != 3) return fail(MBXG_EINVAL, "monomer %d: co2 needs 3 atoms", i);
+        if (kind[i] >= MBXG_F && kind[i] <= MBXG_I && nat[i] != 1)
+            return fail(MBXG_EINVAL, "monomer %d: a halide ion has 1 atom", i);
+        if (kind[i] < MBXG_H2O || kind[i] > MBXG_I) return fail(MBXG_EINVAL, "monomer %d: unknown kind %d", i, kind[i]);
         if (nat[i] < 1) return fail(MBXG_EINVAL, "monomer %d: no atoms", i);
         c->first_atom[i] = acc;
         acc += nat[i];
@@ -386,6 +430,8 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     c->natoms = acc;
     c->n_h2o = (int)std::count(kind, kind + nmon, MBXG_H2O);
     c->n_ch4 = (int)std::count(kind, kind + nmon, MBXG_CH4);
+    c->has_xkind = false;
+    for (int i = 0; i < nmon; ++i) c->has_xkind = c->has_xkind || (kind[i] >= MBXG_CO2 && kind[i] <= MBXG_I);
     const size_t nm = nmon > 0 ? nmon : 1;
     if (c->d_kind.reserve(nm) || c->d_nat.reserve(nm) || c->d_first.reserve(nm) || c->d_tag.reserve(nm) ||
         c->d_members.reserve(nm) || c->d_islocal.reserve(nm) || c->d_in.reserve(3 * (size_t)acc + 3) ||
@@ -420,7 +466,8 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
 }
 
 extern "C" int mbxg_set_families(mbxg_ctx* c, int mask) {
-    if (!c || (mask & ~((1 << kFamilies) - 1))) return fail(MBXG_EINVAL, "mbxg_set_families: bad arguments");
+    if (!c || (mask & ~((1 << (kFamilies + kXFam)) - 1)))
+        return fail(MBXG_EINVAL, "mbxg_set_families: bad arguments");
     if (mask == c->fam_mask) return MBXG_OK;
     c->fam_mask = mask;
     c->need_sync = true;
@@ -450,6 +497,15 @@ extern "C" int mbxg_set_batch(mbxg_ctx* c, int nconf, const int* mon_offsets) {
     if (!c || nconf < 1 || (nconf > 1 && !mon_offsets)) return fail(MBXG_EINVAL, "mbxg_set_batch: bad arguments");
     HIPCHK(hipSetDevice(c->device));
     return set_configs(c, nconf, mon_offsets);
+}
+
+// The further family of a coefficient-table name, or -1.
+static int xfam_index(const std::string& m) {
+    static const char* names[kXFam] = {"f_h2o_2b",  "cl_h2o_2b",  "br_h2o_2b",  "i_h2o_2b",  "co2co2_2b", "f_h2o2_3b",
+                                       "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b", "co2x3_3b", "h2ox4_4b"};
+    for (int f = 0; f < kXFam; ++f)
+        if (m == names[f]) return f;
+    return -1;
 }
 
 // InvertUnitCell (tools/math_tools.cpp:37-64), same operation order.
@@ -556,6 +612,10 @@ extern "C" int mbxg_set_coefficients(mbxg_ctx* c, const char* model, const doubl
     } else if (m == "ch4_1b") {
         dst = c->d_coef1b.p;
         fit = ch4_1b_fit(&need);
+    } else if (xfam_index(m) >= 0) {
+        const int f = xfam_index(m);
+        dst = c->d_xcoef[f].p;
+        fit = xfam_default_coef(f, &need);
     } else {
         return fail(MBXG_EINVAL, "mbxg_set_coefficients: unknown model '%s'", model);
     }
@@ -898,6 +958,87 @@ void drop_graph(mbxg_ctx* c) {
 }
 
 // One compute: lists, then the evaluation of the requested terms.
+// The further MB-nrg families and the 4B term, after the others (synchronous: each order's
+// family counts are read back to size its evaluate lists).  Tetramers are listed when the 4B term
+// is wanted; listed dimers / trimers are classified when the topology holds co2 or halides.
+int run_xfam(mbxg_ctx* c, const NmerCtx& N, bool want2, bool want3, bool want4, bool eval, int do_grads) {
+    hipStream_t st = c->stream;
+    ListState& L = c->L;
+    const int np = c->np;
+    c->n4_listed = 0;
+    for (int f = 0; f < kXFam; ++f) c->x_count[f] = 0;
+    launch_zero(c->d_xfam_tot.p, (size_t)kXFam * kPartials, st);
+    if (c->x_tot.reserve(3 * kXFam)) return fail(MBXG_ENOMEM, "further families: device allocation failed");
+    if (want4 && np >= 4) {
+        const int ns = np * (c->box.periodic ? 8 : 1);
+        if (c->t_counts.reserve(np) || c->t_off.reserve(np + 1) || c->t_ovf.reserve(1))
+            return fail(MBXG_ENOMEM, "tetramers: device allocation failed");
+        launch_zero_int(c->t_ovf.p, 1, st);
+        const double c4sq = c->cut4 * c->cut4;
+        HIPCHK(launch_tetra(L.pinfo.p, L.cx.p, L.cy.p, L.cz.p, np, ns, N, c4sq, 0, c->t_counts.p, c->t_off.p, nullptr,
+                            0, c->t_ovf.p, st));
+        int h[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(h, c->t_off.p + np, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(h + 1, c->t_ovf.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (h[1] > 0)
+            return fail(MBXG_EINVAL, "tetramers: a monomer has %d neighbours within the 4B cutoff (at most 512)", h[1]);
+        c->n4_listed = h[0];
+        if (c->t_list.reserve(4 * (size_t)h[0] + 4)) return fail(MBXG_ENOMEM, "tetramers: device allocation failed");
+        HIPCHK(launch_tetra(L.pinfo.p, L.cx.p, L.cy.p, L.cz.p, np, ns, N, c4sq, 1, c->t_counts.p, c->t_off.p,
+                            c->t_list.p, h[0], c->t_ovf.p, st));
+    }
+    if (eval) {
+        struct Ord {
+            int order;
+            const int* list;
+            int n;
+        };
+        const Ord ords[3] = {{2, L.list2.p, want2 && c->has_xkind ? L.tot.x : 0},
+                             {3, L.list3.p, want3 && c->has_xkind ? L.tot.z : 0},
+                             {4, c->t_list.p, want4 ? (int)c->n4_listed : 0}};
+        double* grad = do_grads ? c->d_grad.p : nullptr;
+        for (const Ord& o : ords) {
+            if (o.n <= 0) continue;
+            const size_t nb = xclass_blocks(o.n);
+            if (c->x_code.reserve(o.n) || c->x_blk.reserve(nb * kXFam) || c->x_off.reserve(nb * kXFam))
+                return fail(MBXG_ENOMEM, "further families: device allocation failed");
+            int* tot = c->x_tot.p + (o.order - 2) * kXFam;
+            HIPCHK(launch_xclass(o.order, o.list, o.n, N, c->fam_mask >> kFamilies, c->x_code.p, c->x_blk.p,
+                                 c->x_off.p, tot, st));
+            int cnt[kXFam];
+            HIPCHK(hipMemcpyAsync(cnt, tot, sizeof(cnt), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            XLists X;
+            for (int f = 0; f < kXFam; ++f) {
+                X.list[f] = nullptr;
+                X.cap[f] = 0;
+                if (xfam_order(f) != o.order || cnt[f] <= 0) continue;
+                c->x_count[f] = cnt[f];
+                EvalState& e = c->xf[f];
+                if (c->x_list[f].reserve((size_t)(o.order + 1) * cnt[f]) || e.e_nmer.reserve(cnt[f]) ||
+                    e.partials.reserve((size_t)kPartials * partials_records(cnt[f])) ||
+                    e.ws.reserve(xfam_workspace(f, cnt[f])))
+                    return fail(MBXG_ENOMEM, "further families: device allocation failed");
+                X.list[f] = c->x_list[f].p;
+                X.cap[f] = cnt[f];
+            }
+            HIPCHK(launch_xemit(o.order, o.list, o.n, c->x_code.p, c->x_off.p, X, st));
+            for (int f = 0; f < kXFam; ++f) {
+                if (!X.cap[f]) continue;
+                EvalState& e = c->xf[f];
+                HIPCHK(launch_xfam(f, X.list[f], tot + f, cnt[f], N, c->d_xcoef[f].p, grad, e.e_nmer.p, e.partials.p,
+                                   e.ws.p, st));
+                hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, e.partials.p, tot + f, cnt[f],
+                                   c->d_xfam_tot.p + (size_t)kPartials * f);
+                HIPCHK(hipGetLastError());
+            }
+        }
+    }
+    HIPCHK(launch_xcombine(c->d_xfam_tot.p, c->d_totals.p, st));
+    return MBXG_OK;
+}
+
 int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     hipStream_t st = c->stream;
     ListState& L = c->L;
@@ -915,12 +1056,15 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const bool want2 = (terms & MBXG_2B) && c->cut2 > 0.0;
     const bool want3 = (terms & MBXG_3B) && c->cut3 > 0.0;
+    const bool want4 = (terms & MBXG_4B) && c->cut4 > 0.0;
     const double c2sq = want2 ? c->cut2 * c->cut2 : 0.0;
     const double c3sq = want3 ? c->cut3 * c->cut3 : 0.0;
-    L.built = (want2 || want3) && c->np >= 2;
+    // the further families read the listed lists, the tetramer search the cloud
+    const bool xrun = (c->has_xkind && (want2 || want3)) || want4;
+    L.built = (want2 || want3 || want4) && c->np >= 2;
     if (L.built) {
-        if (int r = build_lists(c, N, c2sq, c3sq, std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), sync,
-                                (terms & MBXG_LISTS_ONLY) != 0, ev))
+        const double cmax = std::max(std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), want4 ? c->cut4 : 0.0);
+        if (int r = build_lists(c, N, c2sq, c3sq, cmax, sync, (terms & MBXG_LISTS_ONLY) != 0 || xrun, ev))
             return r;
     } else {
         L.tot = make_int4(0, 0, 0, 0);
@@ -1044,6 +1188,10 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     }
     // per-order totals: 2B = MB-pol + CH4-H2O + CH4-CH4, 3B = MB-pol + CH4-H2O-H2O (fixed order)
     hipLaunchKernelGGL(k_combine_families, dim3(1), dim3(64), 0, st, c->d_fam_tot.p, 1, c->d_totals.p);
+    launch_zero(c->d_totals.p + 3 * kPartials, kPartials, st);
+    c->x_last = xrun;
+    if (xrun)  // then the further families' totals on top, and the 4B record
+        if (int r = run_xfam(c, N, want2, want3, want4, !(terms & MBXG_LISTS_ONLY), do_grads)) return r;
     if (c->nconf > 1)
         hipLaunchKernelGGL(k_combine_families, dim3((c->nconf * kPartials + 63) / 64), dim3(64), 0, st,
                            c->d_conf_fam.p, c->nconf, c->d_conf_tot.p);
@@ -1099,22 +1247,25 @@ int finish(mbxg_ctx* c) {
 extern "C" int mbxg_compute(mbxg_ctx* c, int terms, int do_grads) {
     if (!c) return fail(MBXG_EINVAL, "mbxg_compute: null context");
     if (!c->have_coords) return fail(MBXG_ESTATE, "mbxg_compute: coordinates not set");
-    if (terms & ~(MBXG_1B | MBXG_2B | MBXG_3B | MBXG_LISTS_ONLY))
+    if (terms & ~(MBXG_1B | MBXG_2B | MBXG_3B | MBXG_LISTS_ONLY | MBXG_4B))
         return fail(MBXG_EINVAL, "mbxg_compute: unknown terms 0x%x", terms);
     if ((terms & MBXG_1B) && c->nconf > 1)
         return fail(MBXG_EINVAL, "mbxg_compute: the 1B term is not evaluated over batches");
     if (!(terms & MBXG_LISTS_ONLY))
         for (int i = 0; i < c->nmon; ++i)
-            if (c->kind[i] != MBXG_H2O && c->kind[i] != MBXG_CH4)
-                return fail(MBXG_EINVAL, "mbxg_compute: monomer %d: only h2o and ch4 have GPU holders (lists only)", i);
+            if (c->kind[i] == MBXG_OTHER)
+                return fail(MBXG_EINVAL, "mbxg_compute: monomer %d has no GPU holder (lists only)", i);
+    const bool xrun = (c->has_xkind && (terms & (MBXG_2B | MBXG_3B))) || ((terms & MBXG_4B) && c->cut4 > 0.0);
+    if (xrun && c->nconf > 1)
+        return fail(MBXG_EINVAL, "mbxg_compute: co2, halides and the 4B term are not evaluated over batches");
     HIPCHK(hipSetDevice(c->device));
     c->have_results = false;
     c->last_terms = terms;
     c->last_grads = do_grads;
     c->inputs_changed = false;
-    const bool steady = !c->need_sync && !c->timing;
+    const bool steady = !c->need_sync && !c->timing && !xrun;
     if (!(c->use_graph && steady && c->eager_terms == terms && c->eager_grads == do_grads)) {
-        const bool sync = c->need_sync;
+        const bool sync = c->need_sync || xrun;
         if (int r = run(c, terms, do_grads, sync)) return r;
         if (!c->timing) {
             c->eager_terms = terms;
@@ -1292,7 +1443,7 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
     if (!c) return fail(MBXG_EINVAL, "mbxg_get_results: null context");
     if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_results: nothing computed");
     if (int r = finish(c)) return r;
-    double tot[3 * kPartials];
+    double tot[4 * kPartials];
     HIPCHK(hipMemcpyAsync(tot, c->d_totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     if (grad && c->natoms > 0)
         HIPCHK(hipMemcpyAsync(grad, c->d_grad.p, sizeof(double) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream));
@@ -1304,7 +1455,7 @@ extern "C" int mbxg_get_results(mbxg_ctx* c, double* e2b, double* e3b, double* g
         const int slot[6] = {0, 1, 2, 4, 5, 8};
         for (int q = 0; q < 9; ++q) virial[q] = 0.0;
         for (int q = 0; q < 6; ++q)
-            virial[slot[q]] = tot[2 * kPartials + 1 + q] + tot[1 + q] + tot[kPartials + 1 + q];
+            virial[slot[q]] = tot[2 * kPartials + 1 + q] + tot[1 + q] + tot[kPartials + 1 + q] + tot[3 * kPartials + 1 + q];
         virial[3] = virial[1];
         virial[6] = virial[2];
         virial[7] = virial[5];
@@ -1466,5 +1617,49 @@ extern "C" int mbxg_fix_coordinates(mbxg_ctx* c, const double* xyz_in, double* x
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(xyz_out, c->d_wrap_out.p, sizeof(double) * n3, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_set_cutoff4b(mbxg_ctx* c, double c4) {
+    if (!c || !(c4 >= 0.0)) return fail(MBXG_EINVAL, "mbxg_set_cutoff4b: bad arguments");
+    if (c4 == c->cut4) return MBXG_OK;
+    c->cut4 = c4;
+    c->need_sync = true;
+    c->inputs_changed = true;
+    c->L.layout_ok = false;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_term_energies(mbxg_ctx* c, double* e) {
+    if (!c || !e) return fail(MBXG_EINVAL, "mbxg_get_term_energies: bad arguments");
+    if (!c->have_results) return fail(MBXG_ESTATE, "mbxg_get_term_energies: nothing computed");
+    if (int r = finish(c)) return r;
+    double tot[4 * kPartials];
+    HIPCHK(hipMemcpyAsync(tot, c->d_totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    e[0] = tot[2 * kPartials];
+    e[1] = tot[0];
+    e[2] = tot[kPartials];
+    e[3] = tot[3 * kPartials];
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_xcounts(mbxg_ctx* c, int64_t* counts) {
+    if (!c || !counts) return fail(MBXG_EINVAL, "mbxg_get_xcounts: bad arguments");
+    if (int r = finish(c)) return r;
+    for (int f = 0; f < kXFam; ++f) counts[f] = c->have_results && c->x_last ? c->x_count[f] : 0;
+    counts[kXFam] = c->have_results && c->x_last ? c->n4_listed : 0;
+    return MBXG_OK;
+}
+
+extern "C" int mbxg_get_tetramers(mbxg_ctx* c, int32_t* out, int64_t cap, int64_t* n) {
+    if (!c || !n || (cap > 0 && !out)) return fail(MBXG_EINVAL, "mbxg_get_tetramers: bad arguments");
+    if (int r = finish(c)) return r;
+    *n = c->have_results && c->x_last ? c->n4_listed : 0;
+    const int64_t k = std::min<int64_t>(*n, cap);
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(out, c->t_list.p, sizeof(int32_t) * 4 * k, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return MBXG_OK;
 }

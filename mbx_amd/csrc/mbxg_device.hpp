@@ -17,7 +17,16 @@
 namespace mbxg {
 
 // Monomer kinds known to the GPU path (the reference keys them by name, energy2b.cpp:243-254).
-enum MonKind : int { MON_H2O = 0, MON_CH4 = 1, MON_OTHER = 2 };
+enum MonKind : int {
+    MON_H2O = 0,
+    MON_CH4 = 1,
+    MON_OTHER = 2,  // a monomer no holder here takes (n-mers holding one are listed, not evaluated)
+    MON_CO2 = 3,
+    MON_F = 4,  // halide ions f-, cl-, br-, i-
+    MON_CL = 5,
+    MON_BR = 6,
+    MON_I = 7
+};
 
 // Box description passed by value to kernels.  frac[] holds the coefficients of
 // GetFractionalCoordinates (kdtree/kdtree_utils.h:165-194), computed on the host with libm.

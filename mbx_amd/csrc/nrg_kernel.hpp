@@ -73,4 +73,40 @@ hipError_t launch_poly_piece(const double* xs, const int* n_dev, int cap, int ld
     return hipGetLastError();
 }
 
+// A piece of a further family's polynomial (gen/xpoly_<name>_p<k>.hip): always value and
+// gradient.  Up to 48 variables the kernel runs two wavefronts per SIMD; above that (the water
+// 4B polynomial, 66 variables: x and g alone take 264 registers) one.
+template <int NV, nrg_eg_fn FN>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+    k_xpoly_piece_wide(const double* __restrict__ xs, const int* __restrict__ n_dev, int cap, int ld,
+                       uint64_t coef_addr, double* __restrict__ gs, double* __restrict__ ps) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int n = min(*n_dev, cap);
+    if (t >= n) return;
+    const mbxg_cptr C = (mbxg_cptr)coef_addr;
+    double x[NV], g[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        x[j] = xs[(size_t)j * ld + t];
+        g[j] = 0.0;
+    }
+    const double E = FN(C, x, g, 0.0);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) gs[(size_t)j * ld + t] = g[j];
+    ps[t] = E;
+}
+
+template <int NV, nrg_eg_fn FN>
+hipError_t launch_xpoly_piece(const double* xs, const int* n_dev, int cap, int ld, const double* coef, double* gs,
+                              double* ps, hipStream_t st) {
+    if (cap <= 0) return hipSuccess;
+    const int nb = (cap + 63) / 64;
+    const uint64_t ca = (uint64_t)(uintptr_t)coef;
+    if constexpr (NV > 48)
+        hipLaunchKernelGGL((k_xpoly_piece_wide<NV, FN>), dim3(nb), dim3(64), 0, st, xs, n_dev, cap, ld, ca, gs, ps);
+    else
+        hipLaunchKernelGGL((k_poly_piece_eg<NV, FN>), dim3(nb), dim3(64), 0, st, xs, n_dev, cap, ld, ca, gs, ps);
+    return hipGetLastError();
+}
+
 }  // namespace mbxg

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <sstream>
 
 #include "../../include/mbxg.h"
@@ -36,6 +37,11 @@ namespace {
 int kind_of(const std::string& id) {
     if (id == "h2o") return MBXG_H2O;
     if (id == "ch4") return MBXG_CH4;
+    if (id == "co2") return MBXG_CO2;
+    if (id == "f-") return MBXG_F;
+    if (id == "cl-") return MBXG_CL;
+    if (id == "br-") return MBXG_BR;
+    if (id == "i-") return MBXG_I;
     return MBXG_OTHER;
 }
 
@@ -139,10 +145,12 @@ System::System(const System& o)
       box_(o.box_),
       cutoff2b_(o.cutoff2b_),
       cutoff3b_(o.cutoff3b_),
+      cutoff4b_(o.cutoff4b_),
       max2b_(o.max2b_),
       max3b_(o.max3b_),
       ignore2b_(o.ignore2b_),
       ignore3b_(o.ignore3b_),
+      ignore4b_(o.ignore4b_),
       counts_(o.counts_),
       numat_(o.numat_),
       numsites_(o.numsites_) {}
@@ -186,10 +194,12 @@ System& System::operator=(System&& o) noexcept {
     box_ = std::move(o.box_);
     cutoff2b_ = o.cutoff2b_;
     cutoff3b_ = o.cutoff3b_;
+    cutoff4b_ = o.cutoff4b_;
     max2b_ = o.max2b_;
     max3b_ = o.max3b_;
     ignore2b_ = std::move(o.ignore2b_);
     ignore3b_ = std::move(o.ignore3b_);
+    ignore4b_ = std::move(o.ignore4b_);
     counts_ = std::move(o.counts_);
     numat_ = o.numat_;
     numsites_ = o.numsites_;
@@ -340,10 +350,12 @@ void System::SetUpFromJson(std::string json_text) {
     }
     cutoff2b_ = num("twobody_cutoff", cutoff2b_);
     cutoff3b_ = num("threebody_cutoff", cutoff3b_);
+    cutoff4b_ = num("fourbody_cutoff", cutoff4b_);  // system.cpp:1287-1295
     max2b_ = (size_t)num("max_n_eval_2b", (double)max2b_);
     max3b_ = (size_t)num("max_n_eval_3b", (double)max3b_);
     if (mbx && mbx->has("ignore_2b_poly")) ignore2b_ = read_ignore(mbx->at("ignore_2b_poly"));
     if (mbx && mbx->has("ignore_3b_poly")) ignore3b_ = read_ignore(mbx->at("ignore_3b_poly"));
+    if (mbx && mbx->has("ignore_4b_poly")) ignore4b_ = read_ignore(mbx->at("ignore_4b_poly"));
     if (!initialized_ && !in_mon_.empty()) Initialize();
     SetPBC(box);
 }
@@ -383,13 +395,26 @@ double System::run_terms(int terms, bool do_grads, bool use_ghost) {
     if (!ignored(ignore2b_, {"ch4", "ch4"})) fam |= MBXG_FAM_CH4CH4_2B;
     if (!ignored(ignore3b_, {"h2o", "h2o", "h2o"})) fam |= MBXG_FAM_MBPOL_3B;
     if (!ignored(ignore3b_, {"ch4", "h2o", "h2o"})) fam |= MBXG_FAM_CH4H2O2_3B;
+    if (!ignored(ignore2b_, {"f-", "h2o"})) fam |= MBXG_FAM_F_H2O_2B;
+    if (!ignored(ignore2b_, {"cl-", "h2o"})) fam |= MBXG_FAM_CL_H2O_2B;
+    if (!ignored(ignore2b_, {"br-", "h2o"})) fam |= MBXG_FAM_BR_H2O_2B;
+    if (!ignored(ignore2b_, {"i-", "h2o"})) fam |= MBXG_FAM_I_H2O_2B;
+    if (!ignored(ignore2b_, {"co2", "co2"})) fam |= MBXG_FAM_CO2CO2_2B;
+    if (!ignored(ignore3b_, {"f-", "h2o", "h2o"})) fam |= MBXG_FAM_F_H2O2_3B;
+    if (!ignored(ignore3b_, {"cl-", "h2o", "h2o"})) fam |= MBXG_FAM_CL_H2O2_3B;
+    if (!ignored(ignore3b_, {"br-", "h2o", "h2o"})) fam |= MBXG_FAM_BR_H2O2_3B;
+    if (!ignored(ignore3b_, {"i-", "h2o", "h2o"})) fam |= MBXG_FAM_I_H2O2_3B;
+    if (!ignored(ignore3b_, {"co2", "co2", "co2"})) fam |= MBXG_FAM_CO2X3_3B;
+    if (!ignored(ignore4b_, {"h2o", "h2o", "h2o", "h2o"})) fam |= MBXG_FAM_H2OX4_4B;  // system.cpp:3143-3155
     check(mbxg_set_families(c, fam));
     check(mbxg_set_cutoffs(c, cutoff2b_, cutoff3b_));
+    // Get4B runs only with a cutoff above machine epsilon (system.cpp:2084, 2762)
+    check(mbxg_set_cutoff4b(c, cutoff4b_ > std::numeric_limits<double>::epsilon() ? cutoff4b_ : 0.0));
     check(mbxg_set_coords(c, xyz_.data()));
-    double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
     check(mbxg_compute(c, terms, do_grads ? 1 : 0));
     check(mbxg_get_results(c, nullptr, nullptr, do_grads ? grad_.data() : nullptr, virial_.data()));
-    check(mbxg_get_energies(c, &e1, &e2, &e3));
+    check(mbxg_get_term_energies(c, e));
     if (terms & (MBXG_2B | MBXG_3B)) {
         int64_t n[4];
         check(mbxg_get_counts(c, n));
@@ -398,14 +423,15 @@ double System::run_terms(int terms, bool do_grads, bool use_ghost) {
     // SetPBC wraps the working coordinates each call (system.cpp:2015 -> sys_tools.cpp:257-320)
     check(mbxg_get_coords(c, xyz_.data()));
     if (!do_grads) std::fill(virial_.begin(), virial_.end(), 0.0);
-    return e1 + e2 + e3;
+    return e[0] + e[1] + e[2] + e[3];
 }
 
-// System::Energy (system.cpp:1995-2138) for the terms this engine computes: Get1B + Get2B + Get3B.
+// System::Energy (system.cpp:1995-2138) for the terms this engine computes: Get1B + Get2B + Get3B
+// (+ Get4B with a 4B cutoff, :2083-2086).
 double System::Energy(bool do_grads) {
     if (!initialized_) SetUpFromJson(std::string("{}"));
     check_initialized(__func__, __LINE__);
-    return run_terms(MBXG_1B | MBXG_2B | MBXG_3B, do_grads, false);
+    return run_terms(MBXG_1B | MBXG_2B | MBXG_3B | MBXG_4B, do_grads, false);
 }
 
 // System::OneBodyEnergy (system.cpp:2140-2158): the 1B energy of the local monomers (Get1B,
@@ -423,6 +449,14 @@ double System::TwoBodyEnergy(bool do_grads, bool use_ghost) {
 double System::ThreeBodyEnergy(bool do_grads, bool use_ghost) {
     check_initialized(__func__, __LINE__);
     return run_terms(MBXG_3B, do_grads, use_ghost);
+}
+
+// system.cpp:2747-2767: Get4B when the cutoff exceeds machine epsilon, 0 otherwise.
+double System::FourBodyEnergy(bool do_grads, bool use_ghost) {
+    if (!initialized_)
+        throw CUException(__func__, __FILE__, __LINE__,
+                          "System has not been initialized. 4B Energy calculation not possible.");
+    return run_terms(MBXG_4B, do_grads, use_ghost);
 }
 
 std::vector<double> System::GetRealGrads() {
@@ -522,6 +556,22 @@ void System::Set2bIgnorePoly(std::vector<std::vector<std::string>> v) {
 void System::Set3bIgnorePoly(std::vector<std::vector<std::string>> v) {
     for (auto& p : v) std::sort(p.begin(), p.end());
     ignore3b_ = v;
+}
+
+void System::Set4bCutoff(double c) { cutoff4b_ = c; }
+double System::Get4bCutoff() { return cutoff4b_; }
+std::vector<std::vector<std::string>> System::Get4bIgnorePoly() { return ignore4b_; }
+
+void System::Set4bIgnorePoly(std::vector<std::vector<std::string>> v) {
+    for (auto& p : v) std::sort(p.begin(), p.end());
+    ignore4b_ = v;
+}
+
+// system.cpp:981-988: added once, names sorted
+void System::Add4bIgnorePoly(std::string mon1, std::string mon2, std::string mon3, std::string mon4) {
+    std::vector<std::string> p = {mon1, mon2, mon3, mon4};
+    std::sort(p.begin(), p.end());
+    if (std::find(ignore4b_.begin(), ignore4b_.end(), p) == ignore4b_.end()) ignore4b_.push_back(p);
 }
 
 std::vector<size_t> System::GetPairList(size_t nmax, double cutoff, size_t istart, size_t iend, bool use_ghost) {

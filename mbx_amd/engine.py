@@ -12,10 +12,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmbxg.so")
 
-H2O, CH4, OTHER = 0, 1, 2
-TERM_2B, TERM_3B, LISTS_ONLY, TERM_1B = 1, 2, 4, 8
-KIND_OF = {"h2o": H2O, "ch4": CH4}
-NAT_OF = {H2O: 3, CH4: 5}
+H2O, CH4, OTHER, CO2, F, CL, BR, I = 0, 1, 2, 3, 4, 5, 6, 7
+TERM_2B, TERM_3B, LISTS_ONLY, TERM_1B, TERM_4B = 1, 2, 4, 8, 16
+KIND_OF = {"h2o": H2O, "ch4": CH4, "co2": CO2, "f-": F, "cl-": CL, "br-": BR, "i-": I}
+NAT_OF = {H2O: 3, CH4: 5, CO2: 3, F: 1, CL: 1, BR: 1, I: 1}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -58,6 +58,10 @@ PROTOTYPES = {
     "mbxg_get_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip]),
     "mbxg_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mbxg_get_batch_results": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "mbxg_set_cutoff4b": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mbxg_get_term_energies": (ctypes.c_int, [_vp, _dp]),
+    "mbxg_get_xcounts": (ctypes.c_int, [_vp, _i64p]),
+    "mbxg_get_tetramers": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i64p]),
     "mbxg_ad_shape": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
     "mbxg_ad_defaults": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
     "mbxg_ad_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
@@ -67,7 +71,11 @@ PROTOTYPES = {
                                         ctypes.c_size_t, ctypes.c_int, _vp]),
 }
 
-FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b")
+FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b",
+            # further MB-nrg families (bits 5.. of mbxg_set_families)
+            "f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "co2co2_2b", "f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b",
+            "i_h2o2_3b", "co2x3_3b", "h2ox4_4b")
+XFAMILIES = FAMILIES[5:]
 TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
                  "chain_3b", "total", "ch4h2o2_3b")
 
@@ -136,6 +144,32 @@ class Engine:
     def set_families(self, families=FAMILIES):
         """Evaluate only these n-mer families (names as FAMILIES); the others stay listed."""
         _check(lib().mbxg_set_families(self._c, sum(1 << i for i, f in enumerate(FAMILIES) if f in families)))
+
+    def set_cutoff4b(self, c4):
+        """The 4B tetramer cutoff (0 disables the 4B term)."""
+        _check(lib().mbxg_set_cutoff4b(self._c, float(c4)))
+
+    def term_energies(self):
+        """(E1B, E2B, E3B, E4B) of the last compute."""
+        e = np.zeros(4)
+        _check(lib().mbxg_get_term_energies(self._c, e.ctypes.data_as(_dp)))
+        return e
+
+    def xcounts(self):
+        """{further family: evaluated n-mers}, and the listed tetramers under "tetramers"."""
+        c = np.zeros(len(XFAMILIES) + 1, np.int64)
+        _check(lib().mbxg_get_xcounts(self._c, c.ctypes.data_as(_i64p)))
+        out = {f: int(c[i]) for i, f in enumerate(XFAMILIES)}
+        out["tetramers"] = int(c[-1])
+        return out
+
+    def tetramers(self):
+        """Listed tetramers of the last compute, (n, 4) internal monomer indices."""
+        n = ctypes.c_int64()
+        _check(lib().mbxg_get_tetramers(self._c, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 4), np.int32)
+        _check(lib().mbxg_get_tetramers(self._c, out.ctypes.data_as(_i32p), n.value, ctypes.byref(n)))
+        return out[:n.value]
 
     def set_use_ghost(self, on=True):
         """Neighbour search over local and ghost monomers, n-mers found from local ones only

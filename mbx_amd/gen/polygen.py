@@ -811,6 +811,133 @@ def generate_1b(name, out_dir):
     return len(monos)
 
 
+# ------------------------------------------------------------------------------------------
+# Further MB-nrg families (halide-water 2B / 3B, CO2 2B / 3B, water 4B): holders as data
+# (gen/xholder_<holder>.inc), each parameter set's coefficients (gen/xcoef_<holder>.inc) and the
+# polynomial structures in pieces (gen/xpoly_<structure>_p<k>.{inc,hip}, one kernel each).
+XHOLDERS = ("f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b",
+            "co2co2_2b", "co2x3_3b", "h2ox4_4b")
+XPOLY_PIECES = {"halide_h2o_2b": 1, "co2co2_2b": 2, "halide_h2o2_3b": 4, "co2x3_3b": 4, "h2ox4_4b": 8}
+XKIND = {"h2o": "MON_H2O", "ch4": "MON_CH4", "co2": "MON_CO2", "f-": "MON_F", "cl-": "MON_CL", "br-": "MON_BR",
+         "i-": "MON_I"}
+
+
+def emit_trie_piece(fn, nvars, monos, fence_every=48):
+    """emit_trie for monomials of any degree as a piece function (mbxg_cptr coefficients, value
+    returned on top of E0), with a region fence every `fence_every` statements so coefficient
+    loads stay near their FMAs."""
+    body = emit_trie(fn + "_trie", nvars, monos).replace("const double* __restrict__ C", "mbxg_cptr C")
+    lines = body.split("\n")
+    out, n, last = [], 0, "x[0]"
+    for ln in lines:
+        out.append(ln)
+        m = re.match(r"\s+(?:const )?double (\w+) = ", ln) or re.match(r"\s+(\w+) = __fma_rn", ln)
+        if m:
+            last = m.group(1)
+            n += 1
+            if n % fence_every == 0:
+                out.append("    MBXG_NODE_FENCE(C, %s);" % last)
+    src = "\n".join(out)
+    src += ("__device__ __forceinline__ double %s(mbxg_cptr C, const double* x, double* g, double E0) {\n"
+            "    return E0 + %s_trie(C, x, g);\n}\n" % (fn, fn))
+    return src
+
+
+def generate_xpoly(name, out_dir):
+    nvars, coef, monos = load_table(name)
+    q = XPOLY_PIECES[name]
+    deg = max(len(m) for _, m in monos)
+    hdr = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.tbl -- do not edit." % name,
+           "#pragma once", "#include \"poly_support.hpp\"", ""]
+    if deg > 4:
+        assert q == 1 or name == "co2co2_2b"
+        # degree 5-6: reverse mode over the full monomial trie, split by the first variable
+        firsts = sorted({m[0] for _, m in monos})
+        load = [0] * q
+        groups = [[] for _ in range(q)]
+        for v in sorted(firsts, key=lambda v: -sum(1 for _, m in monos if m[0] == v)):
+            k = min(range(q), key=lambda j: load[j])
+            groups[k].append(v)
+            load[k] += sum(1 for _, m in monos if m[0] == v)
+        parts = [[(k, m) for k, m in monos if m[0] in set(g)] for g in groups]
+        for k in range(q):
+            src = "\n".join(hdr) + "// Piece %d of %d: the monomials whose first variable is one of %s (%d of %d).\n" % (
+                k, q, groups[k], len(parts[k]), len(monos)) + emit_trie_piece("xpoly_%s_p%d" % (name, k), nvars,
+                                                                            parts[k])
+            write_if_changed(os.path.join(out_dir, "xpoly_%s_p%d.inc" % (name, k)), src)
+    else:
+        s = Schedule(nvars, [(k, m) for k, m in monos])
+        assign = s.partition(q)
+        c = s.counts()
+        for k in range(q):
+            src = "\n".join(hdr) + "// Schedule: %s\n// Piece %d of %d (op-count balance: %s).\n" % (
+                ", ".join("%s=%d" % kv for kv in c.items()), k, q, ", ".join(str(v) for v in s.part_load)) + \
+                s.emit("xpoly_%s_p%d" % (name, k), grad=True, part=k, assign=assign)
+            write_if_changed(os.path.join(out_dir, "xpoly_%s_p%d.inc" % (name, k)), src)
+    for k in range(q):
+        tu = ["// GENERATED by mbx_amd/gen/polygen.py -- do not edit.",
+              "// Piece %d of %d of the %s polynomial (%d variables, degree %d): its own kernel." % (
+                  k, q, name, nvars, deg),
+              "#include \"gen/xpoly_%s_p%d.inc\"" % (name, k), "#include \"nrg_kernel.hpp\"", "",
+              "namespace mbxg {", "",
+              "hipError_t launch_xpoly_%s_p%d(const double* xs, const int* n_dev, int cap, int ld, const double* coef, "
+              "double* gs, double* ps, hipStream_t st) {" % (name, k),
+              "    return launch_xpoly_piece<%d, xpoly_%s_p%d>(xs, n_dev, cap, ld, coef, gs, ps, st);" % (nvars, name, k),
+              "}", "", "}  // namespace mbxg", ""]
+        write_if_changed(os.path.join(out_dir, "xpoly_%s_p%d.hip" % (name, k)), "\n".join(tu))
+    return dict(pieces=q, degree=deg, monomials=len(monos))
+
+
+def generate_xholder(name, out_dir):
+    """gen/xholder_<name>.inc and gen/xcoef_<name>.inc from mbx_amd/tables/<name>.holder.json."""
+    import json
+
+    h = json.load(open(os.path.join(TABLES, name + ".holder.json")))
+    forms = {"exp0": 0, "coul0": 1, "exp": 2, "coul": 3}
+    nm = len(h["monomers"])
+    vs = h["vsites"]
+    sw = h["switch"]
+    pairs = sw["pairs"]
+    pad = lambda v, n, z: list(v) + [z] * (n - len(v))
+    out = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.holder.json -- do not edit." % name,
+           "#pragma once", "#include \"mbxg_device.hpp\"", "#include \"nrg_kernel.hpp\"", "",
+           "namespace mbxg {", "",
+           "struct XHolder_%s {" % name,
+           "    static constexpr int kMon = %d;" % nm,
+           "    static constexpr int kNat[4] = {%s};" % ", ".join(str(v) for v in pad(h["atoms"], 4, 0)),
+           "    static constexpr int kKind[4] = {%s};" % ", ".join(pad([XKIND[m] for m in h["monomers"]], 4, "-1")),
+           "    static constexpr int kNReal = %d;" % h["nreal"],
+           "    static constexpr int kNSites = %d;" % h["nsites"],
+           "    static constexpr int kNVsite = %d;" % len(vs),
+           "    static constexpr int kVO[2] = {%s};" % ", ".join(str(v) for v in pad([w["oxygen_site"] for w in vs], 2, 0)),
+           "    static constexpr double kInPlane = %.17g;" % (vs[0]["in_plane"] if vs else 0.0),
+           "    static constexpr double kOutOfPlane = %.17g;" % (vs[0]["out_of_plane"] if vs else 0.0),
+           "    static constexpr double kRIn = %.17g;" % sw["inner"],
+           "    static constexpr double kROut = %.17g;" % sw["outer"],
+           "    static constexpr int kNPair = %d;" % len(pairs),
+           "    static constexpr int kPA[6] = {%s};" % ", ".join(str(v) for v in pad([a for a, b in pairs], 6, 0)),
+           "    static constexpr int kPB[6] = {%s};" % ", ".join(str(v) for v in pad([b for a, b in pairs], 6, 0)),
+           "    static constexpr int kNTerm = %d;" % len(sw["terms"]),
+           "    static constexpr double kTCoef[%d] = {%s};" % (len(sw["terms"]), ", ".join("%.17g" % t[0] for t in sw["terms"])),
+           "    static constexpr int kTMask[%d] = {%s};" % (len(sw["terms"]), ", ".join(
+               str(sum(1 << q for q in t[1])) for t in sw["terms"])),
+           "    static constexpr int kNVars = %d;" % len(h["vars"]),
+           "    static constexpr int kPieces = %d;" % XPOLY_PIECES[h["poly"]],
+           "    static constexpr NrgVar kVars[%d] = {" % len(h["vars"])]
+    for f, k, d, s1, s2 in h["vars"]:
+        out.append("        {%d, %.17g, %.17g, %d, %d}," % (forms[f], k, d, s1, s2))
+    out += ["    };", "};", "", "}  // namespace mbxg", ""]
+    write_if_changed(os.path.join(out_dir, "xholder_%s.inc" % name), "\n".join(out))
+    cs = ["// GENERATED by mbx_amd/gen/polygen.py from mbx_amd/tables/%s.holder.json -- do not edit." % name,
+          "#pragma once", "static const int kNCoefX_%s = %d;" % (name, len(h["coef"])),
+          "static const double kFitX_%s[%d] = {" % (name, len(h["coef"]))]
+    for i in range(0, len(h["coef"]), 4):
+        cs.append("    " + ", ".join("%.17g" % v for v in h["coef"][i:i + 4]) + ",")
+    cs.append("};")
+    write_if_changed(os.path.join(out_dir, "xcoef_%s.inc" % name), "\n".join(cs) + "\n")
+    return h["poly"]
+
+
 def main():
     out_dir = os.path.join(ROOT, "mbx_amd", "csrc", "gen")
     os.makedirs(out_dir, exist_ok=True)
@@ -828,6 +955,10 @@ def main():
     print(generate_ps(out_dir))
     for name in ONE_BODY:
         print(name, generate_1b(name, out_dir))
+    for name in XHOLDERS:
+        generate_xholder(name, out_dir)
+    for name in XPOLY_PIECES:
+        print(name, generate_xpoly(name, out_dir))
 
 
 if __name__ == "__main__":

@@ -781,14 +781,18 @@ def holder_order(names):
     return hn, perm
 
 
-def xsystem(xyz, names, box, cutoff2b, cutoff3b, cutoff4b=0.0, tags=None, ignore=(), nmers=False):
+def xsystem(xyz, names, box, cutoff2b, cutoff3b, cutoff4b=0.0, tags=None, ignore=(), nmers=False, cond=False):
     """2B + 3B (+ 4B) energy, gradient and virial of a system of any of the monomers above
     (names in input order, 3 doubles per atom), as System::Energy's n-body part does it:
     FixMonomerCoordinates under PBC, AddClusters lists, each n-mer imaged next to its first
     listed monomer (GetCloseNmerImage, sys_tools.cpp:453-468) and passed to the holder of its
     names in the holder's argument order.  ignore: sorted name tuples whose polynomials are
     skipped (the System's ignore lists).  Returns dict(e2b, e3b, e4b, grad, virial, counts
-    {order: (listed, evaluated)}, [per-n-mer (order, ids, E)])."""
+    {order: (listed, evaluated)}, [per-n-mer (order, ids, E)], [cond]).  cond: per order, the sum
+    over n-mers of the magnitude of the polynomial's cancelling terms (sum_k |a_k| M_k sw, the
+    further families' holders evaluated with |a|; |E| for the MB-pol and CH4 holders), the scale
+    of the rounding error any evaluation order leaves; cond_mon: per monomer, the same sum over
+    the n-mers that hold it."""
     names = list(names)
     nat = np.array([NAT_OF_NAME[n] for n in names], np.int32)
     first = np.concatenate([[0], np.cumsum(nat)[:-1]]).astype(np.int64)
@@ -802,6 +806,8 @@ def xsystem(xyz, names, box, cutoff2b, cutoff3b, cutoff4b=0.0, tags=None, ignore
     vir = np.zeros(9)
     energies = {2: 0.0, 3: 0.0, 4: 0.0}
     counts, per = {}, []
+    conds = {2: 0.0, 3: 0.0, 4: 0.0}
+    cond_mon = np.zeros(len(names))
     ign = {tuple(sorted(t)) for t in ignore}
     for order, cut in ((2, cutoff2b), (3, cutoff3b), (4, cutoff4b)):
         if cut <= 0.0:
@@ -832,6 +838,13 @@ def xsystem(xyz, names, box, cutoff2b, cutoff3b, cutoff4b=0.0, tags=None, ignore
                 e, gs, v, ev = nrg(hn, ws)
             else:
                 e, gs, v, ev = xnmer(hn, ws)
+            if cond:
+                if hn in NRG or hn.startswith("mbpol"):
+                    cn = abs(e)
+                else:
+                    cn = abs(xnmer(hn, ws, a=np.abs(xholder(hn).coef))[0])
+                conds[order] += cn
+                cond_mon[ids] += cn
             nev += ev
             energies[order] += e
             vir += v
@@ -844,6 +857,9 @@ def xsystem(xyz, names, box, cutoff2b, cutoff3b, cutoff4b=0.0, tags=None, ignore
     out = dict(e2b=energies[2], e3b=energies[3], e4b=energies[4], grad=grad, virial=vir, counts=counts, xyz=x)
     if nmers:
         out["nmers"] = per
+    if cond:
+        out["cond"] = conds
+        out["cond_mon"] = cond_mon
     return out
 
 

@@ -68,9 +68,9 @@ def test_write_nrg_roundtrip(tmp_path):
     out = str(tmp_path / "out.nrg")
     print("write_nrg roundtrip", flush=True)
     r = nrg_files.run(src, "--write", out)
-    # the file is written right after reading; the energy of system 0 then fails loudly on its CO2
+    # the file is written right after reading; the energy of system 0 then fails loudly on its dp1
     # monomers (no holder for them here)
-    assert r.returncode == 1 and "only h2o and ch4 have GPU holders" in r.stderr, r.stderr
+    assert r.returncode == 1 and "has no GPU holder" in r.stderr, r.stderr
     a, b = nrg_files.dump(src), nrg_files.dump(out)
     assert len(a) == len(b) == 3
     for sa, sb in zip(a, b):
