@@ -92,9 +92,11 @@ int mbxg_set_use_ghost(mbxg_ctx* ctx, int on);
  * System::OneBodyEnergy / Get1B (bblock/system.cpp:2140-2158, 2269-2401) and e1b::get_1b_energy
  * (potential/1b/energy1b.cpp:118-147): water by the MB-pol Partridge-Schwenke potential
  * ps::pot_nasa (potential/1b/ps.cpp:346), CH4 by the MB-nrg x1b_A1B4_v1x holder
- * (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371).  e1b receives the total (kcal/mol); with do_grads,
- * grad (3 doubles per atom, nullable) is overwritten with the 1B gradient and virial (9, nullable)
- * with the 1B virial.  Synchronous.  Monomers other than h2o and ch4: MBXG_EINVAL. */
+ * (potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp:371), CO2 by x1b_A1B2_v1x
+ * (potential/1b/x1b_A1B2_deg4_v1x.cpp:160); halide ions have no 1B term (get_1b_energy returns 0
+ * for them, energy1b.cpp:175-177).  e1b receives the total (kcal/mol); with do_grads, grad (3
+ * doubles per atom, nullable) is overwritten with the 1B gradient and virial (9, nullable) with
+ * the 1B virial.  Synchronous.  Monomers without a holder (MBXG_OTHER): MBXG_EINVAL. */
 int mbxg_eval_1b(mbxg_ctx* ctx, int do_grads, double* e1b, double* grad, double* virial);
 
 /* Energies of the last compute, per term (each pointer nullable; a term not computed reads 0).
@@ -135,7 +137,8 @@ int mbxg_fix_coordinates(mbxg_ctx* ctx, const double* xyz_in, double* xyz_out);
 int mbxg_set_batch(mbxg_ctx* ctx, int nconf, const int* mon_offsets);
 
 /* Replace the linear coefficients of a polynomial ("mbpol_2b": 1153, "mbpol_3b": 1163,
- * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207, "f_h2o_2b" / "cl_h2o_2b" /
+ * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207, "co2_1b": 21,
+ * "f_h2o_2b" / "cl_h2o_2b" /
  * "br_h2o_2b" / "i_h2o_2b": 906, "co2co2_2b": 2269, "f_h2o2_3b" / "cl_h2o2_3b" / "br_h2o2_3b" /
  * "i_h2o2_3b": 3173, "co2x3_3b": 1067, "h2ox4_4b": 398).  a == NULL restores the fitted values. */
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);

@@ -172,9 +172,9 @@ struct mbxg_ctx {
     int fam_mask = (1 << (kFamilies + kXFam)) - 1;  // evaluated families (mbxg_set_families)
     int use_ghost = 0;                    // the cloud holds ghost monomers too (mbxg_set_use_ghost)
     // 1B term (MBXG_1B): the local waters, then the local CH4s; per monomer energy and virial rows
-    std::vector<int> water_local, ch4_local;
+    std::vector<int> water_local, ch4_local, co2_local;
     DBuf<int> d_w1b;
-    DBuf<double> d_e1b, d_v1b, d_coef1b;
+    DBuf<double> d_e1b, d_v1b, d_coef1b, d_coef1b_co2;
     bool one_body_last = false;  // the last compute evaluated the 1B term
     DBuf<double> d_wrap_in, d_wrap_out;  // mbxg_fix_coordinates
     int n_h2o = 0, n_ch4 = 0;             // monomers of each kind (mbxg_set_topology)
@@ -222,8 +222,9 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
         delete c;
         return fail(MBXG_EHIP, "mbxg_create: stream/event creation failed: %s", hipGetErrorString(e));
     }
-    int n1b = 0;
+    int n1b = 0, n1b_co2 = 0;
     const double* fit1b = ch4_1b_fit(&n1b);
+    const double* fit1b_co2 = co2_1b_fit(&n1b_co2);
     // totals: 2B record, 3B record, 1B record, 4B record (kPartials each)
     bool xbad = false;
     for (int f = 0; f < kXFam; ++f) {
@@ -233,6 +234,7 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
     }
     if (xbad || c->d_xfam_tot.reserve(kXFam * kPartials) || c->d_coef2.reserve(kNCoef_mbpol_2b) ||
         c->d_coef3.reserve(kNCoef_mbpol_3b) || c->d_totals.reserve(4 * kPartials) || c->d_coef1b.reserve(n1b) ||
+        c->d_coef1b_co2.reserve(n1b_co2) ||
         c->d_box.reserve(1) || c->d_fam_tot.reserve(kFamilies * kPartials) ||
         c->d_coefn[FAM_CH4H2O_2B].reserve(kNCoef_ch4h2o_2b) || c->d_coefn[FAM_CH4CH4_2B].reserve(kNCoef_ch4ch4_2b) ||
         c->d_coefn[FAM_CH4H2O2_3B].reserve(kNCoef_ch4h2o2_3b)) {
@@ -241,6 +243,7 @@ extern "C" int mbxg_create(mbxg_ctx** out, int device) {
     }
     HIPCHK(hipMemcpy(c->d_coef2.p, kFit_mbpol_2b, sizeof(double) * kNCoef_mbpol_2b, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_coef1b.p, fit1b, sizeof(double) * n1b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_coef1b_co2.p, fit1b_co2, sizeof(double) * n1b_co2, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->d_totals.p, 0, sizeof(double) * 4 * kPartials));
     for (int f = 0; f < kXFam; ++f) {
         int nx = 0;
@@ -331,6 +334,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     c->d_e1b.release();
     c->d_v1b.release();
     c->d_coef1b.release();
+    c->d_coef1b_co2.release();
     c->d_wrap_in.release();
     c->d_wrap_out.release();
     c->d_coef2.release();
@@ -449,12 +453,15 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     if (int r = set_configs(c, 1, nullptr)) return r;
     c->water_local.clear();
     c->ch4_local.clear();
+    c->co2_local.clear();
     for (int i = 0; i < nmon; ++i) {
         if (c->kind[i] == MBXG_H2O && c->islocal[i]) c->water_local.push_back(i);
         if (c->kind[i] == MBXG_CH4 && c->islocal[i]) c->ch4_local.push_back(i);
+        if (c->kind[i] == MBXG_CO2 && c->islocal[i]) c->co2_local.push_back(i);
     }
     std::vector<int> m1b(c->water_local);
     m1b.insert(m1b.end(), c->ch4_local.begin(), c->ch4_local.end());
+    m1b.insert(m1b.end(), c->co2_local.begin(), c->co2_local.end());
     if (c->d_w1b.reserve(nm) || c->d_e1b.reserve(nm) || c->d_v1b.reserve(6 * nm))
         return fail(MBXG_ENOMEM, "mbxg_set_topology: device allocation failed");
     if (!m1b.empty())
@@ -612,6 +619,9 @@ extern "C" int mbxg_set_coefficients(mbxg_ctx* c, const char* model, const doubl
     } else if (m == "ch4_1b") {
         dst = c->d_coef1b.p;
         fit = ch4_1b_fit(&need);
+    } else if (m == "co2_1b") {
+        dst = c->d_coef1b_co2.p;
+        fit = co2_1b_fit(&need);
     } else if (xfam_index(m) >= 0) {
         const int f = xfam_index(m);
         dst = c->d_xcoef[f].p;
@@ -1077,12 +1087,15 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     // atoms of the zeroed gradient before any 2B/3B kernel adds to it (they start after the fork)
     c->one_body_last = (terms & MBXG_1B) && !(terms & MBXG_LISTS_ONLY);
     if (c->one_body_last) {
-        const int nw = (int)c->water_local.size(), nc = (int)c->ch4_local.size(), n1 = nw + nc;
+        const int nw = (int)c->water_local.size(), nc = (int)c->ch4_local.size();
+        const int no = (int)c->co2_local.size(), n1 = nw + nc + no;
         double* g = do_grads ? c->d_grad.p : nullptr;
         double* v = do_grads ? c->d_v1b.p : nullptr;
         HIPCHK(launch_1b_ps(c->d_xyz.p, c->d_first.p, c->d_w1b.p, nw, c->d_e1b.p, g, v, n1, st));
         HIPCHK(launch_1b_ch4(c->d_xyz.p, c->d_first.p, c->d_w1b.p + nw, nc, c->d_coef1b.p, c->d_e1b.p + nw, g,
                              v ? v + nw : nullptr, n1, st));
+        HIPCHK(launch_1b_co2(c->d_xyz.p, c->d_first.p, c->d_w1b.p + nw + nc, no, c->d_coef1b_co2.p,
+                             c->d_e1b.p + nw + nc, g, v ? v + nw + nc : nullptr, n1, st));
         hipLaunchKernelGGL(k_reduce_1b, dim3(1), dim3(256), 0, st, c->d_e1b.p, v, n1, n1, c->d_totals.p + 2 * kPartials);
     } else {
         launch_zero(c->d_totals.p + 2 * kPartials, kPartials, st);

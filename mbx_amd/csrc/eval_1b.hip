@@ -12,11 +12,19 @@
 // degree-5 polynomial poly_1b_A1B4_deg5_exp0_v1x (gen/poly_ch4_1b.inc, value and gradient in one
 // reverse-mode pass), the chain rule per variable in the reference's order (variable::grads,
 // :56-63) and the virial -sum_atoms x_a g_b (:442-462).
+//
+// MB-nrg CO2: x1b_A1B2_v1x::eval (potential/1b/x1b_A1B2_deg4_v1x.cpp:160-215): three variables
+// exp(-k (r - d)) over C-O1, C-O2 and O1-O2 (v_intra, :41-47; gen/holder_co2_1b.inc), the degree-4
+// polynomial poly_1b_A1B2_deg4_v1x (gen/poly_co2_1b.inc), the chain rule g_intra (:50-61) and the
+// virial -sum_atoms x_a g_b (:188-203), in the same kernel as CH4.
 #include <hip/hip_runtime.h>
 
 #include "gen/coef_ch4_1b.inc"
+#include "gen/coef_co2_1b.inc"
 #include "gen/holder_ch4_1b.inc"
+#include "gen/holder_co2_1b.inc"
 #include "gen/poly_ch4_1b.inc"
+#include "gen/poly_co2_1b.inc"
 #include "gen/ps_1b.inc"
 #include "mbxg_kernels.hpp"
 
@@ -109,8 +117,10 @@ __global__ void __launch_bounds__(64) k_1b_ps(const double* __restrict__ xyz, co
     }
 }
 
-// The CH4 holder: the same outputs for monomers mons[0..n) of H's kind.
-template <class H>
+// An MB-nrg 1B holder H with polynomial POLY: the same outputs for monomers mons[0..n) of H's kind.
+typedef double (*poly1b_fn)(const double* __restrict__, const double*, double*);
+
+template <class H, poly1b_fn POLY>
 __global__ void __launch_bounds__(64) k_1b_nrg(const double* __restrict__ xyz, const int* __restrict__ first_atom,
                                                const int* __restrict__ mons, int n, const double* __restrict__ coef,
                                                double* __restrict__ e_out, double* __restrict__ g_out,
@@ -139,7 +149,7 @@ __global__ void __launch_bounds__(64) k_1b_nrg(const double* __restrict__ xyz, c
         for (int i = 0; i < 3; ++i) dv[j][i] = d[i] * gg;
     }
     double gp[NV];
-    const double E = poly_ch4_1b(coef, v, gp);
+    const double E = POLY(coef, v, gp);
     e_out[t] = E;
     if (!g_out) return;
     double g[3 * NA];
@@ -182,14 +192,27 @@ hipError_t launch_1b_ps(const double* xyz, const int* first_atom, const int* mon
 hipError_t launch_1b_ch4(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
                          double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_1b_nrg<Holder_ch4_1b>, dim3((n + 63) / 64), dim3(64), 0, st, xyz, first_atom, mons, n,
-                       coef, e_out, g_out, vir_out, ld);
+    hipLaunchKernelGGL((k_1b_nrg<Holder_ch4_1b, poly_ch4_1b>), dim3((n + 63) / 64), dim3(64), 0, st, xyz,
+                       first_atom, mons, n, coef, e_out, g_out, vir_out, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_1b_co2(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
+                         double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_1b_nrg<Holder_co2_1b, poly_co2_1b>), dim3((n + 63) / 64), dim3(64), 0, st, xyz,
+                       first_atom, mons, n, coef, e_out, g_out, vir_out, ld);
     return hipGetLastError();
 }
 
 const double* ch4_1b_fit(int* n) {
     *n = kNCoef_ch4_1b;
     return kFit_ch4_1b;
+}
+
+const double* co2_1b_fit(int* n) {
+    *n = kNCoef_co2_1b;
+    return kFit_co2_1b;
 }
 
 // One workgroup: out[0] = sum of e[0..n), out[1 + q] = sum of vir[q * ld + 0..n) (zeros when vir is

@@ -142,7 +142,7 @@ static void decomp(bblock::System& s, double* E1b, double* E2b, double* E3b, dou
     *E1b = s.OneBodyEnergy(false);
     *E2b = s.TwoBodyEnergy(false);
     *E3b = s.ThreeBodyEnergy(false);
-    *E4b = 0.0;
+    *E4b = s.FourBodyEnergy(false);  // external_call.cpp:170 (0 unless a 4B cutoff is set)
     *Edisp = 0.0;
     *Ebuck = 0.0;
     *Eelec = 0.0;

@@ -176,6 +176,10 @@ hipError_t launch_1b_ps(const double* xyz, const int* first_atom, const int* mon
 hipError_t launch_1b_ch4(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
                          double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st);
 const double* ch4_1b_fit(int* n);
+// CO2 monomers (x1b_A1B2_v1x): the same outputs; co2_1b_fit: the fitted coefficients.
+hipError_t launch_1b_co2(const double* xyz, const int* first_atom, const int* mons, int n, const double* coef,
+                         double* e_out, double* g_out, double* vir_out, int ld, hipStream_t st);
+const double* co2_1b_fit(int* n);
 __global__ void k_reduce_1b(const double* e, const double* vir, int n, int ld, double* out);
 hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
                                   const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);

@@ -145,9 +145,9 @@ System::System(const System& o)
       box_(o.box_),
       cutoff2b_(o.cutoff2b_),
       cutoff3b_(o.cutoff3b_),
-      cutoff4b_(o.cutoff4b_),
       max2b_(o.max2b_),
       max3b_(o.max3b_),
+      cutoff4b_(o.cutoff4b_),
       ignore2b_(o.ignore2b_),
       ignore3b_(o.ignore3b_),
       ignore4b_(o.ignore4b_),
@@ -355,7 +355,6 @@ void System::SetUpFromJson(std::string json_text) {
     max3b_ = (size_t)num("max_n_eval_3b", (double)max3b_);
     if (mbx && mbx->has("ignore_2b_poly")) ignore2b_ = read_ignore(mbx->at("ignore_2b_poly"));
     if (mbx && mbx->has("ignore_3b_poly")) ignore3b_ = read_ignore(mbx->at("ignore_3b_poly"));
-    if (mbx && mbx->has("ignore_4b_poly")) ignore4b_ = read_ignore(mbx->at("ignore_4b_poly"));
     if (!initialized_ && !in_mon_.empty()) Initialize();
     SetPBC(box);
 }

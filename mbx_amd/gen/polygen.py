@@ -750,7 +750,7 @@ def generate_ps(out_dir):
 # to n (zero if none).  A depth-first walk returns the adjoint R(n) = c(n) + sum_children
 # x_v' R(child) and adds P(parent) R(n) to g_v on the way back, so the value and gradient come
 # out of one pass with the path's products and adjoints as the only live temporaries.
-ONE_BODY = ("ch4_1b",)
+ONE_BODY = ("ch4_1b", "co2_1b")
 
 
 def emit_trie(fn, nvars, monos):

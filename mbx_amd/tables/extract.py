@@ -43,11 +43,14 @@ POLYS = [
     # MB-nrg CH4 1B (degree 5)
     ("ch4_1b", "src/potential/1b/poly_1b_A1B4_deg5_exp0_v1.cpp",
      "src/potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp", 10, 207),
+    # MB-nrg CO2 1B (degree 4; the value-and-gradient routine, its energy expression)
+    ("co2_1b", "src/potential/1b/poly_1b_A1B2_deg4_v1x.cpp", "src/potential/1b/x1b_A1B2_deg4_v1x.cpp", 3, 21),
 ]
 
 # 1B holders: the monomer, its atom count and the holder text (x1b_A1B4_deg5_exp0_v1x.cpp:309-343)
 HOLDERS_1B = [
     ("ch4_1b", "src/potential/1b/x1b_A1B4_deg5_exp0_v1x.cpp", "ch4", 5),
+    ("co2_1b", "src/potential/1b/x1b_A1B2_deg4_v1x.cpp", "co2", 3),
 ]
 
 # Holders of the MB-nrg families: argument order of the monomers, and how the variables are
@@ -258,19 +261,23 @@ def parse_xholder(text, mons, nats):
 
 
 def parse_holder_1b(text, mon, nat):
-    """Variables of a generated MB-nrg 1B holder: v[i] = vr[i].v_exp0(d, k, P, Q) over the
-    monomer's atoms (sites named A_1, B_1, ... at xcrd + 3 * site), with the parameters m_k_*,
-    m_d_* of the holder's constructor.  Form exp0: exp(k (d - r))."""
+    """Variables of a generated MB-nrg 1B holder over the monomer's atoms (sites named A_1, B_1, ...
+    or A0, B0, ... at xcrd + 3 * site), with the numeric parameters of the holder's constructor:
+    v[i] = vr[i].v_exp0(d, k, P, Q) (x1b_A1B4_deg5_exp0_v1x.cpp) or x[i] = v_intra(k, d, P, Q)
+    (x1b_A1B2_deg4_v1x.cpp:41-47, exp(-k (r - d))).  Both are form exp0: exp(k (d - r))."""
     src = re.sub(r"//[^\n]*", "", text)
     ev = src[src.index("::eval("):]
     ev = ev[:ev.index("::eval(", 10)]
-    sites = {name: int(off) // 3 for name, off in re.findall(r"const double\*\s*([A-Z]_\d+)\s*=\s*xcrd\s*\+\s*(\d+);", ev)}
+    sites = {name: int(off) // 3 for name, off in re.findall(r"const double\*\s*([A-Z]_?\d+)\s*=\s*xcrd\s*\+\s*(\d+);", ev)}
     ctor = src[:src.index("::eval(")]
-    par = {k: float(v) for k, v in re.findall(r"\b(m_\w+)\s*=\s*([-+0-9.eE]+)\s*;", ctor)}
+    par = {k: float(v) for k, v in re.findall(r"\b(\w+)\s*=\s*([-+0-9.eE]+)\s*;", ctor)}
     vars_ = {}
     for idx, args in re.findall(r"v\[(\d+)\]\s*=\s*vr\[\d+\]\.v_exp0\(([^)]*)\);", ev):
         a = [t.strip() for t in args.split(",")]
         vars_[int(idx)] = ["exp0", par[a[1]], par[a[0]], sites[a[2]], sites[a[3]]]
+    for idx, args in re.findall(r"x\[(\d+)\]\s*=\s*v_intra\(([^)]*)\);", ev):
+        a = [t.strip() for t in args.split(",")]
+        vars_[int(idx)] = ["exp0", par[a[0]], par[a[1]], sites[a[2]], sites[a[3]]]
     nv = max(vars_) + 1
     assert sorted(vars_) == list(range(nv)) and len(sites) == nat
     return {"monomers": [mon], "atoms": [nat], "nreal": nat, "nsites": nat, "vsite": None,

@@ -565,7 +565,7 @@ def ps_1b(w, grad=True):
 
 
 def nrg_1b(name, w, a=None, grad=True):
-    """MB-nrg 1B energies (orc_1b_nrg; name "ch4_1b") of monomers w [n][3*atoms]:
+    """MB-nrg 1B energies (orc_1b_nrg; name "ch4_1b" or "co2_1b") of monomers w [n][3*atoms]:
     (E[n], G[n][3*atoms] or None, virial[6])."""
     L = lib()
     L.orc_1b_nrg.restype = ctypes.c_double
@@ -587,13 +587,18 @@ def nrg_1b(name, w, a=None, grad=True):
     return e, g, vir
 
 
+# Atoms per monomer kind (the engine's kind numbers: h2o, ch4, -, co2, f-, cl-, br-, i-).
+NAT_OF_KIND = {0: 3, 1: 5, 3: 3, 4: 1, 5: 1, 6: 1, 7: 1}
+
+
 def one_body(xyz, kinds, box=None):
-    """The 1B term of a mixed H2O (kind 0, 3 atoms) / CH4 (kind 1, 5 atoms) system, monomers in the
-    given order: e1b::get_1b_energy per monomer (potential/1b/energy1b.cpp:118-147) on the
-    coordinates System::SetPBC wraps first (FixMonomerCoordinates) when box is given.
+    """The 1B term of a system of H2O (kind 0, 3 atoms), CH4 (kind 1, 5 atoms), CO2 (kind 3, 3
+    atoms) and halide (kinds 4-7, 1 atom) monomers in the given order: e1b::get_1b_energy per
+    monomer (potential/1b/energy1b.cpp:118-177; ions have no 1B term) on the coordinates
+    System::SetPBC wraps first (FixMonomerCoordinates) when box is given.
     Returns (E, grad[3 * atoms], virial[9]); E sums in monomer order."""
     kinds = np.asarray(kinds)
-    nat = np.where(kinds == 0, 3, 5)
+    nat = np.array([NAT_OF_KIND[int(k)] for k in kinds], np.int32)
     x = _arr(xyz).reshape(-1)
     if box is not None and len(box):
         x = fix_monomer_coordinates(x, box, nat)
@@ -602,8 +607,10 @@ def one_body(xyz, kinds, box=None):
     g = np.zeros_like(x)
     vir = np.zeros(6)
     for m, (k, f) in enumerate(zip(kinds, first)):
+        if k >= 4:
+            continue
         w = x[3 * f:3 * f + 3 * nat[m]]
-        em, gm, vm = ps_1b(w) if k == 0 else nrg_1b("ch4_1b", w)
+        em, gm, vm = ps_1b(w) if k == 0 else nrg_1b("ch4_1b" if k == 1 else "co2_1b", w)
         e += em[0]
         g[3 * f:3 * f + 3 * nat[m]] = gm[0]
         vir += vm
@@ -624,6 +631,21 @@ def ref_ch4_1b(w):
     v = np.zeros(9)
     L.ref_ch4_1b(_d(w), _d(g), n, _d(e), _d(v))
     return e, g.reshape(n, 15), v
+
+
+def ref_co2_1b(w):
+    """The compiled reference's x1b_A1B2_v1x("co2")::eval on CO2 monomers w [n][9]:
+    (E[n], G[n][9], virial[9])."""
+    L = refx()
+    L.ref_co2_1b.restype = None
+    L.ref_co2_1b.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, _dp]
+    w = np.ascontiguousarray(_arr(w).reshape(-1))
+    n = len(w) // 9
+    g = np.zeros(9 * n)
+    e = np.zeros(n)
+    v = np.zeros(9)
+    L.ref_co2_1b(_d(w), _d(g), n, _d(e), _d(v))
+    return e, g.reshape(n, 9), v
 
 
 def ref_ps_1b(w, virial=None):

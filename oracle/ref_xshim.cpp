@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "potential/1b/x1b_A1B2_deg4_v1x.h"
 #include "potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.h"
 #include "potential/2b/x2b_A1B2_A1B2_v1x.h"
 #include "potential/3b/mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.h"
@@ -71,6 +72,16 @@ double ref_x4b_h2o(const double* x1, const double* x2, const double* x3, const d
     const double e = pot.eval(x1, x2, x3, x4, g1, g2, g3, g4, n, &v);
     add_virial(v, vir);
     return e;
+}
+
+// x1b_A1B2_v1x("co2")::eval with gradients and virial (potential/1b/x1b_A1B2_deg4_v1x.cpp:160):
+// energies[n], grad[9 n] (overwritten), virial[9] (accumulated).
+void ref_co2_1b(const double* xyz, double* grad, size_t n, double* energies, double* virial) {
+    x1b_A1B2_deg4::x1b_A1B2_v1x pot("co2");
+    std::vector<double> v(virial, virial + 9);
+    std::vector<double> e = pot.eval(xyz, grad, n, &v);
+    std::copy(e.begin(), e.end(), energies);
+    std::copy(v.begin(), v.end(), virial);
 }
 
 }  // extern "C"

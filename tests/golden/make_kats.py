@@ -12,7 +12,8 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
     and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, the MB-pol
-    1B (ps::pot_nasa) case (see CASES), and the ch4 and h2o rows of unittest-energy1b.cpp.
+    1B (ps::pot_nasa) case, the CH4 and CO2 1B polynomial and holder cases (see CASES), and the
+    ch4, h2o and co2 rows of unittest-energy1b.cpp.
 
 Usage:  python tests/golden/make_kats.py [/root/reference]
 """
@@ -43,6 +44,9 @@ CASES = {
     # MB-nrg CH4 1B: polynomial and holder
     "poly_ch4_1b": ("src/unittests/unittest-poly-1b.cpp", 'poly_1b_A1B4_deg5_exp0_v1x::struct'),
     "holder_ch4_1b": ("src/unittests/unittest-poly-holder-1b.cpp", 'x1b_A1B4_deg5_exp0::struct'),
+    # MB-nrg CO2 1B: polynomial and holder
+    "poly_co2_1b": ("src/unittests/unittest-poly-1b.cpp", 'poly_1b_A1B2_deg4_v1x::struct'),
+    "holder_co2_1b": ("src/unittests/unittest-poly-holder-1b.cpp", 'x1b_A1B2_deg4_v1x::struct'),
     # halide-water 2B / 3B, CO2 2B / 3B and water 4B energy routines ("#n": the n-th section of
     # that name; the first co2-co2 section uses the co2_archive model, the first co2-co2-co2 one is
     # commented out)
@@ -92,7 +96,7 @@ def values(body):
     return out
 
 
-def energy1b(text, mons=("ch4", "h2o")):
+def energy1b(text, mons=("ch4", "h2o", "co2")):
     """unittest-energy1b.cpp keeps one row per monomer kind in nested vectors (xyz, grad, virial,
     grad_expected, virial_expected), indexed by monid, plus nmon and energies_expected: the rows
     of the named kinds."""
@@ -162,7 +166,7 @@ def main():
                                else '%s TEST_CASE("%s")' % (path, name))
     path = "src/unittests/unittest-energy1b.cpp"
     res["energy1b"] = energy1b(open(os.path.join(ref, path)).read())
-    res["energy1b"]["_source"] = path + ' TEST_CASE("energy1b::get_1b_energy_grad"), rows ch4 and h2o'
+    res["energy1b"]["_source"] = path + ' TEST_CASE("energy1b::get_1b_energy_grad"), rows ch4, h2o and co2'
     path = "src/unittests/unittest-system.cpp"
     res["pair_list"] = pair_list(open(os.path.join(ref, path)).read())
     res["pair_list"]["_source"] = path + ' TEST_CASE("Pair List")'

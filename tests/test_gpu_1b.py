@@ -137,3 +137,64 @@ def test_1b_inside_compute(systems):
     with pytest.raises(engine.MbxgError):  # the 1B term is not evaluated over batches
         e.set_batch([0, 10, len(kinds)])
         e.compute(engine.TERM_1B)
+
+
+def test_1b_co2_kat_and_oracle():
+    """The CO2 1B holder (x1b_A1B2_v1x): the reference's three-monomer known answer
+    (unittest-poly-holder-1b.cpp:52-83), then 125 distorted monomers against the oracle."""
+    import json
+    import os
+
+    from conftest import GOLDEN
+
+    k = json.load(open(os.path.join(GOLDEN, "kats.json")))["holder_co2_1b"]
+    print("1b co2 kat", flush=True)
+    e = engine.Engine()
+    e.set_topology([engine.CO2] * 3)
+    e.set_box(None)
+    e.set_cutoffs(9.0, 7.0)
+    e.set_coords(np.array(k["xyz"]))
+    e1, g, v = e.eval_1b()
+    assert e1 == pytest.approx(sum(k["energies_expected"]), abs=3e-9)
+    np.testing.assert_allclose(g, k["grad_expected"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(v, k["virial_expected"], rtol=0, atol=1e-8)
+    rng = np.random.default_rng(4)
+    w = []
+    for _ in range(125):
+        q, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+        c = rng.uniform(-8, 8, 3)
+        th = np.pi - rng.uniform(0.0, 0.2)
+        r1, r2 = 1.16 + rng.uniform(-0.05, 0.05, 2)
+        w.append(np.concatenate([c, c + q @ [r1, 0, 0], c + q @ [r2 * np.cos(th), r2 * np.sin(th), 0]]))
+    w = np.array(w)
+    e = engine.Engine()
+    e.set_topology([engine.CO2] * len(w))
+    e.set_box(None)
+    e.set_cutoffs(9.0, 7.0)
+    e.set_coords(w)
+    e1, g, v = e.eval_1b()
+    er, gr, vr = oracle.nrg_1b("co2_1b", w)
+    cond = oracle.nrg_1b("co2_1b", w, a=np.abs(oracle.table("co2_1b").coef), grad=False)[0]
+    assert abs(e1 - er.sum()) <= 1e-12 * abs(er.sum()) + 4e-16 * cond.sum()
+    assert np.abs(g - gr.reshape(-1)).max() <= 1e-10 + 4e-15 * cond.max()
+    np.testing.assert_allclose(v[[0, 1, 2, 4, 5, 8]], vr, rtol=1e-10, atol=1e-9)
+
+
+def test_1b_mixed_co2_halides_in_compute(systems):
+    """1B + 2B + 3B in one compute over waters, CH4, CO2 and ions: the 1B part equals the
+    oracle's one_body (ions have none), and the total gradient is 1B + 2B + 3B."""
+    import test_gpu_xfam as X
+
+    xyz, names, box = X.mixed(systems)
+    kinds = [engine.KIND_OF[n] for n in names]
+    print("1b mixed co2 + halides", flush=True)
+    e = X.run_gpu(xyz, names, box, 9.0, 4.5, terms=engine.TERM_1B | engine.TERM_2B | engine.TERM_3B)
+    te = e.term_energies()
+    _, _, g, _ = e.results()
+    e1, g1, _ = oracle.one_body(xyz, kinds, box)
+    # 72 CH4 monomers: their 1B terms cancel ~2e7 each, a floor of ~2e-16 * 1.4e9 ~ 3e-7
+    assert te[0] == pytest.approx(e1, rel=1e-10, abs=1e-6)
+    r = oracle.xsystem(xyz, names, box, 9.0, 4.5, tags=np.arange(len(names)), cond=True)
+    nat = np.array([engine.NAT_OF[k] for k in kinds])
+    gtol = X.G_ABS + 1e-7 + 10 * X.COND * np.repeat(r["cond_mon"], 3 * nat)
+    assert np.all(np.abs(g - (g1 + r["grad"])) <= gtol)

@@ -284,3 +284,76 @@ def test_ch4_1b_finite_differences():
             d[i] = h
             fd = (8 * (E(m + d) - E(m - d)) - (E(m + 2 * d) - E(m - 2 * d))) / (12 * h)
             assert fd == pytest.approx(g[0][i], abs=2e-6 * max(1.0, abs(g[0][i])))
+
+
+
+# ---------------------------------------------------------------------------------------------
+# MB-nrg CO2 1B (orc_1b_nrg over mbx_amd/tables/co2_1b.*: x1b_A1B2_v1x, degree 4, 3 variables)
+
+refx_only = pytest.mark.skipif(not oracle.refx_available(),
+                               reason="oracle/_ref/libmbxref_x.so not built (no reference checkout)")
+
+
+def test_co2_1b_poly_kat():
+    """unittest-poly-1b.cpp:49-74 (a, x, e, g of poly_1b_A1B2_deg4_v1x, 11 digits), to the test's
+    own margin of 1e-6: at x ~ 250 the degree-4 terms reach ~1e5 and cancel to ~1.6, so the
+    11-digit inputs alone move the value by ~1e-6."""
+    k = KATS["poly_co2_1b"]
+    t = oracle.table("co2_1b")
+    np.testing.assert_allclose(t.coef, k["a"], rtol=1e-9)
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    assert e == pytest.approx(k["e_expected"], abs=1e-6)
+    np.testing.assert_allclose(g, k["g_expected"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", ["holder_co2_1b", "energy1b"])
+def test_co2_1b_holder_kats(case):
+    """unittest-poly-holder-1b.cpp:52-83 (three CO2 monomers: energies, gradients, virial) and the
+    co2 row of unittest-energy1b.cpp (the same through e1b::get_1b_energy)."""
+    k = KATS[case] if case == "holder_co2_1b" else KATS["energy1b"]["co2"]
+    e, g, v = oracle.nrg_1b("co2_1b", k["xyz"])
+    if case == "holder_co2_1b":
+        np.testing.assert_allclose(e, k["energies_expected"], rtol=0, atol=1e-9)
+    else:
+        assert e.sum() == pytest.approx(k["energy_expected"], abs=1e-9)
+    np.testing.assert_allclose(g.reshape(-1), k["grad_expected"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(_virial9(v), k["virial_expected"], rtol=0, atol=1e-8)
+
+
+def _co2_monomers(n, seed):
+    """CO2 (C, O, O) with C-O lengths 1.16 +- 0.05, bent by up to ~10 degrees, random orientation."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        q, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+        c = rng.uniform(-5, 5, 3)
+        o1 = np.array([1.16 + rng.uniform(-0.05, 0.05), 0.0, 0.0])
+        th = np.pi - rng.uniform(0.0, 0.18)
+        r2 = 1.16 + rng.uniform(-0.05, 0.05)
+        o2 = np.array([r2 * np.cos(th), r2 * np.sin(th), 0.0])
+        out.append(np.concatenate([c, c + q @ o1, c + q @ o2]))
+    return np.array(out)
+
+
+@refx_only
+def test_co2_1b_vs_compiled_reference():
+    w = _co2_monomers(64, 5)
+    e, g, v = oracle.nrg_1b("co2_1b", w)
+    er, gr, vr = oracle.ref_co2_1b(w)
+    cond = oracle.nrg_1b("co2_1b", w, a=np.abs(oracle.table("co2_1b").coef), grad=False)[0]
+    assert np.all(np.abs(e - er) <= 4e-16 * cond)
+    np.testing.assert_allclose(g, gr, rtol=0, atol=4e-15 * cond.max())
+    np.testing.assert_allclose(_virial9(v), vr, rtol=0, atol=4e-14 * cond.max())
+
+
+def test_co2_1b_finite_differences():
+    w = _co2_monomers(4, 11)
+    h = 1e-4
+    E = lambda x: oracle.nrg_1b("co2_1b", x, grad=False)[0][0]
+    for m in w:
+        _, g, _ = oracle.nrg_1b("co2_1b", m)
+        for i in range(9):
+            d = np.zeros(9)
+            d[i] = h
+            fd = (8 * (E(m + d) - E(m - d)) - (E(m + 2 * d) - E(m - 2 * d))) / (12 * h)
+            assert fd == pytest.approx(g[0][i], abs=1e-6 * max(1.0, abs(g[0][i])))

@@ -94,3 +94,45 @@ def test_external_call_orders_mixed_input(systems, tmp_path):
     assert e.value == pytest.approx(e1 + r["e2b"] + r["e3b"], rel=1e-10)
     assert np.abs(g - (r["grad"] + g1)).max() < 1e-8
     L.finalize_system_()
+
+
+ATOMS_OF = {"h2o": ["O", "H", "H"], "ch4": ["C", "H", "H", "H", "H"], "co2": ["C", "O", "O"], "f-": ["F"],
+            "cl-": ["Cl"], "br-": ["Br"], "i-": ["I"]}
+
+
+def test_external_call_further_families_and_4b(systems, tmp_path):
+    """Waters, CH4, CO2 and halide ions in one periodic box with a 4B cutoff, through the
+    reference's C entry points: Energy() = 1B + 2B + 3B + 4B (system.cpp:2060-2087), the
+    decomposition's E4b = FourBodyEnergy(false), gradients in input order."""
+    import test_gpu_xfam as X
+
+    xyz, names, box = X.mixed(systems)
+    n = len(names)
+    kinds = [engine.KIND_OF[m] for m in names]
+    nats = [engine.NAT_OF[k] for k in kinds]
+    cfg = tmp_path / "mbx.json"
+    cfg.write_text('{"MBX": {"box": [%s], "twobody_cutoff": 9.0, "threebody_cutoff": 4.5, "fourbody_cutoff": 4.0}}'
+                   % ",".join(repr(float(b)) for b in box))
+    print("system mixed + 4B", flush=True)
+    L = _lib()
+    _init(L, xyz, names, nats, [a for m in names for a in ATOMS_OF[m]], str(cfg))
+    dp = ctypes.POINTER(ctypes.c_double)
+    e = ctypes.c_double()
+    g = np.zeros(len(xyz))
+    c = np.ascontiguousarray(xyz)
+    b = np.ascontiguousarray(box)
+    L.get_energy_pbc_g_(c.ctypes.data_as(dp), ctypes.byref(ctypes.c_int(len(xyz) // 3)), b.ctypes.data_as(dp),
+                        ctypes.byref(e), g.ctypes.data_as(dp))
+    r = oracle.xsystem(xyz, names, box, 9.0, 4.5, 4.0, cond=True)
+    e1, g1, _ = oracle.one_body(xyz, kinds, box)
+    want = e1 + r["e2b"] + r["e3b"] + r["e4b"]
+    tol = 1e-6 + 1e-10 * abs(want) + X.COND * sum(r["cond"].values())
+    assert abs(e.value - want) <= tol
+    gtol = 1e-7 + 10 * X.COND * np.repeat(r["cond_mon"], 3 * np.array(nats))
+    assert np.all(np.abs(g - (r["grad"] + g1)) <= gtol)
+    terms = [ctypes.c_double() for _ in range(7)]
+    L.get_energy_decomp_pbc_(c.ctypes.data_as(dp), ctypes.byref(ctypes.c_int(len(xyz) // 3)), b.ctypes.data_as(dp),
+                             *[ctypes.byref(t) for t in terms])
+    assert terms[3].value == pytest.approx(r["e4b"], abs=1e-10 + X.COND * r["cond"][4])
+    assert terms[1].value == pytest.approx(r["e2b"], rel=1e-10, abs=X.COND * r["cond"][2])
+    L.finalize_system_()
