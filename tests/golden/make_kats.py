@@ -13,7 +13,9 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
     and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, the MB-pol
     1B (ps::pot_nasa) case, the CH4 and CO2 1B polynomial and holder cases (see CASES), and the
-    ch4, h2o and co2 rows of unittest-energy1b.cpp.
+    ch4, h2o and co2 rows of unittest-energy1b.cpp, the h2o sections of unittest-energy2b.cpp /
+    unittest-energy3b.cpp, and the three-water gas-phase n-body energies of
+    src/tests/unittests/unittest-gas-nbodyterms-mbpol.cpp.
 
 Usage:  python tests/golden/make_kats.py [/root/reference]
 """
@@ -37,6 +39,8 @@ CASES = {
     "holder_ch4ch4_2b": ("src/unittests/unittest-poly-holder-2b.cpp", 'x2b_A1B4_A1B4_v1x::struct'),
     "holder_ch4h2o2_3b": ("src/unittests/unittest-poly-holder-3b.cpp", 'mbnrg_3b_A1B4_C1D2_C1D2_deg3_v1::class'),
     "energy_ch4ch4_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-ch4'),
+    "energy_h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:h2o-h2o'),
+    "energy_h2o_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:h2o-h2o-h2o'),
     "energy_ch4h2o_2b": ("src/unittests/unittest-energy2b.cpp", 'SECTION:ch4-h2o'),
     "energy_ch4h2o2_3b": ("src/unittests/unittest-energy3b.cpp", 'SECTION:ch4-h2o-h2o'),
     # MB-pol 1B (Partridge-Schwenke water monomer)
@@ -93,6 +97,25 @@ def values(body):
         out[m.group(1)] = int(m.group(2))
     for m in re.finditer(r"std::string\s+(mon\d)\s*=\s*\"([^\"]*)\"\s*;", body):
         out[m.group(1)] = m.group(2)
+    return out
+
+
+def gas_nbody(test_text, setup_text):
+    """src/tests/unittests/unittest-gas-nbodyterms-mbpol.cpp over setup_h2o_3.h: the three waters'
+    real coordinates (O H H of each monomer, the setup's coords at first_ind = 4 n) and the 1B / 2B
+    polynomial / 3B polynomial energies the test expects (its REQUIRE ... Approx values)."""
+    src = re.sub(r"\\\n", "\n", setup_text)
+    m = re.search(r"std::vector<double>\s+coords\s*\{([^}]*)\}", src)
+    coords = [float(v) for v in m.group(1).replace("\n", " ").split(",") if v.strip()]
+    assert re.search(r"first_ind\[n\]\s*=\s*4\s*\*\s*n", src)
+    xyz = []
+    for n in range(3):
+        xyz += coords[12 * n:12 * n + 9]
+    out = {"xyz": xyz}
+    for key, sec in (("e1b", "One body (polynomials)"), ("e2b", "Two body (polynomials)"),
+                     ("e3b", "Three body (polynomials)")):
+        b = block(test_text, "SECTION:" + sec)
+        out[key] = float(re.search(r"Approx\(([-+0-9.eE]+)\)", b).group(1))
     return out
 
 
@@ -167,6 +190,10 @@ def main():
     path = "src/unittests/unittest-energy1b.cpp"
     res["energy1b"] = energy1b(open(os.path.join(ref, path)).read())
     res["energy1b"]["_source"] = path + ' TEST_CASE("energy1b::get_1b_energy_grad"), rows ch4, h2o and co2'
+    path = "src/tests/unittests/unittest-gas-nbodyterms-mbpol.cpp"
+    res["gas_nbody_h2o_3"] = gas_nbody(open(os.path.join(ref, path)).read(),
+                                       open(os.path.join(ref, "src/tests/unittests/setup_h2o_3.h")).read())
+    res["gas_nbody_h2o_3"]["_source"] = path + " (over setup_h2o_3.h)"
     path = "src/unittests/unittest-system.cpp"
     res["pair_list"] = pair_list(open(os.path.join(ref, path)).read())
     res["pair_list"]["_source"] = path + ' TEST_CASE("Pair List")'

@@ -279,3 +279,26 @@ def test_graph_replay(systems):
     a = g.results()
     r = oracle.water_system(dense, box, c2, c3)
     assert a[0] == pytest.approx(r["e2b"], rel=E_REL) and a[1] == pytest.approx(r["e3b"], rel=E_REL)
+
+
+def test_gas_nbody_h2o_3_kat():
+    """Three waters in the gas phase (src/tests/unittests/unittest-gas-nbodyterms-mbpol.cpp over
+    setup_h2o_3.h): the 1B, 2B and 3B energies the reference test expects, in one compute."""
+    import json
+    import os
+
+    from conftest import GOLDEN
+
+    k = json.load(open(os.path.join(GOLDEN, "kats.json")))["gas_nbody_h2o_3"]
+    print("gas nbody 3 waters", flush=True)
+    e = engine.Engine()
+    e.set_topology([engine.H2O] * 3)
+    e.set_box(None)
+    e.set_cutoffs(100.0, 10.0)
+    e.set_coords(np.array(k["xyz"]))
+    e.compute(engine.TERM_1B | engine.TERM_2B | engine.TERM_3B)
+    te = e.term_energies()
+    assert te[0] == pytest.approx(k["e1b"], abs=1e-10)
+    assert te[1] == pytest.approx(k["e2b"], abs=1e-9)
+    assert te[2] == pytest.approx(k["e3b"], abs=1e-10)
+    assert tuple(int(v) for v in e.counts()) == (3, 3, 1, 1)

@@ -391,3 +391,55 @@ def test_energy1b_kat():
     v[3], v[6], v[7] = v[1], v[2], v[5]
     np.testing.assert_allclose(v, k["virial_expected"], rtol=1e-6, atol=1e-6)
 
+
+
+# ---------------------------------------------------------------------------------------------
+# The reference's water energy-routine and gas-phase n-body known answers
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_energy2b_h2o_kat(swap):
+    """unittest-energy2b.cpp SECTION("h2o-h2o") (:48-97): two dimers through e2b::get_2b_energy,
+    energy, both monomers' gradients and the virial; swap: the dimers passed as (mon2, mon1), which
+    the test requires to give the same energy."""
+    k = KATS["energy_h2o_2b"]
+    x1, x2 = np.array(k["xyz1"]).reshape(k["nm"], 9), np.array(k["xyz2"]).reshape(k["nm"], 9)
+    E, G1, G2, V = 0.0, np.zeros_like(x1), np.zeros_like(x2), np.zeros(9)
+    for j in range(k["nm"]):
+        if swap:
+            e, g2, g1, v, _ = oracle.mbpol2b(x2[j], x1[j])
+        else:
+            e, g1, g2, v, _ = oracle.mbpol2b(x1[j], x2[j])
+        E += e
+        G1[j] += g1
+        G2[j] += g2
+        V += v
+    assert E == pytest.approx(k["energy_expected"], abs=1e-9)
+    np.testing.assert_allclose(G1.reshape(-1), k["grad1_expected"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(G2.reshape(-1), k["grad2_expected"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(V, k["virial_expected"], rtol=0, atol=1e-8)
+
+
+def test_energy3b_h2o_kat():
+    """unittest-energy3b.cpp SECTION("h2o-h2o-h2o") (:48-145): one trimer through
+    e3b::get_3b_energy, energy, gradients and virial."""
+    k = KATS["energy_h2o_3b"]
+    e, g1, g2, g3, v, ev = oracle.mbpol3b(k["xyz1"], k["xyz2"], k["xyz3"])
+    assert ev == 1
+    assert e == pytest.approx(k["energy_expected"], abs=1e-9)
+    for g, name in ((g1, "grad1_expected"), (g2, "grad2_expected"), (g3, "grad3_expected")):
+        np.testing.assert_allclose(g, k[name], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(v, k["virial_expected"], rtol=0, atol=1e-8)
+
+
+def test_gas_nbody_h2o_3_kat():
+    """src/tests/unittests/unittest-gas-nbodyterms-mbpol.cpp over setup_h2o_3.h: the 1B, 2B
+    (polynomial) and 3B (polynomial) energies of three waters, every dimer and the one trimer."""
+    k = KATS["gas_nbody_h2o_3"]
+    w = np.array(k["xyz"]).reshape(3, 9)
+    assert oracle.ps_1b(w)[0].sum() == pytest.approx(k["e1b"], abs=1e-10)
+    e2 = sum(oracle.mbpol2b(w[i], w[j])[0] for i in range(3) for j in range(i + 1, 3))
+    assert e2 == pytest.approx(k["e2b"], abs=1e-9)
+    assert oracle.mbpol3b(w[0], w[1], w[2])[0] == pytest.approx(k["e3b"], abs=1e-10)
+    r = oracle.water_system(w.reshape(-1), None, 100.0, 10.0)
+    assert r["e2b"] == pytest.approx(k["e2b"], abs=1e-9) and r["e3b"] == pytest.approx(k["e3b"], abs=1e-10)
