@@ -95,7 +95,7 @@ struct ListState {
     DBuf<int> d_cell_base, d_cell_bound;
     DBuf<double> cx, cy, cz, bx, by, bz;
     DBuf<PtInfo> pinfo;           // per cloud point: first atom, tag, kind
-    DBuf<int> cell, cell_count, cell_start, cell_slot;
+    DBuf<int> cell, cell_count, cell_start, cell_cursor, cell_slot;
     DBuf<int4> counts, offsets, counts_x, offsets_x;
     DBuf<int4> wpre, wpre_x;      // per (query, wavefront): prefix within the query
     DBuf<int> list2, list3, evals[kFamilies];
@@ -103,7 +103,7 @@ struct ListState {
     // capacities in n-mers: cap = {list2, MB-pol 2B, list3, MB-pol 3B},
     // capx = {CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, unused}; tot / totx: totals, same layout
     int4 cap = {0, 0, 0, 0}, capx = {0, 0, 0, 0};
-    int match_cap = 256;          // power of two
+    int match_cap = 256;          // a multiple of 64
     int4 tot = {0, 0, 0, 0}, totx = {0, 0, 0, 0};
     bool built = false;           // the last compute built lists
     // one-pass builds: per-(query, wavefront) staging of each family's evaluated n-mers, sized
@@ -181,6 +181,7 @@ struct mbxg_ctx {
     // wavefronts per 64 n-mers in the polynomial stages (mbxg_set_tuning / mbxg_autotune)
     int split2 = 4, split3 = 2, split3n = 4;
     int fuse_poly = 1;       // MB-pol 2B and 3B polynomials in one launch when both are evaluated
+    int fuse_prep = 1;       // FixMonomerCoordinates, cloud and cell counts in one launch (k_cloud_fix)
     bool fused_last = false;  // the last compute did so (its 2B polynomial time is in the 3B one)
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
@@ -279,6 +280,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.bz.release();
     L.cell.release();
     L.cell_count.release();
+    L.cell_cursor.release();
     L.cell_start.release();
     L.cell_slot.release();
     L.counts.release();
@@ -690,11 +692,6 @@ Grid periodic_grid(const Box& box, double cut) {
     return G;
 }
 
-int pow2_at_least(int n) {
-    int p = 1;
-    while (p < n) p <<= 1;
-    return p;
-}
 
 int with_headroom(int n) { return n + n / 4 + 64; }
 
@@ -801,9 +798,10 @@ int prepare_layout(mbxg_ctx* c, double cut) {
         total += bound[k];
     }
     if (L.d_grids.reserve(nconf) || L.d_cell_base.reserve(nconf) || L.d_cell_bound.reserve(nconf) ||
-        L.cell_count.reserve(total) || L.cell_start.reserve(total + 1))
+        L.cell_count.reserve(total) || L.cell_start.reserve(total + 1) || L.cell_cursor.reserve(total))
         return fail(MBXG_ENOMEM, "cell layout: device allocation failed");
     hipStream_t st = c->stream;
+    launch_zero_int(L.cell_count.p, total, st);  // every scan leaves the counts zero again
     HIPCHK(hipMemcpyAsync(L.d_cell_base.p, base.data(), sizeof(int) * nconf, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(L.d_cell_bound.p, bound.data(), sizeof(int) * nconf, hipMemcpyHostToDevice, st));
     if (!grids.empty())
@@ -839,17 +837,29 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         L.wmax.reserve(np) || L.wmax_x.reserve(np) || L.maxw.reserve(2))
         return fail(MBXG_ENOMEM, "list buffers: device allocation failed");
     const int tb = 256, pb = (np + tb - 1) / tb, sb = (ns + tb - 1) / tb;
+    // the gas-phase grids read the input coordinates (the working copy equals them there)
     if (!c->box.periodic)
-        hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p,
+        hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_in.p, c->d_first.p, c->d_members.p,
                            c->d_pt_off.p, L.d_cell_bound.p, cut, L.d_grids.p);
-    launch_zero_int(L.cell_count.p, L.ncell_total, st);
-    hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, c->d_tag.p,
-                       c->d_kind.p, np, c->box, L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p,
-                       L.cell.p, L.pinfo.p);
-    hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p);
-    hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_start.p, L.cell_count.p, L.cx.p,
-                       L.cy.p, L.cz.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
+    if (c->fuse_prep) {  // fix, cloud and cell counts in one launch
+        const int nt = std::max(c->nmon, np);
+        hipLaunchKernelGGL(k_cloud_fix, dim3((nt + tb - 1) / tb), dim3(tb), 0, st, c->d_in.p, c->d_xyz.p,
+                           c->d_first.p, c->d_nat.p, c->nmon, c->d_members.p, c->d_tag.p, c->d_kind.p, np, c->box,
+                           L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.pinfo.p,
+                           L.cell_count.p);
+    } else {
+        if (c->nmon > 0)
+            hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p,
+                               c->d_xyz.p, c->d_first.p, c->d_nat.p, c->nmon, c->box);
+        hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, c->d_tag.p,
+                           c->d_kind.p, np, c->box, L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p,
+                           L.cell.p, L.pinfo.p);
+        hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p);
+    }
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p,
+                       L.cell_cursor.p);
+    hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_cursor.p, L.cx.p, L.cy.p, L.cz.p,
+                       L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
     HIPCHK(hipGetLastError());
     NmersArgs A{};
     A.pinfo = L.pinfo.p;
@@ -894,9 +904,8 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         double* gs = scratch();
         if (L.match_cap > kMaxLdsMatches && !gs) return fail(MBXG_ENOMEM, "match scratch: device allocation failed");
         HIPCHK(launch_nmers(A, N, one_pass ? 2 : 0, none, none, L.match_cap, gs, kScratchBlocks, st));
-        hipLaunchKernelGGL(k_max_counts, dim3(1), dim3(1024), 0, st, L.wmax.p, L.wmax_x.p, np, L.maxw.p);
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts.p, np, L.offsets.p, 0);
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts_x.p, np, L.offsets_x.p, 1);
+        hipLaunchKernelGGL(k_scan_all, dim3(1), dim3(1024), 0, st, L.wmax.p, L.wmax_x.p, L.maxw.p, L.counts.p,
+                           L.offsets.p, L.counts_x.p, L.offsets_x.p, np);
         HIPCHK(hipGetLastError());
         if (!sync) break;
         int4 tot, totx;
@@ -908,7 +917,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         HIPCHK(hipStreamSynchronize(st));
         wave_maxima(m2, mw);
         ovf = totx.w;  // the largest match count of any query
-        const int want = std::max(64, pow2_at_least(ovf + ovf / 4 + 1));
+        const int want = std::max(64, (ovf + ovf / 4 + 64) / 64 * 64);  // LDS per block: 80 bytes a match
         if (ovf > L.match_cap) {
             if (attempt > 0 || want > 2 * ns + 2)
                 return fail(MBXG_EHIP, "neighbour search: match storage did not grow (%d matches)", ovf);
@@ -1056,14 +1065,8 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     if (ev)
         for (int k = 0; k < kNumEvents; ++k) HIPCHK(hipEventRecord(ev[k], st));  // stages skipped read 0
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    if (c->nmon > 0) {
-        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p, c->d_xyz.p,
-                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
-        HIPCHK(hipGetLastError());
-    }
     NmerCtx N{c->d_xyz.p, c->d_first.p, c->d_nat.p, c->d_kind.p, c->d_tag.p, c->d_islocal.p, c->d_members.p, c->d_box.p,
               c->box.periodic};
-    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const bool want2 = (terms & MBXG_2B) && c->cut2 > 0.0;
     const bool want3 = (terms & MBXG_3B) && c->cut3 > 0.0;
     const bool want4 = (terms & MBXG_4B) && c->cut4 > 0.0;
@@ -1072,6 +1075,14 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
     // the further families read the listed lists, the tetramer search the cloud
     const bool xrun = (c->has_xkind && (want2 || want3)) || want4;
     L.built = (want2 || want3 || want4) && c->np >= 2;
+    // FixMonomerCoordinates: here when no lists are built, else inside build_lists (fused with
+    // the cloud for small periodic clouds)
+    if (c->nmon > 0 && !L.built) {
+        hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p, c->d_xyz.p,
+                           c->d_first.p, c->d_nat.p, c->nmon, c->box);
+        HIPCHK(hipGetLastError());
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     if (L.built) {
         const double cmax = std::max(std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), want4 ? c->cut4 : 0.0);
         if (int r = build_lists(c, N, c2sq, c3sq, cmax, sync, (terms & MBXG_LISTS_ONLY) != 0 || xrun, ev))
@@ -1364,7 +1375,8 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
     const std::string k(key);
     if ((k == "split_2b" || k == "split_3b" || k == "split_ch4h2o2_3b") && !valid_split(value))
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
-    if (k == "fuse_poly" && value != 0 && value != 1) return fail(MBXG_EINVAL, "mbxg_set_tuning: fuse_poly is 0 or 1");
+    if ((k == "fuse_poly" || k == "fuse_prep") && value != 0 && value != 1)
+        return fail(MBXG_EINVAL, "mbxg_set_tuning: %s is 0 or 1", key);
     if (k == "split_2b")
         c->split2 = value;
     else if (k == "split_3b")
@@ -1373,6 +1385,8 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
         c->split3n = value;
     else if (k == "fuse_poly")
         c->fuse_poly = value;
+    else if (k == "fuse_prep")
+        c->fuse_prep = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1392,6 +1406,8 @@ extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
         *value = c->split3n;
     else if (k == "fuse_poly")
         *value = c->fuse_poly;
+    else if (k == "fuse_prep")
+        *value = c->fuse_prep;
     else
         return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
     return MBXG_OK;

@@ -42,10 +42,20 @@ namespace mbxg {
 // FixMonomerCoordinates (sys_tools.cpp:287-319): wrap each monomer's first atom into the cell
 // and bring its other atoms next to it.  Reads the caller's coordinates, writes the working
 // copy (a plain copy in the gas phase).  One thread per monomer.
-__global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict__ out,
-                               const int* __restrict__ first_atom, const int* __restrict__ nat, int nmon, Box B) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nmon) return;
+// The first atom's working position (periodic): its fractional coordinates u0 v0 w0 wrapped
+// into [-1/2, 1/2) and y0 = their Cartesian position.
+__device__ __forceinline__ void fixed_first(const Box& B, const double* x, double& u0, double& v0, double& w0,
+                                            double* y0) {
+    to_frac(B, x[0], x[1], x[2], u0, v0, w0);
+    u0 -= floor(u0 + 0.5);
+    v0 -= floor(v0 + 0.5);
+    w0 -= floor(w0 + 0.5);
+    to_cart(B, u0, v0, w0, y0[0], y0[1], y0[2]);
+}
+
+__device__ __forceinline__ void fix_monomer(int i, const double* __restrict__ in, double* __restrict__ out,
+                                            const int* __restrict__ first_atom, const int* __restrict__ nat,
+                                            const Box& B) {
     const double* x = in + 3 * (size_t)first_atom[i];
     double* y = out + 3 * (size_t)first_atom[i];
     const int na = nat[i];
@@ -54,11 +64,7 @@ __global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict
         return;
     }
     double u0, v0, w0;
-    to_frac(B, x[0], x[1], x[2], u0, v0, w0);
-    u0 -= floor(u0 + 0.5);
-    v0 -= floor(v0 + 0.5);
-    w0 -= floor(w0 + 0.5);
-    to_cart(B, u0, v0, w0, y[0], y[1], y[2]);
+    fixed_first(B, x, u0, v0, w0, y);
     for (int a = 1; a < na; ++a) {
         double u, v, w;
         to_frac(B, x[3 * a], x[3 * a + 1], x[3 * a + 2], u, v, w);
@@ -67,6 +73,12 @@ __global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict
         w -= floor(w - w0 + 0.5);
         to_cart(B, u, v, w, y[3 * a], y[3 * a + 1], y[3 * a + 2]);
     }
+}
+
+__global__ void k_fix_monomers(const double* __restrict__ in, double* __restrict__ out,
+                               const int* __restrict__ first_atom, const int* __restrict__ nat, int nmon, Box B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmon) fix_monomer(i, in, out, first_atom, nat, B);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -136,13 +148,14 @@ __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz
 // its point's plus the image shift), so a query only visits slots near it instead of all 8
 // slots of the points near it.  Cell index = cell_base[config] + local cell.  One thread per
 // point.
-__global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
-                        const int* __restrict__ members, const int* __restrict__ tag, const int* __restrict__ kind,
-                        int np, Box B, const Grid* __restrict__ grids, const int* __restrict__ pconf,
-                        const int* __restrict__ cell_base, double* __restrict__ cx, double* __restrict__ cy,
-                        double* __restrict__ cz, int* __restrict__ cell, PtInfo* __restrict__ pinfo) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
+// The slots of point p: emit(slot, x, y, z, cell) for the point's own slot p and, under PBC, for
+// its image slots p + shift * np.  pinfo (nullable) receives the point's data.
+template <class Emit>
+__device__ __forceinline__ void cloud_point(int p, const double* q, int mp, const int* __restrict__ tag,
+                                            const int* __restrict__ kind, int np, const Box& B,
+                                            const Grid* __restrict__ grids, const int* __restrict__ pconf,
+                                            const int* __restrict__ cell_base, PtInfo* __restrict__ pinfo,
+                                            Emit emit) {
     const int conf = pconf[p];
     const Grid G = grids[conf];
     const int base = cell_base[conf];
@@ -155,10 +168,8 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
         cc = cc < 0 ? 0 : (cc >= G.n[2] ? G.n[2] - 1 : cc);
         return base + (cc * G.n[1] + cb) * G.n[0] + ca;
     };
-    const int mp = members[p];
-    const double* x0 = xyz + 3 * (size_t)first_atom[mp];
-    double qx = x0[0], qy = x0[1], qz = x0[2];
-    pinfo[p] = PtInfo{qx, qy, qz, tag[mp], kind[mp]};
+    double qx = q[0], qy = q[1], qz = q[2];
+    if (pinfo) pinfo[p] = PtInfo{qx, qy, qz, tag[mp], kind[mp]};
     if (B.periodic) {
         double u, v, w;
         to_frac(B, qx, qy, qz, u, v, w);
@@ -171,66 +182,111 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
         const double uvw1 = B.f11 * qy + B.f12 * qz;
         const double uvw2 = B.f22 * qz;
         to_frac(B, qx, qy, qz, u, v, w);  // cell placement only
-        const int ind[3] = {0, -1, 1};
-        for (int m2 = 0; m2 < 3; ++m2) {
-            const int m = ind[m2];
-            if ((uvw0 + m) > 1.0 || (uvw0 + m) < -1.0) continue;
-            for (int n2 = 0; n2 < 3; ++n2) {
-                const int n = ind[n2];
-                if ((uvw1 + n) > 1.0 || (uvw1 + n) < -1.0) continue;
-                for (int l2 = 0; l2 < 3; ++l2) {
-                    if (m2 == 0 && n2 == 0 && l2 == 0) continue;
-                    const int l = ind[l2];
-                    if ((uvw2 + l) > 1.0 || (uvw2 + l) < -1.0) continue;
-                    int shift;
-                    if (m == 0 && n == 0)
-                        shift = 1;
-                    else if (m == 0 && l == 0)
-                        shift = 2;
-                    else if (n == 0 && l == 0)
-                        shift = 3;
-                    else if (m != 0 && n != 0 && l != 0)
-                        shift = 4;
-                    else if (m == 0)
-                        shift = 5;
-                    else if (n == 0)
-                        shift = 6;
-                    else
-                        shift = 7;
-                    const double sx = (double)m * B.b[0] + (double)n * B.b[3] + (double)l * B.b[6];
-                    const double sy = (double)m * B.b[1] + (double)n * B.b[4] + (double)l * B.b[7];
-                    const double sz = (double)m * B.b[2] + (double)n * B.b[5] + (double)l * B.b[8];
-                    const size_t s = (size_t)p + (size_t)shift * np;
-                    cx[s] = qx + sx;
-                    cy[s] = qy + sy;
-                    cz[s] = qz + sz;
-                    cell[s] = cell_of(u + m, v + n, w + l);
-                }
-            }
+        // XyzToCloud's loops over m, n, l in {0, -1, 1} (kdtree_utils.h:190-225) write image
+        // slot `shift` once per admissible combination of its non-zero directions, the last one
+        // kept.  Per dimension that is +1 when admissible, else -1: a coordinate of exactly 0
+        // admits both, and the slot must still be emitted (and counted) once.
+        auto dir = [](double f) { return !((f + 1) > 1.0 || (f + 1) < -1.0) ? 1 : (!((f - 1) > 1.0 || (f - 1) < -1.0) ? -1 : 0); };
+        auto zero_ok = [](double f) { return !(f > 1.0 || f < -1.0); };
+        const int dm = dir(uvw0), dn = dir(uvw1), dl = dir(uvw2);
+        const bool zm = zero_ok(uvw0), zn = zero_ok(uvw1), zl = zero_ok(uvw2);
+#pragma unroll 1
+        for (int pat = 1; pat < 8; ++pat) {
+            const int m = (pat & 1) ? dm : 0, n = (pat & 2) ? dn : 0, l = (pat & 4) ? dl : 0;
+            if ((pat & 1) ? !m : !zm) continue;
+            if ((pat & 2) ? !n : !zn) continue;
+            if ((pat & 4) ? !l : !zl) continue;
+            int shift;
+            if (m == 0 && n == 0)
+                shift = 1;
+            else if (m == 0 && l == 0)
+                shift = 2;
+            else if (n == 0 && l == 0)
+                shift = 3;
+            else if (m != 0 && n != 0 && l != 0)
+                shift = 4;
+            else if (m == 0)
+                shift = 5;
+            else if (n == 0)
+                shift = 6;
+            else
+                shift = 7;
+            const double sx = (double)m * B.b[0] + (double)n * B.b[3] + (double)l * B.b[6];
+            const double sy = (double)m * B.b[1] + (double)n * B.b[4] + (double)l * B.b[7];
+            const double sz = (double)m * B.b[2] + (double)n * B.b[5] + (double)l * B.b[8];
+            emit(p + shift * np, qx + sx, qy + sy, qz + sz, cell_of(u + m, v + n, w + l));
         }
-        cell[p] = cell_of(u, v, w);
+        emit(p, qx, qy, qz, cell_of(u, v, w));
     } else {
-        cell[p] = cell_of(qx, qy, qz);
+        emit(p, qx, qy, qz, cell_of(qx, qy, qz));
     }
-    cx[p] = qx;
-    cy[p] = qy;
-    cz[p] = qz;
+}
+
+__global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
+                        const int* __restrict__ members, const int* __restrict__ tag, const int* __restrict__ kind,
+                        int np, Box B, const Grid* __restrict__ grids, const int* __restrict__ pconf,
+                        const int* __restrict__ cell_base, double* __restrict__ cx, double* __restrict__ cy,
+                        double* __restrict__ cz, int* __restrict__ cell, PtInfo* __restrict__ pinfo) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int mp = members[p];
+    const double* x0 = xyz + 3 * (size_t)first_atom[mp];
+    const double q[3] = {x0[0], x0[1], x0[2]};
+    cloud_point(p, q, mp, tag, kind, np, B, grids, pconf, cell_base, pinfo,
+                [&](int s, double x, double y, double z, int c) {
+                    cx[s] = x;
+                    cy[s] = y;
+                    cz[s] = z;
+                    cell[s] = c;
+                });
+}
+
+// FixMonomerCoordinates, the cloud and the cell counts in one launch (one thread per monomer and
+// cloud point): thread i fixes monomer i (into xyz) and emits point i's slots (as k_cloud) with
+// its own copy of the point's fixed first atom, so no thread reads another's output; each slot
+// adds one to its cell's count (zero on entry: k_scan_cells leaves it zero).
+__global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ xyz,
+                            const int* __restrict__ first_atom, const int* __restrict__ nat, int nmon,
+                            const int* __restrict__ members, const int* __restrict__ tag,
+                            const int* __restrict__ kind, int np, Box B, const Grid* __restrict__ grids,
+                            const int* __restrict__ pconf, const int* __restrict__ cell_base,
+                            double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
+                            int* __restrict__ cell, PtInfo* __restrict__ pinfo, int* __restrict__ cell_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmon) fix_monomer(i, in, xyz, first_atom, nat, B);
+    if (i >= np) return;
+    const int mp = members[i];
+    const double* x0 = in + 3 * (size_t)first_atom[mp];
+    double q[3] = {x0[0], x0[1], x0[2]};
+    if (B.periodic) {
+        double u0, v0, w0;
+        fixed_first(B, x0, u0, v0, w0, q);
+    }
+    cloud_point(i, q, mp, tag, kind, np, B, grids, pconf, cell_base, pinfo,
+                [&](int s, double x, double y, double z, int c) {
+                    cx[s] = x;
+                    cy[s] = y;
+                    cz[s] = z;
+                    cell[s] = c;
+                    atomicAdd(&cell_count[c], 1);
+                });
 }
 
 // ---------------------------------------------------------------------------------------------
 // Binning of slots into cells: count, scan, fill.  The fill also writes the slots' coordinates
 // in cell order, so the gather reads them contiguously.  The order of slots inside a cell is
 // left to the atomics; the n-mer kernel sorts each query's matches, which makes the output
-// order independent of it.  cell_count must be zero on entry (k_zero_int).
+// order independent of it.  cell_count is zero on entry: zeroed once when the cell layout is
+// made, and left zero by every scan.
 __global__ void k_bin_count(const int* __restrict__ cell, int ns, int* __restrict__ cell_count) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < ns) atomicAdd(&cell_count[cell[s]], 1);
 }
 
-// Single-workgroup exclusive scan of ncell cell counts; cell_start gets ncell+1 entries and
-// cell_count is reset to zero for use as the fill cursor.
+// Single-workgroup exclusive scan of ncell cell counts; cell_start gets ncell+1 entries, the
+// fill cursor starts at each cell's start, and cell_count is left zero for the next build.
 __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, int ncell,
-                                                     int* __restrict__ cell_start) {
+                                                     int* __restrict__ cell_start, int* __restrict__ cursor) {
     __shared__ int sh[1024];
     const int t = threadIdx.x;
     const int per = (ncell + 1023) / 1024;
@@ -248,20 +304,21 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_coun
     int o = sh[t] - acc;
     for (int c = beg; c < end; ++c) {
         cell_start[c] = o;
+        cursor[c] = o;
         o += cell_count[c];
         cell_count[c] = 0;
     }
     if (t == 1023) cell_start[ncell] = sh[t];
 }
 
-__global__ void k_bin_fill(const int* __restrict__ cell, int ns, const int* __restrict__ cell_start,
-                           int* __restrict__ cursor, const double* __restrict__ cx, const double* __restrict__ cy,
+__global__ void k_bin_fill(const int* __restrict__ cell, int ns, int* __restrict__ cursor,
+                           const double* __restrict__ cx, const double* __restrict__ cy,
                            const double* __restrict__ cz, int* __restrict__ cell_slot, double* __restrict__ bx,
                            double* __restrict__ by, double* __restrict__ bz) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const int c = cell[s];
-    const int at = cell_start[c] + atomicAdd(&cursor[c], 1);
+    const int at = atomicAdd(&cursor[c], 1);
     cell_slot[at] = s;
     bx[at] = cx[s];
     by[at] = cy[s];
@@ -754,9 +811,9 @@ __global__ void __launch_bounds__(64 * kQueryWaves)
     }
 }
 
-__global__ void __launch_bounds__(1024) k_max_counts(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
-                                                     int n, int4* __restrict__ out) {
-    __shared__ int4 sh[2][1024];
+// Block-wide (1024 threads) maxima of the per-(query, wavefront) counts: out[0..1].
+__device__ __forceinline__ void max_counts_block(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
+                                                 int n, int4* __restrict__ out, int4 (*sh)[1024]) {
     const int t = threadIdx.x;
     int4 m = make_int4(0, 0, 0, 0), mx = make_int4(0, 0, 0, 0);
     for (int i = t; i < n; i += 1024) {
@@ -776,16 +833,16 @@ __global__ void __launch_bounds__(1024) k_max_counts(const int4* __restrict__ wm
         __syncthreads();
     }
     if (t < 2) out[t] = sh[t][0];
+    __syncthreads();
 }
 
-// Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).
-__global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ counts, int n,
-                                                      int4* __restrict__ offsets, int max_w) {
-    __shared__ int4 sh[1024];
+// Block-wide (1024 threads) exclusive scan of the per-query counts; totals at offsets[n].  .w
+// is summed, or with max_w maximised (its "offsets" are then running maxima).
+__device__ __forceinline__ void scan_counts_block(const int4* __restrict__ counts, int n, int4* __restrict__ offsets,
+                                                  int max_w, int4* sh) {
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int beg = t * per, end = min(n, beg + per);
-    // .w is summed, or with max_w maximised (its "offsets" are then running maxima)
     auto combine = [max_w](int4 a, int4 c) {
         return make_int4(a.x + c.x, a.y + c.y, a.z + c.z, max_w ? max(a.w, c.w) : a.w + c.w);
     };
@@ -805,6 +862,32 @@ __global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ c
         o = combine(o, counts[i]);
     }
     if (t == 1023) offsets[n] = sh[t];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_max_counts(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
+                                                     int n, int4* __restrict__ out) {
+    __shared__ int4 sh[2][1024];
+    max_counts_block(wmax, wmax_x, n, out, sh);
+}
+
+// Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).
+__global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ counts, int n,
+                                                      int4* __restrict__ offsets, int max_w) {
+    __shared__ int4 sh[1024];
+    scan_counts_block(counts, n, offsets, max_w, sh);
+}
+
+// The three reductions after the n-mer count pass in one launch: the per-wavefront maxima
+// (wmax may be null: skipped), the scan of counts and the scan of counts_x (.w maximised).
+__global__ void __launch_bounds__(1024) k_scan_all(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
+                                                   int4* __restrict__ maxw, const int4* __restrict__ counts,
+                                                   int4* __restrict__ offsets, const int4* __restrict__ counts_x,
+                                                   int4* __restrict__ offsets_x, int n) {
+    __shared__ int4 sh[2][1024];
+    if (wmax) max_counts_block(wmax, wmax_x, n, maxw, sh);
+    scan_counts_block(counts, n, offsets, 0, sh[0]);
+    scan_counts_block(counts_x, n, offsets_x, 1, sh[1]);
 }
 
 // Zero fills (graph-capturable replacements for memset nodes).

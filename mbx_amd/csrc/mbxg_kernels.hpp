@@ -55,9 +55,14 @@ __global__ void k_cloud(const double* xyz, const int* first_atom, const int* mem
                         int np, Box B, const Grid* grids, const int* pconf, const int* cell_base, double* cx,
                         double* cy, double* cz, int* cell, PtInfo* pinfo);
 __global__ void k_bin_count(const int* cell, int ns, int* cell_count);
-__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
-__global__ void k_bin_fill(const int* cell, int ns, const int* cell_start, int* cursor, const double* cx,
-                           const double* cy, const double* cz, int* cell_slot, double* bx, double* by, double* bz);
+// k_fix_monomers, k_cloud and k_bin_count in one launch of max(nmon, np) threads.
+__global__ void k_cloud_fix(const double* in, double* xyz, const int* first_atom, const int* nat, int nmon,
+                            const int* members, const int* tag, const int* kind, int np, Box B, const Grid* grids,
+                            const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell,
+                            PtInfo* pinfo, int* cell_count);
+__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start, int* cursor);
+__global__ void k_bin_fill(const int* cell, int ns, int* cursor, const double* cx, const double* cy,
+                           const double* cz, int* cell_slot, double* bx, double* by, double* bz);
 __global__ void k_scan_counts(const int4* counts, int n, int4* offsets, int max_w);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_int(int* p, size_t n);
@@ -115,6 +120,9 @@ struct NmersArgs {
 
 // out[0] / out[1] = componentwise maxima of wmax[0..n) / wmax_x[0..n) (one workgroup).
 __global__ void k_max_counts(const int4* wmax, const int4* wmax_x, int n, int4* out);
+// k_max_counts (skipped with wmax null) and the scans of counts and counts_x in one launch.
+__global__ void k_scan_all(const int4* wmax, const int4* wmax_x, int4* maxw, const int4* counts, int4* offsets,
+                           const int4* counts_x, int4* offsets_x, int n);
 
 // The staged n-mers of a one-pass build, moved to the evaluate lists: block q (striding),
 // wavefront w copies its family-f n-mers to evals[f] at the family's offset of query q plus the

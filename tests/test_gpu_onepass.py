@@ -91,3 +91,38 @@ def test_paired_polynomials_match_separate_launches():
     for e2, e3, g in ((a2, a3, ag), (b2, b3, bg)):
         assert e2 == pytest.approx(r["e2b"], rel=1e-10) and e3 == pytest.approx(r["e3b"], rel=1e-10)
         assert np.abs(g - r["grad"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("case", ["box", "droplet", "clathrate"])
+def test_fused_prep_matches_separate_launches(systems, case):
+    """FixMonomerCoordinates, the cloud and its binning in one single-workgroup launch
+    (fuse_prep = 1, the default for clouds of at most 65536 slots) against the separate launches:
+    the same working coordinates bit for bit, the same lists in the same order (each query's
+    matches are sorted, so the slots' order inside a cell does not matter) and so bit-identical
+    energies; gradients to rounding.  Eager and graph replays."""
+    from conftest import water_case
+
+    if case == "clathrate":
+        xyz, box = systems["clathrate/xyz"], systems["clathrate/box"]
+        kinds = [engine.H2O if m == "h2o" else engine.CH4 for m in systems["clathrate/mon"]]
+        c2, c3 = 9.0, 6.0
+    else:
+        xyz, box, c2, c3, _ = water_case(systems, "h2o_2048_cubic" if case == "box" else "h2o_2048_droplet")
+        kinds = [engine.H2O] * (len(xyz) // 9)
+    out = {}
+    for fuse in (1, 0):
+        print("fuse_prep", fuse, case, flush=True)
+        e = engine.Engine()
+        e.set_topology(kinds)
+        e.set_box(box)
+        e.set_cutoffs(c2, c3)
+        e.set_tuning("fuse_prep", fuse)
+        assert e.tuning("fuse_prep") == fuse
+        e.set_graph(True)
+        e.set_coords(xyz)
+        for _ in range(3):  # sizing compute, eager, graph replay
+            e.compute()
+        out[fuse] = _state(e) + (e.coords(),)
+    a, b = out[1], out[0]
+    _same(a[:6], b[:6])
+    assert np.array_equal(a[6], b[6])
