@@ -3,15 +3,18 @@
 //
 // Pipeline of mbxg_compute (all on the context's stream, no host synchronisation in steady
 // state):
-//   k_fix_monomers          FixMonomerCoordinates into the working copy (a copy in the gas phase)
-//   k_grid_gas | (upload)   candidate grid at the larger cutoff
-//   k_cloud, binning        cloud points with image slots, cells
-//   k_nmers pass 0          per-query counts {listed 2B, evaluated 2B, listed 3B, evaluated 3B}
-//   k_scan_counts           offsets; totals at offsets[np]
-//   k_nmers pass 1          lists written at the offsets, bounded by the buffer capacities
-//   launch_mbpol2b/3b       variables, polynomial, chain rule over the evaluate lists; n-mer
-//                           counts read on the device
-//   k_reduce_partials       deterministic energy / virial totals
+//   k_grid_gas              (gas phase) candidate grid at the larger cutoff
+//   k_cloud_fix             FixMonomerCoordinates into the working copy, cloud points with image
+//                           slots and their cells, cell counts; zeroes the gradient and records
+//   k_scan_cells, k_bin_fill  slots binned by cell
+//   k_nmers                 per-query counts and (one pass) each family's evaluated n-mers staged
+//                           per (query, wavefront); or count pass + emit pass when synchronised
+//   k_scan_all              offsets; totals at offsets[np]
+//   k_compact_evals         evaluate lists (one-pass builds)
+//   1B, launch_mbpol2b/3b   variables, polynomials, chain rule over the evaluate lists; n-mer
+//   / MB-nrg families       counts read on the device
+//   k_finish                deterministic energy / virial totals per family and per order
+//   run_xfam                (co2, halides, 4B) further families, synchronously
 // Capacities (list buffers, match storage) come from a synchronised first call and carry
 // headroom.  Every read of results synchronises and checks the totals against them; a call that
 // exceeded them is recomputed synchronously with grown buffers, so results are always whole.
@@ -821,7 +824,7 @@ int prepare_layout(mbxg_ctx* c, double cut) {
 // and stages the evaluated n-mers per (query, wavefront), the scan, and a compaction into the
 // evaluate lists (the listed lists are written later, if asked for, by mbxg_get_lists).
 int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double cmax, bool sync, bool need_lists,
-                hipEvent_t* ev) {
+                const ZeroJobs& Z, hipEvent_t* ev) {
     ListState& L = c->L;
     hipStream_t st = c->stream;
     const int np = c->np;
@@ -841,12 +844,13 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     if (!c->box.periodic)
         hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_in.p, c->d_first.p, c->d_members.p,
                            c->d_pt_off.p, L.d_cell_bound.p, cut, L.d_grids.p);
-    if (c->fuse_prep) {  // fix, cloud and cell counts in one launch
+    if (c->fuse_prep) {  // fix, cloud, cell counts and the zero fills Z in one launch (64-thread
+                         // blocks: the cell-count atomics spread over many CUs)
         const int nt = std::max(c->nmon, np);
-        hipLaunchKernelGGL(k_cloud_fix, dim3((nt + tb - 1) / tb), dim3(tb), 0, st, c->d_in.p, c->d_xyz.p,
-                           c->d_first.p, c->d_nat.p, c->nmon, c->d_members.p, c->d_tag.p, c->d_kind.p, np, c->box,
-                           L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.pinfo.p,
-                           L.cell_count.p);
+        hipLaunchKernelGGL(k_cloud_fix, dim3((nt + 63) / 64), dim3(64), 0, st, c->d_in.p, c->d_xyz.p, c->d_first.p,
+                           c->d_nat.p, c->nmon, c->d_members.p, c->d_tag.p, c->d_kind.p, np, c->box, L.d_grids.p,
+                           c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.pinfo.p,
+                           L.cell_count.p, Z);
     } else {
         if (c->nmon > 0)
             hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p,
@@ -1082,21 +1086,41 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                            c->d_first.p, c->d_nat.p, c->nmon, c->box);
         HIPCHK(hipGetLastError());
     }
+    // the gradient, the 1B record when no 1B term runs and (batches) the family records start at
+    // zero: zeroed by the list build's first kernel, or by one launch of their own
+    c->one_body_last = (terms & MBXG_1B) && !(terms & MBXG_LISTS_ONLY);
+    const bool batch = c->nconf > 1;
+    ZeroJobs Z{};
+    if (c->natoms > 0) {
+        Z.p[0] = c->d_grad.p;
+        Z.n[0] = 3 * (size_t)c->natoms;
+    }
+    if (!c->one_body_last) {
+        Z.p[1] = c->d_totals.p + 2 * kPartials;
+        Z.n[1] = kPartials;
+    }
+    if (batch) {
+        Z.p[2] = c->d_fam_tot.p;
+        Z.n[2] = (size_t)kFamilies * kPartials;
+        Z.p[3] = c->d_conf_fam.p;
+        Z.n[3] = (size_t)kFamilies * kPartials * c->nconf;
+    }
+    if (!(L.built && c->fuse_prep)) {
+        hipLaunchKernelGGL(k_zero_multi, dim3(64), dim3(256), 0, st, Z);
+        HIPCHK(hipGetLastError());
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     if (L.built) {
         const double cmax = std::max(std::max(want2 ? c->cut2 : 0.0, want3 ? c->cut3 : 0.0), want4 ? c->cut4 : 0.0);
-        if (int r = build_lists(c, N, c2sq, c3sq, cmax, sync, (terms & MBXG_LISTS_ONLY) != 0 || xrun, ev))
+        if (int r = build_lists(c, N, c2sq, c3sq, cmax, sync, (terms & MBXG_LISTS_ONLY) != 0 || xrun, Z, ev))
             return r;
     } else {
         L.tot = make_int4(0, 0, 0, 0);
         L.totx = make_int4(0, 0, 0, 0);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    if (c->natoms > 0) launch_zero(c->d_grad.p, 3 * (size_t)c->natoms, st);
-    launch_zero(c->d_fam_tot.p, (size_t)kFamilies * kPartials, st);
     // 1B (System::Get1B, system.cpp:2269-2401): every local monomer's gradient is written at its own
     // atoms of the zeroed gradient before any 2B/3B kernel adds to it (they start after the fork)
-    c->one_body_last = (terms & MBXG_1B) && !(terms & MBXG_LISTS_ONLY);
     if (c->one_body_last) {
         const int nw = (int)c->water_local.size(), nc = (int)c->ch4_local.size();
         const int no = (int)c->co2_local.size(), n1 = nw + nc + no;
@@ -1108,18 +1132,15 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
         HIPCHK(launch_1b_co2(c->d_xyz.p, c->d_first.p, c->d_w1b.p + nw + nc, no, c->d_coef1b_co2.p,
                              c->d_e1b.p + nw + nc, g, v ? v + nw + nc : nullptr, n1, st));
         hipLaunchKernelGGL(k_reduce_1b, dim3(1), dim3(256), 0, st, c->d_e1b.p, v, n1, n1, c->d_totals.p + 2 * kPartials);
-    } else {
-        launch_zero(c->d_totals.p + 2 * kPartials, kPartials, st);
     }
-    if (c->nconf > 1) launch_zero(c->d_conf_fam.p, (size_t)kFamilies * kPartials * c->nconf, st);
     HIPCHK(hipGetLastError());
     c->evaluated = L.built && !(terms & MBXG_LISTS_ONLY);
+    FinishJobs J{};  // single configuration: the families' records, reduced after the join
     if (c->evaluated) {
         // the 2B families run on stream2, concurrently with the 3B ones: their small blocks
         // fill the machine while the last 3B blocks drain
         HIPCHK(hipEventRecord(c->fork, st));
         HIPCHK(hipStreamWaitEvent(c->stream2, c->fork, 0));
-        const bool batch = c->nconf > 1;
         const int order_of[kFamilies] = {FAM_MBPOL_3B, FAM_CH4H2O2_3B, FAM_MBPOL_2B, FAM_CH4H2O_2B, FAM_CH4CH4_2B};
         auto active = [&](int f) {
             return (family_order(f) == 2 ? want2 : want3) && family_launched(c, f) && fam_cap(L, f) > 0;
@@ -1143,6 +1164,12 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
             const int ld = (cap + 63) / 64 * 64;
             EvalState& o = c->fam[f];
             const int* nd = fam_ndev(L, c->np, f);
+            if (!batch) {
+                J.partials[f] = o.partials.p;
+                J.n_dev[f] = nd;
+                J.cap[f] = cap;
+                return;
+            }
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, sk, o.partials.p, nd, cap,
                                c->d_fam_tot.p + kPartials * f);
             if (batch) {
@@ -1170,6 +1197,8 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                 const int f2 = FAM_MBPOL_2B, cap2 = fam_cap(L, f2);
                 EvalState& o2 = c->fam[f2];
                 const int* nd2 = fam_ndev(L, c->np, f2);
+                // all on the main stream (the 2B stages on stream2 beside the 3B ones measured
+                // slower: 0.641-0.646 against 0.636-0.637 ms a step at 2048 waters)
                 if (ev) HIPCHK(hipEventRecord(ev[4], st));
                 HIPCHK(launch_mbpol2b_vars(L.evals[f2].p, nd2, cap2, N, o2.ws.p, st));
                 if (ev) HIPCHK(hipEventRecord(ev[5], st));
@@ -1211,8 +1240,12 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
         HIPCHK(hipStreamWaitEvent(st, c->join, 0));
     }
     // per-order totals: 2B = MB-pol + CH4-H2O + CH4-CH4, 3B = MB-pol + CH4-H2O-H2O (fixed order)
-    hipLaunchKernelGGL(k_combine_families, dim3(1), dim3(64), 0, st, c->d_fam_tot.p, 1, c->d_totals.p);
-    launch_zero(c->d_totals.p + 3 * kPartials, kPartials, st);
+    if (!batch) {
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishThreads), 0, st, J, c->d_fam_tot.p, c->d_totals.p);
+    } else {
+        hipLaunchKernelGGL(k_combine_families, dim3(1), dim3(64), 0, st, c->d_fam_tot.p, 1, c->d_totals.p);
+        launch_zero(c->d_totals.p + 3 * kPartials, kPartials, st);
+    }
     c->x_last = xrun;
     if (xrun)  // then the further families' totals on top, and the 4B record
         if (int r = run_xfam(c, N, want2, want3, want4, !(terms & MBXG_LISTS_ONLY), do_grads)) return r;

@@ -241,6 +241,17 @@ __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ 
                 });
 }
 
+__device__ __forceinline__ void zero_jobs(const ZeroJobs& Z, size_t tid, size_t nthreads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (Z.p[k])
+            for (size_t j = tid; j < Z.n[k]; j += nthreads) Z.p[k][j] = 0.0;
+}
+
+__global__ void k_zero_multi(ZeroJobs Z) {
+    zero_jobs(Z, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
+
 // FixMonomerCoordinates, the cloud and the cell counts in one launch (one thread per monomer and
 // cloud point): thread i fixes monomer i (into xyz) and emits point i's slots (as k_cloud) with
 // its own copy of the point's fixed first atom, so no thread reads another's output; each slot
@@ -251,8 +262,10 @@ __global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ 
                             const int* __restrict__ kind, int np, Box B, const Grid* __restrict__ grids,
                             const int* __restrict__ pconf, const int* __restrict__ cell_base,
                             double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
-                            int* __restrict__ cell, PtInfo* __restrict__ pinfo, int* __restrict__ cell_count) {
+                            int* __restrict__ cell, PtInfo* __restrict__ pinfo, int* __restrict__ cell_count,
+                            ZeroJobs Z) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    zero_jobs(Z, i, (size_t)gridDim.x * blockDim.x);
     if (i < nmon) fix_monomer(i, in, xyz, first_atom, nat, B);
     if (i >= np) return;
     const int mp = members[i];
@@ -757,14 +770,17 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 // counts_x[q] = {evaluated CH4-H2O 2B, CH4-CH4 2B, CH4-H2O-H2O 3B, 0};
 // pass 1: writes at offsets[q] / offsets_x[q], bounded by cap / capx (same orders).
 // counts_x[q].w receives query q's match count (the scan maximises it: the host compares it with
-// match_cap).  Blocks stride over the queries; with gscratch == nullptr the match storage is
-// the dynamic LDS, otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.
+// match_cap).  Blocks stride over the queries; with kLds the match storage is the dynamic LDS,
+// otherwise block b uses gscratch + b * match_cap * kMatchDoubles doubles.  The two storages are
+// separate instantiations so that the LDS one addresses its storage as LDS (ds_* instructions)
+// rather than through generic pointers.
+template <bool kLds>
 __global__ void __launch_bounds__(64 * kQueryWaves)
     k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap, double* __restrict__ gscratch) {
     extern __shared__ double smem[];
     __shared__ int sh_count;
     __shared__ int4 sh_w[2 * kQueryWaves];
-    double* store = gscratch ? gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles : smem;
+    double* store = kLds ? smem : gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles;
     const Lds L = carve(store, match_cap);
     for (int q = blockIdx.x; q < A.np; q += gridDim.x) {
         const int conf = A.pconf[q];
@@ -811,83 +827,94 @@ __global__ void __launch_bounds__(64 * kQueryWaves)
     }
 }
 
-// Block-wide (1024 threads) maxima of the per-(query, wavefront) counts: out[0..1].
-__device__ __forceinline__ void max_counts_block(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
-                                                 int n, int4* __restrict__ out, int4 (*sh)[1024]) {
-    const int t = threadIdx.x;
-    int4 m = make_int4(0, 0, 0, 0), mx = make_int4(0, 0, 0, 0);
-    for (int i = t; i < n; i += 1024) {
-        const int4 c = wmax[i], cx = wmax_x[i];
-        m = make_int4(max(m.x, c.x), max(m.y, c.y), max(m.z, c.z), max(m.w, c.w));
-        mx = make_int4(max(mx.x, cx.x), max(mx.y, cx.y), max(mx.z, cx.z), max(mx.w, cx.w));
-    }
-    sh[0][t] = m;
-    sh[1][t] = mx;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s)
-            for (int k = 0; k < 2; ++k) {
-                const int4 a = sh[k][t], b = sh[k][t + s];
-                sh[k][t] = make_int4(max(a.x, b.x), max(a.y, b.y), max(a.z, b.z), max(a.w, b.w));
-            }
-        __syncthreads();
-    }
-    if (t < 2) out[t] = sh[t][0];
-    __syncthreads();
+// The three reductions after the n-mer count pass in one launch: the per-wavefront maxima
+// (wmax may be null: skipped), the exclusive scan of counts and the scan of counts_x (.w
+// maximised: running maxima, the total at n the largest).  Each thread takes a run of
+// consecutive entries; runs are scanned within each wavefront by lane shuffles and across the
+// block's 16 wavefronts through LDS.
+namespace {
+
+__device__ __forceinline__ int4 combine4(int4 a, int4 b, bool max_w) {
+    return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, max_w ? max(a.w, b.w) : a.w + b.w);
 }
 
-// Block-wide (1024 threads) exclusive scan of the per-query counts; totals at offsets[n].  .w
-// is summed, or with max_w maximised (its "offsets" are then running maxima).
-__device__ __forceinline__ void scan_counts_block(const int4* __restrict__ counts, int n, int4* __restrict__ offsets,
-                                                  int max_w, int4* sh) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ int4 shfl_up4(int4 v, int o) {
+    return make_int4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64), __shfl_up(v.w, o, 64));
+}
+
+// Block-wide exclusive scan (1024 threads) of counts[0..n) into offsets[0..n], offsets[n] the total.
+__device__ __forceinline__ void scan_block_fast(const int4* __restrict__ counts, int n, int4* __restrict__ offsets,
+                                                bool max_w, int4* wsum) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int per = (n + 1023) / 1024;
-    const int beg = t * per, end = min(n, beg + per);
-    auto combine = [max_w](int4 a, int4 c) {
-        return make_int4(a.x + c.x, a.y + c.y, a.z + c.z, max_w ? max(a.w, c.w) : a.w + c.w);
-    };
-    int4 acc = make_int4(0, 0, 0, 0);
-    for (int i = beg; i < end; ++i) acc = combine(acc, counts[i]);
-    sh[t] = acc;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int4 v = t >= off ? sh[t - off] : make_int4(0, 0, 0, 0);
-        __syncthreads();
-        sh[t] = combine(sh[t], v);
-        __syncthreads();
+    const int beg = min(n, t * per), end = min(n, beg + per);
+    const int4 zero = make_int4(0, 0, 0, 0);
+    int4 acc = zero;
+    for (int i = beg; i < end; ++i) acc = combine4(acc, counts[i], max_w);
+    int4 incl = acc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int4 y = shfl_up4(incl, o);
+        if (lane >= o) incl = combine4(y, incl, max_w);
     }
-    int4 o = t > 0 ? sh[t - 1] : make_int4(0, 0, 0, 0);
+    int4 excl = shfl_up4(incl, 1);
+    if (lane == 0) excl = zero;
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int4 base = zero;
+    for (int w = 0; w < wv; ++w) base = combine4(base, wsum[w], max_w);
+    int4 o = combine4(base, excl, max_w);
     for (int i = beg; i < end; ++i) {
         offsets[i] = o;
-        o = combine(o, counts[i]);
+        o = combine4(o, counts[i], max_w);
     }
-    if (t == 1023) offsets[n] = sh[t];
+    if (t == 1023) {
+        int4 tot = zero;
+        for (int w = 0; w < 16; ++w) tot = combine4(tot, wsum[w], max_w);
+        offsets[n] = tot;
+    }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_max_counts(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
-                                                     int n, int4* __restrict__ out) {
-    __shared__ int4 sh[2][1024];
-    max_counts_block(wmax, wmax_x, n, out, sh);
-}
+}  // namespace
 
-// Exclusive scan of the per-query counts (single workgroup; totals written at offsets[n]).
-__global__ void __launch_bounds__(1024) k_scan_counts(const int4* __restrict__ counts, int n,
-                                                      int4* __restrict__ offsets, int max_w) {
-    __shared__ int4 sh[1024];
-    scan_counts_block(counts, n, offsets, max_w, sh);
-}
-
-// The three reductions after the n-mer count pass in one launch: the per-wavefront maxima
-// (wmax may be null: skipped), the scan of counts and the scan of counts_x (.w maximised).
 __global__ void __launch_bounds__(1024) k_scan_all(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
                                                    int4* __restrict__ maxw, const int4* __restrict__ counts,
                                                    int4* __restrict__ offsets, const int4* __restrict__ counts_x,
                                                    int4* __restrict__ offsets_x, int n) {
-    __shared__ int4 sh[2][1024];
-    if (wmax) max_counts_block(wmax, wmax_x, n, maxw, sh);
-    scan_counts_block(counts, n, offsets, 0, sh[0]);
-    scan_counts_block(counts_x, n, offsets_x, 1, sh[1]);
+    __shared__ int4 wsum[2][16];
+    __shared__ int4 msum[2][16];
+    scan_block_fast(counts, n, offsets, false, wsum[0]);
+    scan_block_fast(counts_x, n, offsets_x, true, wsum[1]);
+    if (wmax) {
+        const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        int4 m = make_int4(0, 0, 0, 0), mx = make_int4(0, 0, 0, 0);
+        for (int i = t; i < n; i += 1024) {
+            const int4 c = wmax[i], cx = wmax_x[i];
+            m = make_int4(max(m.x, c.x), max(m.y, c.y), max(m.z, c.z), max(m.w, c.w));
+            mx = make_int4(max(mx.x, cx.x), max(mx.y, cx.y), max(mx.z, cx.z), max(mx.w, cx.w));
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            m = make_int4(max(m.x, __shfl_xor(m.x, o, 64)), max(m.y, __shfl_xor(m.y, o, 64)),
+                          max(m.z, __shfl_xor(m.z, o, 64)), max(m.w, __shfl_xor(m.w, o, 64)));
+            mx = make_int4(max(mx.x, __shfl_xor(mx.x, o, 64)), max(mx.y, __shfl_xor(mx.y, o, 64)),
+                           max(mx.z, __shfl_xor(mx.z, o, 64)), max(mx.w, __shfl_xor(mx.w, o, 64)));
+        }
+        if (lane == 0) {
+            msum[0][wv] = m;
+            msum[1][wv] = mx;
+        }
+        __syncthreads();
+        if (t < 2) {
+            int4 r = msum[t][0];
+            for (int w = 1; w < 16; ++w) {
+                const int4 c = msum[t][w];
+                r = make_int4(max(r.x, c.x), max(r.y, c.y), max(r.z, c.z), max(r.w, c.w));
+            }
+            maxw[t] = r;
+        }
+    }
 }
 
 // Zero fills (graph-capturable replacements for memset nodes).
@@ -967,6 +994,41 @@ __global__ void k_reduce_slices(const double* __restrict__ part, int nconf, int 
 // Per-order records from per-family ones (kPartials doubles per record, nconf configurations):
 // out[0][c] = fam[MB-pol 2B][c] + fam[CH4-H2O 2B][c] + fam[CH4-CH4 2B][c],
 // out[1][c] = fam[MB-pol 3B][c] + fam[CH4-H2O-H2O 3B][c].
+__global__ void __launch_bounds__(kFinishThreads) k_finish(FinishJobs J, double* __restrict__ fam, double* __restrict__ totals) {
+    __shared__ double sh[kPartials][kFinishThreads];
+    __shared__ double res[kFamilies][kPartials];
+    const int t = threadIdx.x;
+    for (int f = 0; f < kFamilies; ++f) {
+        if (!J.partials[f]) {  // uniform: the family was not evaluated
+            if (t < kPartials) res[f][t] = 0.0;
+            continue;
+        }
+        int n = *J.n_dev[f];
+        n = n < J.cap[f] ? n : J.cap[f];
+        const int nb = (n + 63) / 64;
+        double acc[kPartials];
+        for (int q = 0; q < kPartials; ++q) acc[q] = 0.0;
+        for (int b = t; b < nb; b += kFinishThreads)
+            for (int q = 0; q < kPartials; ++q) acc[q] += J.partials[f][(size_t)b * kPartials + q];
+        for (int q = 0; q < kPartials; ++q) sh[q][t] = acc[q];
+        __syncthreads();
+        for (int s = kFinishThreads / 2; s > 0; s >>= 1) {
+            if (t < s)
+                for (int q = 0; q < kPartials; ++q) sh[q][t] += sh[q][t + s];
+            __syncthreads();
+        }
+        if (t < kPartials) res[f][t] = sh[t][0];
+        __syncthreads();
+    }
+    __syncthreads();
+    if (t < kPartials) {
+        for (int f = 0; f < kFamilies; ++f) fam[f * kPartials + t] = res[f][t];
+        totals[t] = res[FAM_MBPOL_2B][t] + res[FAM_CH4H2O_2B][t] + res[FAM_CH4CH4_2B][t];
+        totals[kPartials + t] = res[FAM_MBPOL_3B][t] + res[FAM_CH4H2O2_3B][t];
+        totals[3 * kPartials + t] = 0.0;
+    }
+}
+
 __global__ void k_combine_families(const double* __restrict__ fam, int nconf, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nconf * kPartials) return;
@@ -982,7 +1044,12 @@ hipError_t launch_nmers(const NmersArgs& A, const NmerCtx& N, int pass, int4 cap
     if (A.np <= 0) return hipSuccess;
     const int nb = gscratch ? (nblocks < A.np ? nblocks : A.np) : A.np;
     const size_t lds = gscratch ? 0 : nmers_lds_bytes(match_cap);
-    hipLaunchKernelGGL(k_nmers, dim3(nb), dim3(64 * kQueryWaves), lds, st, A, N, pass, cap, capx, match_cap, gscratch);
+    if (gscratch)
+        hipLaunchKernelGGL(k_nmers<false>, dim3(nb), dim3(64 * kQueryWaves), 0, st, A, N, pass, cap, capx, match_cap,
+                           gscratch);
+    else
+        hipLaunchKernelGGL(k_nmers<true>, dim3(nb), dim3(64 * kQueryWaves), lds, st, A, N, pass, cap, capx,
+                           match_cap, gscratch);
     return hipGetLastError();
 }
 

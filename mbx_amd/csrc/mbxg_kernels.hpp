@@ -55,15 +55,22 @@ __global__ void k_cloud(const double* xyz, const int* first_atom, const int* mem
                         int np, Box B, const Grid* grids, const int* pconf, const int* cell_base, double* cx,
                         double* cy, double* cz, int* cell, PtInfo* pinfo);
 __global__ void k_bin_count(const int* cell, int ns, int* cell_count);
-// k_fix_monomers, k_cloud and k_bin_count in one launch of max(nmon, np) threads.
+// Up to four device ranges to zero in one launch (null pointers skipped).
+struct ZeroJobs {
+    double* p[4];
+    size_t n[4];
+};
+// k_zero over every range of Z, grid-stride.
+__global__ void k_zero_multi(ZeroJobs Z);
+// k_fix_monomers, k_cloud and k_bin_count in one launch of max(nmon, np) threads, which also
+// zero the ranges of Z.
 __global__ void k_cloud_fix(const double* in, double* xyz, const int* first_atom, const int* nat, int nmon,
                             const int* members, const int* tag, const int* kind, int np, Box B, const Grid* grids,
                             const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell,
-                            PtInfo* pinfo, int* cell_count);
+                            PtInfo* pinfo, int* cell_count, ZeroJobs Z);
 __global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start, int* cursor);
 __global__ void k_bin_fill(const int* cell, int ns, int* cursor, const double* cx, const double* cy,
                            const double* cz, int* cell_slot, double* bx, double* by, double* bz);
-__global__ void k_scan_counts(const int4* counts, int n, int4* offsets, int max_w);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_int(int* p, size_t n);
 
@@ -119,14 +126,26 @@ struct NmersArgs {
 };
 
 // out[0] / out[1] = componentwise maxima of wmax[0..n) / wmax_x[0..n) (one workgroup).
-__global__ void k_max_counts(const int4* wmax, const int4* wmax_x, int n, int4* out);
-// k_max_counts (skipped with wmax null) and the scans of counts and counts_x in one launch.
+// The per-(query, wavefront) maxima of wmax / wmax_x into maxw[0..1] (skipped with wmax null) and
+// the exclusive scans of counts and counts_x (.w maximised), totals at n, in one launch.
 __global__ void k_scan_all(const int4* wmax, const int4* wmax_x, int4* maxw, const int4* counts, int4* offsets,
                            const int4* counts_x, int4* offsets_x, int n);
 
 // The staged n-mers of a one-pass build, moved to the evaluate lists: block q (striding),
 // wavefront w copies its family-f n-mers to evals[f] at the family's offset of query q plus the
 // wavefront's prefix, bounded by fcap[f] (the list capacity in n-mers) and scap[f].
+// The per-family energy/virial records of one compute (null partials: family not evaluated).
+struct FinishJobs {
+    const double* partials[kFamilies];
+    const int* n_dev[kFamilies];
+    int cap[kFamilies];
+};
+// One workgroup of kFinishThreads: each family's partial records summed in a fixed order into fam
+// (kPartials per family, zeros for the families not evaluated), then the per-order records of
+// totals as k_combine_families does (2B, 3B) and the 4B record zeroed.
+constexpr int kFinishThreads = 512;
+__global__ void k_finish(FinishJobs J, double* fam, double* totals);
+
 struct CompactArgs {
     const int* stage[kFamilies];
     int scap[kFamilies];
