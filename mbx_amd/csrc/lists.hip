@@ -515,8 +515,17 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
     // per-match data are written
     for (int a = tid; a < nm; a += blockDim.x) {
         const int key = L.match[a];
-        int rank = 0;
-        for (int b = 0; b < nm; ++b) rank += L.match[b] < key;
+        int rank = 0, b = 0;
+        // 16 keys a step, four 16-byte reads in flight (L.match is 16-byte aligned: the match
+        // storage holds a multiple of 64 matches)
+        for (; b + 16 <= nm; b += 16) {
+            const int4 k0 = *(const int4*)(L.match + b), k1 = *(const int4*)(L.match + b + 4);
+            const int4 k2 = *(const int4*)(L.match + b + 8), k3 = *(const int4*)(L.match + b + 12);
+            rank += (k0.x < key) + (k0.y < key) + (k0.z < key) + (k0.w < key) + (k1.x < key) + (k1.y < key) +
+                    (k1.z < key) + (k1.w < key) + (k2.x < key) + (k2.y < key) + (k2.z < key) + (k2.w < key) +
+                    (k3.x < key) + (k3.y < key) + (k3.z < key) + (k3.w < key);
+        }
+        for (; b < nm; ++b) rank += L.match[b] < key;
         L.mon[rank] = key;
     }
     __syncthreads();
@@ -777,7 +786,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 template <bool kLds>
 __global__ void __launch_bounds__(64 * kQueryWaves)
     k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap, double* __restrict__ gscratch) {
-    extern __shared__ double smem[];
+    extern __shared__ __attribute__((aligned(16))) double smem[];
     __shared__ int sh_count;
     __shared__ int4 sh_w[2 * kQueryWaves];
     double* store = kLds ? smem : gscratch + (size_t)blockIdx.x * match_cap * kMatchDoubles;
