@@ -126,7 +126,7 @@ struct EvalState {
 };
 
 // Largest match storage kept in LDS (80 bytes per match): 512 matches = 40 KB per block.
-constexpr int kMaxLdsMatches = 512;
+constexpr int kMaxLdsMatches = 384;  // 160 bytes of LDS a match: at most 60 KB a block
 constexpr int kScratchBlocks = 2048;
 // slices per configuration in the per-configuration reductions (blocks = configurations x slices)
 constexpr int kConfSlices = 16;

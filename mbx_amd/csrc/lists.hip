@@ -357,10 +357,13 @@ struct Lds {
     double* iy;
     double* iz;
     double* r;  // |query first atom - imaged partner first atom|
+    double* p3;  // the matches inside the 3B cutoff packed in m3 order, kP3Doubles each: sx sy sz
+                 // ix iy iz r, then int4 {mon, tag, kind, 0} at double 8 (five 16-byte reads)
 };
 
-// 7 doubles + 6 ints per match.
-constexpr int kMatchDoubles = 10;
+// 7 doubles + 6 ints per match, and one packed 3B entry of 10 doubles.
+constexpr int kP3Doubles = 10;
+constexpr int kMatchDoubles = 10 + kP3Doubles;
 
 __device__ __forceinline__ Lds carve(double* d, int cap) {
     Lds L;
@@ -378,6 +381,7 @@ __device__ __forceinline__ Lds carve(double* d, int cap) {
     L.kind = i + 3 * cap;
     L.flags = i + 4 * cap;
     L.m3 = i + 5 * cap;
+    L.p3 = d + 10 * cap;  // 16-byte aligned: cap is a multiple of 64
     return L;
 }
 
@@ -582,6 +586,17 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
     }
     __syncthreads();
     const int n3 = *sh_count;
+    // the trimer loop reads each 3B match's data as one packed entry
+    for (int u = tid; u < n3; u += blockDim.x) {
+        const int a = L.m3[u];
+        double* e = L.p3 + (size_t)kP3Doubles * u;
+        *(double2*)e = make_double2(L.sx[a], L.sy[a]);
+        *(double2*)(e + 2) = make_double2(L.sz[a], L.ix[a]);
+        *(double2*)(e + 4) = make_double2(L.iy[a], L.iz[a]);
+        *(double2*)(e + 6) = make_double2(L.r[a], 0.0);
+        *(int4*)(e + 8) = make_int4(L.mon[a], L.tag[a], L.kind[a], 0);
+    }
+    __syncthreads();
 
     const unsigned long long below = (1ull << lane) - 1ull;
     int4 at = make_int4(0, 0, 0, 0), atx = make_int4(0, 0, 0, 0);
@@ -682,22 +697,26 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
                 while (u < n3 - 2 && (u + 1) * (2 * n3 - u - 2) / 2 <= t) ++u;
                 const int v = t - u * (2 * n3 - u - 1) / 2 + u + 1;
                 {
-                    const int ja = L.m3[u], ka = L.m3[v];
-                    mj = L.mon[ja];
-                    mk = L.mon[ka];
+                    const double* ej = L.p3 + (size_t)kP3Doubles * u;
+                    const double* ek = L.p3 + (size_t)kP3Doubles * v;
+                    const int4 ij = *(const int4*)(ej + 8), ik = *(const int4*)(ek + 8);
+                    mj = ij.x;
+                    mk = ik.x;
                     if (mj != mi && mk != mi && mj != mk) {
-                        const double djk = dist2(L.sx[ja], L.sy[ja], L.sz[ja], L.sx[ka], L.sy[ka], L.sz[ka]);
-                        listed = djk > c3sq || (ti < L.tag[ja] && ti < L.tag[ka]);
-                        const int nch4 = (ki == MON_CH4) + (L.kind[ja] == MON_CH4) + (L.kind[ka] == MON_CH4);
-                        const int nh2o = (ki == MON_H2O) + (L.kind[ja] == MON_H2O) + (L.kind[ka] == MON_H2O);
+                        const double2 j01 = *(const double2*)ej, k01 = *(const double2*)ek;
+                        const double2 j23 = *(const double2*)(ej + 2), k23 = *(const double2*)(ek + 2);
+                        const double djk = dist2(j01.x, j01.y, j23.x, k01.x, k01.y, k23.x);
+                        listed = djk > c3sq || (ti < ij.y && ti < ik.y);
+                        const int nch4 = (ki == MON_CH4) + (ij.z == MON_CH4) + (ik.z == MON_CH4);
+                        const int nh2o = (ki == MON_H2O) + (ij.z == MON_H2O) + (ik.z == MON_H2O);
                         if (listed && (nh2o == 3 || (nh2o == 2 && nch4 == 1))) {
-                            const double e0 = L.ix[ja] - L.ix[ka], e1 = L.iy[ja] - L.iy[ka],
-                                         e2 = L.iz[ja] - L.iz[ka];
+                            const double2 j45 = *(const double2*)(ej + 4), k45 = *(const double2*)(ek + 4);
+                            const double e0 = j23.y - k23.y, e1 = j45.x - k45.x, e2 = j45.y - k45.y;
                             double rr = 0.0;
                             rr += e0 * e0;
                             rr += e1 * e1;
                             rr += e2 * e2;
-                            const double rab = L.r[ja], rac = L.r[ka], rbc = sqrt(rr);
+                            const double rab = ej[6], rac = ek[6], rbc = sqrt(rr);
                             if (nh2o == 3) {
                                 const double f = p3::r3f;
                                 if (!((rab > f && rac > f) || (rab > f && rbc > f) || (rac > f && rbc > f)))
@@ -784,7 +803,7 @@ __device__ void nmers_query(int q, const PtInfo* __restrict__ pinfo, const doubl
 // separate instantiations so that the LDS one addresses its storage as LDS (ds_* instructions)
 // rather than through generic pointers.
 template <bool kLds>
-__global__ void __launch_bounds__(64 * kQueryWaves)
+__global__ void __launch_bounds__(64 * kQueryWaves) __attribute__((amdgpu_waves_per_eu(5)))
     k_nmers(NmersArgs A, NmerCtx N, int pass, int4 cap, int4 capx, int match_cap, double* __restrict__ gscratch) {
     extern __shared__ __attribute__((aligned(16))) double smem[];
     __shared__ int sh_count;
