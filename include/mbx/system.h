@@ -5,9 +5,9 @@
 // same names, by-value arguments and return values, the same internal monomer ordering
 // (types from fewest to most monomers, input order within a type; sys_tools.cpp:44-131), the
 // same site layout (systools::SetUpMonomers, sys_tools.cpp:133-255: h2o 4 sites for 3 atoms,
-// ch4 5/5, co2 3/3, ...) and CUException errors.  Energy() returns the 1B + 2B + 3B terms; the
-// dispersion, electrostatics, 4B and other terms of the reference's Energy (system.cpp:1995-2138)
-// are outside this engine and contribute nothing here.
+// ch4 5/5, co2 3/3, ...) and CUException errors.  Energy() returns the 1B + 2B + 3B terms, plus
+// the 4B term when a 4B cutoff is set; the dispersion, Buckingham and electrostatics terms of the
+// reference's Energy (system.cpp:1995-2138) are outside this engine and contribute nothing here.
 //
 // A System is copyable, as the reference's is (tools::ReadNrg fills a std::vector<System>).
 // Its device context (one HIP stream and the device buffers) is created on the first call that
