@@ -29,8 +29,12 @@ extern "C" {
 
 typedef struct mbxg_ctx mbxg_ctx;
 
-/* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", "co2", "f-", "cl-", "br-", "i-"). */
-enum { MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2, MBXG_CO2 = 3, MBXG_F = 4, MBXG_CL = 5, MBXG_BR = 6, MBXG_I = 7 };
+/* Monomer kinds (the reference keys monomers by name: "h2o", "ch4", "co2", "f-", "cl-", "br-", "i-",
+ * "li+", "na+", "k+", "rb+", "cs+"). */
+enum {
+    MBXG_H2O = 0, MBXG_CH4 = 1, MBXG_OTHER = 2, MBXG_CO2 = 3, MBXG_F = 4, MBXG_CL = 5, MBXG_BR = 6, MBXG_I = 7,
+    MBXG_LI = 8, MBXG_NA = 9, MBXG_K = 10, MBXG_RB = 11, MBXG_CS = 12
+};
 /* Terms for mbxg_compute; MBXG_LISTS_ONLY builds the requested lists without evaluating. */
 enum { MBXG_2B = 1, MBXG_3B = 2, MBXG_LISTS_ONLY = 4, MBXG_1B = 8, MBXG_4B = 16 };
 /* Status codes. */
@@ -45,7 +49,8 @@ int mbxg_destroy(mbxg_ctx* ctx);
 
 /* Topology, in the engine's (internal) monomer order; monomer i's atoms follow monomer i-1's.
  * kind[i]: MBXG_H2O (3 atoms O H H), MBXG_CH4 (5 atoms C H H H H), MBXG_CO2 (3 atoms C O O), the
- * halide ions MBXG_F / MBXG_CL / MBXG_BR / MBXG_I (1 atom) or MBXG_OTHER (nat[i] atoms, listed but
+ * halide ions MBXG_F / MBXG_CL / MBXG_BR / MBXG_I and the alkali-metal ions MBXG_LI / MBXG_NA /
+ * MBXG_K / MBXG_RB / MBXG_CS (1 atom each) or MBXG_OTHER (nat[i] atoms, listed but
  * without a holder here: only MBXG_LISTS_ONLY computes accept it).  tag[i] orders monomers
  * for n-mer ownership (the reference uses the first atom's tag, sys_tools.cpp:558-592).
  * islocal may be NULL (all local).  Replaces System::AddMonomer/Initialize bookkeeping
@@ -75,7 +80,13 @@ enum {
     MBXG_FAM_F_H2O2_3B = 1 << 10, MBXG_FAM_CL_H2O2_3B = 1 << 11, MBXG_FAM_BR_H2O2_3B = 1 << 12,
     MBXG_FAM_I_H2O2_3B = 1 << 13,
     MBXG_FAM_CO2X3_3B = 1 << 14, /* co2 x3, MB-nrg A1B2_A1B2_A1B2 deg4 (energy3b.cpp:247-249) */
-    MBXG_FAM_H2OX4_4B = 1 << 15  /* h2o x4, MB-nrg A1B2_A1B2_A1B2_A1B2 deg4 (energy4b.cpp:218-221) */
+    MBXG_FAM_H2OX4_4B = 1 << 15, /* h2o x4, MB-nrg A1B2_A1B2_A1B2_A1B2 deg4 (energy4b.cpp:218-221) */
+    /* alkali-metal-ion-water 2B, x2b_h2o_ion_v2x (energy2b.cpp:258-282) */
+    MBXG_FAM_LI_H2O_2B = 1 << 16, MBXG_FAM_NA_H2O_2B = 1 << 17, MBXG_FAM_K_H2O_2B = 1 << 18,
+    MBXG_FAM_RB_H2O_2B = 1 << 19, MBXG_FAM_CS_H2O_2B = 1 << 20,
+    /* alkali-metal-ion-water-water 3B, x3b_h2o_ion_v1x_deg4_filtered (energy3b.cpp:223-228) */
+    MBXG_FAM_LI_H2O2_3B = 1 << 21, MBXG_FAM_NA_H2O2_3B = 1 << 22, MBXG_FAM_K_H2O2_3B = 1 << 23,
+    MBXG_FAM_RB_H2O2_3B = 1 << 24, MBXG_FAM_CS_H2O2_3B = 1 << 25
 };
 int mbxg_set_families(mbxg_ctx* ctx, int mask);
 
@@ -107,13 +118,13 @@ int mbxg_get_energies(mbxg_ctx* ctx, double* e1b, double* e2b, double* e3b);
  * cutoff of the tetramer search (System::Set4bCutoff, system.h:842; 0, the default, disables the
  * term).  Tetramers come from AddClusters with nmax = 4 (bblock/sys_tools.cpp:594-634, the
  * center-3-neighbour rule) and are evaluated by e4b::get_4b_energy (potential/4b/energy4b.cpp:120;
- * water x4 only).  A compute with MBXG_4B, or over a topology holding co2 or halide monomers,
+ * water x4 only).  A compute with MBXG_4B, or over a topology holding co2 or ion monomers,
  * runs synchronously (eager, no graph replay). */
 int mbxg_set_cutoff4b(mbxg_ctx* ctx, double cutoff4b);
 /* e[0..3] = the 1B, 2B, 3B and 4B energies of the last compute (2B / 3B include every family). */
 int mbxg_get_term_energies(mbxg_ctx* ctx, double* e);
-/* counts[0..10]: evaluated n-mers of the further families, in the order of their MBXG_FAM_* bits
- * from MBXG_FAM_F_H2O_2B; counts[11]: listed tetramers. */
+/* counts[0..20]: evaluated n-mers of the further families, in the order of their MBXG_FAM_* bits
+ * from MBXG_FAM_F_H2O_2B to MBXG_FAM_CS_H2O2_3B; counts[21]: listed tetramers. */
 int mbxg_get_xcounts(mbxg_ctx* ctx, int64_t* counts);
 /* The last compute's listed tetramers (4 internal monomer indices each, query monomer first), at
  * most cap of them; *n receives the number listed. */
@@ -140,7 +151,9 @@ int mbxg_set_batch(mbxg_ctx* ctx, int nconf, const int* mon_offsets);
  * "ch4h2o_2b": 603, "ch4ch4_2b": 1345, "ch4h2o2_3b": 1254, "ch4_1b": 207, "co2_1b": 21,
  * "f_h2o_2b" / "cl_h2o_2b" /
  * "br_h2o_2b" / "i_h2o_2b": 906, "co2co2_2b": 2269, "f_h2o2_3b" / "cl_h2o2_3b" / "br_h2o2_3b" /
- * "i_h2o2_3b": 3173, "co2x3_3b": 1067, "h2ox4_4b": 398).  a == NULL restores the fitted values. */
+ * "i_h2o2_3b": 3173, "co2x3_3b": 1067, "h2ox4_4b": 398, "li_h2o_2b" / "na_h2o_2b" / "k_h2o_2b" /
+ * "rb_h2o_2b" / "cs_h2o_2b": 429, "li_h2o2_3b" / "na_h2o2_3b" / "k_h2o2_3b" / "rb_h2o2_3b" /
+ * "cs_h2o2_3b": 1016).  a == NULL restores the fitted values. */
 int mbxg_set_coefficients(mbxg_ctx* ctx, const char* model, const double* a, int n);
 
 /* Coordinates of all atoms (3*natoms doubles, internal order) from host or device memory.

@@ -191,7 +191,7 @@ struct mbxg_ctx {
     hipEvent_t ev[kNumEvents] = {};
     // further MB-nrg families and the 4B term (xfam.hip); they run synchronously
     double cut4 = 0.0;
-    bool has_xkind = false;                  // the topology holds co2 or halide monomers
+    bool has_xkind = false;                  // the topology holds co2 or ion monomers
     DBuf<int> x_code, x_blk, x_off, x_tot;   // classification scratch; x_tot: [order 2, 3, 4][kXFam]
     DBuf<int> x_list[kXFam];
     int x_count[kXFam] = {};
@@ -429,9 +429,9 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
         if (kind[i] == MBXG_H2O && nat[i] != 3) return fail(MBXG_EINVAL, "monomer %d: h2o needs 3 atoms", i);
         if (kind[i] == MBXG_CH4 && nat[i] != 5) return fail(MBXG_EINVAL, "monomer %d: ch4 needs 5 atoms", i);
         if (kind[i] == MBXG_CO2 && nat[i] != 3) return fail(MBXG_EINVAL, "monomer %d: co2 needs 3 atoms", i);
-        if (kind[i] >= MBXG_F && kind[i] <= MBXG_I && nat[i] != 1)
-            return fail(MBXG_EINVAL, "monomer %d: a halide ion has 1 atom", i);
-        if (kind[i] < MBXG_H2O || kind[i] > MBXG_I) return fail(MBXG_EINVAL, "monomer %d: unknown kind %d", i, kind[i]);
+        if (kind[i] >= MBXG_F && kind[i] <= MBXG_CS && nat[i] != 1)
+            return fail(MBXG_EINVAL, "monomer %d: an ion has 1 atom", i);
+        if (kind[i] < MBXG_H2O || kind[i] > MBXG_CS) return fail(MBXG_EINVAL, "monomer %d: unknown kind %d", i, kind[i]);
         if (nat[i] < 1) return fail(MBXG_EINVAL, "monomer %d: no atoms", i);
         c->first_atom[i] = acc;
         acc += nat[i];
@@ -440,7 +440,7 @@ extern "C" int mbxg_set_topology(mbxg_ctx* c, int nmon, const int* kind, const i
     c->n_h2o = (int)std::count(kind, kind + nmon, MBXG_H2O);
     c->n_ch4 = (int)std::count(kind, kind + nmon, MBXG_CH4);
     c->has_xkind = false;
-    for (int i = 0; i < nmon; ++i) c->has_xkind = c->has_xkind || (kind[i] >= MBXG_CO2 && kind[i] <= MBXG_I);
+    for (int i = 0; i < nmon; ++i) c->has_xkind = c->has_xkind || (kind[i] >= MBXG_CO2 && kind[i] <= MBXG_CS);
     const size_t nm = nmon > 0 ? nmon : 1;
     if (c->d_kind.reserve(nm) || c->d_nat.reserve(nm) || c->d_first.reserve(nm) || c->d_tag.reserve(nm) ||
         c->d_members.reserve(nm) || c->d_islocal.reserve(nm) || c->d_in.reserve(3 * (size_t)acc + 3) ||
@@ -513,8 +513,11 @@ extern "C" int mbxg_set_batch(mbxg_ctx* c, int nconf, const int* mon_offsets) {
 
 // The further family of a coefficient-table name, or -1.
 static int xfam_index(const std::string& m) {
-    static const char* names[kXFam] = {"f_h2o_2b",  "cl_h2o_2b",  "br_h2o_2b",  "i_h2o_2b",  "co2co2_2b", "f_h2o2_3b",
-                                       "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b", "co2x3_3b", "h2ox4_4b"};
+    static const char* names[kXFam] = {"f_h2o_2b",   "cl_h2o_2b",  "br_h2o_2b",  "i_h2o_2b",   "co2co2_2b",
+                                       "f_h2o2_3b",  "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b",  "co2x3_3b",
+                                       "h2ox4_4b",   "li_h2o_2b",  "na_h2o_2b",  "k_h2o_2b",   "rb_h2o_2b",
+                                       "cs_h2o_2b",  "li_h2o2_3b", "na_h2o2_3b", "k_h2o2_3b",  "rb_h2o2_3b",
+                                       "cs_h2o2_3b"};
     for (int f = 0; f < kXFam; ++f)
         if (m == names[f]) return f;
     return -1;

@@ -25,7 +25,12 @@ enum MonKind : int {
     MON_F = 4,  // halide ions f-, cl-, br-, i-
     MON_CL = 5,
     MON_BR = 6,
-    MON_I = 7
+    MON_I = 7,
+    MON_LI = 8,  // alkali-metal ions li+, na+, k+, rb+, cs+
+    MON_NA = 9,
+    MON_K = 10,
+    MON_RB = 11,
+    MON_CS = 12
 };
 
 // Box description passed by value to kernels.  frac[] holds the coefficients of

@@ -42,6 +42,11 @@ int kind_of(const std::string& id) {
     if (id == "cl-") return MBXG_CL;
     if (id == "br-") return MBXG_BR;
     if (id == "i-") return MBXG_I;
+    if (id == "li+") return MBXG_LI;
+    if (id == "na+") return MBXG_NA;
+    if (id == "k+") return MBXG_K;
+    if (id == "rb+") return MBXG_RB;
+    if (id == "cs+") return MBXG_CS;
     return MBXG_OTHER;
 }
 
@@ -405,6 +410,11 @@ double System::run_terms(int terms, bool do_grads, bool use_ghost) {
     if (!ignored(ignore3b_, {"i-", "h2o", "h2o"})) fam |= MBXG_FAM_I_H2O2_3B;
     if (!ignored(ignore3b_, {"co2", "co2", "co2"})) fam |= MBXG_FAM_CO2X3_3B;
     if (!ignored(ignore4b_, {"h2o", "h2o", "h2o", "h2o"})) fam |= MBXG_FAM_H2OX4_4B;  // system.cpp:3143-3155
+    static const char* const kAlkali[5] = {"li+", "na+", "k+", "rb+", "cs+"};
+    for (int a = 0; a < 5; ++a) {
+        if (!ignored(ignore2b_, {kAlkali[a], "h2o"})) fam |= MBXG_FAM_LI_H2O_2B << a;
+        if (!ignored(ignore3b_, {kAlkali[a], "h2o", "h2o"})) fam |= MBXG_FAM_LI_H2O2_3B << a;
+    }
     check(mbxg_set_families(c, fam));
     check(mbxg_set_cutoffs(c, cutoff2b_, cutoff3b_));
     // Get4B runs only with a cutoff above machine epsilon (system.cpp:2084, 2762)

@@ -1,4 +1,5 @@
-// Further MB-nrg families on the GPU: halide-water 2B / 3B, CO2 2B / 3B and water 4B.
+// Further MB-nrg families on the GPU: halide-water 2B / 3B, CO2 2B / 3B, water 4B and
+// alkali-metal-ion-water 2B / 3B.
 //
 // Restates, per n-mer (one lane each), the generated holders' eval with gradients:
 //   mbnrg_A1_B1C2X2_deg6_v1          potential/2b/mbnrg_2b_A1_B1C2X2_deg6_v1.cpp:3789-3960  (f-, cl-, br-, i- with h2o)
@@ -6,6 +7,8 @@
 //   x2b_A1B2_A1B2_v1x                potential/2b/x2b_A1B2_A1B2_v1x.cpp:18485-18700 (co2, co2)
 //   mbnrg_A1B2_A1B2_A1B2_deg4_v1     potential/3b/mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.cpp:1223-1460
 //   mbnrg_A1B2_A1B2_A1B2_A1B2_deg4_v1 potential/4b/mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp:657-1100 (h2o x4)
+//   x2b_h2o_ion_v2x                  potential/2b/x2b-A1B2Z2_C-v1x.cpp:2546-2770 (h2o with li+, na+, k+, rb+, cs+)
+//   x3b_h2o_ion_v1x_deg4_filtered    potential/3b/x3b-h2o-ion-v1x-deg4-filtered.cpp:206-482
 // with the holders as data (gen/xholder_<name>.inc: sites, variables, switching radii, the pair
 // switch combination), reached through e2b::get_2b_energy / e3b::get_3b_energy /
 // e4b::get_4b_energy: the n-mer's monomer names are sorted by those routines' exchange networks
@@ -28,22 +31,42 @@
 #include "gen/xholder_cl_h2o_2b.inc"
 #include "gen/xholder_co2co2_2b.inc"
 #include "gen/xholder_co2x3_3b.inc"
+#include "gen/xholder_cs_h2o2_3b.inc"
+#include "gen/xholder_cs_h2o_2b.inc"
 #include "gen/xholder_f_h2o2_3b.inc"
 #include "gen/xholder_f_h2o_2b.inc"
 #include "gen/xholder_h2ox4_4b.inc"
 #include "gen/xholder_i_h2o2_3b.inc"
 #include "gen/xholder_i_h2o_2b.inc"
+#include "gen/xholder_k_h2o2_3b.inc"
+#include "gen/xholder_k_h2o_2b.inc"
+#include "gen/xholder_li_h2o2_3b.inc"
+#include "gen/xholder_li_h2o_2b.inc"
+#include "gen/xholder_na_h2o2_3b.inc"
+#include "gen/xholder_na_h2o_2b.inc"
+#include "gen/xholder_rb_h2o2_3b.inc"
+#include "gen/xholder_rb_h2o_2b.inc"
 #include "gen/xcoef_br_h2o2_3b.inc"
 #include "gen/xcoef_br_h2o_2b.inc"
 #include "gen/xcoef_cl_h2o2_3b.inc"
 #include "gen/xcoef_cl_h2o_2b.inc"
 #include "gen/xcoef_co2co2_2b.inc"
 #include "gen/xcoef_co2x3_3b.inc"
+#include "gen/xcoef_cs_h2o2_3b.inc"
+#include "gen/xcoef_cs_h2o_2b.inc"
 #include "gen/xcoef_f_h2o2_3b.inc"
 #include "gen/xcoef_f_h2o_2b.inc"
 #include "gen/xcoef_h2ox4_4b.inc"
 #include "gen/xcoef_i_h2o2_3b.inc"
 #include "gen/xcoef_i_h2o_2b.inc"
+#include "gen/xcoef_k_h2o2_3b.inc"
+#include "gen/xcoef_k_h2o_2b.inc"
+#include "gen/xcoef_li_h2o2_3b.inc"
+#include "gen/xcoef_li_h2o_2b.inc"
+#include "gen/xcoef_na_h2o2_3b.inc"
+#include "gen/xcoef_na_h2o_2b.inc"
+#include "gen/xcoef_rb_h2o2_3b.inc"
+#include "gen/xcoef_rb_h2o_2b.inc"
 #include "mbxg_device.hpp"
 #include "mbxg_kernels.hpp"
 #include "nrg_kernel.hpp"
@@ -55,16 +78,22 @@ namespace mbxg {
 namespace {
 
 // Name rank of each engine kind: the order of the reference's monomer id strings
-// ("br-" < "ch4" < "cl-" < "co2" < "f-" < "h2o" < "i-"); kinds without a term rank last.
+// ("br-" < "ch4" < "cl-" < "co2" < "cs+" < "f-" < "h2o" < "i-" < "k+" < "li+" < "na+" < "rb+");
+// kinds without a term rank last.
 __device__ __forceinline__ int name_rank(int kind) {
     switch (kind) {
         case MON_BR: return 0;
         case MON_CH4: return 1;
         case MON_CL: return 2;
         case MON_CO2: return 3;
-        case MON_F: return 4;
-        case MON_H2O: return 5;
-        case MON_I: return 6;
+        case MON_CS: return 4;
+        case MON_F: return 5;
+        case MON_H2O: return 6;
+        case MON_I: return 7;
+        case MON_K: return 8;
+        case MON_LI: return 9;
+        case MON_NA: return 10;
+        case MON_RB: return 11;
         default: return 100 + kind;
     }
 }
@@ -106,24 +135,49 @@ __device__ __forceinline__ void sort_network(const int* rank_in, int* ix) {
     }
 }
 
+// The families in the order of XFam: enum, holder, polynomial pieces.
+#define MBXG_XFAMILIES(X)                              \
+    X(XF_F_H2O_2B, f_h2o_2b, kPHalide2)                \
+    X(XF_CL_H2O_2B, cl_h2o_2b, kPHalide2)              \
+    X(XF_BR_H2O_2B, br_h2o_2b, kPHalide2)              \
+    X(XF_I_H2O_2B, i_h2o_2b, kPHalide2)                \
+    X(XF_CO2CO2_2B, co2co2_2b, kPCo2_2)                \
+    X(XF_F_H2O2_3B, f_h2o2_3b, kPHalide3)              \
+    X(XF_CL_H2O2_3B, cl_h2o2_3b, kPHalide3)            \
+    X(XF_BR_H2O2_3B, br_h2o2_3b, kPHalide3)            \
+    X(XF_I_H2O2_3B, i_h2o2_3b, kPHalide3)              \
+    X(XF_CO2X3_3B, co2x3_3b, kPCo2_3)                  \
+    X(XF_H2OX4_4B, h2ox4_4b, kPH2o_4)                  \
+    X(XF_LI_H2O_2B, li_h2o_2b, kPAlkali2)              \
+    X(XF_NA_H2O_2B, na_h2o_2b, kPAlkali2)              \
+    X(XF_K_H2O_2B, k_h2o_2b, kPAlkali2)                \
+    X(XF_RB_H2O_2B, rb_h2o_2b, kPAlkali2)              \
+    X(XF_CS_H2O_2B, cs_h2o_2b, kPAlkali2)              \
+    X(XF_LI_H2O2_3B, li_h2o2_3b, kPAlkali3)            \
+    X(XF_NA_H2O2_3B, na_h2o2_3b, kPAlkali3)            \
+    X(XF_K_H2O2_3B, k_h2o2_3b, kPAlkali3)              \
+    X(XF_RB_H2O2_3B, rb_h2o2_3b, kPAlkali3)            \
+    X(XF_CS_H2O2_3B, cs_h2o2_3b, kPAlkali3)
+
 // Family descriptions for the classification: order, the holder's monomer kinds in argument
-// order, the skip radius.
+// order, the skip radius and how many first-atom separations beyond it skip the n-mer (all of
+// them for the generated holders; two of three for x3b_h2o_ion_v1x_deg4_filtered,
+// x3b-h2o-ion-v1x-deg4-filtered.cpp:295).
 struct XDesc {
     int order;
     int hkind[4];
     double rout;
+    int skip_min;
 };
 
 template <class H>
 constexpr XDesc xdesc() {
-    return XDesc{H::kMon, {H::kKind[0], H::kKind[1], H::kKind[2], H::kKind[3]}, H::kROut};
+    return XDesc{H::kMon, {H::kKind[0], H::kKind[1], H::kKind[2], H::kKind[3]}, H::kROut, H::kSkipMin};
 }
 
-__constant__ const XDesc kXDesc[kXFam] = {
-    xdesc<XHolder_f_h2o_2b>(),   xdesc<XHolder_cl_h2o_2b>(),  xdesc<XHolder_br_h2o_2b>(),
-    xdesc<XHolder_i_h2o_2b>(),   xdesc<XHolder_co2co2_2b>(),  xdesc<XHolder_f_h2o2_3b>(),
-    xdesc<XHolder_cl_h2o2_3b>(), xdesc<XHolder_br_h2o2_3b>(), xdesc<XHolder_i_h2o2_3b>(),
-    xdesc<XHolder_co2x3_3b>(),   xdesc<XHolder_h2ox4_4b>()};
+#define MBXG_XDESC(F, name, pieces) xdesc<XHolder_##name>(),
+__constant__ const XDesc kXDesc[kXFam] = {MBXG_XFAMILIES(MBXG_XDESC)};
+#undef MBXG_XDESC
 
 // Family of a listed n-mer (ids in listed order) with its holder order, or -1.  code packs the
 // family (bits 0-7), the anchor (the holder slot of the listed first monomer, bits 8-11) and the
@@ -171,8 +225,9 @@ __device__ __forceinline__ int classify(const NmerCtx& N, const int* ids, int fa
         perm[s] = ix[got];
         if (perm[s] == 0) anchor = s;
     }
-    // skip rule: every first-atom separation (monomers imaged next to the listed first one)
-    // beyond the outer radius (e.g. mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp:13013)
+    // skip rule: at least skip_min first-atom separations (monomers imaged next to the listed
+    // first one) beyond the outer radius (e.g. mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp:13013, all of
+    // them)
     double p[4][3];
     const double* f0 = N.xyz + 3 * (size_t)N.first_atom[ids[0]];
 #pragma unroll
@@ -188,7 +243,7 @@ __device__ __forceinline__ int classify(const NmerCtx& N, const int* ids, int fa
             image_apply(B, dx, dy, dz, p[i][0], p[i][1], p[i][2]);
         }
     }
-    bool beyond = true;
+    int beyond = 0;
 #pragma unroll
     for (int a = 0; a < K; ++a)
 #pragma unroll
@@ -197,9 +252,9 @@ __device__ __forceinline__ int classify(const NmerCtx& N, const int* ids, int fa
             const double* v = p[perm[b]];
             const double d0 = u[0] - v[0], d1 = u[1] - v[1], d2 = u[2] - v[2];
             const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-            beyond = beyond && r > D.rout;
+            beyond += r > D.rout;
         }
-    if (beyond) return -1;
+    if (beyond >= D.skip_min) return -1;
     int code = fam | (anchor << 8);
 #pragma unroll
     for (int s = 0; s < K; ++s) code |= perm[s] << (12 + 4 * s);
@@ -751,6 +806,9 @@ MBXG_XPIECE(h2ox4_4b, 4)
 MBXG_XPIECE(h2ox4_4b, 5)
 MBXG_XPIECE(h2ox4_4b, 6)
 MBXG_XPIECE(h2ox4_4b, 7)
+MBXG_XPIECE(alkali_h2o_2b, 0)
+MBXG_XPIECE(alkali_h2o2_3b, 0)
+MBXG_XPIECE(alkali_h2o2_3b, 1)
 #undef MBXG_XPIECE
 
 namespace {
@@ -795,24 +853,22 @@ const xpiece_launch kPCo2_3[4] = {launch_xpoly_co2x3_3b_p0, launch_xpoly_co2x3_3
 const xpiece_launch kPH2o_4[8] = {launch_xpoly_h2ox4_4b_p0, launch_xpoly_h2ox4_4b_p1, launch_xpoly_h2ox4_4b_p2,
                                   launch_xpoly_h2ox4_4b_p3, launch_xpoly_h2ox4_4b_p4, launch_xpoly_h2ox4_4b_p5,
                                   launch_xpoly_h2ox4_4b_p6, launch_xpoly_h2ox4_4b_p7};
+const xpiece_launch kPAlkali2[1] = {launch_xpoly_alkali_h2o_2b_p0};
+const xpiece_launch kPAlkali3[2] = {launch_xpoly_alkali_h2o2_3b_p0, launch_xpoly_alkali_h2o2_3b_p1};
 
 }  // namespace
 
-__host__ __device__ int xfam_order(int f) { return f < XF_F_H2O2_3B ? 2 : (f < XF_H2OX4_4B ? 3 : 4); }
+__host__ __device__ int xfam_order(int f) {
+    if (f <= XF_CO2CO2_2B || (f >= XF_LI_H2O_2B && f <= XF_CS_H2O_2B)) return 2;
+    return f == XF_H2OX4_4B ? 4 : 3;
+}
 
 size_t xfam_workspace(int f, int cap) {
     switch (f) {
-        case XF_F_H2O_2B: return xworkspace<XHolder_f_h2o_2b>(cap);
-        case XF_CL_H2O_2B: return xworkspace<XHolder_cl_h2o_2b>(cap);
-        case XF_BR_H2O_2B: return xworkspace<XHolder_br_h2o_2b>(cap);
-        case XF_I_H2O_2B: return xworkspace<XHolder_i_h2o_2b>(cap);
-        case XF_CO2CO2_2B: return xworkspace<XHolder_co2co2_2b>(cap);
-        case XF_F_H2O2_3B: return xworkspace<XHolder_f_h2o2_3b>(cap);
-        case XF_CL_H2O2_3B: return xworkspace<XHolder_cl_h2o2_3b>(cap);
-        case XF_BR_H2O2_3B: return xworkspace<XHolder_br_h2o2_3b>(cap);
-        case XF_I_H2O2_3B: return xworkspace<XHolder_i_h2o2_3b>(cap);
-        case XF_CO2X3_3B: return xworkspace<XHolder_co2x3_3b>(cap);
-        case XF_H2OX4_4B: return xworkspace<XHolder_h2ox4_4b>(cap);
+#define MBXG_XW(F, name, pieces) \
+    case F: return xworkspace<XHolder_##name>(cap);
+        MBXG_XFAMILIES(MBXG_XW)
+#undef MBXG_XW
         default: return 0;
     }
 }
@@ -820,30 +876,11 @@ size_t xfam_workspace(int f, int cap) {
 hipError_t launch_xfam(int f, const int* list, const int* n_dev, int cap, const NmerCtx& N, const double* coef,
                        double* grad, double* e_out, double* partials, double* ws, hipStream_t st) {
     switch (f) {
-        case XF_F_H2O_2B:
-            return xlaunch<XHolder_f_h2o_2b>(kPHalide2, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_CL_H2O_2B:
-            return xlaunch<XHolder_cl_h2o_2b>(kPHalide2, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_BR_H2O_2B:
-            return xlaunch<XHolder_br_h2o_2b>(kPHalide2, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_I_H2O_2B:
-            return xlaunch<XHolder_i_h2o_2b>(kPHalide2, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_CO2CO2_2B:
-            return xlaunch<XHolder_co2co2_2b>(kPCo2_2, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_F_H2O2_3B:
-            return xlaunch<XHolder_f_h2o2_3b>(kPHalide3, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_CL_H2O2_3B:
-            return xlaunch<XHolder_cl_h2o2_3b>(kPHalide3, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_BR_H2O2_3B:
-            return xlaunch<XHolder_br_h2o2_3b>(kPHalide3, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_I_H2O2_3B:
-            return xlaunch<XHolder_i_h2o2_3b>(kPHalide3, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_CO2X3_3B:
-            return xlaunch<XHolder_co2x3_3b>(kPCo2_3, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        case XF_H2OX4_4B:
-            return xlaunch<XHolder_h2ox4_4b>(kPH2o_4, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
-        default:
-            return hipErrorInvalidValue;
+#define MBXG_XL(F, name, pieces) \
+    case F: return xlaunch<XHolder_##name>(pieces, list, n_dev, cap, N, coef, grad, e_out, partials, ws, st);
+        MBXG_XFAMILIES(MBXG_XL)
+#undef MBXG_XL
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -906,21 +943,11 @@ hipError_t launch_tetra(const PtInfo* pinfo, const double* cx, const double* cy,
 
 const double* xfam_default_coef(int f, int* n) {
     switch (f) {
-#define MBXG_XC(F, name)       \
-    case F:                    \
-        *n = kNCoefX_##name;   \
+#define MBXG_XC(F, name, pieces) \
+    case F:                      \
+        *n = kNCoefX_##name;     \
         return kFitX_##name;
-        MBXG_XC(XF_F_H2O_2B, f_h2o_2b)
-        MBXG_XC(XF_CL_H2O_2B, cl_h2o_2b)
-        MBXG_XC(XF_BR_H2O_2B, br_h2o_2b)
-        MBXG_XC(XF_I_H2O_2B, i_h2o_2b)
-        MBXG_XC(XF_CO2CO2_2B, co2co2_2b)
-        MBXG_XC(XF_F_H2O2_3B, f_h2o2_3b)
-        MBXG_XC(XF_CL_H2O2_3B, cl_h2o2_3b)
-        MBXG_XC(XF_BR_H2O2_3B, br_h2o2_3b)
-        MBXG_XC(XF_I_H2O2_3B, i_h2o2_3b)
-        MBXG_XC(XF_CO2X3_3B, co2x3_3b)
-        MBXG_XC(XF_H2OX4_4B, h2ox4_4b)
+        MBXG_XFAMILIES(MBXG_XC)
 #undef MBXG_XC
         default:
             *n = 0;

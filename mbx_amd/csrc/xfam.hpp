@@ -1,4 +1,5 @@
-// Further MB-nrg families (xfam.hip): halide-water 2B / 3B, CO2 2B / 3B, water 4B.
+// Further MB-nrg families (xfam.hip): halide-water 2B / 3B, CO2 2B / 3B, water 4B,
+// alkali-metal-ion-water 2B / 3B.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,9 +21,19 @@ enum XFam : int {
     XF_BR_H2O2_3B = 7,
     XF_I_H2O2_3B = 8,
     XF_CO2X3_3B = 9,
-    XF_H2OX4_4B = 10
+    XF_H2OX4_4B = 10,
+    XF_LI_H2O_2B = 11,
+    XF_NA_H2O_2B = 12,
+    XF_K_H2O_2B = 13,
+    XF_RB_H2O_2B = 14,
+    XF_CS_H2O_2B = 15,
+    XF_LI_H2O2_3B = 16,
+    XF_NA_H2O2_3B = 17,
+    XF_K_H2O2_3B = 18,
+    XF_RB_H2O2_3B = 19,
+    XF_CS_H2O2_3B = 20
 };
-constexpr int kXFam = 11;
+constexpr int kXFam = 21;
 
 // Evaluate lists of the families: entries of order + 1 ints (monomer ids in the holder's order,
 // then the holder slot of the listed first monomer); at most cap[f] entries each.

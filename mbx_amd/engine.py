@@ -13,9 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmbxg.so")
 
 H2O, CH4, OTHER, CO2, F, CL, BR, I = 0, 1, 2, 3, 4, 5, 6, 7
+LI, NA, K, RB, CS = 8, 9, 10, 11, 12
 TERM_2B, TERM_3B, LISTS_ONLY, TERM_1B, TERM_4B = 1, 2, 4, 8, 16
-KIND_OF = {"h2o": H2O, "ch4": CH4, "co2": CO2, "f-": F, "cl-": CL, "br-": BR, "i-": I}
-NAT_OF = {H2O: 3, CH4: 5, CO2: 3, F: 1, CL: 1, BR: 1, I: 1}
+KIND_OF = {"h2o": H2O, "ch4": CH4, "co2": CO2, "f-": F, "cl-": CL, "br-": BR, "i-": I, "li+": LI, "na+": NA,
+           "k+": K, "rb+": RB, "cs+": CS}
+NAT_OF = {H2O: 3, CH4: 5, CO2: 3, F: 1, CL: 1, BR: 1, I: 1, LI: 1, NA: 1, K: 1, RB: 1, CS: 1}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -74,7 +76,9 @@ PROTOTYPES = {
 FAMILIES = ("mbpol_2b", "ch4h2o_2b", "ch4ch4_2b", "mbpol_3b", "ch4h2o2_3b",
             # further MB-nrg families (bits 5.. of mbxg_set_families)
             "f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "co2co2_2b", "f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b",
-            "i_h2o2_3b", "co2x3_3b", "h2ox4_4b")
+            "i_h2o2_3b", "co2x3_3b", "h2ox4_4b",
+            "li_h2o_2b", "na_h2o_2b", "k_h2o_2b", "rb_h2o_2b", "cs_h2o_2b",
+            "li_h2o2_3b", "na_h2o2_3b", "k_h2o2_3b", "rb_h2o2_3b", "cs_h2o2_3b")
 XFAMILIES = FAMILIES[5:]
 TIMING_STAGES = ("fix", "lists_count", "lists_emit", "vars_2b", "poly_2b", "chain_2b", "vars_3b", "poly_3b",
                  "chain_3b", "total", "ch4h2o2_3b")

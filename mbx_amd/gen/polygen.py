@@ -812,14 +812,18 @@ def generate_1b(name, out_dir):
 
 
 # ------------------------------------------------------------------------------------------
-# Further MB-nrg families (halide-water 2B / 3B, CO2 2B / 3B, water 4B): holders as data
-# (gen/xholder_<holder>.inc), each parameter set's coefficients (gen/xcoef_<holder>.inc) and the
-# polynomial structures in pieces (gen/xpoly_<structure>_p<k>.{inc,hip}, one kernel each).
+# Further MB-nrg families (halide-water 2B / 3B, CO2 2B / 3B, water 4B, alkali-ion-water 2B / 3B):
+# holders as data (gen/xholder_<holder>.inc), each parameter set's coefficients
+# (gen/xcoef_<holder>.inc) and the polynomial structures in pieces (gen/xpoly_<structure>_p<k>.{inc,hip},
+# one kernel each).
+ALKALI = ("li", "na", "k", "rb", "cs")
 XHOLDERS = ("f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b",
-            "co2co2_2b", "co2x3_3b", "h2ox4_4b")
-XPOLY_PIECES = {"halide_h2o_2b": 1, "co2co2_2b": 2, "halide_h2o2_3b": 4, "co2x3_3b": 4, "h2ox4_4b": 8}
+            "co2co2_2b", "co2x3_3b", "h2ox4_4b") + tuple("%s_h2o_2b" % a for a in ALKALI) + \
+    tuple("%s_h2o2_3b" % a for a in ALKALI)
+XPOLY_PIECES = {"halide_h2o_2b": 1, "co2co2_2b": 2, "halide_h2o2_3b": 4, "co2x3_3b": 4, "h2ox4_4b": 8,
+                "alkali_h2o_2b": 1, "alkali_h2o2_3b": 2}
 XKIND = {"h2o": "MON_H2O", "ch4": "MON_CH4", "co2": "MON_CO2", "f-": "MON_F", "cl-": "MON_CL", "br-": "MON_BR",
-         "i-": "MON_I"}
+         "i-": "MON_I", "li+": "MON_LI", "na+": "MON_NA", "k+": "MON_K", "rb+": "MON_RB", "cs+": "MON_CS"}
 
 
 def emit_trie_piece(fn, nvars, monos, fence_every=48):
@@ -915,6 +919,8 @@ def generate_xholder(name, out_dir):
            "    static constexpr double kRIn = %.17g;" % sw["inner"],
            "    static constexpr double kROut = %.17g;" % sw["outer"],
            "    static constexpr int kNPair = %d;" % len(pairs),
+           "    // skipped when at least kSkipMin first-atom separations exceed kROut",
+           "    static constexpr int kSkipMin = %d;" % h.get("skip_min", len(pairs)),
            "    static constexpr int kPA[6] = {%s};" % ", ".join(str(v) for v in pad([a for a, b in pairs], 6, 0)),
            "    static constexpr int kPB[6] = {%s};" % ", ".join(str(v) for v in pad([b for a, b in pairs], 6, 0)),
            "    static constexpr int kNTerm = %d;" % len(sw["terms"]),

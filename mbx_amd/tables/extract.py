@@ -125,6 +125,93 @@ XHOLDERS = [
     ("h2ox4_4b", "src/potential/4b/mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp", ["h2o"] * 4, [3] * 4, "h2ox4_4b"),
 ]
 
+# Alkali-metal-ion-water holders, in the older hand-written style (named non-linear parameters,
+# one parameter set per ion): x2b_h2o_ion_v2x (x2b-A1B2Z2_C-v1x.cpp:176-2416, eval with
+# gradients :2546-2770) and x3b_h2o_ion_v1x_deg4_filtered (x3b-h2o-ion-v1x-deg4-filtered.cpp:
+# 206-482, parameter sets :484-).  Both take water(s) first and the ion last
+# (energy2b.cpp:258-282, energy3b.cpp:223-228).
+ALKALI = ("li+", "na+", "k+", "rb+", "cs+")
+ALKALI_HOLDERS = [("%s_h2o_2b" % ion[:-1], "src/potential/2b/x2b-A1B2Z2_C-v1x.cpp", ["h2o", ion], [3, 1],
+                   "alkali_h2o_2b") for ion in ALKALI] + \
+                 [("%s_h2o2_3b" % ion[:-1], "src/potential/3b/x3b-h2o-ion-v1x-deg4-filtered.cpp",
+                   ["h2o", "h2o", ion], [3, 3, 1], "alkali_h2o2_3b") for ion in ALKALI]
+
+
+def _named_params(blk):
+    return {k: float(v) for k, v in re.findall(r"\b([A-Za-z_]\w*)\s*=\s*([-+0-9.eE]+)\s*;", blk)}
+
+
+def parse_alkali_2b(text, mons, nats):
+    """x2b_h2o_ion_v2x for one ion as holder data: sites O H H X lp1 lp2, the eight variables of
+    eval() (v_exp: exp(k (r0 - r)), v_coul: the same over r; x2b-A1B2Z2_C-v1x.cpp:58-96), the
+    water lone pairs (monomer::setup, :123-141), the cosine switch on the O-X separation with
+    the dimer skipped beyond r2f, and the ion's twobodyfit coefficients."""
+    ion = mons[1]
+    src = re.sub(r"//[^\n]*", "", text)
+    m = re.search(r'if\s*\(\s*mon1\s*==\s*"h2o"\s*&&\s*mon2\s*==\s*"%s"\s*\)' % re.escape(ion), src)
+    blk = _block(src, m.end())
+    c = re.search(r"twobodyfit\s*=\s*std::vector<double>\s*\{(.*?)\};", blk, re.S)
+    coef = [float(v) for v in c.group(1).replace(",", " ").split()]
+    par = _named_params(blk[c.end():])
+    i = src.index("x2b_h2o_ion_v2x::eval(const double* w1, const double* x, double* g1")
+    ev = src[i:]
+    site = {n: int(o) // 3 for n, o in re.findall(r"const int (\w+) = (\d+);", ev[:ev.index("poly_2b_h2o_ion")])}
+    assert site == {"O": 0, "H1": 1, "H2": 2, "X": 3, "lp1": 4, "lp2": 5}, site
+    assert re.search(r"setup\(xcrd \+ O, in_plane_gamma, out_of_plane_gamma, xcrd \+ lp1, xcrd \+ lp2\)", ev)
+    vars_ = {}
+    for idx, form, d, k, a, b in re.findall(
+            r"v\[(\d+)\]\s*=\s*ctxt\[[^\]]*\]\.(v_exp|v_coul)\((\w+),\s*(\w+),\s*xcrd,\s*(\w+),\s*(\w+)\)", ev):
+        vars_[int(idx)] = [{"v_exp": "exp0", "v_coul": "coul0"}[form], par[k], par[d], site[a], site[b]]
+    assert sorted(vars_) == list(range(8)), vars_
+    assert "if (rab[i] > r2f) continue;" in ev and "const double sw = f_switch(rab[sh], gsw);" in ev
+    return {"monomers": mons, "atoms": nats, "nreal": 4, "nsites": 6,
+            "vsites": [{"oxygen_site": 0, "in_plane": par["in_plane_gamma"], "out_of_plane": par["out_of_plane_gamma"]}],
+            "switch": {"inner": par["r2i"], "outer": par["r2f"], "pairs": [[0, 1]], "terms": [[1.0, [0]]]},
+            "skip": "pairs_beyond_outer", "skip_min": 1, "vars": [vars_[i] for i in range(8)], "coef": coef}
+
+
+def parse_alkali_3b(text, mons, nats):
+    """x3b_h2o_ion_v1x_deg4_filtered for one ion as holder data: sites Oa Ha1 Ha2 Ob Hb1 Hb2 X,
+    the 21 variables of operator() (v_intra / v_main: exp(-k (d - r0)), which is exp(k (r0 - d))
+    bit for bit; x3b-h2o-ion-v1x-deg4-filtered.cpp:43-93), the switch combination
+    s_ab s_ac + s_ab s_bc + s_ac s_bc over the O-O and O-X separations, the trimer skipped when two
+    of them exceed r3f, and the ion's coefficients."""
+    ion = mons[2]
+    src = re.sub(r"//[^\n]*", "", text)
+    m = re.search(r'if\s*\(\s*m3\s*==\s*"%s"\s*\)' % re.escape(ion), src)
+    blk = _block(src, m.end())
+    c = re.search(r"the_poly\[\]\s*=\s*\{(.*?)\};", blk, re.S)
+    coef = [float(v) for v in c.group(1).replace(",", " ").split()]
+    par = _named_params(blk[c.end():])
+    i = src.index("double* grad1, double* grad2, double* grad3")
+    ev = src[i:src.index("x3b_h2o_ion_v1x_deg4_filtered::x3b_h2o_ion_v1x_deg4_filtered")]
+    base = {"w1": 0, "w2": 3, "x": 6}
+    site = {}
+    for n, b, o in re.findall(r"const double\*\s*(\w+)\s*=\s*(w1|w2|x)\s*(?:\+\s*(\d+))?;", ev):
+        if n not in ("w1", "w2", "x"):
+            site[n] = base[b] + int(o or 0) // 3
+    assert site == {"Oa": 0, "Ha1": 1, "Ha2": 2, "Ob": 3, "Hb1": 4, "Hb2": 5, "X": 6}, site
+    vars_ = {}
+    for idx, form, k, d, a, b in re.findall(r"v\[(\d+)\]\s*=\s*(v_intra|v_main)\((m_\w+),\s*(m_\w+),\s*(\w+),\s*(\w+)\)", ev):
+        vars_[int(idx)] = ["exp0", par[k], par[d], site[a], site[b]]
+    assert sorted(vars_) == list(range(21)), vars_
+    assert "const double s = sab * sac + sab * sbc + sac * sbc;" in ev
+    assert re.search(r"if \(\(drab > m_r3f and drac > m_r3f\) or \(drab > m_r3f and drbc > m_r3f\) or "
+                     r"\(drac > m_r3f and drbc > m_r3f\)\)\s*continue;", ev)
+    return {"monomers": mons, "atoms": nats, "nreal": 7, "nsites": 7, "vsites": [],
+            "switch": {"inner": par["m_r3i"], "outer": par["m_r3f"], "pairs": [[0, 1], [0, 2], [1, 2]],
+                       "terms": [[1.0, [0, 1]], [1.0, [0, 2]], [1.0, [1, 2]]]},
+            "skip": "pairs_beyond_outer", "skip_min": 2, "vars": [vars_[i] for i in range(21)], "coef": coef}
+
+
+def xholder_data(path, text, mons, nats):
+    if path.endswith("x2b-A1B2Z2_C-v1x.cpp"):
+        return parse_alkali_2b(text, mons, nats)
+    if path.endswith("x3b-h2o-ion-v1x-deg4-filtered.cpp"):
+        return parse_alkali_3b(text, mons, nats)
+    return parse_xholder(text, mons, nats)
+
+
 # Their polynomial structures: (name, energy-only routine, variables, coefficients)
 XPOLYS = [
     ("halide_h2o_2b", "src/potential/2b/poly_2b_A1_B1C2X2_deg6_nograd_v1.cpp", 8, 906),
@@ -132,6 +219,9 @@ XPOLYS = [
     ("co2co2_2b", "src/potential/2b/poly_2b_A1B2_A1B2_v1.cpp", 15, 2269),
     ("co2x3_3b", "src/potential/3b/poly_3b_A1B2_A1B2_A1B2_deg4_nograd_v1.cpp", 27, 1067),
     ("h2ox4_4b", "src/potential/4b/poly_4b_A1B2_A1B2_A1B2_A1B2_deg4_nograd_v1.cpp", 66, 398),
+    ("alkali_h2o_2b", "src/potential/2b/poly-2b-A1B2Z2_C-v1.cpp", 8, 429),
+    # value-and-gradient routine; its energy expression (double e)
+    ("alkali_h2o2_3b", "src/potential/3b/poly-3b-h2o-ion-v1x_deg4_filtered.cpp", 21, 1016),
 ]
 
 # pairs of an n-mer's monomers in the holders' switch naming (sw12, sw13, sw14, sw23, sw24, sw34)
@@ -504,14 +594,14 @@ def main():
     for name, poly_path, nvars, ncoef in XPOLYS:
         text = open(os.path.join(ref, poly_path)).read()
         poly = expand_tarray(text) if "fmg1(" in text else expand(text)
-        first = [h for h in XHOLDERS if h[4] == name][0]
-        coef = parse_xholder(open(os.path.join(ref, first[1])).read(), first[2], first[3])["coef"]
+        first = [h for h in XHOLDERS + ALKALI_HOLDERS if h[4] == name][0]
+        coef = xholder_data(first[1], open(os.path.join(ref, first[1])).read(), first[2], first[3])["coef"]
         if len(coef) != ncoef:
             raise RuntimeError("%s: %d coefficients, expected %d" % (name, len(coef), ncoef))
         n = write_table(os.path.join(HERE, name + ".tbl"), name, nvars, coef, poly)
         print("%s: %d monomials, %d coefficients" % (name, n, len(coef)))
-    for name, holder_path, mons, nats, poly in XHOLDERS:
-        h = parse_xholder(open(os.path.join(ref, holder_path)).read(), mons, nats)
+    for name, holder_path, mons, nats, poly in XHOLDERS + ALKALI_HOLDERS:
+        h = xholder_data(holder_path, open(os.path.join(ref, holder_path)).read(), mons, nats)
         h["name"], h["poly"] = name, poly
         with open(os.path.join(HERE, name + ".holder.json"), "w") as f:
             json.dump(h, f, indent=0)

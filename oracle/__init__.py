@@ -588,12 +588,13 @@ def nrg_1b(name, w, a=None, grad=True):
 
 
 # Atoms per monomer kind (the engine's kind numbers: h2o, ch4, -, co2, f-, cl-, br-, i-).
-NAT_OF_KIND = {0: 3, 1: 5, 3: 3, 4: 1, 5: 1, 6: 1, 7: 1}
+NAT_OF_KIND = {0: 3, 1: 5, 3: 3, 4: 1, 5: 1, 6: 1, 7: 1, 8: 1, 9: 1, 10: 1, 11: 1, 12: 1}
 
 
 def one_body(xyz, kinds, box=None):
     """The 1B term of a system of H2O (kind 0, 3 atoms), CH4 (kind 1, 5 atoms), CO2 (kind 3, 3
-    atoms) and halide (kinds 4-7, 1 atom) monomers in the given order: e1b::get_1b_energy per
+    atoms) and ion (halides kinds 4-7, alkali metals 8-12, 1 atom) monomers in the given order:
+    e1b::get_1b_energy per
     monomer (potential/1b/energy1b.cpp:118-177; ions have no 1B term) on the coordinates
     System::SetPBC wraps first (FixMonomerCoordinates) when box is given.
     Returns (E, grad[3 * atoms], virial[9]); E sums in monomer order."""
@@ -663,7 +664,7 @@ def ref_ps_1b(w, virial=None):
 
 
 # ------------------------------------------------------------------------------------------
-# Further MB-nrg families (halide-water 2B / 3B, CO2 2B / 3B, water 4B) and the system-level
+# Further MB-nrg families (halide-water 2B / 3B, CO2 2B / 3B, water 4B, alkali-ion-water 2B / 3B) and the system-level
 # orchestration over monomers of any kind.
 
 class _OrcXHolder(ctypes.Structure):
@@ -672,7 +673,7 @@ class _OrcXHolder(ctypes.Structure):
                 ("out_of_plane", ctypes.c_double), ("r_in", ctypes.c_double), ("r_out", ctypes.c_double),
                 ("nvars", ctypes.c_int), ("form", _ip), ("k", _dp), ("d", _dp), ("s1", _ip), ("s2", _ip),
                 ("npair", ctypes.c_int), ("pa", ctypes.c_int * 6), ("pb", ctypes.c_int * 6), ("nterm", ctypes.c_int),
-                ("tcoef", _dp), ("tmask", _ip)]
+                ("tcoef", _dp), ("tmask", _ip), ("skip_min", ctypes.c_int)]
 
 
 XFORMS = {"exp0": 0, "coul0": 1, "exp": 2, "coul": 3}
@@ -706,7 +707,7 @@ class XHolder:
                              sw["inner"], sw["outer"], len(v), _i(self.form), _d(self.k), _d(self.d), _i(self.s1),
                              _i(self.s2), len(pairs), (ctypes.c_int * 6)(*([a for a, b in pairs] + [0] * (6 - len(pairs)))),
                              (ctypes.c_int * 6)(*([b for a, b in pairs] + [0] * (6 - len(pairs)))),
-                             len(self.tcoef), _d(self.tcoef), _i(self.tmask))
+                             len(self.tcoef), _d(self.tcoef), _i(self.tmask), h.get("skip_min", len(pairs)))
 
 
 _XHOLDERS = {}
@@ -737,9 +738,11 @@ def xnmer(name, mons, a=None):
 
 # Atoms per monomer of the ids the n-mer families here know (systools::SetUpMonomers,
 # sys_tools.cpp:133-255).
-NAT_OF_NAME = {"h2o": 3, "ch4": 5, "co2": 3, "f-": 1, "cl-": 1, "br-": 1, "i-": 1}
+NAT_OF_NAME = {"h2o": 3, "ch4": 5, "co2": 3, "f-": 1, "cl-": 1, "br-": 1, "i-": 1, "li+": 1, "na+": 1, "k+": 1,
+               "rb+": 1, "cs+": 1}
 
-# The holder of each sorted name tuple (energy2b.cpp:253-374, energy3b.cpp:219-260,
+# The holder of each sorted name tuple (energy2b.cpp:253-374, energy3b.cpp:219-260 with the
+# alkali ions at energy2b.cpp:261-264, 279-282 and energy3b.cpp:223-228,
 # energy4b.cpp:214-221); the holder's argument order is taken from its monomer list.
 NMER_HOLDERS = {
     ("h2o", "h2o"): "mbpol_2b", ("ch4", "h2o"): "ch4h2o_2b", ("ch4", "ch4"): "ch4ch4_2b",
@@ -749,6 +752,10 @@ NMER_HOLDERS = {
     ("f-", "h2o", "h2o"): "f_h2o2_3b", ("cl-", "h2o", "h2o"): "cl_h2o2_3b", ("br-", "h2o", "h2o"): "br_h2o2_3b",
     ("h2o", "h2o", "i-"): "i_h2o2_3b", ("co2", "co2", "co2"): "co2x3_3b",
     ("h2o", "h2o", "h2o", "h2o"): "h2ox4_4b",
+    ("h2o", "li+"): "li_h2o_2b", ("h2o", "na+"): "na_h2o_2b", ("h2o", "k+"): "k_h2o_2b", ("h2o", "rb+"): "rb_h2o_2b",
+    ("cs+", "h2o"): "cs_h2o_2b",
+    ("h2o", "h2o", "li+"): "li_h2o2_3b", ("h2o", "h2o", "na+"): "na_h2o2_3b", ("h2o", "h2o", "k+"): "k_h2o2_3b",
+    ("h2o", "h2o", "rb+"): "rb_h2o2_3b", ("cs+", "h2o", "h2o"): "cs_h2o2_3b",
 }
 
 
@@ -899,14 +906,16 @@ def refx():
     if _REFX is None:
         L = ctypes.CDLL(REFX_PATH)
         sz = ctypes.c_size_t
-        for name, nargs in (("ref_x2b_halide_h2o", None), ("ref_x3b_halide_h2o2", None), ("ref_x2b_co2", None),
-                            ("ref_x3b_co2", None), ("ref_x4b_h2o", None)):
+        for name in ("ref_x2b_halide_h2o", "ref_x3b_halide_h2o2", "ref_x2b_co2", "ref_x3b_co2", "ref_x4b_h2o",
+                     "ref_x2b_alkali_h2o", "ref_x3b_alkali_h2o2"):
             getattr(L, name).restype = ctypes.c_double
         L.ref_x2b_halide_h2o.argtypes = [ctypes.c_char_p, _dp, _dp, sz, _dp, _dp, _dp]
         L.ref_x3b_halide_h2o2.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, sz, _dp, _dp, _dp, _dp]
         L.ref_x2b_co2.argtypes = [_dp, _dp, sz, _dp, _dp, _dp]
         L.ref_x3b_co2.argtypes = [_dp, _dp, _dp, sz, _dp, _dp, _dp, _dp]
         L.ref_x4b_h2o.argtypes = [_dp, _dp, _dp, _dp, sz, _dp, _dp, _dp, _dp, _dp]
+        L.ref_x2b_alkali_h2o.argtypes = [ctypes.c_char_p, _dp, _dp, sz, _dp, _dp, _dp]
+        L.ref_x3b_alkali_h2o2.argtypes = [ctypes.c_char_p, _dp, _dp, _dp, sz, _dp, _dp, _dp, _dp]
         _REFX = L
     return _REFX
 
@@ -919,7 +928,12 @@ def ref_xnmer(name, mons):
     gs = [np.zeros_like(m) for m in ms]
     vir = np.zeros(9)
     h = xholder(name).desc
-    if name.endswith("_h2o_2b"):
+    if h["poly"] == "alkali_h2o_2b":
+        e = L.ref_x2b_alkali_h2o(h["monomers"][1].encode(), _d(ms[0]), _d(ms[1]), 1, _d(gs[0]), _d(gs[1]), _d(vir))
+    elif h["poly"] == "alkali_h2o2_3b":
+        e = L.ref_x3b_alkali_h2o2(h["monomers"][2].encode(), _d(ms[0]), _d(ms[1]), _d(ms[2]), 1, _d(gs[0]),
+                                  _d(gs[1]), _d(gs[2]), _d(vir))
+    elif name.endswith("_h2o_2b"):
         e = L.ref_x2b_halide_h2o(h["monomers"][0].encode(), _d(ms[0]), _d(ms[1]), 1, _d(gs[0]), _d(gs[1]), _d(vir))
     elif name.endswith("_h2o2_3b"):
         e = L.ref_x3b_halide_h2o2(h["monomers"][0].encode(), _d(ms[0]), _d(ms[1]), _d(ms[2]), 1, _d(gs[0]),

@@ -1184,7 +1184,8 @@ static double xvar(int form, double k, double d, const double* p1, const double*
 
 /* The generated holders' eval with gradients, e.g. mbnrg_2b_A1_B1C2X2_deg6_v1.cpp:3789-3960,
  * mbnrg_3b_A1_B1C2X2_B1C2X2_deg4_v1.cpp:12955-13230, x2b_A1B2_A1B2_v1x.cpp:18485-18700,
- * mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.cpp:1223-1460, mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp:657-1100:
+ * mbnrg_3b_A1B2_A1B2_A1B2_deg4_v1.cpp:1223-1460, mbnrg_4b_A1B2_A1B2_A1B2_A1B2_deg4_v1.cpp:657-1100,
+ * x2b-A1B2Z2_C-v1x.cpp:2546-2770 (alkali ion, water), x3b-h2o-ion-v1x-deg4-filtered.cpp:206-482:
  * first-atom separations of every monomer pair and the skip rule; the water lone-pair sites
  * (tools/water_monomer_lp.cpp:3-21); variables; P and its gradient; the gradient scaled by the
  * switch combination sw; variable gradients to the sites; lone-pair gradients back to O, H, H
@@ -1198,7 +1199,7 @@ double orc_xholder_eval(const orc_xholder* h, const orc_poly* p, const double* a
     for (int m = 0; m < h->nmon; ++m) memcpy(xs + 3 * off[m], mons[m], sizeof(double) * 3 * h->nat[m]);
     *evaluated = 0;
     double dd[6][3], r[6];
-    int beyond = 1;
+    int beyond = 0;
     for (int q = 0; q < h->npair; ++q) {
         const double* u = xs + 3 * off[h->pa[q]];
         const double* v = xs + 3 * off[h->pb[q]];
@@ -1206,9 +1207,9 @@ double orc_xholder_eval(const orc_xholder* h, const orc_poly* p, const double* a
         dd[q][1] = u[1] - v[1];
         dd[q][2] = u[2] - v[2];
         r[q] = sqrt(dd[q][0] * dd[q][0] + dd[q][1] * dd[q][1] + dd[q][2] * dd[q][2]);
-        beyond = beyond && r[q] > h->r_out;
+        beyond += r[q] > h->r_out;
     }
-    if (beyond) return 0.0;
+    if (beyond >= h->skip_min) return 0.0;
     *evaluated = 1;
     double oh1[2][3], oh2[2][3];
     for (int w = 0; w < h->nvsite; ++w)

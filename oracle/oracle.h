@@ -102,7 +102,9 @@ double orc_nrg_holder(const orc_holder* h, const orc_poly* p, const double* a, c
  * in the holder's argument order; sites = real atoms, then two lone-pair sites per water with
  * them (oxygen site vo[w], the holders' w12 / wcross); variables of forms 0-2 as orc_holder
  * and 3: exp(-k r) / r; the pair switches of every monomer pair (pa[q], pb[q]) combined as
- * sum_t tcoef[t] prod_{q in tmask[t]} s_q; skipped when every pair separation exceeds r_out. */
+ * sum_t tcoef[t] prod_{q in tmask[t]} s_q; skipped when at least skip_min pair separations exceed
+ * r_out (every pair for the generated holders; two of three for x3b_h2o_ion_v1x_deg4_filtered,
+ * x3b-h2o-ion-v1x-deg4-filtered.cpp:295). */
 typedef struct {
     int nmon;
     int nat[4];
@@ -122,6 +124,7 @@ typedef struct {
     int nterm;
     const double* tcoef;
     const int* tmask;
+    int skip_min; /* skipped when at least skip_min pair separations exceed r_out */
 } orc_xholder;
 
 /* One n-mer through such a holder; conventions of orc_nrg_holder.  Pinned by the reference's

@@ -11,7 +11,9 @@ Nothing is compiled, imported or executed.  Output: tests/golden/kats.json.
     poly_3b    src/unittests/unittest-poly-3b.cpp         TEST_CASE("poly-3b-v2x::class")
     holder_2b  src/unittests/unittest-poly-holder-2b.cpp  TEST_CASE("x2b_v9x::struct")
     holder_3b  src/unittests/unittest-poly-holder-3b.cpp  TEST_CASE("x3b-v2x::class")
-    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, the MB-pol
+    and the MB-nrg CH4 families' polynomial, holder and energy-routine cases, the alkali-ion-water
+    holder, polynomial and energy-routine cases and the Na+ / Cs+ water setups' per-term energies
+    (src/tests/unittests/setup_h2o_{1,2}_{na,cs}_1.h), the MB-pol
     1B (ps::pot_nasa) case, the CH4 and CO2 1B polynomial and holder cases (see CASES), and the
     ch4, h2o and co2 rows of unittest-energy1b.cpp, the h2o sections of unittest-energy2b.cpp /
     unittest-energy3b.cpp, and the three-water gas-phase n-body energies of
@@ -67,6 +69,20 @@ CASES = {
     "energy_h2ox4_4b": ("src/unittests/unittest-energy4b.cpp", 'SECTION:h2o-h2o-h2o-h2o'),
     "poly_co2co2_2b": ("src/unittests/unittest-poly-2b.cpp", ':poly_2b_A1B2_A1B2_v1x:struct'),
 }
+
+
+# Alkali-metal-ion-water families: x2b_h2o_ion_v2x and x3b_h2o_ion_v1x_deg4_filtered holder cases
+# (every ion), the 3B polynomial case, and the li+ / cs+ sections of the 3B energy routine.
+for _ion in ("li", "na", "k", "rb", "cs"):
+    CASES["holder_%s_h2o_2b" % _ion] = ("src/unittests/unittest-poly-holder-2b.cpp", "SECTION:h2o-%s" % _ion)
+    CASES["holder_%s_h2o2_3b" % _ion] = ("src/unittests/unittest-poly-holder-3b.cpp", "SECTION:h2o-h2o-%s" % _ion)
+CASES["poly_alkali_h2o2_3b"] = ("src/unittests/unittest-poly-3b.cpp", "poly-3b-h2o-ion-v1x_deg4_filtered::class")
+CASES["energy_li_h2o2_3b"] = ("src/unittests/unittest-energy3b.cpp", "SECTION:li-h2o-h2o")
+CASES["energy_cs_h2o2_3b"] = ("src/unittests/unittest-energy3b.cpp", "SECTION:cs-h2o-h2o")
+
+# System-level data of the reference's ion-water setups (src/tests/unittests/setup_h2o_<n>_<ion>_1.h):
+# per-term energies and virials of one or two waters with one Na+ or Cs+.
+SETUPS = ["setup_h2o_1_na_1", "setup_h2o_2_na_1", "setup_h2o_1_cs_1", "setup_h2o_2_cs_1"]
 
 
 def block(text, name):
@@ -180,6 +196,20 @@ def test_data(text):
     return out
 
 
+def setup_macro(text):
+    """The variables a setup_*.h macro declares (src/tests/unittests/setup_h2o_2_na_1.h): numbers,
+    double vectors (with or without '='), and the monomer names."""
+    src = re.sub(r"//[^\n]*", "", re.sub(r"\\\n", "\n", text))
+    out = {}
+    for m in re.finditer(r"std::vector<double>\s+(\w+)\s*=?\s*\{([^}]*)\}\s*;", src):
+        out[m.group(1)] = [float(v) for v in m.group(2).replace("\n", " ").split(",") if v.strip()]
+    for m in re.finditer(r"(?<![\w>])double\s+(\w+)\s*=\s*([-+0-9.eE]+)\s*;", src):
+        out[m.group(1)] = float(m.group(2))
+    m = re.search(r"std::vector<std::string>\s+monomer_names\s*=\s*\{([^}]*)\}", src)
+    out["monomer_names"] = [v.strip().strip('"') for v in m.group(1).split(",") if v.strip()]
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     res = {}
@@ -200,6 +230,13 @@ def main():
     path = "src/unittests/setup_h2o_2_ch4_1.h"
     res["system_h2o_2_ch4_1"] = test_data(open(os.path.join(ref, path)).read())
     res["system_h2o_2_ch4_1"]["_source"] = path + " (TestEnergyData of unittest-system.cpp:215-240)"
+    for name in SETUPS:
+        path = "src/tests/unittests/%s.h" % name
+        d = setup_macro(open(os.path.join(ref, path)).read())
+        keep = ("one_body_energy", "two_body_energy", "three_body_energy", "real_coords", "monomer_names",
+                "virial_1b", "virial_2b", "virial_3b")
+        res[name] = {k: d[k] for k in keep if k in d}  # setup_h2o_1_na_1.h has no virial_3b
+        res[name]["_source"] = path
     
     with open(os.path.join(HERE, "kats.json"), "w") as f:
         json.dump(res, f)

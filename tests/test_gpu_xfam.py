@@ -2,7 +2,9 @@
 oracle.clusters(4)).
 
 Families: halide-water 2B and halide-water-water 3B (f-, cl-, br-, i-), CO2-CO2 2B, CO2 x3 3B,
-water x4 4B, alone and mixed with MB-pol and the CH4 families.
+water x4 4B, alkali-metal-ion-water 2B and 3B (li+, na+, k+, rb+, cs+), alone and mixed with
+MB-pol and the CH4 families; the reference's Na+ / Cs+ water setups against their published
+2B / 3B energies.
 
 Bar: tetramer lists bit-exact as sets; listed and evaluated counts equal; each order's energy
 within 1e-10 relative plus 1e-15 of the sum of the cancelling terms' magnitudes (cond: the CO2
@@ -23,8 +25,11 @@ E_REL = 1e-10
 COND = 1e-15
 G_ABS = 1e-8
 IONS = ("f-", "cl-", "br-", "i-")
-XF_2B = ("f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "co2co2_2b")
-XF_3B = ("f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b", "co2x3_3b")
+ALKALI = ("li+", "na+", "k+", "rb+", "cs+")
+XF_ALK_2B = ("li_h2o_2b", "na_h2o_2b", "k_h2o_2b", "rb_h2o_2b", "cs_h2o_2b")
+XF_ALK_3B = ("li_h2o2_3b", "na_h2o2_3b", "k_h2o2_3b", "rb_h2o2_3b", "cs_h2o2_3b")
+XF_2B = ("f_h2o_2b", "cl_h2o_2b", "br_h2o_2b", "i_h2o_2b", "co2co2_2b") + XF_ALK_2B
+XF_3B = ("f_h2o2_3b", "cl_h2o2_3b", "br_h2o2_3b", "i_h2o2_3b", "co2x3_3b") + XF_ALK_3B
 
 
 def _rot(rng):
@@ -36,13 +41,13 @@ def _rot(rng):
                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
 
 
-def halide_water(systems, nion=16, every=None):
-    """The 256-water cubic box with nion waters replaced by ions (f-, cl-, br-, i- in turn) at
+def halide_water(systems, nion=16, every=None, ions=IONS):
+    """The 256-water cubic box with nion waters replaced by ions (those of `ions` in turn) at
     their oxygen positions."""
     xyz, box, _, _, _ = water_case(systems, "h2o_256_cubic")
     w = xyz.reshape(-1, 3, 3)
     step = every or len(w) // nion
-    pick = {i * step: IONS[i % 4] for i in range(nion)}
+    pick = {i * step: ions[i % len(ions)] for i in range(nion)}
     names, out = [], []
     for i in range(len(w)):
         if i in pick:
@@ -70,15 +75,15 @@ def co2_box(n_side=5, spacing=4.5, seed=7):
 
 
 def mixed(systems, seed=3):
-    """The clathrate (CH4 + water) with some waters turned into ions and some into CO2 (carbon at
-    the oxygen, a random axis)."""
+    """The clathrate (CH4 + water) with some waters turned into ions (halides and alkali metals)
+    and some into CO2 (carbon at the oxygen, a random axis)."""
     rng = np.random.default_rng(seed)
     xyz, mon, box = systems["clathrate/xyz"], list(systems["clathrate/mon"]), systems["clathrate/box"]
     nat = [3 if m == "h2o" else 5 for m in mon]
     first = np.concatenate([[0], np.cumsum(nat)[:-1]])
     waters = [i for i, m in enumerate(mon) if m == "h2o"]
-    pick = rng.choice(waters, 24, replace=False)
-    conv = {int(p): (IONS[i % 4] if i < 12 else "co2") for i, p in enumerate(pick)}
+    pick = rng.choice(waters, 30, replace=False)
+    conv = {int(p): ((IONS + ALKALI)[i % 9] if i < 18 else "co2") for i, p in enumerate(pick)}
     names, out = [], []
     for i, m in enumerate(mon):
         a = xyz[3 * first[i]:3 * (first[i] + nat[i])].reshape(-1, 3)
@@ -182,9 +187,57 @@ def test_co2(periodic):
     assert xc["co2co2_2b"] > 0 and xc["co2x3_3b"] > 0
 
 
+def test_alkali_water_box(systems):
+    """Alkali-metal ions in the 256-water box, 3B cutoff 7 A: the x3b_h2o_ion_v1x_deg4_filtered
+    skip rule (two of three separations beyond 4.5 A) drops most listed ion-water-water trimers."""
+    xyz, names, box = halide_water(systems, nion=20, ions=ALKALI)
+    print("alkali_water 256 box, 20 ions", flush=True)
+    e = run_gpu(xyz, names, box, 9.0, 7.0)
+    r = oracle.xsystem(xyz, names, box, 9.0, 7.0, tags=np.arange(len(names)), cond=True)
+    check(e, r)
+    xc = e.xcounts()
+    assert all(xc[f] > 0 for f in XF_ALK_2B + XF_ALK_3B)
+
+
+def test_alkali_water_cluster_and_ignore_lists(systems):
+    xyz, names, _ = halide_water(systems, nion=10, ions=ALKALI)
+    print("alkali_water 256 gas phase, 10 ions", flush=True)
+    e = run_gpu(xyz, names, None, 9.0, 6.0)
+    r = oracle.xsystem(xyz, names, None, 9.0, 6.0, tags=np.arange(len(names)), cond=True)
+    check(e, r)
+    skip = ("na_h2o_2b", "cs_h2o2_3b")
+    e = run_gpu(xyz, names, None, 9.0, 6.0, families=[f for f in engine.FAMILIES if f not in skip])
+    r = oracle.xsystem(xyz, names, None, 9.0, 6.0, tags=np.arange(len(names)), cond=True,
+                       ignore=[("h2o", "na+"), ("cs+", "h2o", "h2o")])
+    check(e, r)
+    assert e.xcounts()["na_h2o_2b"] == 0 and e.xcounts()["cs_h2o2_3b"] == 0
+
+
+@pytest.mark.parametrize("setup", ["setup_h2o_1_na_1", "setup_h2o_2_na_1", "setup_h2o_1_cs_1", "setup_h2o_2_cs_1"])
+def test_alkali_water_setups(setup):
+    """The reference's Na+ / Cs+ water setups (src/tests/unittests/setup_h2o_<n>_<ion>_1.h): the
+    published 2B and 3B energies (11 digits; plus the rounding the Cs+ trimer's cancelling terms
+    leave), gas phase, with the oracle's gradients."""
+    import json
+    import os
+
+    from conftest import GOLDEN
+
+    d = json.load(open(os.path.join(GOLDEN, "kats.json")))[setup]
+    names, xyz = d["monomer_names"], np.array(d["real_coords"])
+    print(setup, names, flush=True)
+    e = run_gpu(xyz, names, None, 100.0, 100.0)
+    r = oracle.xsystem(xyz, names, None, 100.0, 100.0, tags=np.arange(len(names)), cond=True)
+    check(e, r)
+    te = e.term_energies()
+    assert abs(te[1] - d["two_body_energy"]) <= 6e-11 * abs(d["two_body_energy"]) + COND * r["cond"][2]
+    assert abs(te[2] - d["three_body_energy"]) <= 6e-11 * abs(d["three_body_energy"]) + COND * r["cond"][3]
+
+
 def test_mixed_all_families(systems):
-    """MB-pol, CH4-H2O, CH4-CH4, CH4-H2O-H2O, halide-water and CO2 families in one box; n-mers
-    whose names have no holder (co2-h2o, ion-ion, ...) are listed and skipped."""
+    """MB-pol, CH4-H2O, CH4-CH4, CH4-H2O-H2O, halide-water, alkali-ion-water and CO2 families
+    in one box; n-mers whose names have no holder (co2-h2o, ion-ion, ...) are listed and
+    skipped."""
     xyz, names, box = mixed(systems)
     print("mixed clathrate + ions + co2", flush=True)
     e = run_gpu(xyz, names, box, 9.0, 4.5)

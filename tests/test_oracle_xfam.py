@@ -3,16 +3,24 @@ tetramer lists) to the reference.
 
 Families: halide-water 2B (mbnrg_A1_B1C2X2_deg6_v1: f-, cl-, br-, i-), halide-water-water 3B
 (mbnrg_A1_B1C2X2_B1C2X2_deg4_v1), CO2-CO2 2B (x2b_A1B2_A1B2_v1x, the "co2" model), CO2 x3 3B
-(mbnrg_A1B2_A1B2_A1B2_deg4_v1) and water x4 4B (mbnrg_A1B2_A1B2_A1B2_A1B2_deg4_v1).
+(mbnrg_A1B2_A1B2_A1B2_deg4_v1), water x4 4B (mbnrg_A1B2_A1B2_A1B2_A1B2_deg4_v1), and the
+alkali-metal-ion-water 2B (x2b_h2o_ion_v2x) and 3B (x3b_h2o_ion_v1x_deg4_filtered: li+, na+, k+,
+rb+, cs+).
 
 Pins:
   * energy routines -- the reference's known answers of unittest-energy2b.cpp (f--h2o, cl-h2o,
                        br-h2o, i-h2o, the second co2-co2 section), unittest-energy3b.cpp
-                       (f/cl/br/i-h2o-h2o, co2-co2-co2) and unittest-energy4b.cpp (h2o x4):
+                       (f/cl/br/i-h2o-h2o, co2-co2-co2, li-h2o-h2o, cs-h2o-h2o) and
+                       unittest-energy4b.cpp (h2o x4):
                        energy, gradients and virial, with the names given in every order the
                        reference tests (the name sorts of get_2b/3b/4b_energy);
-  * polynomial      -- unittest-poly-2b.cpp :poly_2b_A1B2_A1B2_v1x:struct (21-digit value and
-                       gradient) and linearity / Euler's identity for every table;
+  * alkali holders  -- unittest-poly-holder-2b.cpp x2b_h2o_ion_v2x::struct and
+                       unittest-poly-holder-3b.cpp x3b-3b-h2o-ion-v1x_deg4_filtered::class, every
+                       ion (energy, gradients, virial); the Na+ and Cs+ water setups'
+                       System-level 2B / 3B energies (src/tests/unittests/setup_h2o_{1,2}_{na,cs}_1.h);
+  * polynomial      -- unittest-poly-2b.cpp :poly_2b_A1B2_A1B2_v1x:struct and unittest-poly-3b.cpp
+                       poly-3b-h2o-ion-v1x_deg4_filtered::class (21-digit value and gradient) and
+                       linearity / Euler's identity for every table;
   * holders         -- the compiled reference holders (oracle/_ref/libmbxref_x.so) on perturbed
                        n-mers, to rounding (1e-15 of the sum of the cancelling terms' magnitudes);
                        five-point central differences of the oracle's energy;
@@ -38,7 +46,13 @@ ref_only = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref no
 FAMS = {"f_h2o_2b": "energy_f_h2o_2b", "cl_h2o_2b": "energy_cl_h2o_2b", "br_h2o_2b": "energy_br_h2o_2b",
         "i_h2o_2b": "energy_i_h2o_2b", "co2co2_2b": "energy_co2co2_2b", "f_h2o2_3b": "energy_f_h2o2_3b",
         "cl_h2o2_3b": "energy_cl_h2o2_3b", "br_h2o2_3b": "energy_br_h2o2_3b", "i_h2o2_3b": "energy_i_h2o2_3b",
-        "co2x3_3b": "energy_co2x3_3b", "h2ox4_4b": "energy_h2ox4_4b"}
+        "co2x3_3b": "energy_co2x3_3b", "h2ox4_4b": "energy_h2ox4_4b", "li_h2o2_3b": "energy_li_h2o2_3b",
+        "cs_h2o2_3b": "energy_cs_h2o2_3b"}
+ALKALI = ("li", "na", "k", "rb", "cs")
+# holder known answers, in the holder's own argument order (water(s), then the ion)
+HOLDER_KATS = {"%s_h2o_2b" % a: "holder_%s_h2o_2b" % a for a in ALKALI}
+HOLDER_KATS.update({"%s_h2o2_3b" % a: "holder_%s_h2o2_3b" % a for a in ALKALI})
+ALL = sorted(set(FAMS) | set(HOLDER_KATS))
 TOL = 1e-6  # the reference's own margin (unittest-energy2b.cpp:45)
 
 
@@ -110,7 +124,8 @@ def test_poly_co2co2_kat():
     np.testing.assert_allclose(g, k["g_expected"], rtol=1e-10, atol=1e-14 * ea)
 
 
-@pytest.mark.parametrize("poly", ["halide_h2o_2b", "halide_h2o2_3b", "co2co2_2b", "co2x3_3b", "h2ox4_4b"])
+@pytest.mark.parametrize("poly", ["halide_h2o_2b", "halide_h2o2_3b", "co2co2_2b", "co2x3_3b", "h2ox4_4b",
+                                  "alkali_h2o_2b", "alkali_h2o2_3b"])
 def test_poly_linear_and_euler(poly):
     t = oracle.table(poly)
     rng = np.random.default_rng(7)
@@ -125,7 +140,65 @@ def test_poly_linear_and_euler(poly):
     assert abs(float(np.dot(a, basis)) - ea) <= 1e-14 * scale
 
 
+def holder_case(name):
+    """A holder known answer: (case, n-mers as lists of monomers in the holder's order)."""
+    k = KATS[HOLDER_KATS[name]]
+    order = 2 if name.endswith("_2b") else 3
+    n = k.get("ndim", k.get("ntrim"))
+    xs = [np.array(k["xyz%d" % (i + 1)]).reshape(n, -1) for i in range(order)]
+    return k, [[x[j] for x in xs] for j in range(n)]
+
+
+@pytest.mark.parametrize("name", sorted(HOLDER_KATS))
+def test_alkali_holder_kats(name):
+    k, nmers = holder_case(name)
+    order = len(nmers[0])
+    E, V = 0.0, np.zeros(9)
+    G = [np.zeros((len(nmers), len(m))) for m in nmers[0]]
+    for j, ws in enumerate(nmers):
+        e, gs, v, _ = oracle.xnmer(name, ws)
+        E += e
+        V += v
+        for i in range(order):
+            G[i][j] = gs[i]
+    want = k["energy_expected"]
+    assert abs(E - want) <= 1e-9 * abs(want) + 1e-11, (E, want)  # printed with 11 digits
+    for i in range(order):
+        np.testing.assert_allclose(G[i].reshape(-1), k["grad%d_expected" % (i + 1)], rtol=0, atol=TOL)
+    np.testing.assert_allclose(V, k["virial_expected"], rtol=0, atol=TOL)
+
+
+def test_alkali_poly_kat():
+    k = KATS["poly_alkali_h2o2_3b"]
+    t = oracle.table("alkali_h2o2_3b")  # the known answer comes with coefficients of its own
+    e, g = oracle.poly_eval(t, k["x"], a=k["a"])
+    ea, _ = oracle.poly_eval(t, np.abs(k["x"]), a=np.abs(k["a"]))
+    assert abs(e - k["e_expected"]) <= 1e-15 * ea + 1e-13 * abs(e)
+    np.testing.assert_allclose(g, k["g_expected"], rtol=1e-10, atol=1e-14 * ea)
+
+
+@pytest.mark.parametrize("setup", ["setup_h2o_1_na_1", "setup_h2o_2_na_1", "setup_h2o_1_cs_1", "setup_h2o_2_cs_1"])
+def test_alkali_water_system_energies(setup):
+    """One or two waters with Na+ or Cs+ through the System-level orchestration (gas phase): the
+    2B energy (water-water MB-pol plus the ion-water dimers) and the 3B energy (the ion-water-water
+    trimer) the reference's setups hold, printed with 11 digits; their virials with 6."""
+    d = KATS[setup]
+    r = oracle.xsystem(d["real_coords"], d["monomer_names"], None, 100.0, 100.0, cond=True)
+    nw = d["monomer_names"].count("h2o")
+    assert r["counts"][2] == (nw * (nw + 1) // 2,) * 2 and r["counts"][3] == ((1, 1) if nw == 2 else (0, 0))
+    # half a unit of the printed 11th digit, plus the rounding any evaluation order leaves (the
+    # Cs+ trimer's cancelling terms sum to 8e5 for E3B = 0.2)
+    assert abs(r["e2b"] - d["two_body_energy"]) <= 6e-11 * abs(d["two_body_energy"]) + 1e-15 * r["cond"][2]
+    assert abs(r["e3b"] - d["three_body_energy"]) <= 6e-11 * abs(d["three_body_energy"]) + 1e-15 * r["cond"][3]
+    vir = np.array(d["virial_2b"]) + np.array(d.get("virial_3b", np.zeros(9)))
+    np.testing.assert_allclose(r["virial"], vir, rtol=1e-5, atol=1e-5)
+
+
 def perturbed(name, rng, sigma=0.05):
+    if name in HOLDER_KATS:
+        _, nmers = holder_case(name)
+        ws = nmers[rng.integers(len(nmers))]
+        return [w + rng.normal(0.0, sigma, w.shape) for w in ws]
     k, names, xs = kat_case(name)
     hn, perm = oracle.holder_order(names)
     j = rng.integers(len(xs[0]))
@@ -133,7 +206,7 @@ def perturbed(name, rng, sigma=0.05):
 
 
 @refx_only
-@pytest.mark.parametrize("name", sorted(FAMS))
+@pytest.mark.parametrize("name", ALL)
 def test_holders_vs_compiled_reference(name):
     rng = np.random.default_rng(11)
     evaluated = 0
@@ -151,11 +224,14 @@ def test_holders_vs_compiled_reference(name):
     assert evaluated > 0
 
 
-@pytest.mark.parametrize("name", sorted(FAMS))
+@pytest.mark.parametrize("name", ALL)
 def test_holder_finite_differences(name):
     rng = np.random.default_rng(5)
-    ws = perturbed(name, rng, sigma=0.02)
-    e, gs, _, ev = oracle.xnmer(name, ws)
+    for _ in range(50):  # an evaluated n-mer (some known-answer cases hold skipped ones)
+        ws = perturbed(name, rng, sigma=0.02)
+        e, gs, _, ev = oracle.xnmer(name, ws)
+        if ev:
+            break
     assert ev == 1
     # five-point stencil; the energy's own rounding (about eps times the sum of the cancelling
     # terms' magnitudes, 2e7 for CO2-CO2) enters the difference quotient divided by h
@@ -175,11 +251,13 @@ def test_holder_finite_differences(name):
 def test_skip_rule_and_switch_ends():
     """Beyond the outer radius on every pair the n-mer is skipped; inside the inner radius on
     every pair the switch combination is 1 (2B: sw12; 3B: s12 s13 + s12 s23 + s13 s23 - 2 s12 s13 s23;
-    4B: the center-3-neighbour combination)."""
-    for name in FAMS:
+    4B: the center-3-neighbour combination), and 3 for the alkali-ion 3B holder, whose combination
+    s_ab s_ac + s_ab s_bc + s_ac s_bc has no cubic term (x3b-h2o-ion-v1x-deg4-filtered.cpp:306)."""
+    for name in ALL:
         h = oracle.xholder(name).desc
         terms = h["switch"]["terms"]
-        assert abs(sum(c for c, _ in terms) - 1.0) < 1e-15  # all switches 1 -> sw = 1
+        want = 3.0 if h["poly"] == "alkali_h2o2_3b" else 1.0
+        assert abs(sum(c for c, _ in terms) - want) < 1e-15  # all switches 1
         ws = perturbed(name, np.random.default_rng(2), sigma=0.0)
         far = [w + 100.0 * i for i, w in enumerate(ws)]
         e, _, _, ev = oracle.xnmer(name, far)
@@ -230,3 +308,24 @@ def test_tetramers_vs_compiled_reference():
         got = sorted_tuples(oracle.clusters(4, 5.0, first, b))
         want = sorted_tuples(oracle.ref_add_clusters(4, 5.0, first, b))
         assert len(got) == len(want) and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("ion", ALKALI)
+def test_alkali_3b_skips_two_pairs_beyond(ion):
+    """x3b_h2o_ion_v1x_deg4_filtered skips a trimer when two of its three O-O / O-X separations
+    exceed r3f = 4.5 A (x3b-h2o-ion-v1x-deg4-filtered.cpp:295); with one beyond it is evaluated
+    (and its switch combination, one product of two switches, is not zero)."""
+    name = "%s_h2o2_3b" % ion
+    _, nmers = holder_case(name)
+    w1, w2, x = nmers[0]
+    o1 = w1[:3]
+    def place(w, target):  # rigid shift of a monomer so its first atom sits at target
+        return w - w[:3].tolist() * (len(w) // 3) + np.tile(target, len(w) // 3)
+    u = np.array([1.0, 0.0, 0.0])
+    v = np.array([0.0, 1.0, 0.0])
+    # one pair beyond (a-b at 5 A, a-x and b-x near 3.5 A): evaluated
+    e1, _, _, ev1 = oracle.xnmer(name, [w1, place(w2, o1 + 5.0 * u), place(x, o1 + 2.5 * u + 2.5 * v)])
+    assert ev1 == 1 and e1 != 0.0
+    # two pairs beyond (x at 6 A from both waters, the waters 3 A apart): skipped
+    e2, _, _, ev2 = oracle.xnmer(name, [w1, place(w2, o1 + 3.0 * u), place(x, o1 + 1.5 * u + 6.0 * v)])
+    assert ev2 == 0 and e2 == 0.0

@@ -97,11 +97,12 @@ def test_external_call_orders_mixed_input(systems, tmp_path):
 
 
 ATOMS_OF = {"h2o": ["O", "H", "H"], "ch4": ["C", "H", "H", "H", "H"], "co2": ["C", "O", "O"], "f-": ["F"],
-            "cl-": ["Cl"], "br-": ["Br"], "i-": ["I"]}
+            "cl-": ["Cl"], "br-": ["Br"], "i-": ["I"], "li+": ["Li"], "na+": ["Na"], "k+": ["K"], "rb+": ["Rb"],
+            "cs+": ["Cs"]}
 
 
 def test_external_call_further_families_and_4b(systems, tmp_path):
-    """Waters, CH4, CO2 and halide ions in one periodic box with a 4B cutoff, through the
+    """Waters, CH4, CO2, halide and alkali-metal ions in one periodic box with a 4B cutoff, through the
     reference's C entry points: Energy() = 1B + 2B + 3B + 4B (system.cpp:2060-2087), the
     decomposition's E4b = FourBodyEnergy(false), gradients in input order."""
     import test_gpu_xfam as X
