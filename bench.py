@@ -404,6 +404,19 @@ def main():
         poly_ms.append(tm["poly_3b"])
         for k, v in tm.items():
             stage[k] = stage.get(k, 0.0) + v
+    # the polynomial launch without the 3B chain rule (fuse_chain 0: the chain rule as a kernel of
+    # its own), for the polynomial's own roofline figure
+    unfused = None
+    if eng.tuning("fuse_chain") == 1:
+        eng.set_tuning("fuse_chain", 0)
+        up, uc = [], []
+        for _ in range(args.steps):
+            step()
+            tm = eng.timings()
+            up.append(tm["poly_3b"])
+            uc.append(tm["chain_3b"])
+        eng.set_tuning("fuse_chain", 1)
+        unfused = (float(np.mean(up)), float(np.mean(uc)))
     eng.set_timing(False)
     # the same step with the gradient (and energies) copied to the host every step: synchronous,
     # PCIe-inclusive (graph mode again)
@@ -426,10 +439,14 @@ def main():
     flops = json.load(open(os.path.join(ROOT, "mbx_amd", "tables", "flops.json")))
     f3, f2 = flops["mbpol_3b_eg"]["flops"], flops["mbpol_2b_eg"]["flops"]
     paired = eng.tuning("fuse_poly") == 1 and counts[1] > 0 and counts[3] > 0
+    chained = paired and eng.tuning("fuse_chain") == 1
+    if chained:  # the launch also runs the 3B chain rule: its algorithmic work counts too
+        f3 += flops["mbpol_3b_chain"]["flops"]
     t_poly = float(np.mean(poly_ms)) * 1e-3
     achieved = (counts[3] * f3 + (counts[1] * f2 if paired else 0)) / t_poly
     traffic = read_traffic()
-    kernel = ("k_poly_eg_pair<36, 31> (MB-pol 3B and 2B polynomials E+G, one launch)" if paired
+    kernel = ("k_poly_eg_pair_chain3<36, 31> (MB-pol 3B and 2B polynomials E+G and the 3B chain rule, one launch)"
+              if chained else "k_poly_eg_pair<36, 31> (MB-pol 3B and 2B polynomials E+G, one launch)" if paired
               else "k_poly_eg<36> (MB-pol 3B polynomial E+G)")
 
     out = {
@@ -459,6 +476,13 @@ def main():
         "cpu_baseline": None,
     }
     # list build (fix, cloud, binning, both n-mer passes): PMC bytes per step over its HIP-event time
+    if chained and unfused:
+        f3p = flops["mbpol_3b_eg"]["flops"]
+        a_u = (counts[3] * f3p + counts[1] * f2) / (unfused[0] * 1e-3)
+        out["config"]["polynomial_alone"] = {
+            "kernel": "k_poly_eg_pair<36, 31> (fuse_chain 0; the 3B chain rule then runs as k_grad_mbpol3b)",
+            "launch_ms": unfused[0], "achieved_TFLOPs": a_u / 1e12, "frac": a_u / FP64_VECTOR_PEAK,
+            "chain_3b_ms": unfused[1]}
     out["config"]["with_grad_d2h"] = {
         "ms_per_step": t_d2h * 1e3, "nmers_per_s": n_listed / t_d2h,
         "step": "the headline step plus mbxg_get_results (energies, virial and the 6144 x 3 gradient copied to "

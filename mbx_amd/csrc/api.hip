@@ -185,6 +185,7 @@ struct mbxg_ctx {
     int split2 = 4, split3 = 2, split3n = 4;
     int fuse_poly = 1;       // MB-pol 2B and 3B polynomials in one launch when both are evaluated
     int fuse_prep = 1;       // FixMonomerCoordinates, cloud and cell counts in one launch (k_cloud_fix)
+    int fuse_chain = 1;      // the 3B chain rule in the paired polynomial launch's 3B blocks
     bool fused_last = false;  // the last compute did so (its 2B polynomial time is in the 3B one)
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
@@ -1208,11 +1209,18 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                 if (ev) HIPCHK(hipEventRecord(ev[8], st));
                 HIPCHK(launch_mbpol3b_vars(L.evals[f].p, nd, cap, N, o.ws.p, st));
                 if (ev) HIPCHK(hipEventRecord(ev[9], st));
-                HIPCHK(launch_poly_mbpol_pair(o.ws.p, nd, cap, c->d_coef3.p, o2.ws.p, nd2, cap2, c->d_coef2.p,
-                                              c->split3, st));
-                if (ev) HIPCHK(hipEventRecord(ev[10], st));
-                HIPCHK(launch_mbpol3b_chain(L.evals[f].p, nd, cap, N, 1, c->d_grad.p, o.e_nmer.p, vir, o.partials.p,
-                                            o.ws.p, st));
+                if (c->fuse_chain) {
+                    HIPCHK(launch_poly_mbpol_pair_chain3(o.ws.p, L.evals[f].p, nd, cap, c->d_coef3.p, o2.ws.p, nd2,
+                                                         cap2, c->d_coef2.p, c->split3, N, c->d_grad.p, o.e_nmer.p,
+                                                         vir, o.partials.p, st));
+                    if (ev) HIPCHK(hipEventRecord(ev[10], st));
+                } else {
+                    HIPCHK(launch_poly_mbpol_pair(o.ws.p, nd, cap, c->d_coef3.p, o2.ws.p, nd2, cap2, c->d_coef2.p,
+                                                  c->split3, st));
+                    if (ev) HIPCHK(hipEventRecord(ev[10], st));
+                    HIPCHK(launch_mbpol3b_chain(L.evals[f].p, nd, cap, N, 1, c->d_grad.p, o.e_nmer.p, vir,
+                                                o.partials.p, o.ws.p, st));
+                }
                 if (ev) HIPCHK(hipEventRecord(ev[11], st));
                 if (ev) HIPCHK(hipEventRecord(ev[6], st));
                 HIPCHK(launch_mbpol2b_chain(L.evals[f2].p, nd2, cap2, N, 1, c->d_grad.p, o2.e_nmer.p,
@@ -1411,7 +1419,7 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
     const std::string k(key);
     if ((k == "split_2b" || k == "split_3b" || k == "split_ch4h2o2_3b") && !valid_split(value))
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
-    if ((k == "fuse_poly" || k == "fuse_prep") && value != 0 && value != 1)
+    if ((k == "fuse_poly" || k == "fuse_prep" || k == "fuse_chain") && value != 0 && value != 1)
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s is 0 or 1", key);
     if (k == "split_2b")
         c->split2 = value;
@@ -1423,6 +1431,8 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
         c->fuse_poly = value;
     else if (k == "fuse_prep")
         c->fuse_prep = value;
+    else if (k == "fuse_chain")
+        c->fuse_chain = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1444,6 +1454,8 @@ extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
         *value = c->fuse_poly;
     else if (k == "fuse_prep")
         *value = c->fuse_prep;
+    else if (k == "fuse_chain")
+        *value = c->fuse_chain;
     else
         return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
     return MBXG_OK;

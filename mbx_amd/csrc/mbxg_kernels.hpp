@@ -210,6 +210,12 @@ const double* co2_1b_fit(int* n);
 __global__ void k_reduce_1b(const double* e, const double* vir, int n, int ld, double* out);
 hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
                                   const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);
+// The same launch with the 3B chain rule (launch_mbpol3b_chain with gradients) in its tail:
+// energies e3_out, gradient added to grad, virials vir3_out (nullable), partial records partials3.
+hipError_t launch_poly_mbpol_pair_chain3(double* ws3, const int* list3, const int* n3_dev, int cap3,
+                                         const double* coef3, double* ws2, const int* n2_dev, int cap2,
+                                         const double* coef2, int split, const NmerCtx& N, double* grad,
+                                         double* e3_out, double* vir3_out, double* partials3, hipStream_t st);
 // MB-nrg families (eval_nrg.hip): the same three stages for FAM_CH4H2O_2B, FAM_CH4CH4_2B and
 // FAM_CH4H2O2_3B; list entries in emission order (the kernels order each n-mer's monomers
 // for its holder).  `split` (1, 2, 4 or 8) is the wavefronts per 64 n-mers of the blocked

@@ -61,4 +61,34 @@ __device__ __forceinline__ void table_flush(const GradTable<TB, NA>& T, const in
     }
 }
 
+// A table that one wavefront owns alone (the other wavefronts of its block never touch it):
+// lane-strided init and flush, no block barrier.  The LDS unit performs one wavefront's
+// operations in issue order, so between the phases only the compiler has to be kept from moving
+// them and the outstanding LDS operations have to be waited for: wave_lds_order.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int TB, int NA>
+__device__ __forceinline__ void table_init_wave(GradTable<TB, NA>& T) {
+    const int lane = threadIdx.x & 63;
+    for (int i = lane; i < TB; i += 64) T.key[i] = -1;
+    for (int i = lane; i < TB * NA; i += 64) T.val[i] = 0.0;
+}
+
+template <int TB, int NA>
+__device__ __forceinline__ void table_flush_wave(const GradTable<TB, NA>& T, const int* first_atom, const int* nat,
+                                                 double* grad) {
+    for (int i = threadIdx.x & 63; i < TB; i += 64) {
+        const int mon = T.key[i];
+        if (mon < 0) continue;
+        double* gp = grad + 3 * (size_t)first_atom[mon];
+        const int cnt = 3 * nat[mon];
+#pragma unroll
+        for (int c = 0; c < NA; ++c)
+            if (c < cnt) unsafeAtomicAdd(gp + c, T.val[i * NA + c]);
+    }
+}
+
 }  // namespace mbxg

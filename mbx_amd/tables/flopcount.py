@@ -11,6 +11,7 @@ of how this project schedules it.
     poly_3b_v2x  E+G   potential/3b/poly-3b-v2x.cpp:1-26980     (f0..f130)
     poly_2b_v6x  E     potential/2b/poly-2b-v6x-nogrd.cpp:41
     poly_3b_v2x  E     potential/3b/poly-3b-v2x-nogrd.cpp:39
+    x3b_v2x chain rule potential/3b/x3b-v2x.cpp:62-76, 1620-1726 (counted by hand, see CHAIN_3B)
 
 Writes mbx_amd/tables/flops.json.  Usage: python mbx_amd/tables/flopcount.py [/root/reference]
 """
@@ -20,6 +21,18 @@ import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# The 3B holder's work after the polynomial, per evaluated trimer (x3b-v2x.cpp:1620-1726), in the
+# same units (multiplications, additions/subtractions, and divisions; sqrt, exp and negations not
+# counted): the 36 gradients scaled by the switch product (36 mul); 36 calls of g_var (:62-76):
+# the separation (3 sub), its square (3 mul + 2 add), gg = -k g exp(-k (d - r0)) / d (3 mul,
+# 1 sub, 1 div) and the two atom updates (6 mul + 6 add/sub), 25 each; the three switch
+# derivatives (3 x (1 add + 2 mul + 1 div)); e += s E (1 mul + 1 add); the switch-derivative
+# gradient of the three oxygens (9 x (2 mul + 2 add)); the virial (6 x (9 mul + 9 add)).
+CHAIN_3B = dict(mul=36 + 36 * 12 + 6 + 1 + 18 + 54, add=36 * 12 + 3 + 1 + 18 + 54, div=36 + 3,
+                source="src/potential/3b/x3b-v2x.cpp:62-76, 1620-1726")
+CHAIN_3B["flops"] = CHAIN_3B["mul"] + CHAIN_3B["add"] + CHAIN_3B["div"]
 
 
 def count_ops(text):
@@ -79,6 +92,7 @@ def main():
         text = open(os.path.join(ref, path)).read()
         m, a = count_ops(nogrd_body(text))
         res[key + "_e"] = dict(mul=m, add=a, flops=m + a, source=path)
+    res["mbpol_3b_chain"] = CHAIN_3B
     with open(os.path.join(HERE, "flops.json"), "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res, indent=1))

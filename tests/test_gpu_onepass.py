@@ -126,3 +126,38 @@ def test_fused_prep_matches_separate_launches(systems, case):
     a, b = out[1], out[0]
     _same(a[:6], b[:6])
     assert np.array_equal(a[6], b[6])
+
+
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+def test_fused_chain_matches_separate_kernel(systems, split):
+    """The MB-pol 3B chain rule in the paired polynomial launch's 3B blocks (fuse_chain = 1, the
+    default) against its own kernel (fuse_chain = 0), for every split of the 64-trimer blocks:
+    bit-identical energies (the same per-trimer arithmetic and the same records of 64 trimers),
+    gradients and virial to the rounding of the atomics' order.  Eager and graph replays; and the
+    oracle's energies and gradients."""
+    from conftest import water_case
+
+    xyz, box, c2, c3, _ = water_case(systems, "h2o_2048_cubic")
+    kinds = [engine.H2O] * (len(xyz) // 9)
+    out = {}
+    for fuse in (1, 0):
+        print("fuse_chain", fuse, "split", split, flush=True)
+        e = engine.Engine()
+        e.set_topology(kinds)
+        e.set_box(box)
+        e.set_cutoffs(c2, c3)
+        e.set_tuning("split_2b", split)
+        e.set_tuning("split_3b", split)
+        e.set_tuning("fuse_chain", fuse)
+        assert e.tuning("fuse_chain") == fuse
+        e.set_graph(True)
+        e.set_coords(xyz)
+        for _ in range(3):  # sizing compute, eager, graph replay
+            e.compute()
+        out[fuse] = _state(e)
+    _same(out[1], out[0])
+    if split == 4:
+        r = oracle.water_system(xyz, box, c2, c3)
+        e2, e3, g, _ = out[1][:4]
+        assert e2 == pytest.approx(r["e2b"], rel=1e-10) and e3 == pytest.approx(r["e3b"], rel=1e-10)
+        assert np.abs(g - r["grad"]).max() <= 1e-8
