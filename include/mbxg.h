@@ -225,7 +225,8 @@ int mbxg_set_graph(mbxg_ctx* ctx, int on);
  * when both run with gradients; "fuse_prep" (0 or 1, default 1): FixMonomerCoordinates, the
  * cloud and the cell counts in one launch; "fuse_chain" (0 or 1, default 1): with "fuse_poly", the
  * MB-pol 3B chain rule runs in the paired polynomial launch (each 3B block's tail) instead of a
- * kernel of its own.  mbxg_autotune times every split of both MB-pol orders
+ * kernel of its own; "fuse_vars" (0 or 1, default 1): with "fuse_poly", the 2B and 3B variables in
+ * one launch.  mbxg_autotune times every split of both MB-pol orders
  * on the current inputs
  * (the polynomial stage, `reps` timed repetitions after one warm-up) and, when CH4-H2O-H2O trimers
  * are evaluated, every split of that family (its whole evaluation), keeps the fastest of each and

@@ -186,6 +186,7 @@ struct mbxg_ctx {
     int fuse_poly = 1;       // MB-pol 2B and 3B polynomials in one launch when both are evaluated
     int fuse_prep = 1;       // FixMonomerCoordinates, cloud and cell counts in one launch (k_cloud_fix)
     int fuse_chain = 1;      // the 3B chain rule in the paired polynomial launch's 3B blocks
+    int fuse_vars = 1;       // the paired path's 2B and 3B variables in one launch
     bool fused_last = false;  // the last compute did so (its 2B polynomial time is in the 3B one)
     // stage timing (mbxg_set_timing): events bracketing each stage of the last compute
     bool timing = false;
@@ -1203,12 +1204,21 @@ int run(mbxg_ctx* c, int terms, int do_grads, bool sync) {
                 const int* nd2 = fam_ndev(L, c->np, f2);
                 // all on the main stream (the 2B stages on stream2 beside the 3B ones measured
                 // slower: 0.641-0.646 against 0.636-0.637 ms a step at 2048 waters)
-                if (ev) HIPCHK(hipEventRecord(ev[4], st));
-                HIPCHK(launch_mbpol2b_vars(L.evals[f2].p, nd2, cap2, N, o2.ws.p, st));
-                if (ev) HIPCHK(hipEventRecord(ev[5], st));
-                if (ev) HIPCHK(hipEventRecord(ev[8], st));
-                HIPCHK(launch_mbpol3b_vars(L.evals[f].p, nd, cap, N, o.ws.p, st));
-                if (ev) HIPCHK(hipEventRecord(ev[9], st));
+                if (c->fuse_vars) {  // both orders' variables in one launch (timed as the 3B stage)
+                    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+                    if (ev) HIPCHK(hipEventRecord(ev[5], st));
+                    if (ev) HIPCHK(hipEventRecord(ev[8], st));
+                    HIPCHK(launch_mbpol_vars_pair(L.evals[f].p, nd, cap, o.ws.p, L.evals[f2].p, nd2, cap2, o2.ws.p,
+                                                  N, st));
+                    if (ev) HIPCHK(hipEventRecord(ev[9], st));
+                } else {
+                    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+                    HIPCHK(launch_mbpol2b_vars(L.evals[f2].p, nd2, cap2, N, o2.ws.p, st));
+                    if (ev) HIPCHK(hipEventRecord(ev[5], st));
+                    if (ev) HIPCHK(hipEventRecord(ev[8], st));
+                    HIPCHK(launch_mbpol3b_vars(L.evals[f].p, nd, cap, N, o.ws.p, st));
+                    if (ev) HIPCHK(hipEventRecord(ev[9], st));
+                }
                 if (c->fuse_chain) {
                     HIPCHK(launch_poly_mbpol_pair_chain3(o.ws.p, L.evals[f].p, nd, cap, c->d_coef3.p, o2.ws.p, nd2,
                                                          cap2, c->d_coef2.p, c->split3, N, c->d_grad.p, o.e_nmer.p,
@@ -1419,7 +1429,7 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
     const std::string k(key);
     if ((k == "split_2b" || k == "split_3b" || k == "split_ch4h2o2_3b") && !valid_split(value))
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s must be 1, 2, 4 or 8", key);
-    if ((k == "fuse_poly" || k == "fuse_prep" || k == "fuse_chain") && value != 0 && value != 1)
+    if ((k == "fuse_poly" || k == "fuse_prep" || k == "fuse_chain" || k == "fuse_vars") && value != 0 && value != 1)
         return fail(MBXG_EINVAL, "mbxg_set_tuning: %s is 0 or 1", key);
     if (k == "split_2b")
         c->split2 = value;
@@ -1433,6 +1443,8 @@ extern "C" int mbxg_set_tuning(mbxg_ctx* c, const char* key, int value) {
         c->fuse_prep = value;
     else if (k == "fuse_chain")
         c->fuse_chain = value;
+    else if (k == "fuse_vars")
+        c->fuse_vars = value;
     else
         return fail(MBXG_EINVAL, "mbxg_set_tuning: unknown key '%s'", key);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1456,6 +1468,8 @@ extern "C" int mbxg_get_tuning(mbxg_ctx* c, const char* key, int* value) {
         *value = c->fuse_prep;
     else if (k == "fuse_chain")
         *value = c->fuse_chain;
+    else if (k == "fuse_vars")
+        *value = c->fuse_vars;
     else
         return fail(MBXG_EINVAL, "mbxg_get_tuning: unknown key '%s'", key);
     return MBXG_OK;

@@ -10,6 +10,7 @@
 //      virial (x3b-v2x.cpp:1622-1726); gradients scattered with hardware FP64 atomics, energy
 //      and virial reduced per wave.
 // Stage buffers are variable-major (entry (j, t) at j*ld + t).  Compiled with -ffp-contract=off.
+#include "mbpol2b_dev.hpp"
 #include "mbpol3b_chain.hpp"
 
 
@@ -19,21 +20,21 @@ namespace mbxg {
 __global__ void __launch_bounds__(64) k_vars_mbpol3b(const int* __restrict__ list, const int* __restrict__ n_dev,
                                                      int cap, NmerCtx N, double* __restrict__ xs,
                                                      double* __restrict__ sw, int ld) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= min(*n_dev, cap)) return;
-    double w[27];
-    load_trimer(N, list[3 * t], list[3 * t + 1], list[3 * t + 2], w);
-#pragma unroll
-    for (int j = 0; j < 36; ++j) {
-        const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
-        const double d0 = w[3 * a1] - w[3 * a2];
-        const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
-        const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
-        const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        xs[(size_t)j * ld + t] = exp(-kset(st) * (d - dset(st)));
-    }
-    double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
-    sw[t] = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
+    vars3_lane(blockIdx.x * 64 + threadIdx.x, list, n_dev, cap, N, xs, sw, ld);
+}
+
+// Stage 1 of both MB-pol orders in one launch: blocks [0, nb3) the trimers, the rest the dimers
+// (the 2B blocks take the slots the last 3B blocks leave, as in the paired polynomial launch).
+__global__ void __launch_bounds__(64) k_vars_mbpol_pair(const int* __restrict__ list3, const int* __restrict__ n3,
+                                                        int cap3, double* __restrict__ xs3, double* __restrict__ sw3,
+                                                        int ld3, int nb3, const int* __restrict__ list2,
+                                                        const int* __restrict__ n2, int cap2,
+                                                        double* __restrict__ xs2, double* __restrict__ sw2, int ld2,
+                                                        NmerCtx N) {
+    if ((int)blockIdx.x < nb3)
+        vars3_lane(blockIdx.x * 64 + threadIdx.x, list3, n3, cap3, N, xs3, sw3, ld3);
+    else
+        vars2_lane((blockIdx.x - nb3) * 64 + threadIdx.x, list2, n2, cap2, N, xs2, sw2, ld2);
 }
 
 // Stage 3: energy, and with GRAD the chain rule, virial and gradient scatter.  Blocks of 128
@@ -70,6 +71,18 @@ hipError_t launch_mbpol3b_vars(const int* list, const int* n_dev, int cap, const
     double* xs = ws;
     double* sw = xs + (size_t)(2 * 36 + 1) * ld;
     hipLaunchKernelGGL(k_vars_mbpol3b, dim3(nb), dim3(64), 0, st, list, n_dev, cap, N, xs, sw, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_mbpol_vars_pair(const int* list3, const int* n3_dev, int cap3, double* ws3, const int* list2,
+                                  const int* n2_dev, int cap2, double* ws2, const NmerCtx& N, hipStream_t st) {
+    if (cap3 <= 0 || cap2 <= 0) return hipErrorInvalidValue;
+    const int nb3 = (cap3 + 63) / 64, nb2 = (cap2 + 63) / 64;
+    const int ld3 = nb3 * 64, ld2 = nb2 * 64;
+    // workspace layouts of launch_mbpol3b_vars / launch_mbpol2b_vars: xs first, sw after xs, gs and ps
+    hipLaunchKernelGGL(k_vars_mbpol_pair, dim3(nb3 + nb2), dim3(64), 0, st, list3, n3_dev, cap3, ws3,
+                       ws3 + (size_t)(2 * 36 + 1) * ld3, ld3, nb3, list2, n2_dev, cap2, ws2,
+                       ws2 + (size_t)(2 * 31 + 1) * ld2, ld2, N);
     return hipGetLastError();
 }
 

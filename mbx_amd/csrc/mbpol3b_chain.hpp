@@ -102,6 +102,27 @@ __device__ __forceinline__ void group_fence(size_t& off, double* v) {
                    "+v"(v[20]), "+v"(v[21]), "+v"(v[22]), "+v"(v[23]), "+v"(v[24]), "+v"(v[25]), "+v"(v[26]));
 }
 
+// Stage 1 for the trimer of lane t: its 36 variables into xs (entry (j, t) at j*ld + t) and its
+// switch product into sw; lanes at or past min(*n_dev, cap) do nothing.
+__device__ __forceinline__ void vars3_lane(int t, const int* __restrict__ list, const int* __restrict__ n_dev,
+                                           int cap, const NmerCtx& N, double* __restrict__ xs,
+                                           double* __restrict__ sw, int ld) {
+    if (t >= min(*n_dev, cap)) return;
+    double w[27];
+    load_trimer(N, list[3 * t], list[3 * t + 1], list[3 * t + 2], w);
+#pragma unroll
+    for (int j = 0; j < 36; ++j) {
+        const int st = kVar3[j].set, a1 = kVar3[j].a1, a2 = kVar3[j].a2;
+        const double d0 = w[3 * a1] - w[3 * a2];
+        const double d1 = w[3 * a1 + 1] - w[3 * a2 + 1];
+        const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
+        const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        xs[(size_t)j * ld + t] = exp(-kset(st) * (d - dset(st)));
+    }
+    double dab[3], dac[3], dbc[3], rab, rac, rbc, sab, sac, sbc, gab, gac, gbc;
+    sw[t] = switches(w, dab, dac, dbc, rab, rac, rbc, sab, sac, sbc, gab, gac, gbc);
+}
+
 // variables per group of the chain rule's software pipeline (a divisor of 36)
 constexpr int kChainGroup3 = 4;
 

@@ -210,6 +210,9 @@ const double* co2_1b_fit(int* n);
 __global__ void k_reduce_1b(const double* e, const double* vir, int n, int ld, double* out);
 hipError_t launch_poly_mbpol_pair(double* ws3, const int* n3_dev, int cap3, const double* coef3, double* ws2,
                                   const int* n2_dev, int cap2, const double* coef2, int split, hipStream_t st);
+// launch_mbpol3b_vars and launch_mbpol2b_vars as one launch (workspaces as theirs).
+hipError_t launch_mbpol_vars_pair(const int* list3, const int* n3_dev, int cap3, double* ws3, const int* list2,
+                                  const int* n2_dev, int cap2, double* ws2, const NmerCtx& N, hipStream_t st);
 // The same launch with the 3B chain rule (launch_mbpol3b_chain with gradients) in its tail:
 // energies e3_out, gradient added to grad, virials vir3_out (nullable), partial records partials3.
 hipError_t launch_poly_mbpol_pair_chain3(double* ws3, const int* list3, const int* n3_dev, int cap3,
