@@ -850,8 +850,8 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         hipLaunchKernelGGL(k_grid_gas, dim3(c->nconf), dim3(256), 0, st, c->d_in.p, c->d_first.p, c->d_members.p,
                            c->d_pt_off.p, L.d_cell_bound.p, cut, L.d_grids.p);
     if (c->fuse_prep) {  // fix, cloud, cell counts and the zero fills Z in one launch (64-thread
-                         // blocks: the cell-count atomics spread over many CUs)
-        const int nt = std::max(c->nmon, np);
+                         // blocks: the cell-count atomics spread over many CUs; eight lanes a point)
+        const int nt = std::max(c->nmon, 8 * np);
         hipLaunchKernelGGL(k_cloud_fix, dim3((nt + 63) / 64), dim3(64), 0, st, c->d_in.p, c->d_xyz.p, c->d_first.p,
                            c->d_nat.p, c->nmon, c->d_members.p, c->d_tag.p, c->d_kind.p, np, c->box, L.d_grids.p,
                            c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.pinfo.p,

@@ -146,16 +146,19 @@ __global__ void __launch_bounds__(256) k_grid_gas(const double* __restrict__ xyz
 // Every slot is also given a cell of its configuration's grid.  Under PBC the grid spans the
 // extended fractional range [-1, 1]^3 that the slots occupy (a slot's fractional position is
 // its point's plus the image shift), so a query only visits slots near it instead of all 8
-// slots of the points near it.  Cell index = cell_base[config] + local cell.  One thread per
-// point.
-// The slots of point p: emit(slot, x, y, z, cell) for the point's own slot p and, under PBC, for
-// its image slots p + shift * np.  pinfo (nullable) receives the point's data.
+// slots of the points near it.  Cell index = cell_base[config] + local cell.  k_cloud runs one
+// thread per point (cloud_point), k_cloud_fix eight, one slot pattern each (cloud_pattern).
+// Slot pattern pat of point p (pat 0: the point's own slot; pat 1..7: the image whose non-zero
+// directions are pat's bits m, n, l): emit(slot, x, y, z, cell) when the slot exists (always for
+// pat 0; under PBC for the admissible images; never for images in the gas phase).  The patterns
+// map to distinct slots (pat -> shift is one to one), so they may run in any order or in parallel.
+// pinfo (nullable) receives the point's data from pattern 0.
 template <class Emit>
-__device__ __forceinline__ void cloud_point(int p, const double* q, int mp, const int* __restrict__ tag,
-                                            const int* __restrict__ kind, int np, const Box& B,
-                                            const Grid* __restrict__ grids, const int* __restrict__ pconf,
-                                            const int* __restrict__ cell_base, PtInfo* __restrict__ pinfo,
-                                            Emit emit) {
+__device__ __forceinline__ void cloud_pattern(int p, int pat, const double* q, int mp, const int* __restrict__ tag,
+                                              const int* __restrict__ kind, int np, const Box& B,
+                                              const Grid* __restrict__ grids, const int* __restrict__ pconf,
+                                              const int* __restrict__ cell_base, PtInfo* __restrict__ pinfo,
+                                              Emit emit) {
     const int conf = pconf[p];
     const Grid G = grids[conf];
     const int base = cell_base[conf];
@@ -169,57 +172,68 @@ __device__ __forceinline__ void cloud_point(int p, const double* q, int mp, cons
         return base + (cc * G.n[1] + cb) * G.n[0] + ca;
     };
     double qx = q[0], qy = q[1], qz = q[2];
-    if (pinfo) pinfo[p] = PtInfo{qx, qy, qz, tag[mp], kind[mp]};
-    if (B.periodic) {
-        double u, v, w;
-        to_frac(B, qx, qy, qz, u, v, w);
-        u -= floor(u + 0.5);
-        v -= floor(v + 0.5);
-        w -= floor(w + 0.5);
-        to_cart(B, u, v, w, qx, qy, qz);
-        // GetFractionalCoordinates (kdtree_utils.h:186-189)
-        const double uvw0 = B.f00 * qx - B.f01 * qy + B.f02 * qz;
-        const double uvw1 = B.f11 * qy + B.f12 * qz;
-        const double uvw2 = B.f22 * qz;
-        to_frac(B, qx, qy, qz, u, v, w);  // cell placement only
-        // XyzToCloud's loops over m, n, l in {0, -1, 1} (kdtree_utils.h:190-225) write image
-        // slot `shift` once per admissible combination of its non-zero directions, the last one
-        // kept.  Per dimension that is +1 when admissible, else -1: a coordinate of exactly 0
-        // admits both, and the slot must still be emitted (and counted) once.
-        auto dir = [](double f) { return !((f + 1) > 1.0 || (f + 1) < -1.0) ? 1 : (!((f - 1) > 1.0 || (f - 1) < -1.0) ? -1 : 0); };
-        auto zero_ok = [](double f) { return !(f > 1.0 || f < -1.0); };
-        const int dm = dir(uvw0), dn = dir(uvw1), dl = dir(uvw2);
-        const bool zm = zero_ok(uvw0), zn = zero_ok(uvw1), zl = zero_ok(uvw2);
-#pragma unroll 1
-        for (int pat = 1; pat < 8; ++pat) {
-            const int m = (pat & 1) ? dm : 0, n = (pat & 2) ? dn : 0, l = (pat & 4) ? dl : 0;
-            if ((pat & 1) ? !m : !zm) continue;
-            if ((pat & 2) ? !n : !zn) continue;
-            if ((pat & 4) ? !l : !zl) continue;
-            int shift;
-            if (m == 0 && n == 0)
-                shift = 1;
-            else if (m == 0 && l == 0)
-                shift = 2;
-            else if (n == 0 && l == 0)
-                shift = 3;
-            else if (m != 0 && n != 0 && l != 0)
-                shift = 4;
-            else if (m == 0)
-                shift = 5;
-            else if (n == 0)
-                shift = 6;
-            else
-                shift = 7;
-            const double sx = (double)m * B.b[0] + (double)n * B.b[3] + (double)l * B.b[6];
-            const double sy = (double)m * B.b[1] + (double)n * B.b[4] + (double)l * B.b[7];
-            const double sz = (double)m * B.b[2] + (double)n * B.b[5] + (double)l * B.b[8];
-            emit(p + shift * np, qx + sx, qy + sy, qz + sz, cell_of(u + m, v + n, w + l));
-        }
-        emit(p, qx, qy, qz, cell_of(u, v, w));
-    } else {
-        emit(p, qx, qy, qz, cell_of(qx, qy, qz));
+    if (pat == 0 && pinfo) pinfo[p] = PtInfo{qx, qy, qz, tag[mp], kind[mp]};
+    if (!B.periodic) {
+        if (pat == 0) emit(p, qx, qy, qz, cell_of(qx, qy, qz));
+        return;
     }
+    double u, v, w;
+    to_frac(B, qx, qy, qz, u, v, w);
+    u -= floor(u + 0.5);
+    v -= floor(v + 0.5);
+    w -= floor(w + 0.5);
+    to_cart(B, u, v, w, qx, qy, qz);
+    to_frac(B, qx, qy, qz, u, v, w);  // cell placement only
+    if (pat == 0) {
+        emit(p, qx, qy, qz, cell_of(u, v, w));
+        return;
+    }
+    // GetFractionalCoordinates (kdtree_utils.h:186-189)
+    const double uvw0 = B.f00 * qx - B.f01 * qy + B.f02 * qz;
+    const double uvw1 = B.f11 * qy + B.f12 * qz;
+    const double uvw2 = B.f22 * qz;
+    // XyzToCloud's loops over m, n, l in {0, -1, 1} (kdtree_utils.h:190-225) write image
+    // slot `shift` once per admissible combination of its non-zero directions, the last one
+    // kept.  Per dimension that is +1 when admissible, else -1: a coordinate of exactly 0
+    // admits both, and the slot must still be emitted (and counted) once.
+    auto dir = [](double f) { return !((f + 1) > 1.0 || (f + 1) < -1.0) ? 1 : (!((f - 1) > 1.0 || (f - 1) < -1.0) ? -1 : 0); };
+    auto zero_ok = [](double f) { return !(f > 1.0 || f < -1.0); };
+    const int m = (pat & 1) ? dir(uvw0) : 0, n = (pat & 2) ? dir(uvw1) : 0, l = (pat & 4) ? dir(uvw2) : 0;
+    if ((pat & 1) ? !m : !zero_ok(uvw0)) return;
+    if ((pat & 2) ? !n : !zero_ok(uvw1)) return;
+    if ((pat & 4) ? !l : !zero_ok(uvw2)) return;
+    int shift;
+    if (m == 0 && n == 0)
+        shift = 1;
+    else if (m == 0 && l == 0)
+        shift = 2;
+    else if (n == 0 && l == 0)
+        shift = 3;
+    else if (m != 0 && n != 0 && l != 0)
+        shift = 4;
+    else if (m == 0)
+        shift = 5;
+    else if (n == 0)
+        shift = 6;
+    else
+        shift = 7;
+    const double sx = (double)m * B.b[0] + (double)n * B.b[3] + (double)l * B.b[6];
+    const double sy = (double)m * B.b[1] + (double)n * B.b[4] + (double)l * B.b[7];
+    const double sz = (double)m * B.b[2] + (double)n * B.b[5] + (double)l * B.b[8];
+    emit(p + shift * np, qx + sx, qy + sy, qz + sz, cell_of(u + m, v + n, w + l));
+}
+
+// Every slot of point p (the image patterns, then the point's own slot), one thread.
+template <class Emit>
+__device__ __forceinline__ void cloud_point(int p, const double* q, int mp, const int* __restrict__ tag,
+                                            const int* __restrict__ kind, int np, const Box& B,
+                                            const Grid* __restrict__ grids, const int* __restrict__ pconf,
+                                            const int* __restrict__ cell_base, PtInfo* __restrict__ pinfo,
+                                            Emit emit) {
+#pragma unroll 1
+    for (int pat = 1; pat < 8; ++pat)
+        cloud_pattern(p, pat, q, mp, tag, kind, np, B, grids, pconf, cell_base, nullptr, emit);
+    cloud_pattern(p, 0, q, mp, tag, kind, np, B, grids, pconf, cell_base, pinfo, emit);
 }
 
 __global__ void k_cloud(const double* __restrict__ xyz, const int* __restrict__ first_atom,
@@ -264,9 +278,11 @@ __global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ 
                             double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
                             int* __restrict__ cell, PtInfo* __restrict__ pinfo, int* __restrict__ cell_count,
                             ZeroJobs Z) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    zero_jobs(Z, i, (size_t)gridDim.x * blockDim.x);
-    if (i < nmon) fix_monomer(i, in, xyz, first_atom, nat, B);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    zero_jobs(Z, g, (size_t)gridDim.x * blockDim.x);
+    if (g < nmon) fix_monomer(g, in, xyz, first_atom, nat, B);
+    // eight lanes per cloud point, one slot pattern each (the grid covers max(nmon, 8 np))
+    const int i = g >> 3, pat = g & 7;
     if (i >= np) return;
     const int mp = members[i];
     const double* x0 = in + 3 * (size_t)first_atom[mp];
@@ -275,14 +291,14 @@ __global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ 
         double u0, v0, w0;
         fixed_first(B, x0, u0, v0, w0, q);
     }
-    cloud_point(i, q, mp, tag, kind, np, B, grids, pconf, cell_base, pinfo,
-                [&](int s, double x, double y, double z, int c) {
-                    cx[s] = x;
-                    cy[s] = y;
-                    cz[s] = z;
-                    cell[s] = c;
-                    atomicAdd(&cell_count[c], 1);
-                });
+    cloud_pattern(i, pat, q, mp, tag, kind, np, B, grids, pconf, cell_base, pinfo,
+                  [&](int s, double x, double y, double z, int c) {
+                      cx[s] = x;
+                      cy[s] = y;
+                      cz[s] = z;
+                      cell[s] = c;
+                      atomicAdd(&cell_count[c], 1);
+                  });
 }
 
 // ---------------------------------------------------------------------------------------------
