@@ -98,7 +98,7 @@ struct ListState {
     DBuf<int> d_cell_base, d_cell_bound;
     DBuf<double> cx, cy, cz, bx, by, bz;
     DBuf<PtInfo> pinfo;           // per cloud point: first atom, tag, kind
-    DBuf<int> cell, cell_count, cell_start, cell_cursor, cell_slot;
+    DBuf<int> cell, cell_count, cell_start, cell_pos, cell_slot;  // cell_pos: a slot's place in its cell
     DBuf<int4> counts, offsets, counts_x, offsets_x;
     DBuf<int4> wpre, wpre_x;      // per (query, wavefront): prefix within the query
     DBuf<int> list2, list3, evals[kFamilies];
@@ -285,7 +285,7 @@ extern "C" int mbxg_destroy(mbxg_ctx* c) {
     L.bz.release();
     L.cell.release();
     L.cell_count.release();
-    L.cell_cursor.release();
+    L.cell_pos.release();
     L.cell_start.release();
     L.cell_slot.release();
     L.counts.release();
@@ -806,7 +806,7 @@ int prepare_layout(mbxg_ctx* c, double cut) {
         total += bound[k];
     }
     if (L.d_grids.reserve(nconf) || L.d_cell_base.reserve(nconf) || L.d_cell_bound.reserve(nconf) ||
-        L.cell_count.reserve(total) || L.cell_start.reserve(total + 1) || L.cell_cursor.reserve(total))
+        L.cell_count.reserve(total) || L.cell_start.reserve(total + 1))
         return fail(MBXG_ENOMEM, "cell layout: device allocation failed");
     hipStream_t st = c->stream;
     launch_zero_int(L.cell_count.p, total, st);  // every scan leaves the counts zero again
@@ -839,7 +839,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
     if (!L.layout_ok || L.layout_cut != cut)
         if (int r = prepare_layout(c, cut)) return r;
     if (L.cx.reserve(ns) || L.cy.reserve(ns) || L.cz.reserve(ns) || L.bx.reserve(ns) || L.by.reserve(ns) ||
-        L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_slot.reserve(ns) || L.pinfo.reserve(np) ||
+        L.bz.reserve(ns) || L.cell.reserve(ns) || L.cell_pos.reserve(ns) || L.cell_slot.reserve(ns) || L.pinfo.reserve(np) ||
         L.counts.reserve(np) || L.wpre.reserve((size_t)np * kQueryWaves) || L.wpre_x.reserve((size_t)np * kQueryWaves) ||
         L.offsets.reserve(np + 1) || L.counts_x.reserve(np) || L.offsets_x.reserve(np + 1) ||
         L.wmax.reserve(np) || L.wmax_x.reserve(np) || L.maxw.reserve(2))
@@ -855,7 +855,7 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         hipLaunchKernelGGL(k_cloud_fix, dim3((nt + 63) / 64), dim3(64), 0, st, c->d_in.p, c->d_xyz.p, c->d_first.p,
                            c->d_nat.p, c->nmon, c->d_members.p, c->d_tag.p, c->d_kind.p, np, c->box, L.d_grids.p,
                            c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p, L.cell.p, L.pinfo.p,
-                           L.cell_count.p, Z);
+                           L.cell_count.p, L.cell_pos.p, Z);
     } else {
         if (c->nmon > 0)
             hipLaunchKernelGGL(k_fix_monomers, dim3((c->nmon + 255) / 256), dim3(256), 0, st, c->d_in.p,
@@ -863,12 +863,11 @@ int build_lists(mbxg_ctx* c, const NmerCtx& N, double c2sq, double c3sq, double 
         hipLaunchKernelGGL(k_cloud, dim3(pb), dim3(tb), 0, st, c->d_xyz.p, c->d_first.p, c->d_members.p, c->d_tag.p,
                            c->d_kind.p, np, c->box, L.d_grids.p, c->d_pconf.p, L.d_cell_base.p, L.cx.p, L.cy.p, L.cz.p,
                            L.cell.p, L.pinfo.p);
-        hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p);
+        hipLaunchKernelGGL(k_bin_count, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_count.p, L.cell_pos.p);
     }
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p,
-                       L.cell_cursor.p);
-    hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_cursor.p, L.cx.p, L.cy.p, L.cz.p,
-                       L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, L.cell_count.p, L.ncell_total, L.cell_start.p);
+    hipLaunchKernelGGL(k_bin_fill, dim3(sb), dim3(tb), 0, st, L.cell.p, ns, L.cell_start.p, L.cell_pos.p, L.cx.p,
+                       L.cy.p, L.cz.p, L.cell_slot.p, L.bx.p, L.by.p, L.bz.p);
     HIPCHK(hipGetLastError());
     NmersArgs A{};
     A.pinfo = L.pinfo.p;

@@ -277,7 +277,7 @@ __global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ 
                             const int* __restrict__ pconf, const int* __restrict__ cell_base,
                             double* __restrict__ cx, double* __restrict__ cy, double* __restrict__ cz,
                             int* __restrict__ cell, PtInfo* __restrict__ pinfo, int* __restrict__ cell_count,
-                            ZeroJobs Z) {
+                            int* __restrict__ cell_pos, ZeroJobs Z) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     zero_jobs(Z, g, (size_t)gridDim.x * blockDim.x);
     if (g < nmon) fix_monomer(g, in, xyz, first_atom, nat, B);
@@ -297,25 +297,27 @@ __global__ void k_cloud_fix(const double* __restrict__ in, double* __restrict__ 
                       cy[s] = y;
                       cz[s] = z;
                       cell[s] = c;
-                      atomicAdd(&cell_count[c], 1);
+                      cell_pos[s] = atomicAdd(&cell_count[c], 1);
                   });
 }
 
 // ---------------------------------------------------------------------------------------------
 // Binning of slots into cells: count, scan, fill.  The fill also writes the slots' coordinates
-// in cell order, so the gather reads them contiguously.  The order of slots inside a cell is
-// left to the atomics; the n-mer kernel sorts each query's matches, which makes the output
-// order independent of it.  cell_count is zero on entry: zeroed once when the cell layout is
-// made, and left zero by every scan.
-__global__ void k_bin_count(const int* __restrict__ cell, int ns, int* __restrict__ cell_count) {
+// in cell order, so the gather reads them contiguously.  A slot's place inside its cell is the
+// count its cell-count atomic returned (cell_pos), so the order inside a cell is left to the
+// atomics; the n-mer kernel sorts each query's matches, which makes the output order independent
+// of it.  cell_count is zero on entry: zeroed once when the cell layout is made, and left zero by
+// every scan.
+__global__ void k_bin_count(const int* __restrict__ cell, int ns, int* __restrict__ cell_count,
+                            int* __restrict__ cell_pos) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < ns) atomicAdd(&cell_count[cell[s]], 1);
+    if (s < ns) cell_pos[s] = atomicAdd(&cell_count[cell[s]], 1);
 }
 
-// Single-workgroup exclusive scan of ncell cell counts; cell_start gets ncell+1 entries, the
-// fill cursor starts at each cell's start, and cell_count is left zero for the next build.
+// Single-workgroup exclusive scan of ncell cell counts; cell_start gets ncell+1 entries, and
+// cell_count is left zero for the next build.
 __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_count, int ncell,
-                                                     int* __restrict__ cell_start, int* __restrict__ cursor) {
+                                                     int* __restrict__ cell_start) {
     __shared__ int sh[1024];
     const int t = threadIdx.x;
     const int per = (ncell + 1023) / 1024;
@@ -333,21 +335,19 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ cell_coun
     int o = sh[t] - acc;
     for (int c = beg; c < end; ++c) {
         cell_start[c] = o;
-        cursor[c] = o;
         o += cell_count[c];
         cell_count[c] = 0;
     }
     if (t == 1023) cell_start[ncell] = sh[t];
 }
 
-__global__ void k_bin_fill(const int* __restrict__ cell, int ns, int* __restrict__ cursor,
-                           const double* __restrict__ cx, const double* __restrict__ cy,
-                           const double* __restrict__ cz, int* __restrict__ cell_slot, double* __restrict__ bx,
-                           double* __restrict__ by, double* __restrict__ bz) {
+__global__ void k_bin_fill(const int* __restrict__ cell, int ns, const int* __restrict__ cell_start,
+                           const int* __restrict__ cell_pos, const double* __restrict__ cx,
+                           const double* __restrict__ cy, const double* __restrict__ cz, int* __restrict__ cell_slot,
+                           double* __restrict__ bx, double* __restrict__ by, double* __restrict__ bz) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const int c = cell[s];
-    const int at = atomicAdd(&cursor[c], 1);
+    const int at = cell_start[cell[s]] + cell_pos[s];
     cell_slot[at] = s;
     bx[at] = cx[s];
     by[at] = cy[s];

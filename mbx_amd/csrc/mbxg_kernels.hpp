@@ -54,7 +54,8 @@ __global__ void k_grid_gas(const double* xyz, const int* first_atom, const int* 
 __global__ void k_cloud(const double* xyz, const int* first_atom, const int* members, const int* tag, const int* kind,
                         int np, Box B, const Grid* grids, const int* pconf, const int* cell_base, double* cx,
                         double* cy, double* cz, int* cell, PtInfo* pinfo);
-__global__ void k_bin_count(const int* cell, int ns, int* cell_count);
+// cell_pos[s] receives slot s's place among its cell's slots (the count before its own).
+__global__ void k_bin_count(const int* cell, int ns, int* cell_count, int* cell_pos);
 // Up to four device ranges to zero in one launch (null pointers skipped).
 struct ZeroJobs {
     double* p[4];
@@ -67,10 +68,10 @@ __global__ void k_zero_multi(ZeroJobs Z);
 __global__ void k_cloud_fix(const double* in, double* xyz, const int* first_atom, const int* nat, int nmon,
                             const int* members, const int* tag, const int* kind, int np, Box B, const Grid* grids,
                             const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell,
-                            PtInfo* pinfo, int* cell_count, ZeroJobs Z);
-__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start, int* cursor);
-__global__ void k_bin_fill(const int* cell, int ns, int* cursor, const double* cx, const double* cy,
-                           const double* cz, int* cell_slot, double* bx, double* by, double* bz);
+                            PtInfo* pinfo, int* cell_count, int* cell_pos, ZeroJobs Z);
+__global__ void k_scan_cells(int* cell_count, int ncell, int* cell_start);
+__global__ void k_bin_fill(const int* cell, int ns, const int* cell_start, const int* cell_pos, const double* cx,
+                           const double* cy, const double* cz, int* cell_slot, double* bx, double* by, double* bz);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_int(int* p, size_t n);
 
