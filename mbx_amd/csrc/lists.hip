@@ -886,58 +886,43 @@ __device__ __forceinline__ int4 shfl_up4(int4 v, int o) {
     return make_int4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64), __shfl_up(v.w, o, 64));
 }
 
-// Block-wide exclusive scan (1024 threads) of counts[0..n) into offsets[0..n], offsets[n] the total.
-__device__ __forceinline__ void scan_block_fast(const int4* __restrict__ counts, int n, int4* __restrict__ offsets,
-                                                bool max_w, int4* wsum) {
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int per = (n + 1023) / 1024;
-    const int beg = min(n, t * per), end = min(n, beg + per);
-    const int4 zero = make_int4(0, 0, 0, 0);
-    int4 acc = zero;
-    for (int i = beg; i < end; ++i) acc = combine4(acc, counts[i], max_w);
-    int4 incl = acc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int4 y = shfl_up4(incl, o);
-        if (lane >= o) incl = combine4(y, incl, max_w);
-    }
-    int4 excl = shfl_up4(incl, 1);
-    if (lane == 0) excl = zero;
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
-    int4 base = zero;
-    for (int w = 0; w < wv; ++w) base = combine4(base, wsum[w], max_w);
-    int4 o = combine4(base, excl, max_w);
-    for (int i = beg; i < end; ++i) {
-        offsets[i] = o;
-        o = combine4(o, counts[i], max_w);
-    }
-    if (t == 1023) {
-        int4 tot = zero;
-        for (int w = 0; w < 16; ++w) tot = combine4(tot, wsum[w], max_w);
-        offsets[n] = tot;
-    }
-    __syncthreads();
-}
-
 }  // namespace
 
 __global__ void __launch_bounds__(1024) k_scan_all(const int4* __restrict__ wmax, const int4* __restrict__ wmax_x,
                                                    int4* __restrict__ maxw, const int4* __restrict__ counts,
                                                    int4* __restrict__ offsets, const int4* __restrict__ counts_x,
                                                    int4* __restrict__ offsets_x, int n) {
+    // Both scans and the two maxima in one pass: every thread's loads are issued together, and one
+    // barrier separates the per-wavefront sums from their use.
     __shared__ int4 wsum[2][16];
     __shared__ int4 msum[2][16];
-    scan_block_fast(counts, n, offsets, false, wsum[0]);
-    scan_block_fast(counts_x, n, offsets_x, true, wsum[1]);
-    if (wmax) {
-        const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-        int4 m = make_int4(0, 0, 0, 0), mx = make_int4(0, 0, 0, 0);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int per = (n + 1023) / 1024;
+    const int beg = min(n, t * per), end = min(n, beg + per);
+    const int4 zero = make_int4(0, 0, 0, 0);
+    auto max4 = [](int4 a, int4 b) { return make_int4(max(a.x, b.x), max(a.y, b.y), max(a.z, b.z), max(a.w, b.w)); };
+    int4 acc = zero, accx = zero, m = zero, mx = zero;
+    for (int i = beg; i < end; ++i) {
+        acc = combine4(acc, counts[i], false);
+        accx = combine4(accx, counts_x[i], true);
+    }
+    if (wmax)
         for (int i = t; i < n; i += 1024) {
-            const int4 c = wmax[i], cx = wmax_x[i];
-            m = make_int4(max(m.x, c.x), max(m.y, c.y), max(m.z, c.z), max(m.w, c.w));
-            mx = make_int4(max(mx.x, cx.x), max(mx.y, cx.y), max(mx.z, cx.z), max(mx.w, cx.w));
+            m = max4(m, wmax[i]);
+            mx = max4(mx, wmax_x[i]);
         }
+    int4 incl = acc, inclx = accx;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int4 y = shfl_up4(incl, o), yx = shfl_up4(inclx, o);
+        if (lane >= o) {
+            incl = combine4(y, incl, false);
+            inclx = combine4(yx, inclx, true);
+        }
+    }
+    int4 excl = shfl_up4(incl, 1), exclx = shfl_up4(inclx, 1);
+    if (lane == 0) excl = exclx = zero;
+    if (wmax) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             m = make_int4(max(m.x, __shfl_xor(m.x, o, 64)), max(m.y, __shfl_xor(m.y, o, 64)),
@@ -945,19 +930,41 @@ __global__ void __launch_bounds__(1024) k_scan_all(const int4* __restrict__ wmax
             mx = make_int4(max(mx.x, __shfl_xor(mx.x, o, 64)), max(mx.y, __shfl_xor(mx.y, o, 64)),
                            max(mx.z, __shfl_xor(mx.z, o, 64)), max(mx.w, __shfl_xor(mx.w, o, 64)));
         }
-        if (lane == 0) {
-            msum[0][wv] = m;
-            msum[1][wv] = mx;
+    }
+    if (lane == 63) {
+        wsum[0][wv] = incl;
+        wsum[1][wv] = inclx;
+    }
+    if (lane == 0) {
+        msum[0][wv] = m;
+        msum[1][wv] = mx;
+    }
+    __syncthreads();
+    int4 base = zero, basex = zero;
+    for (int w = 0; w < wv; ++w) {
+        base = combine4(base, wsum[0][w], false);
+        basex = combine4(basex, wsum[1][w], true);
+    }
+    int4 o = combine4(base, excl, false), ox = combine4(basex, exclx, true);
+    for (int i = beg; i < end; ++i) {
+        offsets[i] = o;
+        offsets_x[i] = ox;
+        o = combine4(o, counts[i], false);
+        ox = combine4(ox, counts_x[i], true);
+    }
+    if (t == 1023) {
+        int4 tot = zero, totx = zero;
+        for (int w = 0; w < 16; ++w) {
+            tot = combine4(tot, wsum[0][w], false);
+            totx = combine4(totx, wsum[1][w], true);
         }
-        __syncthreads();
-        if (t < 2) {
-            int4 r = msum[t][0];
-            for (int w = 1; w < 16; ++w) {
-                const int4 c = msum[t][w];
-                r = make_int4(max(r.x, c.x), max(r.y, c.y), max(r.z, c.z), max(r.w, c.w));
-            }
-            maxw[t] = r;
-        }
+        offsets[n] = tot;
+        offsets_x[n] = totx;
+    }
+    if (wmax && t < 2) {
+        int4 r = msum[t][0];
+        for (int w = 1; w < 16; ++w) r = max4(r, msum[t][w]);
+        maxw[t] = r;
     }
 }
 
