@@ -12,9 +12,12 @@
 //   k_scan_all              offsets; totals at offsets[np]
 //   k_compact_evals         evaluate lists (one-pass builds)
 //   1B, launch_mbpol2b/3b   variables, polynomials, chain rule over the evaluate lists; n-mer
-//   / MB-nrg families       counts read on the device
+//   / MB-nrg families       counts read on the device.  With both MB-pol orders and gradients:
+//                           k_vars_mbpol_pair (both orders' variables), k_poly_eg_pair_chain3
+//                           (both polynomials, the 3B chain rule in the 3B blocks' tail),
+//                           k_grad_mbpol2b
 //   k_finish                deterministic energy / virial totals per family and per order
-//   run_xfam                (co2, halides, 4B) further families, synchronously
+//   run_xfam                (co2, ions, 4B) further families, synchronously
 // Capacities (list buffers, match storage) come from a synchronised first call and carry
 // headroom.  Every read of results synchronises and checks the totals against them; a call that
 // exceeded them is recomputed synchronously with grown buffers, so results are always whole.

@@ -63,8 +63,8 @@ struct ZeroJobs {
 };
 // k_zero over every range of Z, grid-stride.
 __global__ void k_zero_multi(ZeroJobs Z);
-// k_fix_monomers, k_cloud and k_bin_count in one launch of max(nmon, np) threads, which also
-// zero the ranges of Z.
+// k_fix_monomers, k_cloud and k_bin_count in one launch of max(nmon, 8 np) threads (eight per
+// cloud point, one slot pattern each), which also zero the ranges of Z.
 __global__ void k_cloud_fix(const double* in, double* xyz, const int* first_atom, const int* nat, int nmon,
                             const int* members, const int* tag, const int* kind, int np, Box B, const Grid* grids,
                             const int* pconf, const int* cell_base, double* cx, double* cy, double* cz, int* cell,

@@ -3,7 +3,9 @@
 // It runs as its own kernel so the register allocator sees nothing but the generated
 // straight-line polynomial (x and g arrays plus a few trie temporaries); the variable and
 // chain-rule stages, whose exp/sqrt code the instruction selector freely interleaves, would
-// otherwise push the polynomial past the register file.
+// otherwise push the polynomial past the register file.  The one exception,
+// k_poly_eg_pair_chain3 (poly_mbpol_pair.hip), runs the 3B chain rule strictly after a block's
+// polynomial, behind a fence, so the two phases' registers are allocated one after the other.
 //
 // Layout: xs and gs are variable-major ("SoA"), entry (j, t) at j*ld + t, so a wavefront's 64
 // lanes read and write 64 consecutive doubles per variable.
