@@ -229,8 +229,9 @@ int mbxg_set_graph(mbxg_ctx* ctx, int on);
  * one launch.  mbxg_autotune times every split of both MB-pol orders
  * on the current inputs
  * (the polynomial stage, `reps` timed repetitions after one warm-up) and, when CH4-H2O-H2O trimers
- * are evaluated, every split of that family (its whole evaluation), keeps the fastest of each and
- * recomputes once with them.  No counterpart in the reference (its kernels have no
+ * are evaluated, every split of that family (its whole evaluation), keeps the fastest of each,
+ * then, on the paired path, every combination of "fuse_chain" and "fuse_vars" (the whole compute),
+ * and recomputes once with them.  No counterpart in the reference (its kernels have no
  * launch shape). */
 int mbxg_set_tuning(mbxg_ctx* ctx, const char* key, int value);
 int mbxg_get_tuning(mbxg_ctx* ctx, const char* key, int* value);

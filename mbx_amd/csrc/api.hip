@@ -1528,6 +1528,32 @@ extern "C" int mbxg_autotune(mbxg_ctx* c, int terms, int do_grads, int reps) {
         }
         c->split3n = pick3n;
     }
+    // The paired path's fusions (the 3B chain rule in the polynomial launch, both orders'
+    // variables in one launch): every combination, timed over the whole compute.
+    if (c->fuse_poly && do_grads && (terms & MBXG_2B) && (terms & MBXG_3B) && family_launched(c, FAM_MBPOL_2B) &&
+        family_launched(c, FAM_MBPOL_3B)) {
+        double bestf = 1e300;
+        int pickc = c->fuse_chain, pickv = c->fuse_vars;
+        for (int fc = 0; fc < 2; ++fc)
+            for (int fv = 0; fv < 2; ++fv) {
+                c->fuse_chain = fc;
+                c->fuse_vars = fv;
+                double t = 1e300;
+                for (int r = 0; r < reps + 1; ++r) {
+                    if (int e = mbxg_compute(c, terms, do_grads)) return e;
+                    double ms[10];
+                    if (int e = mbxg_get_timings(c, ms, 10)) return e;
+                    if (r > 0) t = std::min(t, ms[9]);
+                }
+                if (t < bestf) {
+                    bestf = t;
+                    pickc = fc;
+                    pickv = fv;
+                }
+            }
+        c->fuse_chain = pickc;
+        c->fuse_vars = pickv;
+    }
     if (int r = mbxg_set_timing(c, was_timing ? 1 : 0)) return r;
     return mbxg_compute(c, terms, do_grads);
 }

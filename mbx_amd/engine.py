@@ -229,9 +229,9 @@ class Engine:
         return v.value
 
     def autotune(self, terms=TERM_2B | TERM_3B, do_grads=True, reps=3):
-        """Pick the fastest polynomial schedule for the current inputs (see mbxg_autotune)."""
+        """Pick the fastest polynomial schedule and fusions for the current inputs (see mbxg_autotune)."""
         _check(lib().mbxg_autotune(self._c, int(terms), int(bool(do_grads)), int(reps)))
-        return {k: self.tuning(k) for k in ("split_2b", "split_3b", "split_ch4h2o2_3b")}
+        return {k: self.tuning(k) for k in ("split_2b", "split_3b", "split_ch4h2o2_3b", "fuse_chain", "fuse_vars")}
 
     def set_box(self, box):
         if box is None or len(box) == 0:

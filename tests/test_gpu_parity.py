@@ -242,6 +242,7 @@ def test_autotune_keeps_results(systems):
     before = e.results()
     picked = e.autotune(reps=1)
     assert picked["split_2b"] in (1, 2, 4, 8) and picked["split_3b"] in (1, 2, 4, 8)
+    assert picked["fuse_chain"] in (0, 1) and picked["fuse_vars"] in (0, 1)
     after = e.results()
     # the chosen schedule may differ from the default one: same values up to rounding order
     assert after[0] == pytest.approx(before[0], rel=E_REL)
