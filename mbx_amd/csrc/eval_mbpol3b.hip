@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(kGradBlock3) __attribute__((amdgpu_waves_per_e
                                                               double* __restrict__ partials) {
     __shared__ GradTable<256, 9> T;
     const int t = blockIdx.x * kGradBlock3 + threadIdx.x;
-    chain3<GRAD, false, 256>(t, blockIdx.x * (kGradBlock3 / 64) + threadIdx.x / 64, min(*n_dev, cap), list, N, gs,
-                             ps, sw, ld, grad, e_out, vir_out, partials, T);
+    chain3<GRAD, false, 256>(t, blockIdx.x * (kGradBlock3 / 64) + threadIdx.x / 64, min(*n_dev, cap), list, N,
+                             nullptr, gs, ps, sw, ld, grad, e_out, vir_out, partials, T);
 }
 
 size_t workspace_mbpol3b(int cap) {

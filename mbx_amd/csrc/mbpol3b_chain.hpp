@@ -130,10 +130,10 @@ constexpr int kChainGroup3 = 4;
 // derivatives, virial and gradient scatter through the LDS table T; the 64 lanes' energy and
 // virial sums go to partial record `rec`.  WAVE: T belongs to this wavefront alone (lane-strided
 // init and flush, no block barrier); otherwise every thread of the block calls this and T is the
-// block's.
-template <bool GRAD, bool WAVE, int TB>
+// block's.  READX: the variables are read back from xs instead of recomputed.
+template <bool GRAD, bool WAVE, int TB, bool READX = false>
 __device__ __forceinline__ void chain3(int t, int rec, int n, const int* __restrict__ list, const NmerCtx& N,
-                                       const double* __restrict__ gs, const double* __restrict__ ps,
+                                       const double* __restrict__ xs, const double* __restrict__ gs, const double* __restrict__ ps,
                                        const double* __restrict__ sw, int ld, double* __restrict__ grad,
                                        double* __restrict__ e_out, double* __restrict__ vir_out,
                                        double* __restrict__ partials, GradTable<TB, 9>& T) {
@@ -169,10 +169,11 @@ __device__ __forceinline__ void chain3(int t, int rec, int n, const int* __restr
             // hoists every load and square root and sinks every accumulation: ~500 registers.
             constexpr int G = kChainGroup3;
             size_t toff = (size_t)t;
-            double cg[G], ng[G];
+            double cg[G], ng[G], cx[G], nx[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 cg[k] = gs[(size_t)k * ld + toff];
+                if (READX) cx[k] = xs[(size_t)k * ld + toff];
             }
 #pragma unroll
             for (int g = 0; g < 36 / G; ++g) {
@@ -182,6 +183,7 @@ __device__ __forceinline__ void chain3(int t, int rec, int n, const int* __restr
 #pragma unroll
                     for (int k = 0; k < G; ++k) {
                         ng[k] = gs[(size_t)(G * g + G + k) * ld + toff];
+                        if (READX) nx[k] = xs[(size_t)(G * g + G + k) * ld + toff];
                     }
                 }
 #pragma unroll
@@ -193,9 +195,9 @@ __device__ __forceinline__ void chain3(int t, int rec, int n, const int* __restr
                     const double d2 = w[3 * a1 + 2] - w[3 * a2 + 2];
                     const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
                     const double gj = cg[k] * s;
-                    // the variable itself, recomputed as k_vars_mbpol3b computed it (same bits; one
-                    // exp is cheaper than reading it back)
-                    const double xv = exp(-kset(st) * (d - dset(st)));
+                    // the variable itself: read back (READX) or recomputed as k_vars_mbpol3b computed
+                    // it (the same bits either way; alone, one exp is cheaper than the load)
+                    const double xv = READX ? cx[k] : exp(-kset(st) * (d - dset(st)));
                     const double gg = -kset(st) * gj * xv / d;
                     gd[3 * a1] += gg * d0;
                     gd[3 * a1 + 1] += gg * d1;
@@ -207,6 +209,7 @@ __device__ __forceinline__ void chain3(int t, int rec, int n, const int* __restr
 #pragma unroll
                 for (int k = 0; k < G; ++k) {
                     cg[k] = ng[k];
+                    if (READX) cx[k] = nx[k];
                 }
             }
             group_fence(toff, gd);

@@ -64,9 +64,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3))
     if ((threadIdx.x >> 6) != 0 || (int)blockIdx.x * 64 >= n) return;
     // this wavefront's gs / ps stores are complete before it loads them back
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    chain3<true, true, kChainTable3>(blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.x, n, C.list, N, A.gs, A.ps, C.sw,
-                                     A.ld, C.grad, C.e_out, C.vir_out, C.partials,
-                                     *reinterpret_cast<GradTable<kChainTable3, 9>*>(red));
+    // the variables are read back (their exponentials would be VALU work in a VALU-bound launch)
+    chain3<true, true, kChainTable3, true>(blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.x, n, C.list, N, A.xs, A.gs,
+                                           A.ps, C.sw, A.ld, C.grad, C.e_out, C.vir_out, C.partials,
+                                           *reinterpret_cast<GradTable<kChainTable3, 9>*>(red));
 }
 
 }  // namespace
