@@ -10,12 +10,14 @@ Throughput is listed dimers + trimers per second (the n-mers the reference's Get
 
 The JSON line also carries
   roofline:     the polynomial launch (the dominant kernel: the MB-pol 3B and 2B polynomials in
-                one launch, k_poly_eg_pair; the 3B one alone when the pairing is off) against
-                MI355X FP64 vector peak, with the reference's algorithmic FLOPs per evaluated
-                trimer (81,549) and dimer (27,208), counted from the reference's generated E+G
-                code (mbx_amd/tables/flops.json), timed with HIP events on the engine stream
-                during the timed steps; traffic from the committed rocprofv3 PMC summary when
-                present;
+                one launch with the 3B chain rule in its 3B blocks, k_poly_eg_pair_chain3; without
+                the chain rule when fuse_chain is off; the 3B one alone when the pairing is off)
+                against MI355X FP64 vector peak, with the reference's algorithmic FLOPs per
+                evaluated trimer (81,549, plus 1,094 for the chain rule) and dimer (27,208),
+                counted from the reference's code (mbx_amd/tables/flops.json), timed with HIP events
+                on the engine stream in a stage-timing pass of the same workload; traffic from the
+                committed rocprofv3 PMC summary when present.  config.polynomial_alone times the
+                launch without the chain rule beside it;
   cpu_baseline: the reference's own compiled 2B/3B leaves (oracle/_ref, OpenMP driver) on this
                 host's cores, on the same workload, in two dispatch modes: the reference's own
                 batch of 8 n-mers per polynomial call and a best-effort batch of 1024; the faster
